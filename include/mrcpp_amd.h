@@ -1,0 +1,75 @@
+/*
+ * mrcpp_amd.h -- C ABI of the MI355X MRCPP operator-application library.
+ *
+ * A flat binding of the C++ API in namespace mrcpp (mrcpp_amd/csrc), for
+ * FFI callers (ctypes, cffi, cgo, JNI).  Each function names the MRCPP C++
+ * call it stands for (paths relative to the MRCPP source tree).  Handles are
+ * opaque; every function returning int returns 0 on success and -1 on error,
+ * with mrcpp_last_error() describing the failure.
+ */
+#ifndef MRCPP_AMD_H
+#define MRCPP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char *mrcpp_last_error(void);
+/* Directory with the L_H0_k, L_G0_k, L_c_left_k, L_c_right_k tables
+ * (MWFILTERS_DIR / details::find_filters, src/utils/details.cpp:53-69). */
+void mrcpp_set_filter_dir(const char *dir);
+
+/* MultiResolutionAnalysis<3>(BoundingBox<3>({lo, hi}), LegendreBasis(order), depth)
+ * (src/trees/MultiResolutionAnalysis.h:51-53) */
+int mrcpp_mra_create(int box_lo, int box_hi, int order, int depth, void **mra);
+void mrcpp_mra_destroy(void *mra);
+/* root scale, scaling factor and max scale of the world box */
+int mrcpp_mra_info(void *mra, int *root_scale, double *scaling_factor, int *max_scale);
+
+/* FunctionTree<3>(mra) (src/trees/FunctionTree.h:81) */
+int mrcpp_tree_create(void *mra, void **tree);
+void mrcpp_tree_destroy(void *tree);
+/* Gaussian sums f = sum_i c_i exp(-a_i |r - R_i|^2), centres R_i = centers[3i..3i+2]:
+ * build_grid(tree, GaussExp) (src/treebuilders/grid.cpp:106-123) and
+ * project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102) */
+int mrcpp_tree_build_grid_gaussians(void *tree, int n, const double *coefs, const double *alphas, const double *centers);
+int mrcpp_tree_project_gaussians(void *tree, double prec, int n, const double *coefs, const double *alphas, const double *centers);
+long long mrcpp_tree_num_nodes(void *tree);
+double mrcpp_tree_square_norm(void *tree);
+/* real nodes depth first: idx[5 i] = scale, lx, ly, lz, is_end;
+ * coefs[8 (k+1)^3 i ...] (may be NULL); norms[9 i] = 8 component norms, square norm */
+int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms);
+/* FunctionTree::evalf, integrate (src/trees/FunctionTree.cpp:438-546); dot (src/treebuilders/multiply.cpp:286-318) */
+double mrcpp_tree_evalf(void *tree, double x, double y, double z);
+double mrcpp_tree_integrate(void *tree);
+double mrcpp_tree_dot(void *bra, void *ket);
+
+/* PoissonOperator(mra, prec) (kind 0) / HelmholtzOperator(mra, mu, prec) (kind 1)
+ * (src/operators/PoissonOperator.h:44, src/operators/HelmholtzOperator.h:44) */
+int mrcpp_operator_create(void *mra, int kind, double mu, double prec, void **oper);
+void mrcpp_operator_destroy(void *oper);
+int mrcpp_operator_size(void *oper);
+double mrcpp_operator_build_seconds(void *oper);
+int mrcpp_operator_kernel(void *oper, double *alpha, double *coef);
+/* band widths at an apply precision: out[(term * max_depth + depth) * 5 + c], c = T,C,B,A,max
+ * (MWOperator::calcBandWidths, src/operators/MWOperator.cpp:78-108) */
+int mrcpp_operator_bandwidths(void *oper, double prec, int *out, int max_depth);
+/* one operator node component (OperatorTree::getNode(depth, l), component c) */
+int mrcpp_operator_node(void *oper, int term, int depth, int l, int comp, double *mat, double *norm);
+/* PoissonKernel / HelmholtzKernel expansion (src/operators/PoissonKernel.cpp:48-86,
+ * src/operators/HelmholtzKernel.cpp:47-80); returns the number of terms */
+int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_max, double *alpha, double *coef, int cap);
+
+/* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
+ * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
+ * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
+ * t_reduce, t_transform, t_host */
+int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats);
+/* nodes per breadth-first level of the last apply; returns the number of levels */
+int mrcpp_last_apply_levels(int *level_nodes, int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRCPP_AMD_H */

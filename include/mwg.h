@@ -1,0 +1,167 @@
+/*
+ * mwg.h -- C ABI of the MI355X (gfx950) band-sum and multiwavelet-transform
+ * kernels behind MRCPP's operator application.
+ *
+ * This is the thin layer the host C++ (mrcpp_amd/csrc) calls in place of
+ * MRCPP's OpenMP node loop.  Every entry point replaces one piece of the
+ * reference's apply path; the citation above each declaration names it
+ * (paths relative to the MRCPP source tree).  Plain C types only: device
+ * pointers are void* / typed pointers obtained from mwg_malloc, sizes are
+ * counts.  Every call returns MWG_OK or an error code; mwg_last_error()
+ * describes the last failure of the calling thread.  Work is enqueued on the
+ * library's stream; calls that return host data synchronise it.
+ */
+#ifndef MWG_H
+#define MWG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MWG_OK = 0,
+    MWG_ERR_ARG = 1,       /* invalid argument / shape */
+    MWG_ERR_DEVICE = 2,    /* HIP runtime failure */
+    MWG_ERR_NOMEM = 3,     /* device allocation failed */
+    MWG_ERR_UNSUPPORTED = 4 /* order or depth outside the supported range */
+} mwg_status;
+
+/* Device selection and stream set-up.  Replaces nothing in MRCPP (the
+ * reference has no device); must precede every other call. */
+mwg_status mwg_init(int device);
+const char *mwg_last_error(void);
+int mwg_device_count(void);
+
+/* Device memory helpers (bytes). */
+mwg_status mwg_malloc(void **ptr, size_t bytes);
+mwg_status mwg_free(void *ptr);
+mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
+mwg_status mwg_download(void *dst, const void *src, size_t bytes);
+mwg_status mwg_copy(void *dst, const void *src, size_t bytes); /* device to device */
+mwg_status mwg_zero(void *dst, size_t bytes);
+mwg_status mwg_synchronize(void);
+
+/* Band tables of a convolution operator at one apply precision: widths and
+ * band-size factors (ConvolutionCalculator::initBandSizes / calcBandSizeFactor,
+ * src/treebuilders/ConvolutionCalculator.cpp:105-139), operator component
+ * matrices and norms indexed by (depth, term, translation, component)
+ * (OperatorTree::getNode + OperatorNode::getComponentNorm,
+ * src/trees/OperatorTree.h:53-57, src/trees/OperatorNode.cpp:56-80), and the
+ * separable pair bound used to skip band nodes. All pointers are device. */
+typedef struct {
+    int M;                 /* separation rank (number of Gaussian terms) */
+    int n_depth;           /* depths 0..n_depth-1 */
+    int kp1;               /* k+1 */
+    int wmax;              /* max band width over depths */
+    const int *band_max;   /* [n_depth] */
+    const int *widths;     /* [n_depth][M][5]: T,C,B,A,max */
+    const int *band_size;  /* [n_depth][M][64] */
+    const long long *mat_start; /* [n_depth][M] first matrix (dl=-w, comp 0) or -1 */
+    const double *mats;    /* matrices (kp1*kp1 each, column-major: O(i_f, j_g)) */
+    const double *mat_norm;/* one norm per matrix */
+    const double *pair_bound; /* [n_depth][2*wmax+1] */
+} mwg_band_tables;
+
+/* Input function tree resident on the device: an open-addressing hash from a
+ * packed node key to a node id, and per-node coefficient offset, 8 component
+ * norms + square norm, and a generated-node flag.  Generated nodes hold the
+ * scaling block only (FunctionNode::genChildren, src/trees/FunctionNode.cpp:293-328). */
+typedef struct {
+    int kp1;
+    int root_scale;
+    int corner[3];
+    int nbox[3];
+    const unsigned long long *hash_keys; /* [hash_cap], 0 = empty */
+    const int *hash_vals;                /* [hash_cap] */
+    int hash_cap;                        /* power of two */
+    double *coef;                        /* coefficient pool */
+    const long long *coef_off;           /* [nodes] offset into coef (doubles) */
+    double *norms;                       /* [nodes][9] */
+    const unsigned char *is_gen;         /* [nodes] */
+} mwg_ftree;
+
+/* One breadth-first level of output nodes, all at the same scale.
+ * Output coefficients go to g_coef + g_off[i]; norms to g_norms + 9 i. */
+typedef struct {
+    int kp1;             /* k+1 of the output coefficients */
+    int count;
+    int scale;
+    int o_depth;         /* scale - operator root scale */
+    double g_thr;        /* screening threshold (<= 0 disables screening) */
+    const int *l;        /* [count][3] translations */
+    double *g_coef;
+    const long long *g_off; /* [count] */
+    double *g_norms;     /* [count][9] */
+} mwg_level;
+
+/* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the
+ * band node id, pair_f < 0 encodes -(ancestor id + 1) when the band node does
+ * not exist yet and must be generated below its nearest existing ancestor
+ * (MWTree::getNode -> MWNode::retrieveNode, src/trees/MWNode.cpp:1097-1120). */
+typedef struct {
+    int capacity;
+    int *count;          /* device scalar: number of pairs written/needed */
+    int *pair_g;         /* [capacity] output-node index in the level */
+    int *pair_f;         /* [capacity] */
+    int *pair_l;         /* [capacity][3] band node translation */
+    int *g_first;        /* [level count] first pair of each output node */
+    int *g_npairs;       /* [level count] */
+} mwg_pairs;
+
+/* Screen the band of every output node of a level with the separable bound
+ * and emit candidate pairs (ConvolutionCalculator::makeOperBand/fillOperBand,
+ * src/treebuilders/ConvolutionCalculator.cpp:142-222, with the norm bound of
+ * applyOperator :297-329 relaxed to a product of per-direction maxima). */
+mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out);
+
+/* Generate scaling-only band nodes below existing parents: node ids[i] gets
+ * the scaling block of child child_idx[i] of node parent[i], written at
+ * coef_off[ids[i]], and its norms (MWNode::threadSafeGenChildren +
+ * giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418;
+ * tree_utils::mw_transform, src/utils/tree_utils.cpp:113-216). */
+mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
+                              int count);
+
+/* Exact screening and the 3-sweep band sum for every candidate pair; writes
+ * one partial 8-block result per pair and adds the number of applied
+ * (f, ft, gt, term) tuples to *n_applied (ConvolutionCalculator::calcNode,
+ * applyOperComp, applyOperator and tensorApplyOperComp,
+ * src/treebuilders/ConvolutionCalculator.cpp:224-382). precision: 0 = FP64. */
+mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs,
+                          int npairs, double *partial, unsigned long long *n_applied, int precision);
+
+/* Sum the partial results of every output node in pair order, store the node
+ * coefficients and its component norms (MWNode::calcNorms,
+ * src/trees/MWNode.cpp:609-655). */
+mwg_status mwg_reduce_level(const mwg_level *lvl, const mwg_pairs *pairs, const double *partial);
+
+/* Add each parent's reconstructed s/d into the scaling block of its 8 children
+ * (MWTree::mwTransform(TopDown, false) -> tree_utils::mw_transform,
+ * src/trees/MWTree.cpp:194-216, src/utils/tree_utils.cpp:113-216).  Children
+ * of parent i start at child_off[i] and are stride apart. */
+mwg_status mwg_reconstruct_add(int kp1, const double *filters, double *coef, const long long *parent_off,
+                               const long long *child_off, int count, long long stride);
+
+/* Compress the children's scaling blocks into the parent's s/d
+ * (MWTree::mwTransform(BottomUp) -> FunctionNode<3>::reCompress ->
+ * tree_utils::mw_transform_back, src/trees/FunctionNode.cpp:391-409,
+ * src/utils/tree_utils.cpp:229-301). */
+mwg_status mwg_compress(int kp1, const double *filters, double *coef, const long long *parent_off,
+                        const long long *child_off, int count, long long stride);
+
+/* Component norms of count nodes at coef + off[i], written to norms[9 i]. */
+mwg_status mwg_node_norms(int kp1, const double *coef, const long long *off, double *norms, int count);
+
+/* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
+ * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;
+ * returns a device pointer (MWFilter::getSubFilter, src/core/MWFilter.cpp:100-133). */
+mwg_status mwg_filters_upload(int kp1, const double *host_filters, double **dev_filters);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MWG_H */

@@ -1,1 +1,279 @@
-"""mrcpp_amd: MI355X-native MRCPP operator-application path."""
+"""mrcpp_amd: MRCPP's convolution-operator application on the AMD MI355X.
+
+Python mirror of the MRCPP interface for this path (names, argument order and
+meaning as in MRCPP's C++ API):
+
+    mra = MultiResolutionAnalysis((-32, 32), order=7)     # BoundingBox + LegendreBasis
+    f = FunctionTree(mra)
+    g = GaussExp(); g.append(GaussFunc(beta, coef, pos))
+    build_grid(f, g); project(prec, f, g)
+    P = PoissonOperator(mra, prec)        # or HelmholtzOperator(mra, mu, prec)
+    out = FunctionTree(mra); apply(prec, out, P, f)
+    dot(out, f), out.evalf(r), out.integrate(), out.getSquareNorm()
+
+Everything calls the C ABI of include/mrcpp_amd.h in
+mrcpp_amd/lib/libmrcpp_amd.so; the band sums run as gfx950 kernels.  There is
+no CPU fallback for apply: without the library or a GPU it raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libmrcpp_amd.so")
+FILTER_DIR = os.path.join(HERE, "share", "mwfilters")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+_lib = None
+
+# name: (restype, argtypes) of every symbol declared in include/mrcpp_amd.h
+SIGNATURES = {
+    "mrcpp_last_error": (ctypes.c_char_p, []),
+    "mrcpp_set_filter_dir": (None, [ctypes.c_char_p]),
+    "mrcpp_mra_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mrcpp_mra_destroy": (None, [_vp]),
+    "mrcpp_mra_info": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
+    "mrcpp_tree_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "mrcpp_tree_destroy": (None, [_vp]),
+    "mrcpp_tree_build_grid_gaussians": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
+    "mrcpp_tree_project_gaussians": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]),
+    "mrcpp_tree_num_nodes": (ctypes.c_longlong, [_vp]),
+    "mrcpp_tree_square_norm": (ctypes.c_double, [_vp]),
+    "mrcpp_tree_export": (ctypes.c_int, [_vp, _ip, _dp, _dp]),
+    "mrcpp_tree_evalf": (ctypes.c_double, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "mrcpp_tree_integrate": (ctypes.c_double, [_vp]),
+    "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
+    "mrcpp_operator_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
+    "mrcpp_operator_destroy": (None, [_vp]),
+    "mrcpp_operator_size": (ctypes.c_int, [_vp]),
+    "mrcpp_operator_build_seconds": (ctypes.c_double, [_vp]),
+    "mrcpp_operator_kernel": (ctypes.c_int, [_vp, _dp, _dp]),
+    "mrcpp_operator_bandwidths": (ctypes.c_int, [_vp, ctypes.c_double, _ip, ctypes.c_int]),
+    "mrcpp_operator_node": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "mrcpp_kernel_terms": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                          _dp, _dp, ctypes.c_int]),
+    "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "mrcpp_last_apply_levels": (ctypes.c_int, [_ip, ctypes.c_int]),
+}
+
+
+class MRCPPError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libmrcpp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MRCPPError("libmrcpp_amd.so is missing: run python -m mrcpp_amd.build")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        L.mrcpp_set_filter_dir(FILTER_DIR.encode())
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MRCPPError(lib().mrcpp_last_error().decode())
+
+
+def _p(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class GaussFunc:
+    """coef * exp(-alpha |r - pos|^2) (src/functions/GaussFunc.h)."""
+
+    def __init__(self, alpha, coef, pos=(0.0, 0.0, 0.0)):
+        self.alpha = float(alpha)
+        self.coef = float(coef)
+        self.pos = tuple(float(x) for x in pos)
+
+
+class GaussExp:
+    """Sum of GaussFunc terms (src/functions/GaussExp.h)."""
+
+    def __init__(self, funcs=()):
+        self.funcs = list(funcs)
+
+    def append(self, g):
+        self.funcs.append(g)
+
+    def size(self):
+        return len(self.funcs)
+
+    def arrays(self):
+        c = np.array([g.coef for g in self.funcs], dtype=np.float64)
+        a = np.array([g.alpha for g in self.funcs], dtype=np.float64)
+        r = np.array([g.pos for g in self.funcs], dtype=np.float64).reshape(-1, 3)
+        return c, a, np.ascontiguousarray(r)
+
+    @classmethod
+    def from_sources(cls, src):
+        """From a workloads.GaussianSources set."""
+        return cls(GaussFunc(b, c, p) for c, b, p in zip(src.coefs, src.exponents, src.centers))
+
+
+class MultiResolutionAnalysis:
+    """MultiResolutionAnalysis<3>(BoundingBox<3>({lo, hi}), LegendreBasis(order), depth)."""
+
+    def __init__(self, box, order, depth=30):
+        self.box = (int(box[0]), int(box[1]))
+        self.order = int(order)
+        h = _vp()
+        _check(lib().mrcpp_mra_create(self.box[0], self.box[1], self.order, int(depth), ctypes.byref(h)))
+        self._h = h
+        n0, sf, ms = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
+        _check(lib().mrcpp_mra_info(h, ctypes.byref(n0), ctypes.byref(sf), ctypes.byref(ms)))
+        self.root_scale, self.scaling_factor, self.max_scale = n0.value, sf.value, ms.value
+
+    def getOrder(self):
+        return self.order
+
+    def getMaxScale(self):
+        return self.max_scale
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mrcpp_mra_destroy(self._h)
+            self._h = None
+
+
+class FunctionTree:
+    """FunctionTree<3>(mra)."""
+
+    def __init__(self, mra, name="nn"):
+        self.mra = mra
+        self.name = name
+        h = _vp()
+        _check(lib().mrcpp_tree_create(mra._h, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mrcpp_tree_destroy(self._h)
+            self._h = None
+
+    def getNNodes(self):
+        return int(lib().mrcpp_tree_num_nodes(self._h))
+
+    def getSquareNorm(self):
+        return float(lib().mrcpp_tree_square_norm(self._h))
+
+    def evalf(self, r):
+        return float(lib().mrcpp_tree_evalf(self._h, float(r[0]), float(r[1]), float(r[2])))
+
+    def integrate(self):
+        return float(lib().mrcpp_tree_integrate(self._h))
+
+    def export(self, with_coefs=True):
+        """(idx[N,5] = scale, lx, ly, lz, is_end; coefs[N, 8 (k+1)^3] or None; norms[N, 9])."""
+        n = self.getNNodes()
+        kp1_d = (self.mra.order + 1) ** 3
+        idx = np.zeros((n, 5), dtype=np.int32)
+        coefs = np.zeros((n, 8 * kp1_d)) if with_coefs else None
+        norms = np.zeros((n, 9))
+        got = lib().mrcpp_tree_export(self._h, _p(idx, _ip), _p(coefs) if with_coefs else None, _p(norms))
+        if got < 0:
+            raise MRCPPError(lib().mrcpp_last_error().decode())
+        return idx, coefs, norms
+
+
+class ConvolutionOperator:
+    POISSON, HELMHOLTZ = 0, 1
+
+    def __init__(self, mra, kind, mu, prec):
+        self.mra = mra
+        h = _vp()
+        _check(lib().mrcpp_operator_create(mra._h, kind, float(mu), float(prec), ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mrcpp_operator_destroy(self._h)
+            self._h = None
+
+    def size(self):
+        return int(lib().mrcpp_operator_size(self._h))
+
+    def build_seconds(self):
+        return float(lib().mrcpp_operator_build_seconds(self._h))
+
+    def kernel(self):
+        m = self.size()
+        a, c = np.zeros(m), np.zeros(m)
+        _check(lib().mrcpp_operator_kernel(self._h, _p(a), _p(c)))
+        return a, c
+
+    def bandwidths(self, prec, max_depth=32):
+        out = np.zeros((self.size(), max_depth, 5), dtype=np.int32)
+        if lib().mrcpp_operator_bandwidths(self._h, float(prec), _p(out, _ip), max_depth) < 0:
+            raise MRCPPError(lib().mrcpp_last_error().decode())
+        return out
+
+    def node(self, term, depth, l, comp):
+        kp1 = self.mra.order + 1
+        mat = np.zeros(kp1 * kp1)
+        nrm = ctypes.c_double()
+        if lib().mrcpp_operator_node(self._h, term, depth, l, comp, _p(mat), ctypes.byref(nrm)) != 0:
+            return None, None
+        return mat.reshape(kp1, kp1, order="F"), nrm.value
+
+
+class PoissonOperator(ConvolutionOperator):
+    """PoissonOperator(mra, prec)."""
+
+    def __init__(self, mra, prec):
+        super().__init__(mra, self.POISSON, 0.0, prec)
+
+
+class HelmholtzOperator(ConvolutionOperator):
+    """HelmholtzOperator(mra, mu, prec)."""
+
+    def __init__(self, mra, mu, prec):
+        super().__init__(mra, self.HELMHOLTZ, mu, prec)
+
+
+def kernel_terms(kind, mu, eps, r_min, r_max):
+    a, c = np.zeros(2000), np.zeros(2000)
+    n = lib().mrcpp_kernel_terms(int(kind), float(mu), float(eps), float(r_min), float(r_max), _p(a), _p(c), 2000)
+    if n < 0:
+        raise MRCPPError(lib().mrcpp_last_error().decode())
+    return a[:n].copy(), c[:n].copy()
+
+
+def build_grid(out, gexp):
+    c, a, r = gexp.arrays()
+    _check(lib().mrcpp_tree_build_grid_gaussians(out._h, len(c), _p(c), _p(a), _p(r)))
+
+
+def project(prec, out, gexp):
+    c, a, r = gexp.arrays()
+    _check(lib().mrcpp_tree_project_gaussians(out._h, float(prec), len(c), _p(c), _p(a), _p(r)))
+
+
+def dot(bra, ket):
+    return float(lib().mrcpp_tree_dot(bra._h, ket._h))
+
+
+STAT_KEYS = ("n_applied", "n_pairs", "n_generated", "n_output_nodes", "seconds", "n_levels", "t_screen", "t_generate",
+             "t_band", "t_reduce", "t_transform", "t_host")
+
+
+def apply(prec, out, oper, inp, maxIter=-1, absPrec=False):
+    """apply(prec, out, oper, inp, maxIter, absPrec); returns the run statistics."""
+    st = np.zeros(16)
+    _check(lib().mrcpp_apply(float(prec), out._h, oper._h, inp._h, int(maxIter), int(bool(absPrec)), _p(st)))
+    stats = {k: float(v) for k, v in zip(STAT_KEYS, st)}
+    lv = np.zeros(128, dtype=np.int32)
+    n = lib().mrcpp_last_apply_levels(_p(lv, _ip), 128)
+    stats["level_nodes"] = lv[:n].tolist()
+    return stats

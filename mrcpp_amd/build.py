@@ -1,0 +1,63 @@
+"""Build libmrcpp_amd.so: host C++ with g++, gfx950 kernels with hipcc.
+
+Objects go to mrcpp_amd/_build/, the library to mrcpp_amd/lib/.  Usage:
+    python -m mrcpp_amd.build          (or mrcpp_amd.build.build())
+"""
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB_DIR = os.path.join(HERE, "lib")
+LIB = os.path.join(LIB_DIR, "libmrcpp_amd.so")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+ARCH = os.environ.get("MRCPP_AMD_ARCH", "gfx950")
+
+HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "apply.cpp", "capi.cpp"]
+HIP_SOURCES = ["kernels.hip"]
+
+# IEEE evaluation order on the host (no FMA contraction) keeps host results
+# reproducible across compilers and identical to the CPU checker.
+CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-march=x86-64-v3", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-Wno-unused-result"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("build failed:\n" + " ".join(cmd) + "\n" + r.stdout + r.stderr)
+    return r
+
+
+def build(verbose=False, jobs=8):
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB_DIR, exist_ok=True)
+    cmds = []
+    objs = []
+    for s in HOST_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        cmds.append(["g++"] + CXXFLAGS + ["-c", os.path.join(SRC, s), "-o", o])
+        objs.append(o)
+    for s in HIP_SOURCES:
+        o = os.path.join(OBJ, s + ".o")
+        cmds.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(SRC, s), "-o", o])
+        objs.append(o)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for r in ex.map(_run, cmds):
+            if verbose and (r.stdout or r.stderr):
+                print(r.stdout + r.stderr)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "-fopenmp", "--offload-arch=" + ARCH, "-o", tmp] + objs
+         + ["-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    shutil.move(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,183 @@
+// include/mrcpp_amd.h over the C++ API.
+#include "../../include/mrcpp_amd.h"
+
+#include <cstring>
+#include <string>
+
+#include "function_tree.h"
+
+using namespace mrcpp;
+
+namespace {
+thread_local std::string g_error;
+
+template <typename F> int guarded(F &&f) {
+    try {
+        f();
+        return 0;
+    } catch (std::exception &e) {
+        g_error = e.what();
+        return -1;
+    }
+}
+} // namespace
+
+extern "C" {
+
+const char *mrcpp_last_error(void) { return g_error.c_str(); }
+
+void mrcpp_set_filter_dir(const char *dir) { set_filter_dir(dir ? dir : ""); }
+
+int mrcpp_mra_create(int box_lo, int box_hi, int order, int depth, void **mra) {
+    return guarded([&] { *mra = new MultiResolutionAnalysis<3>({box_lo, box_hi}, order, depth); });
+}
+
+void mrcpp_mra_destroy(void *mra) { delete static_cast<MultiResolutionAnalysis<3> *>(mra); }
+
+int mrcpp_mra_info(void *mra, int *root_scale, double *scaling_factor, int *max_scale) {
+    return guarded([&] {
+        auto *m = static_cast<MultiResolutionAnalysis<3> *>(mra);
+        *root_scale = m->getRootScale();
+        *scaling_factor = m->getWorldBox().getScalingFactor(0);
+        *max_scale = m->getMaxScale();
+    });
+}
+
+int mrcpp_tree_create(void *mra, void **tree) {
+    return guarded([&] { *tree = new FunctionTree<3>(*static_cast<MultiResolutionAnalysis<3> *>(mra)); });
+}
+
+void mrcpp_tree_destroy(void *tree) { delete static_cast<FunctionTree<3> *>(tree); }
+
+static GaussExp<3> gauss_sum(int n, const double *coefs, const double *alphas, const double *centers) {
+    GaussExp<3> ge;
+    for (int i = 0; i < n; i++) ge.append(GaussFunc<3>(alphas[i], coefs[i], {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]}));
+    return ge;
+}
+
+int mrcpp_tree_build_grid_gaussians(void *tree, int n, const double *coefs, const double *alphas, const double *centers) {
+    return guarded([&] { build_grid(*static_cast<FunctionTree<3> *>(tree), gauss_sum(n, coefs, alphas, centers)); });
+}
+
+int mrcpp_tree_project_gaussians(void *tree, double prec, int n, const double *coefs, const double *alphas, const double *centers) {
+    return guarded([&] { project(prec, *static_cast<FunctionTree<3> *>(tree), gauss_sum(n, coefs, alphas, centers)); });
+}
+
+long long mrcpp_tree_num_nodes(void *tree) { return static_cast<FunctionTree<3> *>(tree)->getNNodes(); }
+
+double mrcpp_tree_square_norm(void *tree) { return static_cast<FunctionTree<3> *>(tree)->getSquareNorm(); }
+
+int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms) {
+    int n = -1;
+    int rc = guarded([&] {
+        auto &t = *static_cast<FunctionTree<3> *>(tree);
+        auto &s = coefs ? t.nodes() : t.topology();
+        auto all = detail::real_nodes(s);
+        const int nc = s.ncoef();
+        for (size_t i = 0; i < all.size(); i++) {
+            const int u = all[i];
+            idx[5 * i] = s.scale(u);
+            for (int d = 0; d < 3; d++) idx[5 * i + 1 + d] = s.transl(u)[d];
+            idx[5 * i + 4] = s.isEnd(u) ? 1 : 0;
+            if (coefs) std::memcpy(coefs + (size_t)i * nc, s.coefs(u), sizeof(double) * nc);
+            if (norms) {
+                for (int c = 0; c < 8; c++) norms[9 * i + c] = s.compNorm(u, c);
+                norms[9 * i + 8] = s.squareNorm(u);
+            }
+        }
+        n = (int)all.size();
+    });
+    return rc < 0 ? -1 : n;
+}
+
+double mrcpp_tree_evalf(void *tree, double x, double y, double z) { return static_cast<FunctionTree<3> *>(tree)->evalf({x, y, z}); }
+
+double mrcpp_tree_integrate(void *tree) { return static_cast<FunctionTree<3> *>(tree)->integrate(); }
+
+double mrcpp_tree_dot(void *bra, void *ket) { return dot(*static_cast<FunctionTree<3> *>(bra), *static_cast<FunctionTree<3> *>(ket)); }
+
+int mrcpp_operator_create(void *mra, int kind, double mu, double prec, void **oper) {
+    return guarded([&] {
+        auto &m = *static_cast<MultiResolutionAnalysis<3> *>(mra);
+        if (kind == 0)
+            *oper = static_cast<ConvolutionOperator<3> *>(new PoissonOperator(m, prec));
+        else if (kind == 1)
+            *oper = static_cast<ConvolutionOperator<3> *>(new HelmholtzOperator(m, mu, prec));
+        else
+            throw Error("unknown operator kind");
+    });
+}
+
+void mrcpp_operator_destroy(void *oper) { delete static_cast<ConvolutionOperator<3> *>(oper); }
+
+int mrcpp_operator_size(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->size(); }
+
+double mrcpp_operator_build_seconds(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->buildSeconds(); }
+
+int mrcpp_operator_kernel(void *oper, double *alpha, double *coef) {
+    return guarded([&] {
+        const auto &k = static_cast<ConvolutionOperator<3> *>(oper)->kernel();
+        for (size_t i = 0; i < k.size(); i++) {
+            alpha[i] = k[i].alpha;
+            coef[i] = k[i].coef;
+        }
+    });
+}
+
+int mrcpp_operator_bandwidths(void *oper, double prec, int *out, int max_depth) {
+    int n = -1;
+    int rc = guarded([&] {
+        auto &o = *static_cast<ConvolutionOperator<3> *>(oper);
+        o.calcBandWidths(prec);
+        for (int i = 0; i < o.size(); i++)
+            for (int d = 0; d < max_depth; d++)
+                for (int c = 0; c < 5; c++) out[((size_t)i * max_depth + d) * 5 + c] = (c < 4) ? o.term(i).width(d, c) : o.term(i).maxWidth(d);
+        n = (int)o.bandMax().size();
+    });
+    return rc < 0 ? -1 : n;
+}
+
+int mrcpp_operator_node(void *oper, int term, int depth, int l, int comp, double *mat, double *norm) {
+    return guarded([&] {
+        const auto &t = static_cast<ConvolutionOperator<3> *>(oper)->term(term);
+        if (depth < 0 || depth >= (int)t.cache.size() || std::abs(l) > t.maxTransl[depth]) throw Error("no such operator node");
+        const int u = t.node(depth, l);
+        const int n2 = t.tree->kp1_d();
+        std::memcpy(mat, t.tree->coefs(u) + (size_t)comp * n2, sizeof(double) * n2);
+        *norm = t.tree->compNorm(u, comp);
+    });
+}
+
+int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_max, double *alpha, double *coef, int cap) {
+    int n = -1;
+    int rc = guarded([&] {
+        auto k = (kind == 0) ? poisson_kernel(eps, r_min, r_max) : helmholtz_kernel(mu, eps, r_min, r_max);
+        for (size_t i = 0; i < k.size() && (int)i < cap; i++) {
+            alpha[i] = k[i].alpha;
+            coef[i] = k[i].coef;
+        }
+        n = (int)k.size();
+    });
+    return rc < 0 ? -1 : n;
+}
+
+int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats) {
+    return guarded([&] {
+        apply(prec, *static_cast<FunctionTree<3> *>(out), *static_cast<ConvolutionOperator<3> *>(oper), *static_cast<FunctionTree<3> *>(inp),
+              max_iter, abs_prec != 0);
+        if (stats) {
+            const ApplyStats &s = last_apply_stats();
+            double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,
+                            (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tReduce, s.tTransform, s.tHost, 0, 0, 0, 0};
+            std::memcpy(stats, v, sizeof(v));
+        }
+    });
+}
+
+int mrcpp_last_apply_levels(int *level_nodes, int cap) {
+    const ApplyStats &s = last_apply_stats();
+    for (size_t i = 0; i < s.levelNodes.size() && (int)i < cap; i++) level_nodes[i] = s.levelNodes[i];
+    return (int)s.levelNodes.size();
+}
+
+} // extern "C"

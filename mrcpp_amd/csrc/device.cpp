@@ -1,0 +1,161 @@
+// Host-side device management: initialisation, band tables, filter banks,
+// the key table mirrored on the device.
+#include "device.h"
+
+#include <map>
+#include <mutex>
+
+#include "keys.h"
+
+namespace mrcpp {
+namespace device {
+
+void check(mwg_status s, const char *what) {
+    if (s != MWG_OK) throw Error(std::string(what) + ": " + mwg_last_error());
+}
+
+void ensure_init() {
+    static std::once_flag once;
+    static mwg_status st = MWG_OK;
+    std::call_once(once, [] { st = mwg_init(0); });
+    check(st, "device initialisation");
+}
+
+detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
+    ensure_init();
+    auto *d = new detail::DeviceBandTables();
+    auto put_i = [](DeviceArray<int> &a, const std::vector<int> &v) {
+        a.reserve(std::max<size_t>(v.size(), 1));
+        a.upload(v.data(), v.size());
+    };
+    auto put_d = [](DeviceArray<double> &a, const std::vector<double> &v) {
+        a.reserve(std::max<size_t>(v.size(), 1));
+        a.upload(v.data(), v.size());
+    };
+    put_i(d->bandMax, t.bandMax);
+    put_i(d->widths, t.widths);
+    put_i(d->bandSize, t.bandSize);
+    d->matStart.reserve(std::max<size_t>(t.matStart.size(), 1));
+    d->matStart.upload(t.matStart.data(), t.matStart.size());
+    put_d(d->mats, t.mats);
+    put_d(d->matNorm, t.matNorm);
+    put_d(d->pairBound, t.pairBound);
+    mwg_band_tables &v = d->view;
+    v.M = t.M;
+    v.n_depth = t.nDepth;
+    v.kp1 = t.kp1;
+    v.wmax = t.Wmax;
+    v.band_max = d->bandMax.get();
+    v.widths = d->widths.get();
+    v.band_size = d->bandSize.get();
+    v.mat_start = d->matStart.get();
+    v.mats = d->mats.get();
+    v.mat_norm = d->matNorm.get();
+    v.pair_bound = d->pairBound.get();
+    return d;
+}
+
+void destroy_band_tables(detail::DeviceBandTables *t) { delete t; }
+
+const double *filters(int order) {
+    static std::map<int, double *> memo;
+    static std::mutex m;
+    std::lock_guard<std::mutex> lk(m);
+    auto it = memo.find(order);
+    if (it != memo.end()) return it->second;
+    ensure_init();
+    const auto &fb = core::filters(order);
+    const int n2 = (order + 1) * (order + 1);
+    std::vector<double> h(8 * (size_t)n2);
+    for (int i = 0; i < 4; i++) {
+        std::copy(fb.sub(i, Compression).a.begin(), fb.sub(i, Compression).a.end(), h.begin() + (size_t)i * n2);
+        std::copy(fb.sub(i, Reconstruction).a.begin(), fb.sub(i, Reconstruction).a.end(), h.begin() + (size_t)(4 + i) * n2);
+    }
+    double *dp = nullptr;
+    check(mwg_filters_upload(order + 1, h.data(), &dp), "filter upload");
+    memo[order] = dp;
+    return dp;
+}
+
+void KeyTable::reset(int min_entries) {
+    log2cap_ = 10;
+    while ((1 << log2cap_) < 2 * std::max(min_entries, 1)) log2cap_++;
+    keys_.assign((size_t)1 << log2cap_, 0ull);
+    vals_.assign((size_t)1 << log2cap_, -1);
+    count_ = 0;
+}
+
+int KeyTable::insert(uint64_t key, int id) {
+    if (2 * (count_ + 1) > capacity()) {
+        // grow and rehash
+        std::vector<unsigned long long> ok = keys_;
+        std::vector<int> ov = vals_;
+        log2cap_++;
+        keys_.assign((size_t)1 << log2cap_, 0ull);
+        vals_.assign((size_t)1 << log2cap_, -1);
+        count_ = 0;
+        for (size_t i = 0; i < ok.size(); i++)
+            if (ok[i]) insert(ok[i], ov[i]);
+    }
+    const uint32_t mask = (uint32_t)capacity() - 1;
+    uint32_t s = mwgkey::slot(key, log2cap_);
+    while (true) {
+        if (keys_[s] == 0) {
+            keys_[s] = key;
+            vals_[s] = id;
+            count_++;
+            return id;
+        }
+        if (keys_[s] == key) return vals_[s];
+        s = (s + 1) & mask;
+    }
+}
+
+int KeyTable::find(uint64_t key) const {
+    const uint32_t mask = (uint32_t)capacity() - 1;
+    uint32_t s = mwgkey::slot(key, log2cap_);
+    while (true) {
+        if (keys_[s] == 0) return -1;
+        if (keys_[s] == key) return vals_[s];
+        s = (s + 1) & mask;
+    }
+}
+
+uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l) {
+    const int d = scale - mra.getRootScale();
+    if (d < 0 || d > mwgkey::kMaxDepth) throw Error("node depth outside the device key range (max depth 18)");
+    int o[3];
+    for (int i = 0; i < 3; i++) {
+        o[i] = l[i] - mra.getWorldBox().getCornerIndex()[i] * (1 << d);
+        if (o[i] < 0 || o[i] >= (1 << mwgkey::kOffsetBits)) throw Error("node translation outside the device key range");
+    }
+    return mwgkey::pack(d, o[0], o[1], o[2]);
+}
+
+void TreeMirror::uploadTable() {
+    keys.reserve(table.capacity());
+    vals.reserve(table.capacity());
+    keys.upload(table.keys().data(), table.capacity());
+    vals.upload(table.vals().data(), table.capacity());
+}
+
+mwg_ftree TreeMirror::view(const MultiResolutionAnalysis<3> &mra) const {
+    mwg_ftree f{};
+    f.kp1 = kp1;
+    f.root_scale = mra.getRootScale();
+    for (int i = 0; i < 3; i++) {
+        f.corner[i] = mra.getWorldBox().getCornerIndex()[i];
+        f.nbox[i] = mra.getWorldBox().size(i);
+    }
+    f.hash_keys = keys.get();
+    f.hash_vals = vals.get();
+    f.hash_cap = table.capacity();
+    f.coef = coef.get();
+    f.coef_off = coefOff.get();
+    f.norms = norms.get();
+    f.is_gen = isGen.get();
+    return f;
+}
+
+} // namespace device
+} // namespace mrcpp

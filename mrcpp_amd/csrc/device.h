@@ -1,0 +1,129 @@
+// Host-side owners of device memory for the MI355X path: growable device
+// arrays, operator band tables and the device mirror of an input tree.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/mwg.h"
+#include "operators.h"
+#include "trees.h"
+
+namespace mrcpp {
+namespace device {
+
+void ensure_init();
+void check(mwg_status s, const char *what);
+
+template <typename T> class DeviceArray {
+public:
+    DeviceArray() = default;
+    ~DeviceArray() { release(); }
+    DeviceArray(const DeviceArray &) = delete;
+    DeviceArray &operator=(const DeviceArray &) = delete;
+    DeviceArray(DeviceArray &&o) noexcept
+            : ptr_(o.ptr_)
+            , cap_(o.cap_) {
+        o.ptr_ = nullptr;
+        o.cap_ = 0;
+    }
+
+    T *get() const { return ptr_; }
+    size_t capacity() const { return cap_; }
+    // grow to at least n elements; keeps the first `keep` elements
+    void reserve(size_t n, size_t keep = 0) {
+        if (n <= cap_) return;
+        size_t nc = cap_ ? cap_ : 1024;
+        while (nc < n) nc *= 2;
+        void *p = nullptr;
+        check(mwg_malloc(&p, nc * sizeof(T)), "device allocation");
+        if (keep && ptr_) check(mwg_copy(p, ptr_, keep * sizeof(T)), "device copy");
+        release();
+        ptr_ = static_cast<T *>(p);
+        cap_ = nc;
+    }
+    void upload(const T *src, size_t n, size_t at = 0) {
+        if (n) check(mwg_upload(ptr_ + at, src, n * sizeof(T)), "upload");
+    }
+    void download(T *dst, size_t n, size_t at = 0) const {
+        if (n) check(mwg_download(dst, ptr_ + at, n * sizeof(T)), "download");
+    }
+    void zero(size_t n, size_t at = 0) {
+        if (n) check(mwg_zero(ptr_ + at, n * sizeof(T)), "memset");
+    }
+    void release() {
+        if (ptr_) mwg_free(ptr_);
+        ptr_ = nullptr;
+        cap_ = 0;
+    }
+
+private:
+    T *ptr_ = nullptr;
+    size_t cap_ = 0;
+};
+
+} // namespace device
+
+namespace detail {
+struct DeviceBandTables {
+    device::DeviceArray<int> bandMax, widths, bandSize;
+    device::DeviceArray<long long> matStart;
+    device::DeviceArray<double> mats, matNorm, pairBound;
+    mwg_band_tables view{};
+};
+} // namespace detail
+
+namespace device {
+
+detail::DeviceBandTables *create_band_tables(const detail::BandTables &t);
+void destroy_band_tables(detail::DeviceBandTables *t);
+
+// Device filter banks per order: 4 compression then 4 reconstruction blocks.
+const double *filters(int order);
+
+// Host mirror of the device open-addressing hash.
+class KeyTable {
+public:
+    void reset(int min_entries);
+    int log2cap() const { return log2cap_; }
+    int capacity() const { return 1 << log2cap_; }
+    // insert, or return the existing id
+    int insert(uint64_t key, int id);
+    int find(uint64_t key) const;
+    int size() const { return count_; }
+    const std::vector<unsigned long long> &keys() const { return keys_; }
+    const std::vector<int> &vals() const { return vals_; }
+
+private:
+    int log2cap_ = 0, count_ = 0;
+    std::vector<unsigned long long> keys_;
+    std::vector<int> vals_;
+};
+
+// Device mirror of a 3D function tree used as apply input: real nodes keep
+// their host ids, generated band nodes are appended after them.
+struct TreeMirror {
+    int kp1 = 0, ncoef = 0;
+    int nReal = 0, nGen = 0;
+    long long genBase = 0; // coefficient offset of the generated-node region
+    KeyTable table;
+    DeviceArray<unsigned long long> keys;
+    DeviceArray<int> vals;
+    DeviceArray<double> coef, norms;
+    DeviceArray<long long> coefOff;
+    DeviceArray<unsigned char> isGen;
+    std::vector<long long> hostOff;    // mirrors coefOff
+    std::vector<int> genParent;        // host bookkeeping for generated nodes
+    std::vector<int> genScale;
+    std::vector<std::array<int, 3>> genL;
+    bool valid = false;
+    mwg_ftree view(const MultiResolutionAnalysis<3> &mra) const;
+    void uploadTable();
+};
+
+uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l);
+
+} // namespace device
+} // namespace mrcpp

@@ -1,0 +1,85 @@
+// FunctionTree<3>: a multiwavelet function on the host node store with an
+// optional device-resident copy of its coefficients (src/trees/FunctionTree.h:55-183).
+#pragma once
+
+#include <memory>
+#include <string>
+
+#include "device.h"
+#include "functions.h"
+#include "operators.h"
+#include "trees.h"
+
+namespace mrcpp {
+
+template <int D> class FunctionTree;
+
+template <> class FunctionTree<3> {
+public:
+    explicit FunctionTree(const MultiResolutionAnalysis<3> &mra, const std::string &name = "nn");
+    FunctionTree(const FunctionTree &) = delete;
+    FunctionTree &operator=(const FunctionTree &) = delete;
+
+    const MultiResolutionAnalysis<3> &getMRA() const { return mra_; }
+    const std::string &getName() const { return name_; }
+    double getSquareNorm() const { return store_->treeSquareNorm; }
+    int getNNodes() const;
+    int getNEndNodes() const;
+    double integrate();
+    double evalf(const Coord<3> &r);
+
+    // host node store with coefficients brought up to date
+    detail::NodeStore<3> &nodes();
+    // host node store without touching coefficients (topology and norms)
+    detail::NodeStore<3> &topology() { return *store_; }
+    void clear();
+
+    // device copy of the coefficients (node id u at u * 8 (k+1)^3)
+    device::DeviceArray<double> &deviceCoefs() { return dev_coef_; }
+    void markDeviceResident() {
+        host_stale_ = true;
+        mirror_.reset();
+    }
+    bool hostStale() const { return host_stale_; }
+    void syncHost();
+    // device mirror used when this tree is the input of apply
+    device::TreeMirror &inputMirror();
+    void invalidate() { mirror_.reset(); }
+
+private:
+    MultiResolutionAnalysis<3> mra_;
+    std::string name_;
+    std::unique_ptr<detail::NodeStore<3>> store_;
+    device::DeviceArray<double> dev_coef_;
+    bool host_stale_ = false;
+    std::unique_ptr<device::TreeMirror> mirror_;
+};
+
+// build_grid(tree, GaussExp) (src/treebuilders/grid.cpp:106-123)
+void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp);
+// project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102)
+void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter = -1, bool absPrec = false);
+// dot(bra, ket) (src/treebuilders/multiply.cpp:286-318)
+double dot(FunctionTree<3> &bra, FunctionTree<3> &ket);
+
+// Statistics of the last apply on this thread.
+struct ApplyStats {
+    long long nApplied = 0;    // applied (f, ft, gt, term) tuples
+    long long nPairs = 0;      // candidate (g, f) pairs after the separable bound
+    long long nGenerated = 0;  // generated band nodes
+    long long nOutputNodes = 0;
+    std::vector<int> levelNodes;
+    double seconds = 0.0;      // host wall time of apply()
+    double tScreen = 0.0, tGenerate = 0.0, tBand = 0.0, tReduce = 0.0, tTransform = 0.0, tHost = 0.0;
+};
+const ApplyStats &last_apply_stats();
+
+// Precision of the band sweeps (FP64 is the drop-in default).
+enum class SweepPrecision { FP64 = 0 };
+
+// apply(prec, out, oper, inp, maxIter, absPrec) (src/treebuilders/apply.cpp:68-93),
+// the band sum running on the MI355X.
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter = -1,
+           bool absPrec = false);
+
+} // namespace mrcpp

@@ -1,0 +1,721 @@
+// gfx950 kernels for MRCPP's operator application and the mwg_* C ABI.
+//
+// Launch shapes: one 256-thread workgroup (4 wave64) per output node, per
+// candidate (output node, band node) pair or per parent node; coefficient
+// blocks of (k+1)^3 doubles are staged in LDS and every thread owns a fixed
+// set of output coefficients across the whole pair.  Layout of a node: 8
+// blocks (block t = wavelet bits t, x->bit 0), element i0 + (k+1) i1 + (k+1)^2 i2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <string>
+
+#include "../../include/mwg.h"
+#include "keys.h"
+
+namespace {
+
+thread_local std::string g_err;
+bool g_ready = false;
+
+mwg_status fail(mwg_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+#define MWG_HIP(call)                                                                                                   \
+    do {                                                                                                                \
+        hipError_t e_ = (call);                                                                                         \
+        if (e_ != hipSuccess) return fail(MWG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));          \
+    } while (0)
+
+#define MWG_LAUNCHED()                                                                                                  \
+    do {                                                                                                                \
+        hipError_t e_ = hipGetLastError();                                                                              \
+        if (e_ != hipSuccess) return fail(MWG_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_));     \
+    } while (0)
+
+constexpr int kThreads = 256;
+constexpr int kMaxTerms = 512;
+constexpr double kMachineZero = 1.0e-14;
+// relative slack of the separable pair bound against rounding
+constexpr double kBoundSlack = 1.0 + 1.0e-9;
+
+__device__ inline int f_lookup(const mwg_ftree &f, int depth, int x, int y, int z) {
+    const int ox = x - f.corner[0] * (1 << depth);
+    const int oy = y - f.corner[1] * (1 << depth);
+    const int oz = z - f.corner[2] * (1 << depth);
+    const uint64_t key = mwgkey::pack(depth, ox, oy, oz);
+    const int log2cap = 31 - __clz(f.hash_cap);
+    const uint32_t mask = (uint32_t)f.hash_cap - 1;
+    uint32_t s = mwgkey::slot(key, log2cap);
+    for (;;) {
+        const unsigned long long k = f.hash_keys[s];
+        if (k == key) return f.hash_vals[s];
+        if (k == 0ull) return -1;
+        s = (s + 1) & mask;
+    }
+}
+
+// Block-wide exclusive scan of one int per thread; returns the total.
+__device__ inline int block_exclusive_scan(int v, int *tmp, int &excl) {
+    const int tid = threadIdx.x;
+    tmp[tid] = v;
+    __syncthreads();
+    for (int off = 1; off < kThreads; off <<= 1) {
+        int add = (tid >= off) ? tmp[tid - off] : 0;
+        __syncthreads();
+        tmp[tid] += add;
+        __syncthreads();
+    }
+    const int incl = tmp[tid];
+    const int total = tmp[kThreads - 1];
+    excl = incl - v;
+    __syncthreads();
+    return total;
+}
+
+// ---------------------------------------------------------------------------
+// Pair screen: one workgroup per output node of the level.
+// ---------------------------------------------------------------------------
+__device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f, const mwg_band_tables &op, const int *gl, int p,
+                                       const int *s, const int *n, int *out_f, int *out_l) {
+    const int d = lvl.o_depth;
+    const int x = s[0] + p % n[0];
+    const int y = s[1] + (p / n[0]) % n[1];
+    const int z = s[2] + p / (n[0] * n[1]);
+    const double *pb = op.pair_bound + (size_t)d * (2 * op.wmax + 1) + op.wmax;
+    const double a = pb[x - gl[0]] * pb[y - gl[1]] * pb[z - gl[2]];
+    // a zero bound cannot pass a non-negative threshold: skip the lookup
+    if (a == 0.0 && lvl.g_thr >= 0.0) return false;
+    int id = f_lookup(f, d, x, y, z);
+    double maxf;
+    int code;
+    if (id >= 0) {
+        const double *nr = f.norms + (size_t)id * 9;
+        maxf = 0.0;
+        for (int t = 0; t < 8; t++) maxf = fmax(maxf, nr[t]);
+        code = id;
+    } else {
+        int dd = d, xx = x, yy = y, zz = z, anc = -1;
+        while (anc < 0 && dd > 0) {
+            dd--;
+            xx >>= 1;
+            yy >>= 1;
+            zz >>= 1;
+            anc = f_lookup(f, dd, xx, yy, zz);
+        }
+        if (anc < 0) return false; // outside the world (not reachable for clamped boxes)
+        const double *nr = f.norms + (size_t)anc * 9;
+        maxf = f.is_gen[anc] ? nr[0] : sqrt(nr[8]);
+        code = -(anc + 1);
+    }
+    if (!(a * maxf * kBoundSlack > lvl.g_thr)) return false;
+    *out_f = code;
+    out_l[0] = x;
+    out_l[1] = y;
+    out_l[2] = z;
+    return true;
+}
+
+__global__ __launch_bounds__(kThreads) void k_pair_screen(mwg_level lvl, mwg_ftree f, mwg_band_tables op, mwg_pairs out) {
+    __shared__ int tmp[kThreads];
+    __shared__ int base;
+    const int g = blockIdx.x;
+    const int d = lvl.o_depth;
+    const int W = op.band_max[d];
+    int gl[3], s[3], n[3];
+    for (int i = 0; i < 3; i++) {
+        gl[i] = lvl.l[3 * g + i];
+        const int c = f.corner[i] * (1 << d);
+        const int nb = f.nbox[i] * (1 << d);
+        s[i] = max(gl[i] - W, c);
+        const int e = min(gl[i] + W, c + nb - 1);
+        n[i] = max(e - s[i] + 1, 0);
+    }
+    const int P = n[0] * n[1] * n[2];
+    int cnt = 0, fid, fl[3];
+    for (int p = threadIdx.x; p < P; p += kThreads)
+        if (screen_position(lvl, f, op, gl, p, s, n, &fid, fl)) cnt++;
+    int excl;
+    const int total = block_exclusive_scan(cnt, tmp, excl);
+    if (threadIdx.x == 0) {
+        base = atomicAdd(out.count, total);
+        out.g_first[g] = base;
+        out.g_npairs[g] = total;
+    }
+    __syncthreads();
+    int w = base + excl;
+    for (int p = threadIdx.x; p < P; p += kThreads)
+        if (screen_position(lvl, f, op, gl, p, s, n, &fid, fl)) {
+            if (w < out.capacity) {
+                out.pair_g[w] = g;
+                out.pair_f[w] = fid;
+                out.pair_l[3 * w + 0] = fl[0];
+                out.pair_l[3 * w + 1] = fl[1];
+                out.pair_l[3 * w + 2] = fl[2];
+            }
+            w++;
+        }
+}
+
+// ---------------------------------------------------------------------------
+// One separable 3D multiwavelet pass set over 8 blocks held in LDS.
+// dir_base 0: compression sub-filters, 4: reconstruction sub-filters.
+// pass dim: out[gt] = sum over the two ft sharing gt's other bits, ascending
+// ft, of in[ft]^T F(2*bit(gt)+bit(ft)), in viewed (N1 x N2) column-major.
+// ---------------------------------------------------------------------------
+template <int N1> __device__ inline void mw_pass(double *out, const double *in, const double *filt, int dir_base, int dim) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    const int bit = 1 << dim;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int gt = o / N3, e = o % N3;
+        const int r = e % N2, j = e / N2;
+        const int ft0 = gt & ~bit, ft1 = gt | bit;
+        const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
+        const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
+        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
+        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; i++) {
+            s0 += x0[i] * F0[i];
+            s1 += x1[i] * F1[i];
+        }
+        out[o] = s0 + s1;
+    }
+}
+
+// Last reconstruction pass accumulating into the children, each of the two
+// contributions added separately (apply_filter with accumulate).
+template <int N1>
+__device__ inline void mw_pass_add_children(double *coef, long long child_off, long long stride, const double *in, const double *filt) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    const int dim = 2, bit = 4;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int gt = o / N3, e = o % N3;
+        const int r = e % N2, j = e / N2;
+        const int ft0 = gt & ~bit, ft1 = gt | bit;
+        const double *F0 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
+        const double *F1 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
+        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
+        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; i++) {
+            s0 += x0[i] * F0[i];
+            s1 += x1[i] * F1[i];
+        }
+        double *dst = coef + child_off + (long long)gt * stride + e;
+        double v = *dst;
+        v += s0;
+        v += s1;
+        *dst = v;
+    }
+}
+
+// Component norms of 8 blocks (LDS or global), written to nrm[0..8].
+template <int N1> __device__ inline void block_norms(const double *c, double *nrm, double *red) {
+    constexpr int N3 = N1 * N1 * N1;
+    double ss[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) ss[t] = 0.0;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const double v = c[o];
+        ss[o / N3] += v * v;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        double v = ss[t];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        ss[t] = v;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int t = 0; t < 8; t++) red[wave * 8 + t] = ss[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int t = 0; t < 8; t++) {
+            double v = 0.0;
+            for (int w = 0; w < kThreads / 64; w++) v += red[w * 8 + t];
+            const double nv = sqrt(v);
+            nrm[t] = nv;
+            tot += nv * nv;
+        }
+        nrm[8] = tot;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Generated band nodes: scaling block of one child of an existing node.
+// ---------------------------------------------------------------------------
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double nrm[9];
+    const int k = blockIdx.x;
+    const int pid = parent[k], id = ids[k], c = child[k];
+    const double *src = f.coef + f.coef_off[pid];
+    const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = (o < nblk * N3) ? src[o] : 0.0;
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 4, 1);
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 2);
+    __syncthreads();
+    double *dst = f.coef + f.coef_off[id];
+    double ss = 0.0;
+    for (int e = threadIdx.x; e < N3; e += kThreads) {
+        const double v = b[c * N3 + e];
+        dst[e] = v;
+        ss += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) v += red[w];
+        const double n0 = sqrt(v);
+        double *nr = f.norms + (size_t)id * 9;
+        nr[0] = n0;
+        for (int t = 1; t < 8; t++) nr[t] = 0.0;
+        nr[8] = n0 * n0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Band sum for one candidate pair: exact screening of every (ft, term, gt)
+// and the three 1D sweeps, x and y sweeps shared between the gt that need
+// the same intermediate.
+// ---------------------------------------------------------------------------
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_band_apply(mwg_level lvl, mwg_ftree f, mwg_band_tables op, const int *pair_g, const int *pair_f,
+                                                        const int *pair_l, double *partial, unsigned long long *n_applied) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    constexpr int R = (8 * N3 + kThreads - 1) / kThreads;
+    __shared__ double sF[N3];
+    __shared__ double sT1[2 * N3];
+    __shared__ double sT2[4 * N3];
+    __shared__ double sO[6 * N2];
+    __shared__ unsigned char smask[8 * kMaxTerms];
+    __shared__ int sgroup[8 * kMaxTerms];
+    __shared__ int tmp[kThreads];
+    __shared__ int ngroups;
+    __shared__ unsigned int napp;
+
+    const int p = blockIdx.x;
+    const int g = pair_g[p];
+    const int fid = pair_f[p];
+    const int d = lvl.o_depth;
+    const int M = op.M;
+    int dl[3], maxdl = 0;
+    for (int i = 0; i < 3; i++) {
+        dl[i] = pair_l[3 * p + i] - lvl.l[3 * g + i];
+        maxdl = max(maxdl, abs(dl[i]));
+    }
+    const double *fn = f.norms + (size_t)fid * 9;
+    const int *wd = op.widths + (size_t)d * M * 5;
+    const int *bs = op.band_size + (size_t)d * M * 64;
+    const long long *ms = op.mat_start + (size_t)d * M;
+    if (threadIdx.x == 0) napp = 0;
+    __syncthreads();
+
+    // --- exact screening: item q = ft * M + i -------------------------------
+    unsigned int local_app = 0;
+    for (int q = threadIdx.x; q < 8 * M; q += kThreads) {
+        const int ft = q / M, i = q - ft * M;
+        unsigned int m = 0;
+        const double fnorm = fn[ft];
+        const int *w = wd + i * 5;
+        if (!(fnorm < kMachineZero) && maxdl <= w[4]) {
+            for (int gt = 0; gt < 8; gt++) {
+                if (!(d == 0 || gt != 0 || ft != 0)) continue;
+                double onorm = 1.0;
+                bool inside = true;
+                for (int dim = 0; dim < 3; dim++) {
+                    const int c = 2 * ((gt >> dim) & 1) + ((ft >> dim) & 1);
+                    if (abs(dl[dim]) > w[c]) {
+                        inside = false;
+                        break;
+                    }
+                    onorm *= op.mat_norm[ms[i] + (long long)(dl[dim] + w[4]) * 4 + c];
+                }
+                if (!inside) continue;
+                const double fthr = bs[i * 64 + gt * 8 + ft] * fnorm;
+                if (onorm * fthr > lvl.g_thr) m |= 1u << gt;
+            }
+        }
+        smask[q] = (unsigned char)m;
+        local_app += __popc(m);
+    }
+    atomicAdd(&napp, local_app);
+    __syncthreads();
+    // compact groups in q order (ft-major), contiguous chunks per thread
+    const int nq = 8 * M;
+    const int chunk = (nq + kThreads - 1) / kThreads;
+    const int q0 = threadIdx.x * chunk, q1 = min(nq, q0 + chunk);
+    int cnt = 0;
+    for (int q = q0; q < q1; q++) cnt += smask[q] != 0;
+    int excl;
+    const int total = block_exclusive_scan(cnt, tmp, excl);
+    for (int q = q0; q < q1; q++)
+        if (smask[q]) sgroup[excl++] = q;
+    if (threadIdx.x == 0) {
+        ngroups = total;
+        if (napp) atomicAdd(n_applied, (unsigned long long)napp);
+    }
+    __syncthreads();
+
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0;
+
+    const double *fsrc = f.coef + f.coef_off[fid];
+    int curft = -1;
+    const int G = ngroups;
+    for (int k = 0; k < G; k++) {
+        const int q = sgroup[k];
+        const int ft = q / M, i = q - ft * M;
+        const unsigned int m = smask[q];
+        if (ft != curft) {
+            for (int e = threadIdx.x; e < N3; e += kThreads) sF[e] = fsrc[(size_t)ft * N3 + e];
+            curft = ft;
+        }
+        // operator blocks: x (gx=0,1), y (gy=0,1), z (gz=0,1)
+        const int w4 = wd[i * 5 + 4];
+        for (int e = threadIdx.x; e < 6 * N2; e += kThreads) {
+            const int which = e / N2, el = e % N2;
+            const int dim = which >> 1, gb = which & 1;
+            const int c = 2 * gb + ((ft >> dim) & 1);
+            sO[e] = op.mats[(size_t)(ms[i] + (long long)(dl[dim] + w4) * 4 + c) * N2 + el];
+        }
+        __syncthreads();
+        // x sweep: T1[gx](r + N2 j0) = sum_i0 F(i0 + N1 r) Ox[gx](i0, j0)
+        unsigned int xneed = 0, yneed = 0;
+        for (int gt = 0; gt < 8; gt++)
+            if (m >> gt & 1) {
+                xneed |= 1u << (gt & 1);
+                yneed |= 1u << (gt & 3);
+            }
+        for (int e = threadIdx.x; e < 2 * N3; e += kThreads) {
+            const int gx = e / N3;
+            if (!(xneed >> gx & 1)) continue;
+            const int o = e % N3, r = o % N2, j = o / N2;
+            const double *x = sF + N1 * r;
+            const double *O = sO + gx * N2 + N1 * j;
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < N1; a++) s += x[a] * O[a];
+            sT1[e] = s;
+        }
+        __syncthreads();
+        // y sweep: T2[gx + 2 gy](r + N2 j1) = sum_i1 T1[gx](i1 + N1 r) Oy[gy](i1, j1)
+        for (int e = threadIdx.x; e < 4 * N3; e += kThreads) {
+            const int cb = e / N3;
+            if (!(yneed >> cb & 1)) continue;
+            const int gx = cb & 1, gy = cb >> 1;
+            const int o = e % N3, r = o % N2, j = o / N2;
+            const double *x = sT1 + gx * N3 + N1 * r;
+            const double *O = sO + (2 + gy) * N2 + N1 * j;
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < N1; a++) s += x[a] * O[a];
+            sT2[e] = s;
+        }
+        __syncthreads();
+        // z sweep into the owned outputs: G[gt](r + N2 j2) += sum_i2 T2[gt&3](i2 + N1 r) Oz[gz](i2, j2)
+#pragma unroll
+        for (int rr = 0; rr < R; rr++) {
+            const int o = threadIdx.x + rr * kThreads;
+            if (o < 8 * N3) {
+                const int gt = o / N3;
+                if (m >> gt & 1) {
+                    const int e = o % N3, r = e % N2, j = e / N2;
+                    const double *x = sT2 + (gt & 3) * N3 + N1 * r;
+                    const double *O = sO + (4 + (gt >> 2)) * N2 + N1 * j;
+                    double s = 0.0;
+#pragma unroll
+                    for (int a = 0; a < N1; a++) s += x[a] * O[a];
+                    acc[rr] += s;
+                }
+            }
+        }
+        __syncthreads();
+    }
+    double *dst = partial + (size_t)p * 8 * N3;
+#pragma unroll
+    for (int rr = 0; rr < R; rr++) {
+        const int o = threadIdx.x + rr * kThreads;
+        if (o < 8 * N3) dst[o] = acc[rr];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per output node: sum the pair partials in pair order, store, norms.
+// ---------------------------------------------------------------------------
+template <int N1> __global__ __launch_bounds__(kThreads) void k_reduce(mwg_level lvl, mwg_pairs pr, const double *partial) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double red[8 * (kThreads / 64)];
+    const int g = blockIdx.x;
+    const int first = pr.g_first[g], n = pr.g_npairs[g];
+    double *dst = lvl.g_coef + lvl.g_off[g];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        double s = 0.0;
+        for (int k = 0; k < n; k++) s += partial[(size_t)(first + k) * 8 * N3 + o];
+        dst[o] = s;
+    }
+    __syncthreads();
+    block_norms<N1>(dst, lvl.g_norms + (size_t)g * 9, red);
+}
+
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    const int k = blockIdx.x;
+    const double *src = coef + poff[k];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = src[o];
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 4, 1);
+    __syncthreads();
+    mw_pass_add_children<N1>(coef, coff[k], stride, a, filt);
+}
+
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_compress(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    const int k = blockIdx.x;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int t = o / N3, e = o % N3;
+        a[o] = coef[coff[k] + (long long)t * stride + e];
+    }
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 0, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 0, 1);
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 0, 2);
+    __syncthreads();
+    double *dst = coef + poff[k];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = b[o];
+}
+
+template <int N1> __global__ __launch_bounds__(kThreads) void k_node_norms(const double *coef, const long long *off, double *norms) {
+    __shared__ double red[8 * (kThreads / 64)];
+    const int k = blockIdx.x;
+    block_norms<N1>(coef + off[k], norms + (size_t)k * 9, red);
+}
+
+template <template <int> class Launcher, typename... Args> mwg_status dispatch(int kp1, Args... args) {
+    switch (kp1) {
+    case 4:
+        return Launcher<4>::run(args...);
+    case 6:
+        return Launcher<6>::run(args...);
+    case 8:
+        return Launcher<8>::run(args...);
+    case 10:
+        return Launcher<10>::run(args...);
+    default:
+        return fail(MWG_ERR_UNSUPPORTED, "unsupported order: k+1 = " + std::to_string(kp1));
+    }
+}
+
+template <int N1> struct GenerateL {
+    static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count) {
+        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct BandApplyL {
+    static mwg_status run(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, int npairs,
+                          double *partial, unsigned long long *napp) {
+        hipLaunchKernelGGL(k_band_apply<N1>, dim3(npairs), dim3(kThreads), 0, 0, *lvl, *f, *op, pr->pair_g, pr->pair_f, pr->pair_l, partial,
+                           napp);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct ReduceL {
+    static mwg_status run(const mwg_level *lvl, const mwg_pairs *pr, const double *partial) {
+        hipLaunchKernelGGL(k_reduce<N1>, dim3(lvl->count), dim3(kThreads), 0, 0, *lvl, *pr, partial);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct ReconL {
+    static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        hipLaunchKernelGGL(k_reconstruct_add<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct CompressL {
+    static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        hipLaunchKernelGGL(k_compress<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct NormsL {
+    static mwg_status run(const double *coef, const long long *off, double *norms, int count) {
+        hipLaunchKernelGGL(k_node_norms<N1>, dim3(count), dim3(kThreads), 0, 0, coef, off, norms);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+
+mwg_status need_init() { return g_ready ? MWG_OK : fail(MWG_ERR_ARG, "mwg_init has not been called"); }
+
+} // namespace
+
+extern "C" {
+
+const char *mwg_last_error(void) { return g_err.c_str(); }
+
+int mwg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+mwg_status mwg_init(int device) {
+    int n = 0;
+    MWG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MWG_ERR_ARG, "no such device");
+    MWG_HIP(hipSetDevice(device));
+    g_ready = true;
+    return MWG_OK;
+}
+
+mwg_status mwg_malloc(void **ptr, size_t bytes) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!ptr) return fail(MWG_ERR_ARG, "null pointer");
+    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(MWG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    }
+    return MWG_OK;
+}
+
+mwg_status mwg_free(void *ptr) {
+    if (ptr) MWG_HIP(hipFree(ptr));
+    return MWG_OK;
+}
+
+mwg_status mwg_upload(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MWG_OK;
+}
+
+mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MWG_OK;
+}
+
+mwg_status mwg_copy(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return MWG_OK;
+}
+
+mwg_status mwg_zero(void *dst, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemset(dst, 0, bytes));
+    return MWG_OK;
+}
+
+mwg_status mwg_synchronize(void) {
+    MWG_HIP(hipDeviceSynchronize());
+    return MWG_OK;
+}
+
+mwg_status mwg_filters_upload(int kp1, const double *host_filters, double **dev_filters) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!host_filters || !dev_filters || kp1 < 1) return fail(MWG_ERR_ARG, "bad filter upload arguments");
+    const size_t bytes = sizeof(double) * 8 * (size_t)kp1 * kp1;
+    void *p = nullptr;
+    if (mwg_malloc(&p, bytes) != MWG_OK) return MWG_ERR_NOMEM;
+    MWG_HIP(hipMemcpy(p, host_filters, bytes, hipMemcpyHostToDevice));
+    *dev_filters = static_cast<double *>(p);
+    return MWG_OK;
+}
+
+mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!lvl || !f || !op || !out || !out->count) return fail(MWG_ERR_ARG, "null argument");
+    if (lvl->o_depth < 0 || lvl->o_depth >= op->n_depth) return fail(MWG_ERR_ARG, "level depth outside the band tables");
+    if (lvl->o_depth > mwgkey::kMaxDepth) return fail(MWG_ERR_UNSUPPORTED, "level deeper than the device key range");
+    MWG_HIP(hipMemset(out->count, 0, sizeof(int)));
+    if (lvl->count == 0) return MWG_OK;
+    hipLaunchKernelGGL(k_pair_screen, dim3(lvl->count), dim3(kThreads), 0, 0, *lvl, *f, *op, *out);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+
+mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!f || !filters || !ids || !parent || !child_idx) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count);
+}
+
+mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs, int npairs, double *partial,
+                          unsigned long long *n_applied, int precision) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (npairs <= 0) return MWG_OK;
+    if (!lvl || !f || !op || !pairs || !partial || !n_applied) return fail(MWG_ERR_ARG, "null argument");
+    if (precision != 0) return fail(MWG_ERR_UNSUPPORTED, "only FP64 sweeps are available");
+    if (op->M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 512");
+    if (op->kp1 != f->kp1) return fail(MWG_ERR_ARG, "operator and tree orders differ");
+    return dispatch<BandApplyL>(f->kp1, lvl, f, op, pairs, npairs, partial, n_applied);
+}
+
+mwg_status mwg_reduce_level(const mwg_level *lvl, const mwg_pairs *pairs, const double *partial) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!lvl || !pairs) return fail(MWG_ERR_ARG, "null argument");
+    if (lvl->count == 0) return MWG_OK;
+    return dispatch<ReduceL>(lvl->kp1, lvl, pairs, partial);
+}
+
+mwg_status mwg_reconstruct_add(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,
+                               long long stride) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!filters || !coef || !parent_off || !child_off) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<ReconL>(kp1, filters, coef, parent_off, child_off, count, stride);
+}
+
+mwg_status mwg_compress(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,
+                        long long stride) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!filters || !coef || !parent_off || !child_off) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<CompressL>(kp1, filters, coef, parent_off, child_off, count, stride);
+}
+
+mwg_status mwg_node_norms(int kp1, const double *coef, const long long *off, double *norms, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!coef || !off || !norms) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<NormsL>(kp1, coef, off, norms, count);
+}
+
+} // extern "C"

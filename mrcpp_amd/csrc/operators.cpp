@@ -1,0 +1,331 @@
+// Operator construction (host FP64) and band tables for the device kernels.
+#include "operators.h"
+
+#include <chrono>
+#include <cmath>
+
+#include "device.h"
+
+namespace mrcpp {
+
+std::vector<KernelTerm> poisson_kernel(double epsilon, double r_min, double r_max) {
+    // Trapezoidal rule on the sinh-substituted integral representation of
+    // 1/r, built on [r_min/r_max, 1] and rescaled to [r_min, r_max].  The
+    // long-double literals reproduce the reference's mixed-precision steps.
+    const double r0 = r_min / r_max;
+    const double r1 = r_max;
+    double t1 = 1.0L;
+    while ((2.0 * t1 * std::exp(-t1)) > epsilon) t1 *= 1.1L;
+    double t2 = 1.0L;
+    while ((std::sqrt(t2) * std::exp(-t2) / r0) > epsilon) t2 *= 1.1L;
+    const double s1 = -std::log(2.0 * t1);
+    const double s2 = std::log(t2 / (r0 * r0)) / 2.0;
+    const double h = 1.0 / (0.2L - 0.47L * std::log10(epsilon));
+    const int n = static_cast<int>(std::ceil((s2 - s1) / h) + 1);
+    if (n > MaxSepRank) throw Error("Maximum separation rank exceeded.");
+    std::vector<KernelTerm> out;
+    out.reserve(n);
+    for (int i = 0; i < n; i++) {
+        const double s = s1 + h * i;
+        const double sh = std::sinh(s), ch = std::cosh(s);
+        const double e = 1.0 + std::exp(-sh);
+        double expo = 4.0L * (sh + std::log(e)) * (sh + std::log(e));
+        double coef = h * (4.0L / root_pi) * ch / e;
+        expo *= 1.0 / (r1 * r1);
+        coef *= 1.0 / r1;
+        if (i == 0 || i == n - 1) coef *= 1.0 / 2.0;
+        out.push_back({expo, coef});
+    }
+    return out;
+}
+
+std::vector<KernelTerm> helmholtz_kernel(double mu, double epsilon, double r_min, double r_max) {
+    const double r0 = r_min / r_max;
+    const double r1 = r_max;
+    const double mt = mu * r1;
+    const double t = std::max((-2.5L * std::log(epsilon)), 5.0L);
+    const double s1 = -std::log(4 * t / (mt * mt)) / 2;
+    const double s2 = std::log(t / (r0 * r0)) / 2;
+    const double h = 1.0 / (0.20L - 0.47L * std::log10(epsilon));
+    const int n = static_cast<int>(std::ceil((s2 - s1) / h) + 1);
+    if (n > MaxSepRank) throw Error("Maximum separation rank exceeded.");
+    std::vector<KernelTerm> out;
+    out.reserve(n);
+    for (int i = 0; i < n; i++) {
+        const double s = s1 + h * i;
+        const double e2 = -s * 2.0;
+        const double e3 = -mt * mt * std::exp(e2) / 4.0 + s;
+        double beta = (h * (2.0 / root_pi) * std::exp(e3));
+        const double two_s = 2.0L * s;
+        double alpha = std::exp(two_s);
+        alpha *= 1.0 / (r1 * r1);
+        beta *= 1.0 / r1;
+        if (i == 0 || i == n - 1) beta *= 1.0 / 2.0;
+        out.push_back({alpha, beta});
+    }
+    return out;
+}
+
+namespace {
+
+using detail::NodeStore;
+
+// Cross-correlation of the projected 1D kernel into the children of one 2D
+// node, then local compression (CrossCorrelationCalculator::calcNode/applyCcc,
+// src/treebuilders/CrossCorrelationCalculator.cpp:36-96).
+void cross_correlate_node(NodeStore<2> &ot, int u, NodeStore<1> &kt) {
+    ot.zeroCoefs(u);
+    const int K = ot.kp1(), kd = ot.kp1_d(), K2 = 2 * K;
+    const auto &cc = core::cross_correlation(ot.mra().getOrder());
+    const int scale = ot.scale(u) + 1;
+    const auto &l = ot.transl(u);
+    std::vector<double> v(4 * kd, 0.0);
+    for (int c = 0; c < 4; c++) {
+        const int l0 = 2 * l[0] + (c & 1), l1 = 2 * l[1] + ((c >> 1) & 1);
+        const int a = detail::get_or_generate(kt, scale, {l1 - l0 - 1});
+        const int b = detail::get_or_generate(kt, scale, {l1 - l0});
+        const double *sa = kt.coefs(a), *sb = kt.coefs(b);
+        for (int r = 0; r < kd; r++) {
+            const double *L = cc.left() + (size_t)r * K2, *R = cc.right() + (size_t)r * K2;
+            double xa = 0.0, xb = 0.0;
+            for (int p = 0; p < K2; p++) {
+                xa += L[p] * sa[p];
+                xb += R[p] * sb[p];
+            }
+            v[(size_t)c * kd + r] = xa + xb;
+        }
+    }
+    const double two_n = std::pow(2.0, -scale / 2.0);
+    const double sf = ot.mra().getWorldBox().getScalingFactor(0);
+    double *coef = ot.coefs(u);
+    for (int i = 0; i < 4 * kd; i++) coef[i] = std::sqrt(sf) * two_n * v[i];
+    detail::local_transform(ot, u, Compression);
+    ot.setHasCoefs(u, true);
+    detail::compute_norms(ot, u);
+}
+
+std::vector<int> max_translations(const NodeStore<2> &t) {
+    std::vector<int> mt(t.getDepth(), 0);
+    for (int u : detail::real_nodes(t)) {
+        const int d = t.depth(u);
+        mt[d] = std::max(mt[d], std::max(std::abs(t.transl(u)[0]), std::abs(t.transl(u)[1])));
+    }
+    return mt;
+}
+
+// One Gaussian term: 1D kernel projection, 2D tree by cross-correlation,
+// bottom-up transform and translation cache (ConvolutionOperator::initialize,
+// src/operators/ConvolutionOperator.cpp:78-108).
+std::unique_ptr<detail::OperatorTerm> build_term(const MultiResolutionAnalysis<3> &mra, const MultiResolutionAnalysis<1> &kmra,
+                                                  const KernelTerm &term, double k_prec, double o_prec, int reach) {
+    GaussFunc<1> kf(term.alpha, std::copysign(std::pow(std::abs(term.coef), 1.0 / 3), term.coef));
+    NodeStore<1> kt(kmra);
+    GaussExp<1> ge;
+    ge.append(kf);
+    build_grid_nodes(kt, ge);
+    project_nodes(kt, k_prec, kf);
+
+    auto term_out = std::make_unique<detail::OperatorTerm>();
+    const double sf = mra.getWorldBox().getScalingFactor(0);
+    term_out->mra = std::make_unique<MultiResolutionAnalysis<2>>(
+        BoundingBox<2>(mra.getRootScale(), {0, 0}, {reach, reach}, {sf, sf}), LegendreBasis(mra.getOrder()));
+    term_out->tree = std::make_unique<NodeStore<2>>(*term_out->mra, true, o_prec);
+    NodeStore<2> &ot = *term_out->tree;
+
+    detail::BuildHooks<2> h;
+    h.initial = [](NodeStore<2> &st) { return detail::end_nodes(st); };
+    h.compute = [&kt](NodeStore<2> &st, const std::vector<int> &v) {
+        for (int u : v) cross_correlate_node(st, u, kt);
+    };
+    // OperatorAdaptor::splitNode (src/treebuilders/OperatorAdaptor.h:38-48)
+    h.split = [](const NodeStore<2> &st, int u) {
+        const auto &l = st.transl(u);
+        if (!(l[0] == 0 || l[1] == 0)) return false;
+        for (int c = 1; c < 4; c++)
+            if (st.compNorm(u, c) > 0.0) return true;
+        return false;
+    };
+    detail::build_tree(ot, h, -1);
+    detail::transform_up(ot);
+    detail::calc_square_norm(ot);
+
+    // translation cache: l > 0 -> node (l, 0), l <= 0 -> node (0, -l)
+    term_out->maxTransl = max_translations(ot);
+    const int ns = ot.getDepth();
+    term_out->cache.resize(ns);
+    for (int d = 0; d < ns; d++) {
+        const int nt = term_out->maxTransl[d], scale = ot.rootScale() + d;
+        auto &row = term_out->cache[d];
+        row.resize(2 * nt + 1);
+        for (int i = nt; i >= 0; i--) row[nt - i] = detail::get_or_generate(ot, scale, {0, i});
+        for (int i = 1; i <= nt; i++) row[nt + i] = detail::get_or_generate(ot, scale, {i, 0});
+    }
+    return term_out;
+}
+
+// OperatorTree::calcBandWidth (src/trees/OperatorTree.cpp:109-134)
+void term_band_width(detail::OperatorTerm &t, double prec) {
+    NodeStore<2> &ot = *t.tree;
+    const int depthN = ot.getDepth();
+    t.widths.assign(depthN + 1, {-1, -1, -1, -1, -1});
+    const std::vector<int> mt = max_translations(ot);
+    if (prec < 0.0) prec = ot.normPrec();
+    for (int d = 0; d < depthN; d++) {
+        const double thr = std::max(MachinePrec, prec / (8.0 * (1 << d)));
+        for (int l = 0;; l++) {
+            if (l > t.maxTransl[d]) throw Error("band width beyond the operator translation cache");
+            const int u = t.node(d, l);
+            bool any = false;
+            for (int c = 0; c < 4; c++)
+                if (ot.compNorm(u, c) > thr) {
+                    t.widths[d][c] = l;
+                    t.widths[d][4] = std::max(t.widths[d][4], l);
+                    any = true;
+                }
+            if (!any || l + 1 > mt[d]) break;
+        }
+    }
+}
+
+} // namespace
+
+ConvolutionOperator<3>::ConvolutionOperator(const MultiResolutionAnalysis<3> &mra)
+        : mra_(mra) {}
+
+ConvolutionOperator<3>::ConvolutionOperator(const MultiResolutionAnalysis<3> &mra, const std::vector<KernelTerm> &kernel, double prec)
+        : mra_(mra) {
+    initialize(kernel, prec);
+}
+
+ConvolutionOperator<3>::~ConvolutionOperator() { device::destroy_band_tables(dev_tables_); }
+
+void ConvolutionOperator<3>::initialize(const std::vector<KernelTerm> &kernel, double prec) {
+    auto t0 = std::chrono::steady_clock::now();
+    kernel_ = kernel;
+    build_prec_ = prec;
+    const double o_prec = prec, k_prec = prec / 10.0;
+    // kernel MRA: order 2k+1 on [-reach, reach) root boxes (ConvolutionOperator::getKernelMRA :110-142)
+    int reach = 0;
+    for (int d = 0; d < 3; d++) reach = std::max(reach, mra_.getWorldBox().size(d));
+    const double sf = mra_.getWorldBox().getScalingFactor(0);
+    MultiResolutionAnalysis<1> kmra(BoundingBox<1>(mra_.getRootScale(), {-reach}, {2 * reach}, {sf}), LegendreBasis(2 * mra_.getOrder() + 1));
+    // prime the shared caches before the parallel region
+    core::cross_correlation(mra_.getOrder());
+    core::filters(2 * mra_.getOrder() + 1);
+    terms_.resize(kernel.size());
+    const int M = (int)kernel.size();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < M; i++) terms_[i] = build_term(mra_, kmra, kernel[i], k_prec, o_prec, reach);
+    build_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+void ConvolutionOperator<3>::calcBandWidths(double prec) {
+    int maxDepth = 0;
+    for (auto &t : terms_) {
+        term_band_width(*t, prec);
+        maxDepth = std::max(maxDepth, t->bwDepth());
+    }
+    band_max_.assign(maxDepth + 1, -1);
+    for (auto &t : terms_)
+        for (int n = 0; n <= t->bwDepth(); n++)
+            for (int c = 0; c < 4; c++) band_max_[n] = std::max(band_max_[n], t->width(n, c));
+}
+
+int ConvolutionOperator<3>::getMaxBandWidth(int depth) const {
+    if (depth < 0) return band_max_.empty() ? -1 : *std::max_element(band_max_.begin(), band_max_.end());
+    if (depth < (int)band_max_.size()) return band_max_[depth];
+    return -1;
+}
+
+const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
+    if (tables_ && table_prec_ == prec) return *tables_;
+    calcBandWidths(prec);
+    auto T = std::make_unique<detail::BandTables>();
+    const int M = size(), kp1 = mra_.getKp1(), n2 = kp1 * kp1;
+    T->M = M;
+    T->kp1 = kp1;
+    T->nDepth = (int)band_max_.size();
+    T->bandMax = band_max_;
+    T->Wmax = std::max(0, getMaxBandWidth(-1));
+    const int ND = T->nDepth, W = T->Wmax;
+    T->widths.assign((size_t)ND * M * 5, -1);
+    T->bandSize.assign((size_t)ND * M * 64, 0);
+    T->matStart.assign((size_t)ND * M, -1);
+    T->pairBound.assign((size_t)ND * (2 * W + 1), 0.0);
+    long long nmat = 0;
+    for (int d = 0; d < ND; d++)
+        for (int i = 0; i < M; i++) {
+            const auto &t = *terms_[i];
+            for (int c = 0; c < 5; c++) T->widths[((size_t)d * M + i) * 5 + c] = (c < 4) ? t.width(d, c) : t.maxWidth(d);
+            // ConvolutionCalculator::calcBandSizeFactor (:116-139): a negative
+            // width drops only its own factor
+            int bsmax = 0;
+            for (int gt = 0; gt < 8; gt++)
+                for (int ft = 0; ft < 8; ft++) {
+                    int tot = 1;
+                    for (int dim = 0; dim < 3; dim++) {
+                        const int w = t.width(d, 2 * ((gt >> dim) & 1) + ((ft >> dim) & 1));
+                        if (w < 0) continue;
+                        tot *= 2 * w + 1;
+                    }
+                    T->bandSize[((size_t)d * M + i) * 64 + gt * 8 + ft] = tot * 64;
+                    bsmax = std::max(bsmax, tot * 64);
+                }
+            const int w = t.maxWidth(d);
+            if (w < 0) continue;
+            T->matStart[(size_t)d * M + i] = nmat;
+            nmat += (long long)(2 * w + 1) * 4;
+            const double cb = std::cbrt((double)bsmax);
+            for (int dl = -w; dl <= w; dl++) {
+                const int u = t.node(d, dl);
+                double mx = 0.0;
+                for (int c = 0; c < 4; c++) mx = std::max(mx, t.tree->compNorm(u, c));
+                double &pb = T->pairBound[(size_t)d * (2 * W + 1) + dl + W];
+                pb = std::max(pb, cb * mx);
+            }
+        }
+    T->mats.resize((size_t)nmat * n2);
+    T->matNorm.resize(nmat);
+    for (int d = 0; d < ND; d++)
+        for (int i = 0; i < M; i++) {
+            const long long s = T->matStart[(size_t)d * M + i];
+            if (s < 0) continue;
+            const auto &t = *terms_[i];
+            const int w = t.maxWidth(d);
+            for (int dl = -w; dl <= w; dl++) {
+                const int u = t.node(d, dl);
+                for (int c = 0; c < 4; c++) {
+                    const long long m = s + (long long)(dl + w) * 4 + c;
+                    std::memcpy(&T->mats[(size_t)m * n2], t.tree->coefs(u) + (size_t)c * n2, sizeof(double) * n2);
+                    T->matNorm[m] = t.tree->compNorm(u, c);
+                }
+            }
+        }
+    tables_ = std::move(T);
+    table_prec_ = prec;
+    device::destroy_band_tables(dev_tables_);
+    dev_tables_ = nullptr;
+    return *tables_;
+}
+
+detail::DeviceBandTables *ConvolutionOperator<3>::deviceTables(double prec) {
+    if (!(tables_ && table_prec_ == prec && dev_tables_)) {
+        bandTables(prec);
+        dev_tables_ = device::create_band_tables(*tables_);
+    }
+    return dev_tables_;
+}
+
+PoissonOperator::PoissonOperator(const MultiResolutionAnalysis<3> &mra, double prec)
+        : ConvolutionOperator<3>(mra) {
+    const double k_prec = prec / 10.0;
+    initialize(poisson_kernel(k_prec, mra.calcMinDistance(k_prec), mra.calcMaxDistance()), prec);
+}
+
+HelmholtzOperator::HelmholtzOperator(const MultiResolutionAnalysis<3> &mra, double mu, double prec)
+        : ConvolutionOperator<3>(mra) {
+    const double k_prec = prec / 10.0;
+    initialize(helmholtz_kernel(mu, k_prec, mra.calcMinDistance(k_prec), mra.calcMaxDistance()), prec);
+}
+
+} // namespace mrcpp
