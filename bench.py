@@ -1,0 +1,152 @@
+"""Benchmark: Poisson apply on the MI355X (order 7, prec 1e-6, 16 Gaussians).
+
+One step = one full apply(prec, out, P, f) on a fresh output tree: every
+breadth-first level's screening, generated band nodes, band sums and
+reductions, then the top-down/bottom-up transforms and norms.  Inputs (the
+projected source tree and the operator with its device band tables) are
+resident in HBM before the timed region.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload gauss16_k7] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def flops_per_tuple(order):
+    """Algorithmic FLOP of one applied (f, ft, gt, term) tuple: three
+    [(k+1)^2 x (k+1)] x [(k+1) x (k+1)] sweeps = 6 (k+1)^4."""
+    return 6 * (order + 1) ** 4
+
+
+def setup(workload):
+    import mrcpp_amd as mr
+    from mrcpp_amd import workloads as W
+    w = W.WORKLOADS[workload]
+    src = w["sources"]()
+    mra = mr.MultiResolutionAnalysis((-w["box"], w["box"]), w["order"])
+    f = mr.FunctionTree(mra)
+    g = mr.GaussExp.from_sources(src)
+    mr.build_grid(f, g)
+    mr.project(w["prec"], f, g)
+    if w["mu"] is None:
+        P = mr.PoissonOperator(mra, w["prec"])
+    else:
+        P = mr.HelmholtzOperator(mra, w["mu"], w["prec"])
+    return mr, w, src, mra, f, P
+
+
+def cpu_baseline(workload, w, src):
+    """The CPU oracle's apply on the same workload (one full tree), timed on
+    this host's cores."""
+    import oracle as O
+    omra = O.OracleMRA((-w["box"], w["box"]), w["order"])
+    of = O.OracleTree(omra)
+    of.project_gaussians(w["prec"], src.coefs, src.exponents, src.centers)
+    op = O.OracleOperator(omra, w["prec"], mu=w["mu"])
+    og = O.OracleTree(omra)
+    st = O.apply(w["prec"], og, op, of)
+    gflop = st["n_applied"] * flops_per_tuple(w["order"]) / 1e9
+    return {
+        "value": gflop / st["seconds"],
+        "unit": "GFLOP/s",
+        "cores": O.Oracle.max_threads(),
+        "kind": "port",
+        "sample": "one full apply of %s on the host (OpenMP over output nodes): %.3f s/tree, %d applied tuples"
+                  % (workload, st["seconds"], st["n_applied"]),
+        "s_per_tree": st["seconds"],
+        "n_applied": st["n_applied"],
+        "n_output_nodes": st["n_g"],
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="gauss16_k7")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the apply path runs on a single GPU "
+              "(multi-GPU sharding is described in DESIGN.md, not launched); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    mr, w, src, mra, f, P = setup(args.workload)
+    order = w["order"]
+    for _ in range(max(args.warmup, 1)):
+        out = mr.FunctionTree(mra)
+        st = mr.apply(w["prec"], out, P, f)
+        del out
+    mr.device_synchronize()
+
+    # timed region: K full applies
+    times, stats = [], []
+    mr.profile(True)
+    mr.device_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ts = time.perf_counter()
+        out = mr.FunctionTree(mra)
+        st = mr.apply(w["prec"], out, P, f)
+        mr.device_synchronize()
+        times.append(time.perf_counter() - ts)
+        stats.append(st)
+        del out
+    mr.device_synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = mr.profile_read()
+    mr.profile(False)
+
+    n_applied = stats[-1]["n_applied"]
+    flop = n_applied * flops_per_tuple(order)
+    s_per_tree = elapsed / args.steps
+    gflops = flop / s_per_tree / 1e9
+    band_ms, band_launches = prof["band"]
+    band_ms_per_apply = band_ms / args.steps
+    band_achieved = flop / (band_ms_per_apply / 1e3) / 1e12  # TFLOP/s while the band kernel runs
+    res = {
+        "metric": "Poisson apply GFLOP/s (order 7, prec 1e-6; s/tree alongside)",
+        "value": round(gflops, 3),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_per_tree * 1e3, 4),
+        "s_per_tree": s_per_tree,
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic seeded Gaussian sources",
+        "config": {"workload": args.workload, "order": order, "prec": w["prec"], "box": [-w["box"], w["box"]],
+                   "n_gaussians": len(src), "operator": "Poisson" if w["mu"] is None else "Helmholtz mu=%g" % w["mu"],
+                   "separation_rank": P.size(), "n_applied": int(n_applied),
+                   "n_output_nodes": int(stats[-1]["n_output_nodes"]), "levels": int(stats[-1]["n_levels"]),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "mfma", "kernel": "k_band_apply", "achieved": round(band_achieved, 4),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(band_achieved / FP64_PEAK_TFLOPS, 5),
+                     "traffic": None, "launches_per_step": band_launches / args.steps,
+                     "avg_launch_ms": band_ms / max(band_launches, 1),
+                     "flop_per_launch": flop / max(band_launches / args.steps, 1)},
+        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
+        "host_breakdown_s": {k: stats[-1][k] for k in ("t_screen", "t_generate", "t_band", "t_reduce", "t_transform", "t_host")},
+        "step_times_s": [round(t, 6) for t in times],
+    }
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
+        res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / s_per_tree
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -35,7 +35,8 @@ mwg_status mwg_init(int device);
 const char *mwg_last_error(void);
 int mwg_device_count(void);
 
-/* Device memory helpers (bytes). */
+/* Device memory helpers (bytes).  Freed blocks are kept in a per-size pool
+ * and reused by later allocations; mwg_release_pool returns them. */
 mwg_status mwg_malloc(void **ptr, size_t bytes);
 mwg_status mwg_free(void *ptr);
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
@@ -43,6 +44,23 @@ mwg_status mwg_download(void *dst, const void *src, size_t bytes);
 mwg_status mwg_copy(void *dst, const void *src, size_t bytes); /* device to device */
 mwg_status mwg_zero(void *dst, size_t bytes);
 mwg_status mwg_synchronize(void);
+mwg_status mwg_release_pool(void);
+
+/* Per-kernel timing: while enabled, every launch is bracketed by HIP events
+ * on its stream; mwg_profile_read sums the durations (ms) and launch counts
+ * per kernel kind.  mwg_profile(0/1) clears the record. */
+enum {
+    MWG_KERNEL_SCREEN = 0,
+    MWG_KERNEL_GENERATE = 1,
+    MWG_KERNEL_BAND = 2,
+    MWG_KERNEL_REDUCE = 3,
+    MWG_KERNEL_RECONSTRUCT = 4,
+    MWG_KERNEL_COMPRESS = 5,
+    MWG_KERNEL_NORMS = 6,
+    MWG_KERNEL_KINDS = 7
+};
+mwg_status mwg_profile(int enable);
+mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds);
 
 /* Band tables of a convolution operator at one apply precision: widths and
  * band-size factors (ConvolutionCalculator::initBandSizes / calcBandSizeFactor,
@@ -63,6 +81,8 @@ typedef struct {
     const double *mats;    /* matrices (kp1*kp1 each, column-major: O(i_f, j_g)) */
     const double *mat_norm;/* one norm per matrix */
     const double *pair_bound; /* [n_depth][2*wmax+1] */
+    const int *band_size_max; /* [n_depth][M] row maximum of band_size */
+    const double *term_norm;  /* [matrix/4] max component norm per (depth, term, dl) */
 } mwg_band_tables;
 
 /* Input function tree resident on the device: an open-addressing hash from a
@@ -109,12 +129,15 @@ typedef struct {
     int *pair_l;         /* [capacity][3] band node translation */
     int *g_first;        /* [level count] first pair of each output node */
     int *g_npairs;       /* [level count] */
+    int *pair_work;      /* [capacity] set by mwg_band_apply: nonzero if the pair applied any tuple */
 } mwg_pairs;
 
-/* Screen the band of every output node of a level with the separable bound
- * and emit candidate pairs (ConvolutionCalculator::makeOperBand/fillOperBand,
- * src/treebuilders/ConvolutionCalculator.cpp:142-222, with the norm bound of
- * applyOperator :297-329 relaxed to a product of per-direction maxima). */
+/* Screen the band of every output node of a level and emit candidate pairs
+ * (ConvolutionCalculator::makeOperBand/fillOperBand,
+ * src/treebuilders/ConvolutionCalculator.cpp:142-222).  Two upper bounds of
+ * the norm test of applyOperator (:297-329) discard band nodes: a product of
+ * per-direction maxima over all terms, then a per-term bound over all
+ * components; neither can reject a tuple the exact test would apply. */
 mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out);
 
 /* Generate scaling-only band nodes below existing parents: node ids[i] gets

@@ -57,7 +57,16 @@ SIGNATURES = {
                                           _dp, _dp, ctypes.c_int]),
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
     "mrcpp_last_apply_levels": (ctypes.c_int, [_ip, ctypes.c_int]),
+    # device layer (include/mwg.h) used by the harness
+    "mwg_last_error": (ctypes.c_char_p, []),
+    "mwg_device_count": (ctypes.c_int, []),
+    "mwg_synchronize": (ctypes.c_int, []),
+    "mwg_profile": (ctypes.c_int, [ctypes.c_int]),
+    "mwg_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
+    "mwg_release_pool": (ctypes.c_int, []),
 }
+
+KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms")
 
 
 class MRCPPError(RuntimeError):
@@ -266,6 +275,27 @@ def dot(bra, ket):
 
 STAT_KEYS = ("n_applied", "n_pairs", "n_generated", "n_output_nodes", "seconds", "n_levels", "t_screen", "t_generate",
              "t_band", "t_reduce", "t_transform", "t_host")
+
+
+def device_synchronize():
+    if lib().mwg_synchronize() != 0:
+        raise MRCPPError(lib().mwg_last_error().decode())
+
+
+def profile(enable):
+    """Start (True) or stop recording per-kernel HIP-event timings."""
+    if lib().mwg_profile(int(bool(enable))) != 0:
+        raise MRCPPError(lib().mwg_last_error().decode())
+
+
+def profile_read():
+    """{kind: (total ms, launches)} of the kernels launched since profile(True)."""
+    n = len(KERNEL_KINDS)
+    ms = np.zeros(n)
+    cnt = np.zeros(n, dtype=np.int64)
+    if lib().mwg_profile_read(_p(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n) != 0:
+        raise MRCPPError(lib().mwg_last_error().decode())
+    return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_KINDS)}
 
 
 def apply(prec, out, oper, inp, maxIter=-1, absPrec=False):

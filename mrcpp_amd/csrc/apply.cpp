@@ -22,7 +22,7 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild;
+    device::DeviceArray<int> l, pairG, pairF, pairL, pairWork, gFirst, gCount, count, genIds, genParent, genChild;
     device::DeviceArray<long long> gOff, pOff, cOff;
     device::DeviceArray<double> gNorms, partial;
     device::DeviceArray<unsigned long long> nApplied;
@@ -167,6 +167,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
             S.pairG.reserve(cap);
             S.pairF.reserve(cap);
             S.pairL.reserve(3 * cap);
+            S.pairWork.reserve(cap);
             pr.capacity = (int)std::min<size_t>(S.pairF.capacity(), S.pairL.capacity() / 3);
             pr.count = S.count.get();
             pr.pair_g = S.pairG.get();
@@ -174,6 +175,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
             pr.pair_l = S.pairL.get();
             pr.g_first = S.gFirst.get();
             pr.g_npairs = S.gCount.get();
+            pr.pair_work = S.pairWork.get();
             device::check(mwg_pair_screen(&lvl, &fv, &c.bt, &pr), "pair screen");
             S.count.download(&npairs, 1);
             if (npairs <= pr.capacity) break;
@@ -318,7 +320,8 @@ void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, Func
     S.nApplied.zero(1);
 
     // device output storage for the nodes that exist now
-    out.deviceCoefs().reserve((size_t)g.size() * c.ncoef);
+    // output pool sized for an output tree as large as the input (grown on demand)
+    out.deviceCoefs().reserve((size_t)std::max(g.size(), fm.nReal) * c.ncoef);
 
     const int M = oper.size();
     const int maxScale = mra.getMaxScale();

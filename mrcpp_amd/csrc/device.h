@@ -70,7 +70,8 @@ namespace detail {
 struct DeviceBandTables {
     device::DeviceArray<int> bandMax, widths, bandSize;
     device::DeviceArray<long long> matStart;
-    device::DeviceArray<double> mats, matNorm, pairBound;
+    device::DeviceArray<double> mats, matNorm, pairBound, termNorm;
+    device::DeviceArray<int> bandSizeMax;
     mwg_band_tables view{};
 };
 } // namespace detail

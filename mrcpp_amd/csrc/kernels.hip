@@ -8,7 +8,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/mwg.h"
 #include "keys.h"
@@ -34,6 +37,33 @@ mwg_status fail(mwg_status s, const std::string &msg) {
         hipError_t e_ = hipGetLastError();                                                                              \
         if (e_ != hipSuccess) return fail(MWG_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_));     \
     } while (0)
+
+// Optional per-kernel timing with HIP events on the launch stream.
+struct TimedLaunch {
+    int kind;
+    hipEvent_t start, stop;
+};
+bool g_profile = false;
+std::vector<TimedLaunch> &timed_launches() {
+    static std::vector<TimedLaunch> v;
+    return v;
+}
+struct ProfileScope {
+    int kind;
+    hipEvent_t start{}, stop{};
+    explicit ProfileScope(int k)
+            : kind(k) {
+        if (!g_profile) return;
+        (void)hipEventCreate(&start);
+        (void)hipEventCreate(&stop);
+        (void)hipEventRecord(start, 0);
+    }
+    ~ProfileScope() {
+        if (!g_profile) return;
+        (void)hipEventRecord(stop, 0);
+        timed_launches().push_back({kind, start, stop});
+    }
+};
 
 constexpr int kThreads = 256;
 constexpr int kMaxTerms = 512;
@@ -111,6 +141,22 @@ __device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f,
         code = -(anc + 1);
     }
     if (!(a * maxf * kBoundSlack > lvl.g_thr)) return false;
+    // per-term bound: max band size x max component norm per direction
+    const int M = op.M;
+    const int dx = x - gl[0], dy = y - gl[1], dz = z - gl[2];
+    const int maxdl = max(abs(dx), max(abs(dy), abs(dz)));
+    const int *wd = op.widths + (size_t)d * M * 5;
+    const long long *ms = op.mat_start + (size_t)d * M;
+    const int *bsm = op.band_size_max + (size_t)d * M;
+    bool any = false;
+    for (int i = 0; i < M && !any; i++) {
+        const int w = wd[i * 5 + 4];
+        if (maxdl > w) continue;
+        const double *tn = op.term_norm + ms[i] / 4 + w;
+        const double t = (double)bsm[i] * tn[dx] * tn[dy] * tn[dz] * maxf;
+        any = t * kBoundSlack > lvl.g_thr;
+    }
+    if (!any) return false;
     *out_f = code;
     out_l[0] = x;
     out_l[1] = y;
@@ -297,7 +343,7 @@ __global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double
 // ---------------------------------------------------------------------------
 template <int N1>
 __global__ __launch_bounds__(kThreads) void k_band_apply(mwg_level lvl, mwg_ftree f, mwg_band_tables op, const int *pair_g, const int *pair_f,
-                                                        const int *pair_l, double *partial, unsigned long long *n_applied) {
+                                                        const int *pair_l, int *pair_work, double *partial, unsigned long long *n_applied) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1;
     constexpr int R = (8 * N3 + kThreads - 1) / kThreads;
     __shared__ double sF[N3];
@@ -449,6 +495,8 @@ __global__ __launch_bounds__(kThreads) void k_band_apply(mwg_level lvl, mwg_ftre
         }
         __syncthreads();
     }
+    if (threadIdx.x == 0) pair_work[p] = G > 0;
+    if (G == 0) return; // nothing applied: the reduction skips this pair
     double *dst = partial + (size_t)p * 8 * N3;
 #pragma unroll
     for (int rr = 0; rr < R; rr++) {
@@ -468,7 +516,8 @@ template <int N1> __global__ __launch_bounds__(kThreads) void k_reduce(mwg_level
     double *dst = lvl.g_coef + lvl.g_off[g];
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
         double s = 0.0;
-        for (int k = 0; k < n; k++) s += partial[(size_t)(first + k) * 8 * N3 + o];
+        for (int k = 0; k < n; k++)
+            if (pr.pair_work[first + k]) s += partial[(size_t)(first + k) * 8 * N3 + o];
         dst[o] = s;
     }
     __syncthreads();
@@ -533,6 +582,7 @@ template <template <int> class Launcher, typename... Args> mwg_status dispatch(i
 
 template <int N1> struct GenerateL {
     static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count) {
+        ProfileScope prof_(MWG_KERNEL_GENERATE);
         hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -541,7 +591,8 @@ template <int N1> struct GenerateL {
 template <int N1> struct BandApplyL {
     static mwg_status run(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, int npairs,
                           double *partial, unsigned long long *napp) {
-        hipLaunchKernelGGL(k_band_apply<N1>, dim3(npairs), dim3(kThreads), 0, 0, *lvl, *f, *op, pr->pair_g, pr->pair_f, pr->pair_l, partial,
+        ProfileScope prof_(MWG_KERNEL_BAND);
+        hipLaunchKernelGGL(k_band_apply<N1>, dim3(npairs), dim3(kThreads), 0, 0, *lvl, *f, *op, pr->pair_g, pr->pair_f, pr->pair_l, pr->pair_work, partial,
                            napp);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -549,6 +600,7 @@ template <int N1> struct BandApplyL {
 };
 template <int N1> struct ReduceL {
     static mwg_status run(const mwg_level *lvl, const mwg_pairs *pr, const double *partial) {
+        ProfileScope prof_(MWG_KERNEL_REDUCE);
         hipLaunchKernelGGL(k_reduce<N1>, dim3(lvl->count), dim3(kThreads), 0, 0, *lvl, *pr, partial);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -556,6 +608,7 @@ template <int N1> struct ReduceL {
 };
 template <int N1> struct ReconL {
     static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        ProfileScope prof_(MWG_KERNEL_RECONSTRUCT);
         hipLaunchKernelGGL(k_reconstruct_add<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -563,6 +616,7 @@ template <int N1> struct ReconL {
 };
 template <int N1> struct CompressL {
     static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        ProfileScope prof_(MWG_KERNEL_COMPRESS);
         hipLaunchKernelGGL(k_compress<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -570,6 +624,7 @@ template <int N1> struct CompressL {
 };
 template <int N1> struct NormsL {
     static mwg_status run(const double *coef, const long long *off, double *norms, int count) {
+        ProfileScope prof_(MWG_KERNEL_NORMS);
         hipLaunchKernelGGL(k_node_norms<N1>, dim3(count), dim3(kThreads), 0, 0, coef, off, norms);
         MWG_LAUNCHED();
         return MWG_OK;
@@ -577,6 +632,21 @@ template <int N1> struct NormsL {
 };
 
 mwg_status need_init() { return g_ready ? MWG_OK : fail(MWG_ERR_ARG, "mwg_init has not been called"); }
+
+// Device blocks cached by exact size for reuse (freed blocks are not returned
+// to the runtime until mwg_release_pool).
+std::mutex &pool_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::map<size_t, std::vector<void *>> &free_blocks() {
+    static std::map<size_t, std::vector<void *>> m;
+    return m;
+}
+std::map<void *, size_t> &block_size() {
+    static std::map<void *, size_t> m;
+    return m;
+}
 
 } // namespace
 
@@ -602,15 +672,76 @@ mwg_status mwg_init(int device) {
 mwg_status mwg_malloc(void **ptr, size_t bytes) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     if (!ptr) return fail(MWG_ERR_ARG, "null pointer");
-    if (hipMalloc(ptr, bytes ? bytes : 1) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(MWG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    if (!bytes) bytes = 1;
+    {
+        std::lock_guard<std::mutex> lk(pool_mutex());
+        auto &fl = free_blocks()[bytes];
+        if (!fl.empty()) {
+            *ptr = fl.back();
+            fl.pop_back();
+            block_size()[*ptr] = bytes;
+            return MWG_OK;
+        }
     }
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        // return cached blocks to the runtime and retry once
+        mwg_release_pool();
+        if (hipMalloc(ptr, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MWG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    }
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    block_size()[*ptr] = bytes;
     return MWG_OK;
 }
 
 mwg_status mwg_free(void *ptr) {
-    if (ptr) MWG_HIP(hipFree(ptr));
+    if (!ptr) return MWG_OK;
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    auto it = block_size().find(ptr);
+    if (it == block_size().end()) return fail(MWG_ERR_ARG, "mwg_free of an unknown pointer");
+    free_blocks()[it->second].push_back(ptr);
+    block_size().erase(it);
+    return MWG_OK;
+}
+
+mwg_status mwg_profile(int enable) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    MWG_HIP(hipDeviceSynchronize());
+    for (auto &t : timed_launches()) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    timed_launches().clear();
+    g_profile = enable != 0;
+    return MWG_OK;
+}
+
+mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!ms || !launches) return fail(MWG_ERR_ARG, "null argument");
+    MWG_HIP(hipDeviceSynchronize());
+    for (int k = 0; k < nkinds; k++) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    for (auto &t : timed_launches()) {
+        if (t.kind < 0 || t.kind >= nkinds) continue;
+        float e = 0.0f;
+        MWG_HIP(hipEventElapsedTime(&e, t.start, t.stop));
+        ms[t.kind] += e;
+        launches[t.kind]++;
+    }
+    return MWG_OK;
+}
+
+mwg_status mwg_release_pool(void) {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    for (auto &kv : free_blocks())
+        for (void *p : kv.second) MWG_HIP(hipFree(p));
+    free_blocks().clear();
     return MWG_OK;
 }
 
@@ -665,6 +796,7 @@ mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_b
     if (lvl->o_depth > mwgkey::kMaxDepth) return fail(MWG_ERR_UNSUPPORTED, "level deeper than the device key range");
     MWG_HIP(hipMemset(out->count, 0, sizeof(int)));
     if (lvl->count == 0) return MWG_OK;
+    ProfileScope prof_(MWG_KERNEL_SCREEN);
     hipLaunchKernelGGL(k_pair_screen, dim3(lvl->count), dim3(kThreads), 0, 0, *lvl, *f, *op, *out);
     MWG_LAUNCHED();
     return MWG_OK;

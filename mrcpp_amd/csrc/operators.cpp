@@ -252,6 +252,7 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
     T->bandSize.assign((size_t)ND * M * 64, 0);
     T->matStart.assign((size_t)ND * M, -1);
     T->pairBound.assign((size_t)ND * (2 * W + 1), 0.0);
+    T->bandSizeMax.assign((size_t)ND * M, 0);
     long long nmat = 0;
     for (int d = 0; d < ND; d++)
         for (int i = 0; i < M; i++) {
@@ -271,6 +272,7 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
                     T->bandSize[((size_t)d * M + i) * 64 + gt * 8 + ft] = tot * 64;
                     bsmax = std::max(bsmax, tot * 64);
                 }
+            T->bandSizeMax[(size_t)d * M + i] = bsmax;
             const int w = t.maxWidth(d);
             if (w < 0) continue;
             T->matStart[(size_t)d * M + i] = nmat;
@@ -286,6 +288,7 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
         }
     T->mats.resize((size_t)nmat * n2);
     T->matNorm.resize(nmat);
+    T->termNorm.assign(nmat / 4, 0.0);
     for (int d = 0; d < ND; d++)
         for (int i = 0; i < M; i++) {
             const long long s = T->matStart[(size_t)d * M + i];
@@ -298,6 +301,7 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
                     const long long m = s + (long long)(dl + w) * 4 + c;
                     std::memcpy(&T->mats[(size_t)m * n2], t.tree->coefs(u) + (size_t)c * n2, sizeof(double) * n2);
                     T->matNorm[m] = t.tree->compNorm(u, c);
+                    T->termNorm[m / 4] = std::max(T->termNorm[m / 4], T->matNorm[m]);
                 }
             }
         }

@@ -50,6 +50,8 @@ struct BandTables {
     std::vector<double> mats;        // matrices, column-major (k+1)^2 each
     std::vector<double> matNorm;     // per matrix
     std::vector<double> pairBound;   // [nDepth][2*Wmax+1]
+    std::vector<int> bandSizeMax;    // [nDepth][M] row maximum of bandSize
+    std::vector<double> termNorm;    // [matrix / 4]: max over the 4 components
 };
 
 struct DeviceBandTables; // owned by the device layer

@@ -1643,6 +1643,26 @@ int orc_tree_project_gaussians(void *tp, double prec, int n, const double *coefs
     }
 }
 
+// build_grid only (an empty grid refined around each Gaussian)
+int orc_tree_build_grid_gaussians(void *tp, int n, const double *coefs, const double *alphas, const double *centers) {
+    try {
+        Tree<3> &t = *static_cast<OrcTree *>(tp)->t;
+        GaussExp<3> ge;
+        for (int i = 0; i < n; i++) {
+            GaussFunc<3> g;
+            g.alpha = {alphas[i], alphas[i], alphas[i]};
+            g.coef = coefs[i];
+            g.pos = {centers[3 * i], centers[3 * i + 1], centers[3 * i + 2]};
+            ge.funcs.push_back(g);
+        }
+        build_grid(t, ge);
+        return 0;
+    } catch (std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
 long long orc_tree_num_nodes(void *tp) { return (long long)node_table(*static_cast<OrcTree *>(tp)->t).size(); }
 
 double orc_tree_square_norm(void *tp) { return static_cast<OrcTree *>(tp)->t->squareNorm; }

@@ -45,6 +45,7 @@ class Oracle:
             L.orc_tree_create.argtypes = [ctypes.c_void_p]
             L.orc_tree_destroy.argtypes = [ctypes.c_void_p]
             L.orc_tree_project_gaussians.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+            L.orc_tree_build_grid_gaussians.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
             L.orc_tree_num_nodes.restype = ctypes.c_longlong
             L.orc_tree_num_nodes.argtypes = [ctypes.c_void_p]
             L.orc_tree_square_norm.restype = ctypes.c_double
@@ -130,6 +131,14 @@ class OracleTree:
         alphas = np.ascontiguousarray(alphas, dtype=np.float64)
         centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
         rc = Oracle.lib().orc_tree_project_gaussians(self.h, prec, len(coefs), _ptr(coefs), _ptr(alphas), _ptr(centers))
+        if rc != 0:
+            raise RuntimeError(Oracle.error())
+
+    def build_grid_gaussians(self, coefs, alphas, centers):
+        coefs = np.ascontiguousarray(coefs, dtype=np.float64)
+        alphas = np.ascontiguousarray(alphas, dtype=np.float64)
+        centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        rc = Oracle.lib().orc_tree_build_grid_gaussians(self.h, len(coefs), _ptr(coefs), _ptr(alphas), _ptr(centers))
         if rc != 0:
             raise RuntimeError(Oracle.error())
 
