@@ -131,6 +131,7 @@ def main():
         "config": {"workload": args.workload, "order": order, "prec": w["prec"], "box": [-w["box"], w["box"]],
                    "n_gaussians": len(src), "operator": "Poisson" if w["mu"] is None else "Helmholtz mu=%g" % w["mu"],
                    "separation_rank": P.size(), "n_applied": int(n_applied),
+                   "n_candidate_pairs": int(stats[-1]["n_pairs"]), "n_generated_nodes": int(stats[-1]["n_generated"]),
                    "n_output_nodes": int(stats[-1]["n_output_nodes"]), "levels": int(stats[-1]["n_levels"]),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "mfma", "kernel": "k_band_apply", "achieved": round(band_achieved, 4),

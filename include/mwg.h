@@ -81,6 +81,7 @@ typedef struct {
     const double *mats;    /* matrices (kp1*kp1 each, column-major: O(i_f, j_g)) */
     const double *mat_norm;/* one norm per matrix */
     const double *pair_bound; /* [n_depth][2*wmax+1] */
+    const double *pair_bound_w; /* [n_depth][2*wmax+1], wavelet components only */
     const int *band_size_max; /* [n_depth][M] row maximum of band_size */
     const double *term_norm;  /* [matrix/4] max component norm per (depth, term, dl) */
 } mwg_band_tables;

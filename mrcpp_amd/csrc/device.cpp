@@ -40,6 +40,7 @@ detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
     put_d(d->mats, t.mats);
     put_d(d->matNorm, t.matNorm);
     put_d(d->pairBound, t.pairBound);
+    put_d(d->pairBoundW, t.pairBoundW);
     put_d(d->termNorm, t.termNorm);
     put_i(d->bandSizeMax, t.bandSizeMax);
     mwg_band_tables &v = d->view;
@@ -54,6 +55,7 @@ detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
     v.mats = d->mats.get();
     v.mat_norm = d->matNorm.get();
     v.pair_bound = d->pairBound.get();
+    v.pair_bound_w = d->pairBoundW.get();
     v.band_size_max = d->bandSizeMax.get();
     v.term_norm = d->termNorm.get();
     return d;

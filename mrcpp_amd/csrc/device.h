@@ -70,7 +70,7 @@ namespace detail {
 struct DeviceBandTables {
     device::DeviceArray<int> bandMax, widths, bandSize;
     device::DeviceArray<long long> matStart;
-    device::DeviceArray<double> mats, matNorm, pairBound, termNorm;
+    device::DeviceArray<double> mats, matNorm, pairBound, pairBoundW, termNorm;
     device::DeviceArray<int> bandSizeMax;
     mwg_band_tables view{};
 };

@@ -115,16 +115,27 @@ __device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f,
     const int y = s[1] + (p / n[0]) % n[1];
     const int z = s[2] + p / (n[0] * n[1]);
     const double *pb = op.pair_bound + (size_t)d * (2 * op.wmax + 1) + op.wmax;
-    const double a = pb[x - gl[0]] * pb[y - gl[1]] * pb[z - gl[2]];
+    double a = pb[x - gl[0]] * pb[y - gl[1]] * pb[z - gl[2]];
+    if (d > 0) {
+        // below the root scale every tuple has a wavelet component (C, B or A)
+        // in at least one direction
+        const double *pw = op.pair_bound_w + (size_t)d * (2 * op.wmax + 1) + op.wmax;
+        const double mx = pb[x - gl[0]], my = pb[y - gl[1]], mz = pb[z - gl[2]];
+        a = fmax(pw[x - gl[0]] * my * mz, fmax(mx * pw[y - gl[1]] * mz, mx * my * pw[z - gl[2]]));
+    }
     // a zero bound cannot pass a non-negative threshold: skip the lookup
     if (a == 0.0 && lvl.g_thr >= 0.0) return false;
     int id = f_lookup(f, d, x, y, z);
     double maxf;
+    double fnv[8]; // per-component norms of the band node (or their bound)
     int code;
     if (id >= 0) {
         const double *nr = f.norms + (size_t)id * 9;
         maxf = 0.0;
-        for (int t = 0; t < 8; t++) maxf = fmax(maxf, nr[t]);
+        for (int t = 0; t < 8; t++) {
+            fnv[t] = nr[t];
+            maxf = fmax(maxf, nr[t]);
+        }
         code = id;
     } else {
         int dd = d, xx = x, yy = y, zz = z, anc = -1;
@@ -137,24 +148,50 @@ __device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f,
         }
         if (anc < 0) return false; // outside the world (not reachable for clamped boxes)
         const double *nr = f.norms + (size_t)anc * 9;
+        // a generated node holds a scaling block only, whose norm is bounded
+        // by its nearest existing ancestor's (orthogonal reconstruction)
         maxf = f.is_gen[anc] ? nr[0] : sqrt(nr[8]);
+        fnv[0] = maxf;
+        for (int t = 1; t < 8; t++) fnv[t] = 0.0;
         code = -(anc + 1);
     }
     if (!(a * maxf * kBoundSlack > lvl.g_thr)) return false;
-    // per-term bound: max band size x max component norm per direction
+    // Per-term bound, resolved per f component ft: the direction-d factor of
+    // a tuple is the norm of component 2*gbit_d + ftbit_d, so over all gt it is
+    // at most max(n_d(ftbit_d), n_d(2+ftbit_d)); at depth > 0 the (ft,gt) =
+    // (0,0) tuple is skipped, so some direction carries the gt bit.
     const int M = op.M;
-    const int dx = x - gl[0], dy = y - gl[1], dz = z - gl[2];
-    const int maxdl = max(abs(dx), max(abs(dy), abs(dz)));
+    const int dl3[3] = {x - gl[0], y - gl[1], z - gl[2]};
+    const int maxdl = max(abs(dl3[0]), max(abs(dl3[1]), abs(dl3[2])));
     const int *wd = op.widths + (size_t)d * M * 5;
     const long long *ms = op.mat_start + (size_t)d * M;
     const int *bsm = op.band_size_max + (size_t)d * M;
     bool any = false;
     for (int i = 0; i < M && !any; i++) {
-        const int w = wd[i * 5 + 4];
+        const int *wi = wd + i * 5;
+        const int w = wi[4];
         if (maxdl > w) continue;
-        const double *tn = op.term_norm + ms[i] / 4 + w;
-        const double t = (double)bsm[i] * tn[dx] * tn[dy] * tn[dz] * maxf;
-        any = t * kBoundSlack > lvl.g_thr;
+        const double *mn = op.mat_norm + ms[i];
+        double n[3][4];
+        for (int dim = 0; dim < 3; dim++)
+            for (int c = 0; c < 4; c++) n[dim][c] = (abs(dl3[dim]) <= wi[c]) ? mn[(dl3[dim] + w) * 4 + c] : 0.0;
+        const double bs = (double)bsm[i];
+        for (int ft = 0; ft < 8 && !any; ft++) {
+            if (fnv[ft] < kMachineZero) continue;
+            double g0[3], g1[3], gm[3];
+            for (int dim = 0; dim < 3; dim++) {
+                const int fb = (ft >> dim) & 1;
+                g0[dim] = n[dim][fb];
+                g1[dim] = n[dim][2 + fb];
+                gm[dim] = fmax(g0[dim], g1[dim]);
+            }
+            double prod;
+            if (d > 0 && ft == 0)
+                prod = fmax(g1[0] * gm[1] * gm[2], fmax(gm[0] * g1[1] * gm[2], gm[0] * gm[1] * g1[2]));
+            else
+                prod = gm[0] * gm[1] * gm[2];
+            any = bs * prod * fnv[ft] * kBoundSlack > lvl.g_thr;
+        }
     }
     if (!any) return false;
     *out_f = code;
@@ -506,19 +543,240 @@ __global__ __launch_bounds__(kThreads) void k_band_apply(mwg_level lvl, mwg_ftre
 }
 
 // ---------------------------------------------------------------------------
+// Band sum, one wave64 per candidate pair, for (k+1)^2 <= 64.
+//
+// Lane t screens term c0 + t for every (ft, gt) and keeps one gt mask per ft;
+// groups (ft, term) are then taken in (term chunk, ft, term) order through a
+// wave ballot, so the whole wave works on one group at a time with uniform
+// control flow and wave-uniform operator blocks.  In every sweep a lane owns
+// one line of the block: it reads N1 values, does N1^2 FMAs and writes N1
+// values.  LDS arrays are [contracted index][line] with a row pitch of 68
+// doubles (64 + 4), so the writes of one sweep (stride 68 across one lane
+// index, 1 across the other) and the reads of the next (stride 1) hit 32
+// distinct double banks in each half wave.
+// ---------------------------------------------------------------------------
+template <int N1>
+__global__ __launch_bounds__(64) void k_band_apply_wave(mwg_level lvl, mwg_ftree f, mwg_band_tables op, const int *pair_g, const int *pair_f,
+                                                       const int *pair_l, int *pair_work, double *partial, unsigned long long *n_applied) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    constexpr int P = 68;
+    static_assert(N2 <= 64, "one line per lane");
+    __shared__ double sF[N1 * P];
+    __shared__ double sT1[N1 * P];
+    __shared__ double sT2[N1 * P];
+
+    const int lane = threadIdx.x;
+    const int p = blockIdx.x;
+    const int g = pair_g[p];
+    const int fid = pair_f[p];
+    const int d = lvl.o_depth;
+    const int M = op.M;
+    int dl[3], maxdl = 0;
+    for (int i = 0; i < 3; i++) {
+        dl[i] = pair_l[3 * p + i] - lvl.l[3 * g + i];
+        maxdl = max(maxdl, abs(dl[i]));
+    }
+    double fn[8];
+    for (int t = 0; t < 8; t++) fn[t] = f.norms[(size_t)fid * 9 + t];
+    const double *fsrc = f.coef + f.coef_off[fid];
+    const int *wd = op.widths + (size_t)d * M * 5;
+    const int *bsz = op.band_size + (size_t)d * M * 64;
+    const long long *ms = op.mat_start + (size_t)d * M;
+
+    double acc[8][N1];
+#pragma unroll
+    for (int t = 0; t < 8; t++)
+#pragma unroll
+        for (int j = 0; j < N1; j++) acc[t][j] = 0.0;
+
+    unsigned int napp = 0;
+    bool any_work = false;
+    int curft = -1;
+    for (int c0 = 0; c0 < M; c0 += 64) {
+        // ---- exact screening of term c0 + lane (applyOperComp / applyOperator) ----
+        const int i = c0 + lane;
+        unsigned int m8[8];
+#pragma unroll
+        for (int ft = 0; ft < 8; ft++) m8[ft] = 0;
+        if (i < M && maxdl <= wd[i * 5 + 4]) {
+            const int *w = wd + i * 5;
+            const int w4 = w[4];
+            double nrm[3][4];
+#pragma unroll
+            for (int dim = 0; dim < 3; dim++)
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    nrm[dim][c] = (abs(dl[dim]) <= w[c]) ? op.mat_norm[ms[i] + (long long)(dl[dim] + w4) * 4 + c] : -1.0;
+#pragma unroll
+            for (int ft = 0; ft < 8; ft++) {
+                if (fn[ft] < kMachineZero) continue;
+                unsigned int m = 0;
+#pragma unroll
+                for (int gt = 0; gt < 8; gt++) {
+                    if (!(d == 0 || gt != 0 || ft != 0)) continue;
+                    const double nx = nrm[0][2 * (gt & 1) + (ft & 1)];
+                    const double ny = nrm[1][2 * ((gt >> 1) & 1) + ((ft >> 1) & 1)];
+                    const double nz = nrm[2][2 * ((gt >> 2) & 1) + ((ft >> 2) & 1)];
+                    if (nx < 0.0 || ny < 0.0 || nz < 0.0) continue;
+                    double onorm = 1.0;
+                    onorm *= nx;
+                    onorm *= ny;
+                    onorm *= nz;
+                    const double fthr = bsz[i * 64 + gt * 8 + ft] * fn[ft];
+                    if (onorm * fthr > lvl.g_thr) m |= 1u << gt;
+                }
+                m8[ft] = m;
+                napp += __popc(m);
+            }
+        }
+        // gt masks packed one byte per ft, so no register array is indexed at run time
+        const unsigned int mlo = m8[0] | (m8[1] << 8) | (m8[2] << 16) | (m8[3] << 24);
+        const unsigned int mhi = m8[4] | (m8[5] << 8) | (m8[6] << 16) | (m8[7] << 24);
+        // ---- groups of this term chunk, ft-major ----
+        for (int ft = 0; ft < 8; ft++) {
+            const unsigned int mine = ((ft < 4 ? mlo : mhi) >> (8 * (ft & 3))) & 0xffu;
+            unsigned long long bal = __ballot(mine != 0);
+            while (bal) {
+                const int src = __ffsll((long long)bal) - 1;
+                bal &= bal - 1;
+                const int ti = __builtin_amdgcn_readfirstlane(c0 + src);
+                const unsigned int gm = (unsigned int)__builtin_amdgcn_readlane((int)mine, src);
+                any_work = true;
+                if (ft != curft) {
+                    __syncthreads();
+                    for (int e = lane; e < N3; e += 64) sF[(e % N1) * P + e / N1] = fsrc[(size_t)ft * N3 + e];
+                    curft = ft;
+                    __syncthreads();
+                }
+                const int w4 = __builtin_amdgcn_readfirstlane(wd[ti * 5 + 4]);
+                const long long mbase = ms[ti];
+                unsigned int xneed = 0, yneed = 0;
+#pragma unroll
+                for (int gt = 0; gt < 8; gt++)
+                    if (gm >> gt & 1) {
+                        xneed |= 1u << (gt & 1);
+                        yneed |= 1u << (gt & 3);
+                    }
+#pragma unroll
+                for (int gx = 0; gx < 2; gx++) {
+                    if (!(xneed >> gx & 1)) continue;
+                    // x sweep: T1[i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox(i0, j0)
+                    const double *Ox = op.mats + (size_t)(mbase + (long long)(dl[0] + w4) * 4 + 2 * gx + (ft & 1)) * N2;
+                    if (lane < N2) {
+                        double x[N1];
+#pragma unroll
+                        for (int a = 0; a < N1; a++) x[a] = sF[a * P + lane];
+                        const int i1 = lane % N1, i2 = lane / N1;
+#pragma unroll
+                        for (int j = 0; j < N1; j++) {
+                            double s = 0.0;
+#pragma unroll
+                            for (int a = 0; a < N1; a++) s += x[a] * Ox[a + N1 * j];
+                            sT1[i1 * P + i2 + N1 * j] = s;
+                        }
+                    }
+                    __syncthreads();
+#pragma unroll
+                    for (int gy = 0; gy < 2; gy++) {
+                        if (!(yneed >> (gx + 2 * gy) & 1)) continue;
+                        // y sweep: T2[i2][j0 + N1 j1] = sum_i1 T1[i1][i2 + N1 j0] Oy(i1, j1)
+                        const double *Oy = op.mats + (size_t)(mbase + (long long)(dl[1] + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
+                        if (lane < N2) {
+                            double y[N1];
+#pragma unroll
+                            for (int a = 0; a < N1; a++) y[a] = sT1[a * P + lane];
+                            const int i2 = lane % N1, j0 = lane / N1;
+#pragma unroll
+                            for (int j = 0; j < N1; j++) {
+                                double s = 0.0;
+#pragma unroll
+                                for (int a = 0; a < N1; a++) s += y[a] * Oy[a + N1 * j];
+                                sT2[i2 * P + j0 + N1 * j] = s;
+                            }
+                        }
+                        __syncthreads();
+                        // z sweep into the owned line (j0, j1): G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[i2][line] Oz(i2, j2)
+                        if (lane < N2) {
+                            double z[N1];
+#pragma unroll
+                            for (int a = 0; a < N1; a++) z[a] = sT2[a * P + lane];
+#pragma unroll
+                            for (int gz = 0; gz < 2; gz++) {
+                                const int gt = gx + 2 * gy + 4 * gz;
+                                if (!(gm >> gt & 1)) continue;
+                                const double *Oz = op.mats + (size_t)(mbase + (long long)(dl[2] + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
+#pragma unroll
+                                for (int j = 0; j < N1; j++) {
+                                    double s = 0.0;
+#pragma unroll
+                                    for (int a = 0; a < N1; a++) s += z[a] * Oz[a + N1 * j];
+                                    acc[gt][j] += s;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+                }
+            }
+        }
+    }
+    // applied-tuple count of this pair
+    for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
+    if (lane == 0) {
+        if (napp) atomicAdd(n_applied, (unsigned long long)napp);
+        pair_work[p] = any_work ? 1 : 0;
+    }
+    if (!any_work) return;
+    double *dst = partial + (size_t)p * 8 * N3;
+    if (lane < N2) {
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+#pragma unroll
+            for (int j = 0; j < N1; j++) dst[t * N3 + lane + N2 * j] = acc[t][j];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Per output node: sum the pair partials in pair order, store, norms.
 // ---------------------------------------------------------------------------
 template <int N1> __global__ __launch_bounds__(kThreads) void k_reduce(mwg_level lvl, mwg_pairs pr, const double *partial) {
     constexpr int N3 = N1 * N1 * N1;
+    constexpr int R = (8 * N3 + kThreads - 1) / kThreads;
+    constexpr int kChunk = 4 * kThreads;
     __shared__ double red[8 * (kThreads / 64)];
+    __shared__ int list[kChunk];
+    __shared__ int tmp[kThreads];
     const int g = blockIdx.x;
     const int first = pr.g_first[g], n = pr.g_npairs[g];
     double *dst = lvl.g_coef + lvl.g_off[g];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
-        double s = 0.0;
-        for (int k = 0; k < n; k++)
-            if (pr.pair_work[first + k]) s += partial[(size_t)(first + k) * 8 * N3 + o];
-        dst[o] = s;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) acc[r] = 0.0;
+    // pairs with work, in pair order, a chunk at a time
+    for (int c0 = 0; c0 < n; c0 += kChunk) {
+        const int m = min(kChunk, n - c0);
+        const int q0 = threadIdx.x * 4, q1 = min(m, q0 + 4);
+        int cnt = 0;
+        for (int q = q0; q < q1; q++) cnt += pr.pair_work[first + c0 + q] != 0;
+        int excl;
+        const int tot = block_exclusive_scan(cnt, tmp, excl);
+        for (int q = q0; q < q1; q++)
+            if (pr.pair_work[first + c0 + q]) list[excl++] = first + c0 + q;
+        __syncthreads();
+        for (int k = 0; k < tot; k++) {
+            const double *src = partial + (size_t)list[k] * 8 * N3;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const int o = threadIdx.x + r * kThreads;
+                if (o < 8 * N3) acc[r] += src[o];
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int o = threadIdx.x + r * kThreads;
+        if (o < 8 * N3) dst[o] = acc[r];
     }
     __syncthreads();
     block_norms<N1>(dst, lvl.g_norms + (size_t)g * 9, red);
@@ -592,6 +850,12 @@ template <int N1> struct BandApplyL {
     static mwg_status run(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, int npairs,
                           double *partial, unsigned long long *napp) {
         ProfileScope prof_(MWG_KERNEL_BAND);
+        if constexpr (N1 * N1 <= 64) {
+            hipLaunchKernelGGL(k_band_apply_wave<N1>, dim3(npairs), dim3(64), 0, 0, *lvl, *f, *op, pr->pair_g, pr->pair_f, pr->pair_l,
+                               pr->pair_work, partial, napp);
+            MWG_LAUNCHED();
+            return MWG_OK;
+        }
         hipLaunchKernelGGL(k_band_apply<N1>, dim3(npairs), dim3(kThreads), 0, 0, *lvl, *f, *op, pr->pair_g, pr->pair_f, pr->pair_l, pr->pair_work, partial,
                            napp);
         MWG_LAUNCHED();

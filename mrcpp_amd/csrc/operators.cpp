@@ -252,6 +252,7 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
     T->bandSize.assign((size_t)ND * M * 64, 0);
     T->matStart.assign((size_t)ND * M, -1);
     T->pairBound.assign((size_t)ND * (2 * W + 1), 0.0);
+    T->pairBoundW.assign((size_t)ND * (2 * W + 1), 0.0);
     T->bandSizeMax.assign((size_t)ND * M, 0);
     long long nmat = 0;
     for (int d = 0; d < ND; d++)
@@ -280,10 +281,13 @@ const detail::BandTables &ConvolutionOperator<3>::bandTables(double prec) {
             const double cb = std::cbrt((double)bsmax);
             for (int dl = -w; dl <= w; dl++) {
                 const int u = t.node(d, dl);
-                double mx = 0.0;
+                double mx = 0.0, mw = 0.0;
                 for (int c = 0; c < 4; c++) mx = std::max(mx, t.tree->compNorm(u, c));
+                for (int c = 1; c < 4; c++) mw = std::max(mw, t.tree->compNorm(u, c));
                 double &pb = T->pairBound[(size_t)d * (2 * W + 1) + dl + W];
                 pb = std::max(pb, cb * mx);
+                double &pw = T->pairBoundW[(size_t)d * (2 * W + 1) + dl + W];
+                pw = std::max(pw, cb * mw);
             }
         }
     T->mats.resize((size_t)nmat * n2);

@@ -49,7 +49,8 @@ struct BandTables {
     std::vector<long long> matStart; // [nDepth][M] index of matrix (dl=-w, comp 0), -1 if none
     std::vector<double> mats;        // matrices, column-major (k+1)^2 each
     std::vector<double> matNorm;     // per matrix
-    std::vector<double> pairBound;   // [nDepth][2*Wmax+1]
+    std::vector<double> pairBound;   // [nDepth][2*Wmax+1] max over terms of cbrt(bs_max) x max component norm
+    std::vector<double> pairBoundW;  // same over the wavelet components (C, B, A) only
     std::vector<int> bandSizeMax;    // [nDepth][M] row maximum of bandSize
     std::vector<double> termNorm;    // [matrix / 4]: max over the 4 components
 };

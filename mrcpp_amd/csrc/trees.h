@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <memory>
@@ -300,7 +301,11 @@ private:
     int add_node(int n, const Index &l, int parent, bool gen) {
         int u = size();
         if ((u >> kChunkBits) >= (int)chunks_.size()) {
-            chunks_.emplace_back(new double[(size_t)(1 << kChunkBits) * ncoef_]);
+            // zero pages are mapped lazily: trees whose coefficients live on
+            // the device never touch this memory
+            void *p = std::calloc((size_t)(1 << kChunkBits) * ncoef_, sizeof(double));
+            if (!p) throw Error("host allocation failed");
+            chunks_.emplace_back(static_cast<double *>(p));
         }
         n_.push_back(n);
         l_.push_back(l);
@@ -310,7 +315,6 @@ private:
         has_coefs_.push_back(0);
         for (int t = 0; t < TD; t++) norm_.push_back(-1.0);
         sq_.push_back(-1.0);
-        std::memset(coefs(u), 0, sizeof(double) * ncoef_);
         index_[{n, l}] = u;
         if (!gen) {
             int dep = n - rootScale();
@@ -330,7 +334,10 @@ private:
     std::vector<int> parent_, child0_;
     std::vector<uint8_t> gen_, has_coefs_;
     std::vector<double> norm_, sq_;
-    std::vector<std::unique_ptr<double[]>> chunks_;
+    struct FreeDeleter {
+        void operator()(double *p) const { std::free(p); }
+    };
+    std::vector<std::unique_ptr<double, FreeDeleter>> chunks_;
     std::unordered_map<std::pair<int, Index>, int, KeyHash<D>> index_;
     std::vector<int> nodes_at_depth_;
 };
