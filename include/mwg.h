@@ -57,7 +57,8 @@ enum {
     MWG_KERNEL_RECONSTRUCT = 4,
     MWG_KERNEL_COMPRESS = 5,
     MWG_KERNEL_NORMS = 6,
-    MWG_KERNEL_KINDS = 7
+    MWG_KERNEL_GROUPS = 7, /* exact group screen and work-unit construction */
+    MWG_KERNEL_KINDS = 8
 };
 mwg_status mwg_profile(int enable);
 mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds);
@@ -112,6 +113,7 @@ typedef struct {
     int scale;
     int o_depth;         /* scale - operator root scale */
     double g_thr;        /* screening threshold (<= 0 disables screening) */
+    int band_w;          /* band half-width at this depth: band_max[o_depth] of the tables */
     const int *l;        /* [count][3] translations */
     double *g_coef;
     const long long *g_off; /* [count] */
@@ -130,7 +132,6 @@ typedef struct {
     int *pair_l;         /* [capacity][3] band node translation */
     int *g_first;        /* [level count] first pair of each output node */
     int *g_npairs;       /* [level count] */
-    int *pair_work;      /* [capacity] set by mwg_band_apply: nonzero if the pair applied any tuple */
 } mwg_pairs;
 
 /* Screen the band of every output node of a level and emit candidate pairs
@@ -138,7 +139,11 @@ typedef struct {
  * src/treebuilders/ConvolutionCalculator.cpp:142-222).  Two upper bounds of
  * the norm test of applyOperator (:297-329) discard band nodes: a product of
  * per-direction maxima over all terms, then a per-term bound over all
- * components; neither can reject a tuple the exact test would apply. */
+ * components; neither can reject a tuple the exact test would apply.  The
+ * pairs of one output node are contiguous and in band order (x fastest);
+ * *count receives the number of pairs (pairs beyond capacity are counted but
+ * not written: grow the arrays and screen again), or -1 when lvl->band_w is
+ * not the tables' band width. */
 mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out);
 
 /* Generate scaling-only band nodes below existing parents: node ids[i] gets
@@ -149,18 +154,16 @@ mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_b
 mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
                               int count);
 
-/* Exact screening and the 3-sweep band sum for every candidate pair; writes
- * one partial 8-block result per pair and adds the number of applied
- * (f, ft, gt, term) tuples to *n_applied (ConvolutionCalculator::calcNode,
- * applyOperComp, applyOperator and tensorApplyOperComp,
- * src/treebuilders/ConvolutionCalculator.cpp:224-382). precision: 0 = FP64. */
+/* Exact screening and the 3-sweep band sum of every candidate pair, then the
+ * output coefficients of the level and their component norms
+ * (ConvolutionCalculator::calcNode, applyOperComp, applyOperator and
+ * tensorApplyOperComp, src/treebuilders/ConvolutionCalculator.cpp:224-382;
+ * MWNode::calcNorms, src/trees/MWNode.cpp:609-655).  Every pair_f must be a
+ * node id (missing band nodes generated first).  Adds the number of applied
+ * (f, ft, gt, term) tuples to *n_applied.  Summation order is fixed by the
+ * pair order, so results are reproducible.  precision: 0 = FP64. */
 mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs,
-                          int npairs, double *partial, unsigned long long *n_applied, int precision);
-
-/* Sum the partial results of every output node in pair order, store the node
- * coefficients and its component norms (MWNode::calcNorms,
- * src/trees/MWNode.cpp:609-655). */
-mwg_status mwg_reduce_level(const mwg_level *lvl, const mwg_pairs *pairs, const double *partial);
+                          int npairs, unsigned long long *n_applied, int precision);
 
 /* Add each parent's reconstructed s/d into the scaling block of its 8 children
  * (MWTree::mwTransform(TopDown, false) -> tree_utils::mw_transform,

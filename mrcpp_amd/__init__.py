@@ -66,7 +66,7 @@ SIGNATURES = {
     "mwg_release_pool": (ctypes.c_int, []),
 }
 
-KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms")
+KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups")
 
 
 class MRCPPError(RuntimeError):

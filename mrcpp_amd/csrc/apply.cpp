@@ -22,9 +22,9 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> l, pairG, pairF, pairL, pairWork, gFirst, gCount, count, genIds, genParent, genChild;
+    device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild;
     device::DeviceArray<long long> gOff, pOff, cOff;
-    device::DeviceArray<double> gNorms, partial;
+    device::DeviceArray<double> gNorms;
     device::DeviceArray<unsigned long long> nApplied;
 };
 
@@ -33,8 +33,9 @@ Scratch &scratch() {
     return s;
 }
 
-// Partial-result budget per batch of output nodes (bytes).
-constexpr double kPartialBudget = 16.0e9;
+// Device bytes per batch of output nodes for the screen candidates and the
+// per-pair group lists.
+constexpr double kBatchBudget = 16.0e9;
 
 struct LevelContext {
     const MultiResolutionAnalysis<3> &mra;
@@ -152,6 +153,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     lvl.scale = scale;
     lvl.o_depth = o_depth;
     lvl.g_thr = gthr;
+    lvl.band_w = empty_band ? 0 : c.ht.bandMax[o_depth];
     lvl.l = S.l.get();
     lvl.g_coef = c.out.deviceCoefs().get();
     lvl.g_off = S.gOff.get();
@@ -160,26 +162,29 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     mwg_ftree fv = c.fm.view(c.mra);
     mwg_pairs pr{};
     int npairs = 0;
+    size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
+    auto bind_pairs = [&]() {
+        S.pairG.reserve(cap);
+        S.pairF.reserve(cap);
+        S.pairL.reserve(3 * cap);
+        pr.capacity = (int)std::min<size_t>(S.pairF.capacity(), S.pairL.capacity() / 3);
+        pr.count = S.count.get();
+        pr.pair_g = S.pairG.get();
+        pr.pair_f = S.pairF.get();
+        pr.pair_l = S.pairL.get();
+        pr.g_first = S.gFirst.get();
+        pr.g_npairs = S.gCount.get();
+    };
+    bind_pairs();
     if (!empty_band) {
         auto t0 = clk::now();
-        size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
         for (;;) {
-            S.pairG.reserve(cap);
-            S.pairF.reserve(cap);
-            S.pairL.reserve(3 * cap);
-            S.pairWork.reserve(cap);
-            pr.capacity = (int)std::min<size_t>(S.pairF.capacity(), S.pairL.capacity() / 3);
-            pr.count = S.count.get();
-            pr.pair_g = S.pairG.get();
-            pr.pair_f = S.pairF.get();
-            pr.pair_l = S.pairL.get();
-            pr.g_first = S.gFirst.get();
-            pr.g_npairs = S.gCount.get();
-            pr.pair_work = S.pairWork.get();
             device::check(mwg_pair_screen(&lvl, &fv, &c.bt, &pr), "pair screen");
             S.count.download(&npairs, 1);
+            if (npairs < 0) throw Error("pair screen: band width does not match the operator tables");
             if (npairs <= pr.capacity) break;
             cap = (size_t)npairs + npairs / 4;
+            bind_pairs();
         }
         g_stats.tScreen += since(t0);
         g_stats.nPairs += npairs;
@@ -197,24 +202,13 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
             fv = c.fm.view(c.mra);
         }
         g_stats.tGenerate += since(t0);
-        t0 = clk::now();
-        S.partial.reserve((size_t)npairs * c.ncoef);
-        device::check(mwg_band_apply(&lvl, &fv, &c.bt, &pr, npairs, S.partial.get(), S.nApplied.get(), 0), "band apply");
-        device::check(mwg_synchronize(), "band apply");
-        g_stats.tBand += since(t0);
-    } else {
-        // no band node survives: every output node is zero
-        std::vector<int> z(L, 0);
-        S.gCount.upload(z.data(), L);
-        S.gFirst.upload(z.data(), L);
-        pr.g_first = S.gFirst.get();
-        pr.g_npairs = S.gCount.get();
     }
     auto t0 = clk::now();
-    device::check(mwg_reduce_level(&lvl, &pr, S.partial.get()), "reduce level");
+    // band sums, output coefficients and norms (zero when no pair survives)
+    device::check(mwg_band_apply(&lvl, &fv, &c.bt, &pr, npairs, S.nApplied.get(), 0), "band apply");
     std::vector<double> nrm(9 * (size_t)L);
     S.gNorms.download(nrm.data(), nrm.size());
-    g_stats.tReduce += since(t0);
+    g_stats.tBand += since(t0);
     for (int i = 0; i < L; i++) {
         double *cn = g.norms(nodes[i]);
         for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)i + t];
@@ -223,12 +217,16 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     }
 }
 
-// Estimated candidate pairs per output node, to size batches.
+// One scale of the work vector, in batches that bound the device scratch.
 void run_level(LevelContext &c, const std::vector<int> &nodes, int scale, double gthr) {
-    // batches sized so that the partial results fit the budget with a
-    // generous per-node pair estimate; run_batch grows buffers as needed
-    const size_t per_node = 512;
-    size_t batch = std::max<size_t>(1, (size_t)(kPartialBudget / (8.0 * c.ncoef * per_node)));
+    // screen candidates take 8 bytes per band position; group lists 32 M
+    // bytes per pair, with a generous per-node pair estimate (buffers grow
+    // when a batch needs more)
+    const int o_depth = scale - c.oper.getOperatorRoot();
+    const int W = (o_depth >= 0 && o_depth < c.ht.nDepth) ? std::max(c.ht.bandMax[o_depth], 0) : 0;
+    const double box = std::pow(2.0 * W + 1.0, 3.0);
+    const double per_node = 8.0 * (box + 256.0) + 128.0 * (32.0 * c.ht.M + 24.0);
+    size_t batch = std::max<size_t>(1, (size_t)(kBatchBudget / per_node));
     for (size_t i = 0; i < nodes.size(); i += batch) {
         std::vector<int> part(nodes.begin() + i, nodes.begin() + std::min(nodes.size(), i + batch));
         run_batch(c, part, scale, gthr);

@@ -1,0 +1,215 @@
+// Device helpers shared by the gfx950 kernels: error plumbing of the C ABI,
+// per-kernel timing, key lookup, block scans and multiwavelet filter passes.
+//
+// Node layout: 8 blocks (block t = wavelet bits t, x -> bit 0), element
+// i0 + N1 i1 + N1^2 i2 with N1 = k+1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/mwg.h"
+#include "keys.h"
+
+namespace mwgdev {
+
+constexpr int kThreads = 256;
+constexpr int kMaxTerms = 512;
+constexpr double kMachineZero = 1.0e-14;
+// relative slack of the screening bounds against rounding
+constexpr double kBoundSlack = 1.0 + 1.0e-9;
+
+mwg_status fail(mwg_status s, const std::string &msg);
+mwg_status need_init();
+
+// Per-kernel timing with HIP events on the launch stream (mwg_profile).
+struct ProfileScope {
+    int kind;
+    hipEvent_t start{}, stop{};
+    explicit ProfileScope(int k);
+    ~ProfileScope();
+};
+
+// Library-owned device scratch, grown on demand from the mwg_malloc pool
+// (contents are not kept when it grows).
+struct Buffer {
+    void *p = nullptr;
+    size_t bytes = 0;
+    mwg_status ensure(size_t need);
+    template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+
+// Compute units of the current device.
+int device_cus();
+
+// Exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in and
+// out may alias.  Enqueued on the null stream.
+mwg_status scan_exclusive(const int *in, int *out, int n);
+mwg_status scan_exclusive(const long long *in, long long *out, int n);
+
+#define MWG_HIP(call)                                                                                                   \
+    do {                                                                                                                \
+        hipError_t e_ = (call);                                                                                         \
+        if (e_ != hipSuccess) return mwgdev::fail(MWG_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+#define MWG_LAUNCHED()                                                                                                  \
+    do {                                                                                                                \
+        hipError_t e_ = hipGetLastError();                                                                              \
+        if (e_ != hipSuccess) return mwgdev::fail(MWG_ERR_DEVICE, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    } while (0)
+
+// Read-only view of device memory that no kernel writes while the reader
+// runs: loads through it at wave-uniform addresses become scalar loads.
+template <typename T> using const_as4 = const __attribute__((address_space(4))) T;
+template <typename T> __device__ inline const_as4<T> *ro(const T *p) { return (const_as4<T> *)p; }
+
+__device__ inline int f_lookup(const mwg_ftree &f, int depth, int x, int y, int z) {
+    const int ox = x - f.corner[0] * (1 << depth);
+    const int oy = y - f.corner[1] * (1 << depth);
+    const int oz = z - f.corner[2] * (1 << depth);
+    const uint64_t key = mwgkey::pack(depth, ox, oy, oz);
+    const int log2cap = 31 - __clz(f.hash_cap);
+    const uint32_t mask = (uint32_t)f.hash_cap - 1;
+    uint32_t s = mwgkey::slot(key, log2cap);
+    for (;;) {
+        const unsigned long long k = f.hash_keys[s];
+        if (k == key) return f.hash_vals[s];
+        if (k == 0ull) return -1;
+        s = (s + 1) & mask;
+    }
+}
+
+// Block-wide exclusive scan of one value per thread (NT threads, a multiple
+// of 64): wave64 shuffles, then the wave totals through LDS (tmp holds NT/64
+// entries).  Returns the block total.
+template <typename T, int NT> __device__ inline T block_exclusive_scan(T v, T *tmp, T &excl) {
+    static_assert(NT % 64 == 0 && NT <= 1024, "whole waves");
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T incl = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T t = __shfl_up(incl, off);
+        if (lane >= off) incl += t;
+    }
+    if (lane == 63) tmp[wave] = incl;
+    __syncthreads();
+    T before = 0, total = 0;
+    for (int w = 0; w < NT / 64; w++) {
+        const T s = tmp[w];
+        if (w < wave) before += s;
+        total += s;
+    }
+    excl = before + incl - v;
+    __syncthreads();
+    return total;
+}
+
+// Kernel dispatch over the supported orders k+1 in {4, 6, 8, 10}.
+template <template <int> class Launcher, typename... Args> mwg_status dispatch(int kp1, Args... args) {
+    switch (kp1) {
+    case 4:
+        return Launcher<4>::run(args...);
+    case 6:
+        return Launcher<6>::run(args...);
+    case 8:
+        return Launcher<8>::run(args...);
+    case 10:
+        return Launcher<10>::run(args...);
+    default:
+        return fail(MWG_ERR_UNSUPPORTED, "unsupported order: k+1 = " + std::to_string(kp1));
+    }
+}
+
+// One pass of the separable 3D multiwavelet transform over 8 blocks in LDS.
+// dir_base 0: compression sub-filters, 4: reconstruction sub-filters (the
+// device filter array holds H0t,H1t,G0t,G1t then H0,G0,H1,G1).  out[gt] =
+// in[ft0]^T F(2 b + 0) + in[ft1]^T F(2 b + 1), ft0 < ft1 the two blocks that
+// agree with gt outside direction dim, b = bit dim of gt, in viewed
+// (N1 x N1^2) column-major (math_utils::apply_filter).
+template <int N1> __device__ inline void mw_pass(double *out, const double *in, const double *filt, int dir_base, int dim) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    const int bit = 1 << dim;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int gt = o / N3, e = o % N3;
+        const int r = e % N2, j = e / N2;
+        const int ft0 = gt & ~bit, ft1 = gt | bit;
+        const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
+        const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
+        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
+        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; i++) {
+            s0 += x0[i] * F0[i];
+            s1 += x1[i] * F1[i];
+        }
+        out[o] = s0 + s1;
+    }
+}
+
+// Last reconstruction pass accumulating into the children's scaling blocks,
+// each of the two contributions added separately.
+template <int N1>
+__device__ inline void mw_pass_add_children(double *coef, long long child_off, long long stride, const double *in, const double *filt) {
+    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+    const int dim = 2, bit = 4;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int gt = o / N3, e = o % N3;
+        const int r = e % N2, j = e / N2;
+        const int ft0 = gt & ~bit, ft1 = gt | bit;
+        const double *F0 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
+        const double *F1 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
+        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
+        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int i = 0; i < N1; i++) {
+            s0 += x0[i] * F0[i];
+            s1 += x1[i] * F1[i];
+        }
+        double *dst = coef + child_off + (long long)gt * stride + e;
+        double v = *dst;
+        v += s0;
+        v += s1;
+        *dst = v;
+    }
+}
+
+// Component norms of 8 blocks, written to nrm[0..8] (8 norms + square norm).
+template <int N1> __device__ inline void block_norms(const double *c, double *nrm, double *red) {
+    constexpr int N3 = N1 * N1 * N1;
+    double ss[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) ss[t] = 0.0;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const double v = c[o];
+        ss[o / N3] += v * v;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+        double v = ss[t];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+        ss[t] = v;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0)
+        for (int t = 0; t < 8; t++) red[wave * 8 + t] = ss[t];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int t = 0; t < 8; t++) {
+            double v = 0.0;
+            for (int w = 0; w < kThreads / 64; w++) v += red[w * 8 + t];
+            const double nv = sqrt(v);
+            nrm[t] = nv;
+            tot += nv * nv;
+        }
+        nrm[8] = tot;
+    }
+    __syncthreads();
+}
+
+} // namespace mwgdev

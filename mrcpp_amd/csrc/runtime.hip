@@ -1,0 +1,229 @@
+// Device runtime of the mwg_* C ABI: device selection, the pooled allocator,
+// transfers and per-kernel timing with HIP events.
+#include <map>
+#include <mutex>
+
+#include "device_common.h"
+
+namespace mwgdev {
+
+namespace {
+
+thread_local std::string g_err;
+bool g_ready = false;
+bool g_profile = false;
+
+struct TimedLaunch {
+    int kind;
+    hipEvent_t start, stop;
+};
+std::vector<TimedLaunch> &timed_launches() {
+    static std::vector<TimedLaunch> v;
+    return v;
+}
+
+// Device blocks cached by exact size for reuse (freed blocks are not returned
+// to the runtime until mwg_release_pool).
+std::mutex &pool_mutex() {
+    static std::mutex m;
+    return m;
+}
+std::map<size_t, std::vector<void *>> &free_blocks() {
+    static std::map<size_t, std::vector<void *>> m;
+    return m;
+}
+std::map<void *, size_t> &block_size() {
+    static std::map<void *, size_t> m;
+    return m;
+}
+
+} // namespace
+
+mwg_status fail(mwg_status s, const std::string &msg) {
+    g_err = msg;
+    return s;
+}
+
+mwg_status need_init() { return g_ready ? MWG_OK : fail(MWG_ERR_ARG, "mwg_init has not been called"); }
+
+ProfileScope::ProfileScope(int k)
+        : kind(k) {
+    if (!g_profile) return;
+    (void)hipEventCreate(&start);
+    (void)hipEventCreate(&stop);
+    (void)hipEventRecord(start, 0);
+}
+
+ProfileScope::~ProfileScope() {
+    if (!g_profile) return;
+    (void)hipEventRecord(stop, 0);
+    timed_launches().push_back({kind, start, stop});
+}
+
+mwg_status Buffer::ensure(size_t need) {
+    if (need <= bytes) return MWG_OK;
+    if (p) {
+        mwg_free(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    size_t b = need + need / 4;
+    if (mwg_malloc(&p, b) != MWG_OK) return MWG_ERR_NOMEM;
+    bytes = b;
+    return MWG_OK;
+}
+
+int device_cus() {
+    static int cus = 0;
+    if (!cus) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    }
+    return cus;
+}
+
+} // namespace mwgdev
+
+using namespace mwgdev;
+
+extern "C" {
+
+const char *mwg_last_error(void) { return g_err.c_str(); }
+
+int mwg_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+mwg_status mwg_init(int device) {
+    int n = 0;
+    MWG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(MWG_ERR_ARG, "no such device");
+    MWG_HIP(hipSetDevice(device));
+    g_ready = true;
+    return MWG_OK;
+}
+
+mwg_status mwg_malloc(void **ptr, size_t bytes) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!ptr) return fail(MWG_ERR_ARG, "null pointer");
+    if (!bytes) bytes = 1;
+    {
+        std::lock_guard<std::mutex> lk(pool_mutex());
+        auto &fl = free_blocks()[bytes];
+        if (!fl.empty()) {
+            *ptr = fl.back();
+            fl.pop_back();
+            block_size()[*ptr] = bytes;
+            return MWG_OK;
+        }
+    }
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        // return cached blocks to the runtime and retry once
+        mwg_release_pool();
+        if (hipMalloc(ptr, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(MWG_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed");
+        }
+    }
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    block_size()[*ptr] = bytes;
+    return MWG_OK;
+}
+
+mwg_status mwg_free(void *ptr) {
+    if (!ptr) return MWG_OK;
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    auto it = block_size().find(ptr);
+    if (it == block_size().end()) return fail(MWG_ERR_ARG, "mwg_free of an unknown pointer");
+    free_blocks()[it->second].push_back(ptr);
+    block_size().erase(it);
+    return MWG_OK;
+}
+
+mwg_status mwg_profile(int enable) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    MWG_HIP(hipDeviceSynchronize());
+    for (auto &t : timed_launches()) {
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    timed_launches().clear();
+    g_profile = enable != 0;
+    return MWG_OK;
+}
+
+mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!ms || !launches) return fail(MWG_ERR_ARG, "null argument");
+    MWG_HIP(hipDeviceSynchronize());
+    for (int k = 0; k < nkinds; k++) {
+        ms[k] = 0.0;
+        launches[k] = 0;
+    }
+    for (auto &t : timed_launches()) {
+        if (t.kind < 0 || t.kind >= nkinds) continue;
+        float e = 0.0f;
+        MWG_HIP(hipEventElapsedTime(&e, t.start, t.stop));
+        ms[t.kind] += e;
+        launches[t.kind]++;
+    }
+    return MWG_OK;
+}
+
+mwg_status mwg_release_pool(void) {
+    std::lock_guard<std::mutex> lk(pool_mutex());
+    for (auto &kv : free_blocks())
+        for (void *p : kv.second) MWG_HIP(hipFree(p));
+    free_blocks().clear();
+    return MWG_OK;
+}
+
+mwg_status mwg_upload(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MWG_OK;
+}
+
+mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return MWG_OK;
+}
+
+mwg_status mwg_copy(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    return MWG_OK;
+}
+
+mwg_status mwg_zero(void *dst, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipMemset(dst, 0, bytes));
+    return MWG_OK;
+}
+
+mwg_status mwg_synchronize(void) {
+    MWG_HIP(hipDeviceSynchronize());
+    return MWG_OK;
+}
+
+mwg_status mwg_filters_upload(int kp1, const double *host_filters, double **dev_filters) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!host_filters || !dev_filters || kp1 < 1) return fail(MWG_ERR_ARG, "bad filter upload arguments");
+    const size_t bytes = sizeof(double) * 8 * (size_t)kp1 * kp1;
+    void *p = nullptr;
+    if (mwg_malloc(&p, bytes) != MWG_OK) return MWG_ERR_NOMEM;
+    MWG_HIP(hipMemcpy(p, host_filters, bytes, hipMemcpyHostToDevice));
+    *dev_filters = static_cast<double *>(p);
+    return MWG_OK;
+}
+
+} // extern "C"

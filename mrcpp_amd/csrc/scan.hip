@@ -1,0 +1,79 @@
+// Device exclusive prefix sums used to place candidate pairs and work units
+// in a deterministic order: 1024 items per 256-thread block, block totals
+// scanned recursively, block offsets added back.
+#include "device_common.h"
+
+namespace mwgdev {
+
+namespace {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4 * kScanThreads;
+
+// out[i] = exclusive scan of in within the block's 1024 items; bsum[block] = block total.
+template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_block(const T *in, T *out, T *bsum, int n) {
+    __shared__ T tmp[kScanThreads / 64];
+    const int base = blockIdx.x * kScanItems + 4 * threadIdx.x;
+    T v[4], s = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = (base + i < n) ? in[base + i] : T(0);
+        s += v[i];
+    }
+    T excl;
+    const T total = block_exclusive_scan<T, kScanThreads>(s, tmp, excl);
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (base + i < n) out[base + i] = excl;
+        excl += v[i];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// out[i] += boff[block]; out[n] = boff[nblocks].
+template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_add(T *out, const T *boff, int n, int nblocks) {
+    const T add = boff[blockIdx.x];
+    const int base = blockIdx.x * kScanItems;
+    for (int i = threadIdx.x; i < kScanItems; i += kScanThreads)
+        if (base + i < n) out[base + i] += add;
+    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = boff[nblocks];
+}
+
+template <typename T> Buffer &scan_temp(int level) {
+    static Buffer b[8];
+    return b[level];
+}
+
+template <typename T> mwg_status scan_rec(const T *in, T *out, int n, int level) {
+    const int nb = (n + kScanItems - 1) / kScanItems;
+    if (nb <= 1) {
+        hipLaunchKernelGGL(k_scan_block<T>, dim3(1), dim3(kScanThreads), 0, 0, in, out, out + n, n);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+    if (level >= 8) return fail(MWG_ERR_ARG, "scan too long");
+    Buffer &tb = scan_temp<T>(level);
+    if (tb.ensure(sizeof(T) * ((size_t)nb + 1)) != MWG_OK) return MWG_ERR_NOMEM;
+    T *bs = tb.as<T>();
+    hipLaunchKernelGGL(k_scan_block<T>, dim3(nb), dim3(kScanThreads), 0, 0, in, out, bs, n);
+    MWG_LAUNCHED();
+    const mwg_status s = scan_rec<T>(bs, bs, nb, level + 1);
+    if (s != MWG_OK) return s;
+    hipLaunchKernelGGL(k_scan_add<T>, dim3(nb), dim3(kScanThreads), 0, 0, out, bs, n, nb);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+
+} // namespace
+
+mwg_status scan_exclusive(const int *in, int *out, int n) {
+    if (n < 0) return fail(MWG_ERR_ARG, "negative scan length");
+    return scan_rec<int>(in, out, n, 0);
+}
+
+mwg_status scan_exclusive(const long long *in, long long *out, int n) {
+    if (n < 0) return fail(MWG_ERR_ARG, "negative scan length");
+    return scan_rec<long long>(in, out, n, 0);
+}
+
+} // namespace mwgdev

@@ -1,0 +1,162 @@
+// Multiwavelet transforms on the device: generated band nodes, top-down
+// reconstruction, bottom-up compression and component norms.  One 256-thread
+// workgroup per node; the 8 blocks of a node are staged in LDS.
+#include "device_common.h"
+
+namespace mwgdev {
+
+namespace {
+
+// Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
+// + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double red[8 * (kThreads / 64)];
+    const int k = blockIdx.x;
+    const int pid = parent[k], id = ids[k], c = child[k];
+    const double *src = f.coef + f.coef_off[pid];
+    const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = (o < nblk * N3) ? src[o] : 0.0;
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 4, 1);
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 2);
+    __syncthreads();
+    double *dst = f.coef + f.coef_off[id];
+    double ss = 0.0;
+    for (int e = threadIdx.x; e < N3; e += kThreads) {
+        const double v = b[c * N3 + e];
+        dst[e] = v;
+        ss += v * v;
+    }
+    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) v += red[w];
+        const double n0 = sqrt(v);
+        double *nr = f.norms + (size_t)id * 9;
+        nr[0] = n0;
+        for (int t = 1; t < 8; t++) nr[t] = 0.0;
+        nr[8] = n0 * n0;
+    }
+}
+
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    const int k = blockIdx.x;
+    const double *src = coef + poff[k];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = src[o];
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 4, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 4, 1);
+    __syncthreads();
+    mw_pass_add_children<N1>(coef, coff[k], stride, a, filt);
+}
+
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_compress(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[8 * N3], b[8 * N3];
+    const int k = blockIdx.x;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+        const int t = o / N3, e = o % N3;
+        a[o] = coef[coff[k] + (long long)t * stride + e];
+    }
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 0, 0);
+    __syncthreads();
+    mw_pass<N1>(a, b, filt, 0, 1);
+    __syncthreads();
+    mw_pass<N1>(b, a, filt, 0, 2);
+    __syncthreads();
+    double *dst = coef + poff[k];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = b[o];
+}
+
+template <int N1> __global__ __launch_bounds__(kThreads) void k_node_norms(const double *coef, const long long *off, double *norms) {
+    __shared__ double red[8 * (kThreads / 64)];
+    const int k = blockIdx.x;
+    block_norms<N1>(coef + off[k], norms + (size_t)k * 9, red);
+}
+
+template <int N1> struct GenerateL {
+    static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count) {
+        ProfileScope prof_(MWG_KERNEL_GENERATE);
+        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct ReconL {
+    static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        ProfileScope prof_(MWG_KERNEL_RECONSTRUCT);
+        hipLaunchKernelGGL(k_reconstruct_add<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct CompressL {
+    static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
+        ProfileScope prof_(MWG_KERNEL_COMPRESS);
+        hipLaunchKernelGGL(k_compress<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+template <int N1> struct NormsL {
+    static mwg_status run(const double *coef, const long long *off, double *norms, int count) {
+        ProfileScope prof_(MWG_KERNEL_NORMS);
+        hipLaunchKernelGGL(k_node_norms<N1>, dim3(count), dim3(kThreads), 0, 0, coef, off, norms);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+
+} // namespace
+
+} // namespace mwgdev
+
+using namespace mwgdev;
+
+extern "C" {
+
+mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!f || !filters || !ids || !parent || !child_idx) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count);
+}
+
+mwg_status mwg_reconstruct_add(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,
+                               long long stride) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!filters || !coef || !parent_off || !child_off) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<ReconL>(kp1, filters, coef, parent_off, child_off, count, stride);
+}
+
+mwg_status mwg_compress(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,
+                        long long stride) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!filters || !coef || !parent_off || !child_off) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<CompressL>(kp1, filters, coef, parent_off, child_off, count, stride);
+}
+
+mwg_status mwg_node_norms(int kp1, const double *coef, const long long *off, double *norms, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!coef || !off || !norms) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<NormsL>(kp1, coef, off, norms, count);
+}
+
+} // extern "C"
