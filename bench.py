@@ -7,6 +7,11 @@ projected source tree and the operator with its device band tables) are
 resident in HBM before the timed region.  Prints one JSON line.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--workload gauss16_k7] [--no-cpu-baseline]
+    python bench.py --copy-probe     (three 256 MiB device copies: HBM counter calibration)
+
+roofline.traffic (HBM bytes per band-kernel launch) comes from the rocprofv3
+FETCH_SIZE / WRITE_SIZE summary committed under profiles/ for the same
+workload, scaled by the copy-probe calibration; it is null when absent.
 """
 import argparse
 import json
@@ -19,6 +24,44 @@ sys.path.insert(0, HERE)
 
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
+METRIC = "Poisson apply s/tree + GFLOP/s (order 7, \u03b5=1e-6), chosen vs plain FP64 GPU"
+BAND_KERNEL = "k_band_units"
+COPY_PROBE_BYTES = 256 << 20
+
+
+def copy_probe():
+    """Three device-to-device copies of COPY_PROBE_BYTES through the library:
+    the known traffic against which FETCH_SIZE / WRITE_SIZE are scaled."""
+    import ctypes
+    import mrcpp_amd as mr
+    L = mr.lib()
+    L.mwg_init.argtypes = [ctypes.c_int]
+    L.mwg_malloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    L.mwg_copy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    assert L.mwg_init(0) == 0
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert L.mwg_malloc(ctypes.byref(a), COPY_PROBE_BYTES) == 0
+    assert L.mwg_malloc(ctypes.byref(b), COPY_PROBE_BYTES) == 0
+    for _ in range(3):
+        assert L.mwg_copy(b, a, COPY_PROBE_BYTES) == 0
+    L.mwg_synchronize()
+    print("copied %d bytes x3" % COPY_PROBE_BYTES, flush=True)
+
+
+def measured_traffic(workload):
+    """HBM bytes per band-kernel launch from the committed counter summary
+    (lines 'workload: <name>' and 'traffic_bytes_per_launch: <int>')."""
+    path = os.path.join(HERE, "profiles", "%s_%s_traffic.md" % (BAND_KERNEL, workload))
+    if not os.path.exists(path):
+        return None, None
+    kv = {}
+    for line in open(path):
+        if ":" in line and not line.startswith(("#", "`", " ")):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    if kv.get("workload") != workload or "traffic_bytes_per_launch" not in kv:
+        return None, None
+    return int(float(kv["traffic_bytes_per_launch"])), os.path.relpath(path, HERE)
 
 
 def flops_per_tuple(order):
@@ -75,7 +118,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="gauss16_k7")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe()
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
               "(multi-GPU sharding is described in DESIGN.md, not launched); use --gpus 1", file=sys.stderr)
@@ -114,8 +161,9 @@ def main():
     band_ms, band_launches = prof["band"]
     band_ms_per_apply = band_ms / args.steps
     band_achieved = flop / (band_ms_per_apply / 1e3) / 1e12  # TFLOP/s while the band kernel runs
+    traffic, traffic_src = measured_traffic(args.workload)
     res = {
-        "metric": "Poisson apply GFLOP/s (order 7, prec 1e-6; s/tree alongside)",
+        "metric": METRIC,
         "value": round(gflops, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
@@ -134,9 +182,9 @@ def main():
                    "n_candidate_pairs": int(stats[-1]["n_pairs"]), "n_generated_nodes": int(stats[-1]["n_generated"]),
                    "n_output_nodes": int(stats[-1]["n_output_nodes"]), "levels": int(stats[-1]["n_levels"]),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": "k_band_apply", "achieved": round(band_achieved, 4),
+        "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(band_achieved, 4),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(band_achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": None, "launches_per_step": band_launches / args.steps,
+                     "traffic": traffic, "traffic_source": traffic_src, "launches_per_step": band_launches / args.steps,
                      "avg_launch_ms": band_ms / max(band_launches, 1),
                      "flop_per_launch": flop / max(band_launches / args.steps, 1)},
         "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},

@@ -21,6 +21,30 @@ std::vector<TimedLaunch> &timed_launches() {
     static std::vector<TimedLaunch> v;
     return v;
 }
+// Events are created once and reused, so timing adds no creation cost to the
+// measured launches.
+std::vector<hipEvent_t> &event_pool() {
+    static std::vector<hipEvent_t> v;
+    return v;
+}
+hipEvent_t take_event() {
+    auto &pool = event_pool();
+    if (!pool.empty()) {
+        hipEvent_t e = pool.back();
+        pool.pop_back();
+        return e;
+    }
+    hipEvent_t e{};
+    (void)hipEventCreate(&e);
+    return e;
+}
+void recycle_events() {
+    for (auto &t : timed_launches()) {
+        event_pool().push_back(t.start);
+        event_pool().push_back(t.stop);
+    }
+    timed_launches().clear();
+}
 
 // Device blocks cached by exact size for reuse (freed blocks are not returned
 // to the runtime until mwg_release_pool).
@@ -49,8 +73,8 @@ mwg_status need_init() { return g_ready ? MWG_OK : fail(MWG_ERR_ARG, "mwg_init h
 ProfileScope::ProfileScope(int k)
         : kind(k) {
     if (!g_profile) return;
-    (void)hipEventCreate(&start);
-    (void)hipEventCreate(&stop);
+    start = take_event();
+    stop = take_event();
     (void)hipEventRecord(start, 0);
 }
 
@@ -147,12 +171,14 @@ mwg_status mwg_free(void *ptr) {
 mwg_status mwg_profile(int enable) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     MWG_HIP(hipDeviceSynchronize());
-    for (auto &t : timed_launches()) {
-        (void)hipEventDestroy(t.start);
-        (void)hipEventDestroy(t.stop);
-    }
-    timed_launches().clear();
+    recycle_events();
     g_profile = enable != 0;
+    if (g_profile) // enough events for a few thousand launches up front
+        while (event_pool().size() < 8192) {
+            hipEvent_t e{};
+            MWG_HIP(hipEventCreate(&e));
+            event_pool().push_back(e);
+        }
     return MWG_OK;
 }
 
