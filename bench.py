@@ -25,7 +25,7 @@ sys.path.insert(0, HERE)
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 METRIC = "Poisson apply s/tree + GFLOP/s (order 7, \u03b5=1e-6), chosen vs plain FP64 GPU"
-BAND_KERNEL = "k_band_units"
+BAND_KERNEL = "k_band_mfma"
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -111,6 +111,31 @@ def cpu_baseline(workload, w, src):
     }
 
 
+def precision_error(mr, w, mra, f, P, precision):
+    """Relative L2 error of one reduced-precision apply against the FP64 apply
+    (itself equal to the FP64 CPU oracle to ~1e-15), and whether the output
+    trees have the same node set."""
+    import numpy as np
+    out = {}
+    for name in ("fp64", precision):
+        mr.set_sweep_precision(name)
+        g = mr.FunctionTree(mra)
+        mr.apply(w["prec"], g, P, f)
+        out[name] = g.export()
+    mr.set_sweep_precision(precision)
+    (i0, c0, _), (i1, c1, _) = out["fp64"], out[precision]
+    same = bool(np.array_equal(i0, i1))
+    if same:
+        rel = float(np.linalg.norm(c1 - c0) / np.linalg.norm(c0))
+    else:
+        k0 = {tuple(x): j for j, x in enumerate(i0)}
+        pairs = [(k0[tuple(x)], j) for j, x in enumerate(i1) if tuple(x) in k0]
+        a = np.array([c0[j] for j, _ in pairs])
+        b = np.array([c1[j] for _, j in pairs])
+        rel = float(np.linalg.norm(b - a) / np.linalg.norm(a))
+    return {"rel_l2": rel, "same_node_set": same, "nodes_fp64": int(len(i0)), "nodes": int(len(i1))}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -119,6 +144,8 @@ def main():
     ap.add_argument("--workload", default="gauss16_k7")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp64_vector"],
+                    help="band sweep precision (fp64 = the drop-in path)")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
@@ -130,6 +157,7 @@ def main():
 
     mr, w, src, mra, f, P = setup(args.workload)
     order = w["order"]
+    mr.set_sweep_precision(args.precision)
     for _ in range(max(args.warmup, 1)):
         out = mr.FunctionTree(mra)
         st = mr.apply(w["prec"], out, P, f)
@@ -174,7 +202,8 @@ def main():
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": {"fp64": "f64", "fp64_vector": "f64", "fp32": "f32 sweeps, f64 band sum", "bf16x3": "bf16x3 sweeps, f64 band sum"}[args.precision],
+        "precision": args.precision,
         "data": "synthetic seeded Gaussian sources",
         "config": {"workload": args.workload, "order": order, "prec": w["prec"], "box": [-w["box"], w["box"]],
                    "n_gaussians": len(src), "operator": "Poisson" if w["mu"] is None else "Helmholtz mu=%g" % w["mu"],
@@ -191,6 +220,8 @@ def main():
         "host_breakdown_s": {k: stats[-1][k] for k in ("t_screen", "t_generate", "t_band", "t_reduce", "t_transform", "t_host")},
         "step_times_s": [round(t, 6) for t in times],
     }
+    if args.precision not in ("fp64", "fp64_vector"):
+        res["error_vs_fp64"] = precision_error(mr, w, mra, f, P, args.precision)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / s_per_tree

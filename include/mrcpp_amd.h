@@ -64,6 +64,11 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
  * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
  * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
  * t_reduce, t_transform, t_host */
+/* Sweep precision of later mrcpp_apply calls on this thread: 0 FP64 (default,
+ * matrix cores), 1 FP32, 2 BF16x3, 3 FP64 on the vector ALUs.  No reference
+ * counterpart (the reference computes in FP64 only). */
+int mrcpp_set_sweep_precision(int precision);
+int mrcpp_get_sweep_precision(void);
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats);
 /* nodes per breadth-first level of the last apply; returns the number of levels */
 int mrcpp_last_apply_levels(int *level_nodes, int cap);

@@ -41,6 +41,10 @@ mwg_status mwg_malloc(void **ptr, size_t bytes);
 mwg_status mwg_free(void *ptr);
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
 mwg_status mwg_download(void *dst, const void *src, size_t bytes);
+/* Upload through a pinned staging ring and return without waiting: src may be
+ * reused on return; the copy is ordered before later work of the library's
+ * stream.  One host thread at a time. */
+mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes);
 mwg_status mwg_copy(void *dst, const void *src, size_t bytes); /* device to device */
 mwg_status mwg_zero(void *dst, size_t bytes);
 mwg_status mwg_synchronize(void);
@@ -85,7 +89,25 @@ typedef struct {
     const double *pair_bound_w; /* [n_depth][2*wmax+1], wavelet components only */
     const int *band_size_max; /* [n_depth][M] row maximum of band_size */
     const double *term_norm;  /* [matrix/4] max component norm per (depth, term, dl) */
+    long long n_mat;          /* number of matrices in mats */
+    const void *frag[3];      /* MFMA operand tables per sweep precision (MWG_PREC_*), NULL until built */
 } mwg_band_tables;
+
+/* Sweep precision of the band sum.  The norm screening, the operator
+ * construction and the transforms stay FP64 in every mode. */
+enum {
+    MWG_PREC_FP64 = 0,   /* FP64 MFMA */
+    MWG_PREC_FP32 = 1,   /* FP32 MFMA, band sum accumulated in FP64 */
+    MWG_PREC_BF16X3 = 2  /* split BF16 (hi*hi + hi*lo + lo*hi) MFMA with FP32 products, FP64 band sum */
+};
+
+/* Size and construction of the MFMA operand table of one precision: every
+ * (depth, term, translation) matrix pair of one f-bit, [O_(gbit 0) | O_(gbit 1)],
+ * laid out in the lane order of the matrix instruction's B operand.  No
+ * reference counterpart (layout of the operator blocks of
+ * OperatorTree::getNode, src/trees/OperatorTree.h:53-57, for the device). */
+size_t mwg_fragment_bytes(int kp1, long long n_mat, int precision);
+mwg_status mwg_build_fragments(const mwg_band_tables *op, int precision, void *dst);
 
 /* Input function tree resident on the device: an open-addressing hash from a
  * packed node key to a node id, and per-node coefficient offset, 8 component
@@ -126,7 +148,7 @@ typedef struct {
  * (MWTree::getNode -> MWNode::retrieveNode, src/trees/MWNode.cpp:1097-1120). */
 typedef struct {
     int capacity;
-    int *count;          /* device scalar: number of pairs written/needed */
+    int *count;          /* device int[2]: pairs written/needed, pairs whose band node is missing */
     int *pair_g;         /* [capacity] output-node index in the level */
     int *pair_f;         /* [capacity] */
     int *pair_l;         /* [capacity][3] band node translation */
@@ -161,7 +183,8 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
  * MWNode::calcNorms, src/trees/MWNode.cpp:609-655).  Every pair_f must be a
  * node id (missing band nodes generated first).  Adds the number of applied
  * (f, ft, gt, term) tuples to *n_applied.  Summation order is fixed by the
- * pair order, so results are reproducible.  precision: 0 = FP64. */
+ * pair order, so results are reproducible.  precision: MWG_PREC_*; the
+ * table op->frag[precision] must be built. */
 mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs,
                           int npairs, unsigned long long *n_applied, int precision);
 

@@ -56,6 +56,8 @@ SIGNATURES = {
     "mrcpp_kernel_terms": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _dp, _dp, ctypes.c_int]),
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "mrcpp_set_sweep_precision": (ctypes.c_int, [ctypes.c_int]),
+    "mrcpp_get_sweep_precision": (ctypes.c_int, []),
     "mrcpp_last_apply_levels": (ctypes.c_int, [_ip, ctypes.c_int]),
     # device layer (include/mwg.h) used by the harness
     "mwg_last_error": (ctypes.c_char_p, []),
@@ -296,6 +298,23 @@ def profile_read():
     if lib().mwg_profile_read(_p(ms), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n) != 0:
         raise MRCPPError(lib().mwg_last_error().decode())
     return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_KINDS)}
+
+
+SWEEP_PRECISIONS = {"fp64": 0, "fp32": 1, "bf16x3": 2, "fp64_vector": 3}
+
+
+def set_sweep_precision(name):
+    """Sweep precision of later apply() calls on this thread: 'fp64' (default,
+    FP64 matrix cores), 'fp32', 'bf16x3' (band sum accumulated in FP64 in
+    both) or 'fp64_vector' (FP64 sweeps on the vector ALUs)."""
+    if name not in SWEEP_PRECISIONS:
+        raise ValueError("sweep precision must be one of %s" % sorted(SWEEP_PRECISIONS))
+    _check(lib().mrcpp_set_sweep_precision(SWEEP_PRECISIONS[name]))
+
+
+def sweep_precision():
+    code = lib().mrcpp_get_sweep_precision()
+    return {v: k for k, v in SWEEP_PRECISIONS.items()}[code]
 
 
 def apply(prec, out, oper, inp, maxIter=-1, absPrec=False):

@@ -16,6 +16,7 @@ namespace mrcpp {
 namespace {
 
 thread_local ApplyStats g_stats;
+thread_local SweepPrecision g_sweep = SweepPrecision::FP64;
 
 using clk = std::chrono::steady_clock;
 double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
@@ -145,7 +146,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     S.gNorms.reserve(9 * (size_t)L);
     S.gFirst.reserve(L);
     S.gCount.reserve(L);
-    S.count.reserve(1);
+    S.count.reserve(2);
 
     mwg_level lvl{};
     lvl.kp1 = c.kp1;
@@ -161,7 +162,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
 
     mwg_ftree fv = c.fm.view(c.mra);
     mwg_pairs pr{};
-    int npairs = 0;
+    int npairs = 0, nmissing = 0;
     size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
     auto bind_pairs = [&]() {
         S.pairG.reserve(cap);
@@ -180,7 +181,10 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
         auto t0 = clk::now();
         for (;;) {
             device::check(mwg_pair_screen(&lvl, &fv, &c.bt, &pr), "pair screen");
-            S.count.download(&npairs, 1);
+            int cnt[2];
+            S.count.download(cnt, 2);
+            npairs = cnt[0];
+            nmissing = cnt[1];
             if (npairs < 0) throw Error("pair screen: band width does not match the operator tables");
             if (npairs <= pr.capacity) break;
             cap = (size_t)npairs + npairs / 4;
@@ -189,13 +193,11 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
         g_stats.tScreen += since(t0);
         g_stats.nPairs += npairs;
     }
-    if (npairs > 0) {
+    if (npairs > 0 && nmissing > 0) {
         auto t0 = clk::now();
         std::vector<int> pf(npairs), pl(3 * (size_t)npairs);
         S.pairF.download(pf.data(), npairs);
-        bool missing = false;
-        for (int v : pf) missing |= (v < 0);
-        if (missing) {
+        {
             S.pairL.download(pl.data(), pl.size());
             generate_band_nodes(c, pf, pl, scale);
             S.pairF.upload(pf.data(), npairs);
@@ -205,7 +207,7 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     }
     auto t0 = clk::now();
     // band sums, output coefficients and norms (zero when no pair survives)
-    device::check(mwg_band_apply(&lvl, &fv, &c.bt, &pr, npairs, S.nApplied.get(), 0), "band apply");
+    device::check(mwg_band_apply(&lvl, &fv, &c.bt, &pr, npairs, S.nApplied.get(), (int)g_sweep), "band apply");
     std::vector<double> nrm(9 * (size_t)L);
     S.gNorms.download(nrm.data(), nrm.size());
     g_stats.tBand += since(t0);
@@ -299,6 +301,9 @@ void drop_generated(device::TreeMirror &fm, const MultiResolutionAnalysis<3> &mr
 
 const ApplyStats &last_apply_stats() { return g_stats; }
 
+void set_sweep_precision(SweepPrecision p) { g_sweep = p; }
+SweepPrecision sweep_precision() { return g_sweep; }
+
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter, bool absPrec) {
     if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
     const auto t_start = clk::now();
@@ -311,7 +316,7 @@ void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, Func
     const detail::BandTables &ht = oper.bandTables(prec);
     device::TreeMirror &fm = inp.inputMirror();
     detail::NodeStore<3> &g = out.nodes();
-    LevelContext c{mra, oper, ht, dbt->view, fm, &inp.topology(), out, device::filters(mra.getOrder()), kp1, kp1 * kp1 * kp1, 8 * kp1 * kp1 * kp1};
+    LevelContext c{mra, oper, ht, dbt->viewFor((int)g_sweep), fm, &inp.topology(), out, device::filters(mra.getOrder()), kp1, kp1 * kp1 * kp1, 8 * kp1 * kp1 * kp1};
 
     Scratch &S = scratch();
     S.nApplied.reserve(1);

@@ -20,6 +20,7 @@
 // Every summation order is fixed by the pair and group order, so results do
 // not depend on scheduling.
 #include "device_common.h"
+#include "mfma.h"
 
 namespace mwgdev {
 
@@ -165,13 +166,109 @@ __global__ __launch_bounds__(256) void k_unit_write(int npairs, mwg_pairs pr, co
 }
 
 // ---------------------------------------------------------------------------
-// 3. persistent unit kernel
+// 3. persistent unit kernels
 //
-// A workgroup of NT threads (one wave for (k+1)^2 <= 64, two for k = 9) owns
-// one line of every block per thread: in each 1D sweep a thread reads N1
-// values, does N1^2 FMAs with wave-uniform operator entries (scalar loads)
-// and writes N1 values.  LDS arrays are [contracted index][line] with a row
-// pitch chosen so the strided writes of one sweep spread over the banks.
+// Workgroups take units from an atomic counter until every unit is taken; a
+// unit walks its groups in order (UnitCursor), accumulates the tensor
+// products of every group in registers and writes one 8-block result.
+// ---------------------------------------------------------------------------
+
+// Read-only views a unit kernel walks (nothing here is written by the band
+// kernels, so loads at wave-uniform addresses are scalar loads).
+struct UnitView {
+    const_as4<int> *lv, *pg, *pf, *pl, *gs, *wd, *unit;
+    const_as4<long long> *ms, *coff;
+    const_as4<unsigned> *cd;
+    const double *coef;
+    int stride, total;
+    __device__ UnitView(const mwg_level &lvl, const mwg_ftree &f, const mwg_band_tables &op, const mwg_pairs &pr, const unsigned *codes, int stride_,
+                        const int *gstart, const int4 *units, const long long *uscan, int L)
+            : lv(ro(lvl.l)), pg(ro(pr.pair_g)), pf(ro(pr.pair_f)), pl(ro(pr.pair_l)), gs(ro(gstart)),
+              wd(ro(op.widths + (size_t)lvl.o_depth * op.M * 5)), unit(ro(reinterpret_cast<const int *>(units))),
+              ms(ro(op.mat_start + (size_t)lvl.o_depth * op.M)), coff(ro(f.coef_off)), cd(ro(codes)), coef(f.coef), stride(stride_),
+              total(unit_first(ro(uscan)[L])) {}
+};
+
+// Next unit for the workgroup (all threads get the same index), or -1.
+__device__ inline int take_unit(int *counter, int *s_unit, int total) {
+    __syncthreads();
+    if (threadIdx.x == 0) *s_unit = atomicAdd(counter, 1);
+    __syncthreads();
+    const int u = __builtin_amdgcn_readfirstlane(*s_unit);
+    return u < total ? u : -1;
+}
+
+// One group of a unit: term, f component, gt mask, and the band node it acts on.
+struct Group {
+    int term, ft;
+    unsigned mask;
+    bool fresh_f; // band node or ft differs from the previous group's
+};
+
+struct UnitCursor {
+    int g, slot, left;
+    int gl[3], dl[3];
+    int p, k, pend, loaded, curft;
+    const double *fsrc;
+
+    __device__ void start(const UnitView &v, int u) {
+        const auto ud = v.unit + 4 * (size_t)u;
+        p = ud[0];
+        k = ud[1];
+        left = ud[2];
+        slot = ud[3];
+        g = v.pg[p];
+        for (int i = 0; i < 3; i++) gl[i] = v.lv[3 * g + i];
+        pend = v.gs[p + 1] - v.gs[p];
+        loaded = -1;
+        curft = -1;
+        fsrc = nullptr;
+    }
+    __device__ bool next(const UnitView &v, Group &grp) {
+        if (left == 0) return false;
+        left--;
+        while (k >= pend) { // next pair of the same output node that has groups
+            p++;
+            k = 0;
+            pend = v.gs[p + 1] - v.gs[p];
+        }
+        bool fresh = false;
+        if (p != loaded) { // a pair with groups always has a band node id
+            for (int i = 0; i < 3; i++) dl[i] = v.pl[3 * p + i] - gl[i];
+            fsrc = v.coef + v.coff[v.pf[p]];
+            loaded = p;
+            fresh = true;
+        }
+        const unsigned code = v.cd[(size_t)p * v.stride + k];
+        k++;
+        grp.term = (int)(code & 0xffffu);
+        grp.ft = (int)((code >> 16) & 7u);
+        grp.mask = code >> 24;
+        grp.fresh_f = fresh || grp.ft != curft;
+        curft = grp.ft;
+        return true;
+    }
+};
+
+// x / y needs of a gt mask: bit gx of xneed, bit gx + 2 gy of yneed.
+__device__ inline void sweep_needs(unsigned mask, unsigned &xneed, unsigned &yneed) {
+    xneed = 0;
+    yneed = 0;
+#pragma unroll
+    for (int gt = 0; gt < 8; gt++)
+        if (mask >> gt & 1) {
+            xneed |= 1u << (gt & 1);
+            yneed |= 1u << (gt & 3);
+        }
+}
+
+// ---------------------------------------------------------------------------
+// 3a. FP64 on the vector ALUs: a workgroup of NT threads (one wave for
+// (k+1)^2 <= 64, two for k = 9) owns one line of every block per thread: in
+// each 1D sweep a thread reads N1 values, does N1^2 FMAs with wave-uniform
+// operator entries (scalar loads) and writes N1 values.  LDS arrays are
+// [contracted index][line] with a row pitch chosen so the strided writes of
+// one sweep spread over the banks.
 // ---------------------------------------------------------------------------
 template <int N1> struct UnitShape {
     static constexpr int N2 = N1 * N1;
@@ -192,82 +289,36 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
 
     const int tid = threadIdx.x;
     const bool owner = tid < N2;
-    const int d = lvl.o_depth;
-    const int M = op.M;
-    // tables, pairs and units are not written by this kernel: scalar loads
-    const auto wd = ro(op.widths + (size_t)d * M * 5);
-    const auto ms = ro(op.mat_start + (size_t)d * M);
+    const UnitView v(lvl, f, op, pr, codes, stride, gstart, units, uscan, L);
     const auto mats = ro(op.mats);
-    const auto lv = ro(lvl.l);
-    const auto pg = ro(pr.pair_g);
-    const auto pf = ro(pr.pair_f);
-    const auto pl = ro(pr.pair_l);
-    const auto gs = ro(gstart);
-    const auto cd = ro(codes);
-    const auto coff = ro(f.coef_off);
-    const int total = unit_first(ro(uscan)[L]);
 
-    for (;;) {
-        __syncthreads();
-        if (tid == 0) s_unit = atomicAdd(counter, 1);
-        __syncthreads();
-        const int u = __builtin_amdgcn_readfirstlane(s_unit);
-        if (u >= total) break;
-        const auto ud = ro(reinterpret_cast<const int *>(units)) + 4 * (size_t)u;
-        int p = ud[0], k = ud[1];
-        const int n = ud[2], slot = ud[3];
-        const int g = pg[p];
-        const int gl0 = lv[3 * g], gl1 = lv[3 * g + 1], gl2 = lv[3 * g + 2];
-
+    for (int u; (u = take_unit(counter, &s_unit, v.total)) >= 0;) {
+        UnitCursor cur;
+        cur.start(v, u);
         double acc[8][N1];
 #pragma unroll
         for (int t = 0; t < 8; t++)
 #pragma unroll
             for (int j = 0; j < N1; j++) acc[t][j] = 0.0;
 
-        int pend = gs[p + 1] - gs[p];
-        int loaded = -1, curft = -1;
-        int dl0 = 0, dl1 = 0, dl2 = 0;
-        const double *fsrc = nullptr;
-        for (int it = 0; it < n; it++) {
-            while (k >= pend) { // next pair of the same output node that has groups
-                p++;
-                k = 0;
-                pend = gs[p + 1] - gs[p];
-            }
-            if (p != loaded) { // a pair with groups always has a band node id
-                dl0 = pl[3 * p] - gl0;
-                dl1 = pl[3 * p + 1] - gl1;
-                dl2 = pl[3 * p + 2] - gl2;
-                fsrc = f.coef + coff[pf[p]];
-                loaded = p;
-                curft = -1;
-            }
-            const unsigned code = cd[(size_t)p * stride + k];
-            k++;
-            const int ti = (int)(code & 0xffffu);
-            const int ft = (int)((code >> 16) & 7u);
-            const unsigned gm = code >> 24;
-            if (ft != curft) {
+        Group grp;
+        while (cur.next(v, grp)) {
+            const int ft = grp.ft;
+            const unsigned gm = grp.mask;
+            if (grp.fresh_f) {
                 __syncthreads();
-                for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = fsrc[(size_t)ft * N3 + e];
-                curft = ft;
+                for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = cur.fsrc[(size_t)ft * N3 + e];
                 __syncthreads();
             }
-            const int w4 = wd[ti * 5 + 4];
-            const long long mbase = ms[ti];
-            unsigned xneed = 0, yneed = 0;
-#pragma unroll
-            for (int gt = 0; gt < 8; gt++)
-                if (gm >> gt & 1) {
-                    xneed |= 1u << (gt & 1);
-                    yneed |= 1u << (gt & 3);
-                }
+            const int w4 = v.wd[grp.term * 5 + 4];
+            const long long mbase = v.ms[grp.term];
+            unsigned xneed, yneed;
+            sweep_needs(gm, xneed, yneed);
 #pragma unroll
             for (int gx = 0; gx < 2; gx++) {
                 if (!(xneed >> gx & 1)) continue;
                 // x sweep: T1[i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox(i0, j0)
-                const auto Ox = mats + (size_t)(mbase + (long long)(dl0 + w4) * 4 + 2 * gx + (ft & 1)) * N2;
+                const auto Ox = mats + (size_t)(mbase + (long long)(cur.dl[0] + w4) * 4 + 2 * gx + (ft & 1)) * N2;
                 if (owner) {
                     double x[N1];
 #pragma unroll
@@ -286,7 +337,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
                 for (int gy = 0; gy < 2; gy++) {
                     if (!(yneed >> (gx + 2 * gy) & 1)) continue;
                     // y sweep: T2[i2][j0 + N1 j1] = sum_i1 T1[i1][i2 + N1 j0] Oy(i1, j1)
-                    const auto Oy = mats + (size_t)(mbase + (long long)(dl1 + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
+                    const auto Oy = mats + (size_t)(mbase + (long long)(cur.dl[1] + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
                     if (owner) {
                         double y[N1];
 #pragma unroll
@@ -310,7 +361,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
                         for (int gz = 0; gz < 2; gz++) {
                             const int gt = gx + 2 * gy + 4 * gz;
                             if (!(gm >> gt & 1)) continue;
-                            const auto Oz = mats + (size_t)(mbase + (long long)(dl2 + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
+                            const auto Oz = mats + (size_t)(mbase + (long long)(cur.dl[2] + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
 #pragma unroll
                             for (int j = 0; j < N1; j++) {
                                 double s = 0.0;
@@ -324,13 +375,269 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
                 }
             }
         }
-        double *dst = (slot < 0) ? lvl.g_coef + ro(lvl.g_off)[g] : partial + (size_t)slot * 8 * N3;
+        double *dst = (cur.slot < 0) ? lvl.g_coef + ro(lvl.g_off)[cur.g] : partial + (size_t)cur.slot * 8 * N3;
         if (owner) {
 #pragma unroll
             for (int t = 0; t < 8; t++)
 #pragma unroll
                 for (int j = 0; j < N1; j++) dst[t * N3 + tid + N2 * j] = acc[t][j];
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// 3b. the sweeps on the matrix cores
+//
+// Four waves per workgroup.  Per group (ft, term, gt mask): the x sweep
+// F^T [Ox_0 | Ox_1] (tiles shared out over the waves), the y sweep for each
+// needed gx, then wave w = gx + 2 gy runs the z sweep of its own (gx, gy)
+// into its accumulators, the columns of a gz outside the mask zeroed in the
+// operand.  LDS holds F, T1 = [gx][i1][i2 + N1 j0], T2 = [gx][gy][i2][j0 + N1 j1].
+//
+// PREC selects the operand types:
+//   FP64: 16x16x4 f64, LDS [contracted index][row], accumulate in place;
+//   FP32: 16x16x4 f32, same layout, each group's product added to FP64;
+//   BF16X3: 16x16x32 bf16 with the contraction concatenated as
+//     [A_hi | A_hi | A_lo | 0] x [B_hi ; B_lo ; B_hi ; 0] (8 wide each), so one
+//     instruction forms hi*hi + hi*lo + lo*hi with FP32 products; LDS in FP32
+//     [row][contracted index] with 4 floats of padding after every N1 rows;
+//     each group's product added to FP64.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+template <int N1, int PREC> struct BandGeom {
+    using S = SweepShape<N1>;
+    static constexpr bool kBF = PREC == MWG_PREC_BF16X3;
+    static constexpr int KB = (N1 + 7) / 8;                  // 8-wide contraction blocks (BF16X3)
+    static constexpr int KSTEPS = kBF ? KB : S::KS;         // instructions per tile
+    static constexpr int PK = 8 * KB;                        // row length (BF16X3 layout)
+    static constexpr int ROWS = 16 * S::MT;
+    // elements of one [contracted][row] or [row][contracted] block in LDS
+    static constexpr int BLK = kBF ? ROWS * PK + (ROWS / N1 + 1) * 4 : S::KP * S::PM;
+    // LDS index of (contracted index kk, row m)
+    __device__ static inline int at(int kk, int m) { return kBF ? m * PK + (m / N1) * 4 + kk : kk * S::PM + m; }
+};
+
+template <int PREC> struct BandTypes;
+template <> struct BandTypes<MWG_PREC_FP64> {
+    using L = double; // LDS element
+    using B = double; // operand table element
+    using MM = Mfma<double>;
+};
+template <> struct BandTypes<MWG_PREC_FP32> {
+    using L = float;
+    using B = float;
+    using MM = Mfma<float>;
+};
+template <> struct BandTypes<MWG_PREC_BF16X3> {
+    using L = float;
+    using B = uint4; // 8 bf16
+    using MM = Mfma<float>;
+};
+
+// One 16x16 tile of a sweep: rows 16 mt.. of the LDS block src times the
+// operand fragments b[0..KSTEPS) of column tile nt.
+template <int N1, int PREC>
+__device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename BandTypes<PREC>::L *src, const typename BandTypes<PREC>::B *b, int mt,
+                                                             int lane) {
+    using G = BandGeom<N1, PREC>;
+    using MM = typename BandTypes<PREC>::MM;
+    typename MM::C c = {};
+    const int li = lane & 15, lq = lane >> 4;
+    if constexpr (PREC == MWG_PREC_BF16X3) {
+        const int m = 16 * mt + li;
+#pragma unroll
+        for (int kb = 0; kb < G::KB; kb++) {
+            const float *row = src + G::at(8 * kb, m);
+            const float4 v0 = *reinterpret_cast<const float4 *>(row);
+            const float4 v1 = *reinterpret_cast<const float4 *>(row + 4);
+            const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
+            bf16x8 a;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const __bf16 hi = (__bf16)x[e];
+                const __bf16 lo = (__bf16)(x[e] - (float)hi);
+                a[e] = lq == 3 ? (__bf16)0.0f : (lq == 2 ? lo : hi);
+            }
+            const uint4 bb = b[kb];
+            bf16x8 bv;
+            __builtin_memcpy(&bv, &bb, sizeof(bv));
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, c, 0, 0, 0);
+        }
+    } else {
+#pragma unroll
+        for (int ks = 0; ks < G::KSTEPS; ks++) c = MM::mma(src[G::at(4 * ks + lq, 16 * mt + li)], b[ks], c);
+    }
+    return c;
+}
+
+template <int N1, int PREC>
+__global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, mwg_band_tables op, mwg_pairs pr, const unsigned *codes, int stride,
+                                                   const int *gstart, const int4 *units, const long long *uscan, int L, double *partial, int *counter,
+                                                   const typename BandTypes<PREC>::B *frag) {
+    using S = SweepShape<N1>;
+    using G = BandGeom<N1, PREC>;
+    using LT = typename BandTypes<PREC>::L;
+    using BT = typename BandTypes<PREC>::B;
+    using MM = typename BandTypes<PREC>::MM;
+    constexpr int N2 = S::N2, N3 = S::N3, MT = S::MT, NT = S::NT, KSTEPS = G::KSTEPS, BLK = G::BLK;
+    constexpr int FRAG = KSTEPS * NT * 64; // operand elements per (matrix block, f bit)
+    __shared__ __align__(16) LT sF[BLK];
+    __shared__ __align__(16) LT sT1[2 * BLK];
+    __shared__ __align__(16) LT sT2[4 * BLK];
+    __shared__ int s_unit;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int gxw = w & 1, gyw = w >> 1;
+    const int li = lane & 15;
+    const UnitView v(lvl, f, op, pr, codes, stride, gstart, units, uscan, L);
+
+    for (int i = tid; i < BLK; i += 256) sF[i] = LT(0);
+    for (int i = tid; i < 2 * BLK; i += 256) sT1[i] = LT(0);
+    for (int i = tid; i < 4 * BLK; i += 256) sT2[i] = LT(0);
+
+    for (int u; (u = take_unit(counter, &s_unit, v.total)) >= 0;) {
+        UnitCursor cur;
+        cur.start(v, u);
+        dvec4 acc[MT][NT];
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) acc[mt][nt] = dvec4{0.0, 0.0, 0.0, 0.0};
+
+        Group grp;
+        while (cur.next(v, grp)) {
+            const int ft = grp.ft;
+            const unsigned gm = grp.mask;
+            unsigned xneed, yneed;
+            sweep_needs(gm, xneed, yneed);
+            // operand fragments: matrix block b4 = mat_start/4 + w + dl of each direction
+            const long long b40 = v.ms[grp.term] / 4 + v.wd[grp.term * 5 + 4];
+            BT bx[NT][KSTEPS], by[NT][KSTEPS], bz[NT][KSTEPS];
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++) {
+                const int nn = 16 * nt + li;
+                const int gz = nn / N1;
+                const bool keep = nn < 2 * N1 && ((gm >> (gxw + 2 * gyw + 4 * gz)) & 1);
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) {
+                    const size_t o = ((size_t)ks * NT + nt) * 64 + lane;
+                    bx[nt][ks] = frag[((b40 + cur.dl[0]) * 2 + (ft & 1)) * FRAG + o];
+                    by[nt][ks] = frag[((b40 + cur.dl[1]) * 2 + ((ft >> 1) & 1)) * FRAG + o];
+                    const BT zv = frag[((b40 + cur.dl[2]) * 2 + ((ft >> 2) & 1)) * FRAG + o];
+                    bz[nt][ks] = keep ? zv : BT{};
+                }
+            }
+            if (grp.fresh_f) {
+                // F[i0][i1 + N1 i2] for element e = i0 + N1 (i1 + N1 i2)
+                for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)cur.fsrc[(size_t)ft * N3 + e];
+                __syncthreads();
+            }
+            // x sweep: T1[gx][i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox_gx(i0, j0)
+            for (int t = w; t < MT * NT; t += 4) {
+                const int mt = t / NT, nt = t % NT;
+                const auto c = sweep_tile<N1, PREC>(sF, bx[nt], mt, lane);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int line = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
+                    if (line < N2 && nn < 2 * N1) {
+                        const int gx = nn / N1, j0 = nn % N1, i1 = line % N1, i2 = line / N1;
+                        sT1[gx * BLK + G::at(i1, i2 + N1 * j0)] = c[r];
+                    }
+                }
+            }
+            __syncthreads();
+            // y sweep: T2[gx][gy][i2][j0 + N1 j1] = sum_i1 T1[gx][i1][i2 + N1 j0] Oy_gy(i1, j1)
+            for (int t = w; t < 2 * MT * NT; t += 4) {
+                const int gx = t / (MT * NT), tt = t % (MT * NT);
+                if (!(xneed >> gx & 1)) continue;
+                const int mt = tt / NT, nt = tt % NT;
+                const auto c = sweep_tile<N1, PREC>(sT1 + gx * BLK, by[nt], mt, lane);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
+                    if (m < N2 && nn < 2 * N1) {
+                        const int i2 = m % N1, j0 = m / N1, gy = nn / N1, j1 = nn % N1;
+                        sT2[(gx * 2 + gy) * BLK + G::at(i2, j0 + N1 * j1)] = c[r];
+                    }
+                }
+            }
+            __syncthreads();
+            // z sweep of this wave's (gx, gy): G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[gx][gy][i2][j0 + N1 j1] Oz_gz(i2, j2)
+            if (yneed >> (gxw + 2 * gyw) & 1) {
+                const LT *src = sT2 + (gxw * 2 + gyw) * BLK;
+#pragma unroll
+                for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+                    for (int nt = 0; nt < NT; nt++) {
+                        if constexpr (PREC == MWG_PREC_FP64) {
+#pragma unroll
+                            for (int ks = 0; ks < KSTEPS; ks++)
+                                acc[mt][nt] = MM::mma(src[G::at(4 * ks + (lane >> 4), 16 * mt + li)], bz[nt][ks], acc[mt][nt]);
+                        } else {
+                            const auto c = sweep_tile<N1, PREC>(src, bz[nt], mt, lane);
+#pragma unroll
+                            for (int r = 0; r < 4; r++) acc[mt][nt][r] += (double)c[r];
+                        }
+                    }
+            }
+        }
+        // this wave's blocks gt = gx + 2 gy + 4 gz of the unit result
+        double *dst = (cur.slot < 0) ? lvl.g_coef + ro(lvl.g_off)[cur.g] : partial + (size_t)cur.slot * 8 * N3;
+#pragma unroll
+        for (int mt = 0; mt < MT; mt++)
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int m = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
+                    if (m < N2 && nn < 2 * N1) {
+                        const int gz = nn / N1, j2 = nn % N1;
+                        dst[(gxw + 2 * gyw + 4 * gz) * N3 + m + N2 * j2] = acc[mt][nt][r];
+                    }
+                }
+    }
+}
+
+// Operand table of one precision from the FP64 operator matrices.  Element
+// (matrix block b4, f bit, column tile nt, step ks, lane) of P[k][n] =
+// O_(2 (n / N1) + f)[k][n % N1], zero in the padding; FP64/FP32 hold
+// k = 4 ks + lane/16, n = 16 nt + lane%16; BF16X3 holds 8 bf16 for
+// k = 8 ks + 0..7 in the slot lane/16 of [hi ; lo ; hi ; 0].
+template <int N1, int PREC> __global__ __launch_bounds__(256) void k_build_frag(const double *mats, long long nb4, typename BandTypes<PREC>::B *out) {
+    using S = SweepShape<N1>;
+    using G = BandGeom<N1, PREC>;
+    constexpr int FRAG = G::KSTEPS * S::NT * 64;
+    const size_t total = (size_t)nb4 * 2 * FRAG;
+    const size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x;
+    if (idx >= total) return;
+    const int lane = (int)(idx % 64);
+    size_t rest = idx / 64;
+    const int nt = (int)(rest % S::NT);
+    rest /= S::NT;
+    const int ks = (int)(rest % G::KSTEPS);
+    rest /= G::KSTEPS;
+    const int fb = (int)(rest % 2);
+    const long long b4 = (long long)(rest / 2);
+    const int nn = 16 * nt + lane % 16;
+    auto P = [&](int kk) {
+        if (kk >= N1 || nn >= 2 * N1) return 0.0;
+        return mats[(size_t)(b4 * 4 + 2 * (nn / N1) + fb) * S::N2 + kk + N1 * (nn % N1)];
+    };
+    if constexpr (PREC == MWG_PREC_BF16X3) {
+        const int slot = lane / 16;
+        bf16x8 val;
+        for (int e = 0; e < 8; e++) {
+            const float x = (float)P(8 * ks + e);
+            const __bf16 hi = (__bf16)x;
+            const __bf16 lo = (__bf16)(x - (float)hi);
+            val[e] = slot == 3 ? (__bf16)0.0f : (slot == 1 ? lo : hi);
+        }
+        uint4 o;
+        __builtin_memcpy(&o, &val, sizeof(o));
+        out[idx] = o;
+    } else {
+        out[idx] = (typename BandTypes<PREC>::B)P(4 * ks + lane / 16);
     }
 }
 
@@ -381,22 +688,33 @@ BandWork &work() {
     return w;
 }
 
-template <int N1> int unit_grid() {
-    static int grid = 0;
-    if (!grid) {
-        int per_cu = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_band_units<N1>, UnitShape<N1>::NT, 0) != hipSuccess || per_cu <= 0) {
-            (void)hipGetLastError();
-            per_cu = 4;
-        }
-        grid = per_cu * device_cus();
+// Persistent grid: as many workgroups as fit on the device at once.
+template <typename K> int resident_grid(K kernel, int threads) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, 0) != hipSuccess || per_cu <= 0) {
+        (void)hipGetLastError();
+        per_cu = 2;
     }
+    return per_cu * device_cus();
+}
+template <int N1> int unit_grid() {
+    static const int grid = resident_grid(k_band_units<N1>, UnitShape<N1>::NT);
     return grid;
+}
+template <int N1, int PREC> int mfma_grid() {
+    static const int grid = resident_grid(k_band_mfma<N1, PREC>, 256);
+    return grid;
+}
+template <int N1, int PREC> void launch_mfma(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, BandWork &w,
+                                             int stride, int L) {
+    hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, *lvl, *f, *op, *pr, w.codes.as<unsigned>(), stride,
+                       w.gstart.as<int>(), w.units.as<int4>(), w.uscan.as<long long>(), L, w.partial.as<double>(), w.counter.as<int>(),
+                       static_cast<const typename BandTypes<PREC>::B *>(op->frag[PREC]));
 }
 
 template <int N1> struct BandL {
     static mwg_status run(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, int npairs,
-                          unsigned long long *n_applied) {
+                          unsigned long long *n_applied, int precision) {
         constexpr int N3 = N1 * N1 * N1;
         const int L = lvl->count;
         if (npairs == 0) {
@@ -433,8 +751,16 @@ template <int N1> struct BandL {
         }
         {
             ProfileScope prof_(MWG_KERNEL_BAND);
-            hipLaunchKernelGGL(k_band_units<N1>, dim3(unit_grid<N1>()), dim3(UnitShape<N1>::NT), 0, 0, *lvl, *f, *op, *pr, w.codes.as<unsigned>(),
-                               stride, gstart, w.units.as<int4>(), uscan, L, w.partial.as<double>(), w.counter.as<int>());
+            if (precision == MWG_PREC_FP64)
+                launch_mfma<N1, MWG_PREC_FP64>(lvl, f, op, pr, w, stride, L);
+            else if (precision == MWG_PREC_FP32)
+                launch_mfma<N1, MWG_PREC_FP32>(lvl, f, op, pr, w, stride, L);
+            else if (precision == MWG_PREC_BF16X3)
+                launch_mfma<N1, MWG_PREC_BF16X3>(lvl, f, op, pr, w, stride, L);
+            else
+                hipLaunchKernelGGL(k_band_units<N1>, dim3(unit_grid<N1>()), dim3(UnitShape<N1>::NT), 0, 0, *lvl, *f, *op, *pr,
+                                   w.codes.as<unsigned>(), stride, gstart, w.units.as<int4>(), uscan, L, w.partial.as<double>(),
+                                   w.counter.as<int>());
             MWG_LAUNCHED();
         }
         {
@@ -448,11 +774,64 @@ template <int N1> struct BandL {
     }
 };
 
+template <int N1, int PREC> size_t frag_count(long long n_mat) {
+    return (size_t)(n_mat / 4) * 2 * BandGeom<N1, PREC>::KSTEPS * SweepShape<N1>::NT * 64;
+}
+template <int N1, int PREC> mwg_status build_frag(const mwg_band_tables *op, void *dst) {
+    const size_t total = frag_count<N1, PREC>(op->n_mat);
+    if (total == 0) return MWG_OK;
+    hipLaunchKernelGGL((k_build_frag<N1, PREC>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, 0, op->mats, op->n_mat / 4,
+                       static_cast<typename BandTypes<PREC>::B *>(dst));
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+template <int N1> struct FragL {
+    static mwg_status run(const mwg_band_tables *op, int precision, void *dst) {
+        if (precision == MWG_PREC_FP64) return build_frag<N1, MWG_PREC_FP64>(op, dst);
+        if (precision == MWG_PREC_FP32) return build_frag<N1, MWG_PREC_FP32>(op, dst);
+        return build_frag<N1, MWG_PREC_BF16X3>(op, dst);
+    }
+};
+template <int N1> size_t frag_bytes(long long n_mat, int precision) {
+    if (precision == MWG_PREC_FP64) return frag_count<N1, MWG_PREC_FP64>(n_mat) * sizeof(double);
+    if (precision == MWG_PREC_FP32) return frag_count<N1, MWG_PREC_FP32>(n_mat) * sizeof(float);
+    if (precision == MWG_PREC_BF16X3) return frag_count<N1, MWG_PREC_BF16X3>(n_mat) * sizeof(uint4);
+    return 0;
+}
+
 } // namespace
 
 } // namespace mwgdev
 
 using namespace mwgdev;
+
+// Precision 3 selects the FP64 sweeps on the vector ALUs (comparison path).
+constexpr int kPrecValu = 3;
+
+extern "C" size_t mwg_fragment_bytes(int kp1, long long n_mat, int precision) {
+    if (n_mat < 0) return 0;
+    switch (kp1) {
+    case 4:
+        return frag_bytes<4>(n_mat, precision);
+    case 6:
+        return frag_bytes<6>(n_mat, precision);
+    case 8:
+        return frag_bytes<8>(n_mat, precision);
+    case 10:
+        return frag_bytes<10>(n_mat, precision);
+    default:
+        return 0;
+    }
+}
+
+extern "C" mwg_status mwg_build_fragments(const mwg_band_tables *op, int precision, void *dst) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!op || !dst) return fail(MWG_ERR_ARG, "null argument");
+    if (op->n_mat % 4) return fail(MWG_ERR_ARG, "matrix count is not a multiple of 4");
+    if (precision != MWG_PREC_FP64 && precision != MWG_PREC_FP32 && precision != MWG_PREC_BF16X3)
+        return fail(MWG_ERR_UNSUPPORTED, "no operand table for this precision");
+    return dispatch<FragL>(op->kp1, op, precision, dst);
+}
 
 extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs, int npairs,
                                      unsigned long long *n_applied, int precision) {
@@ -460,9 +839,10 @@ extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, c
     if (!lvl || !f || !op || !pairs || !n_applied) return fail(MWG_ERR_ARG, "null argument");
     if (npairs < 0) return fail(MWG_ERR_ARG, "negative pair count");
     if (lvl->count <= 0) return MWG_OK;
-    if (precision != 0) return fail(MWG_ERR_UNSUPPORTED, "only FP64 sweeps are available");
+    if (precision < MWG_PREC_FP64 || precision > kPrecValu) return fail(MWG_ERR_UNSUPPORTED, "unsupported sweep precision");
+    if (npairs > 0 && precision != kPrecValu && !op->frag[precision]) return fail(MWG_ERR_ARG, "operand table of this precision not built");
     if (op->M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 512");
     if (op->kp1 != f->kp1 || lvl->kp1 != f->kp1) return fail(MWG_ERR_ARG, "operator, level and tree orders differ");
     if (npairs > pairs->capacity) return fail(MWG_ERR_ARG, "more pairs than the pair arrays hold");
-    return dispatch<BandL>(f->kp1, lvl, f, op, pairs, npairs, n_applied);
+    return dispatch<BandL>(f->kp1, lvl, f, op, pairs, npairs, n_applied, precision);
 }

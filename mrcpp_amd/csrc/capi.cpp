@@ -161,6 +161,15 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
     return rc < 0 ? -1 : n;
 }
 
+int mrcpp_set_sweep_precision(int precision) {
+    return guarded([&] {
+        if (precision < 0 || precision > 3) throw Error("sweep precision must be 0 (FP64), 1 (FP32), 2 (BF16x3) or 3 (FP64 vector)");
+        set_sweep_precision(static_cast<SweepPrecision>(precision));
+    });
+}
+
+int mrcpp_get_sweep_precision(void) { return (int)sweep_precision(); }
+
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats) {
     return guarded([&] {
         apply(prec, *static_cast<FunctionTree<3> *>(out), *static_cast<ConvolutionOperator<3> *>(oper), *static_cast<FunctionTree<3> *>(inp),

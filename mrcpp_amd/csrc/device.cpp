@@ -58,6 +58,7 @@ detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
     v.pair_bound_w = d->pairBoundW.get();
     v.band_size_max = d->bandSizeMax.get();
     v.term_norm = d->termNorm.get();
+    v.n_mat = (long long)t.matNorm.size();
     return d;
 }
 
@@ -164,4 +165,20 @@ mwg_ftree TreeMirror::view(const MultiResolutionAnalysis<3> &mra) const {
 }
 
 } // namespace device
+
+namespace detail {
+
+const mwg_band_tables &DeviceBandTables::viewFor(int precision) {
+    if (precision != MWG_PREC_FP64 && precision != MWG_PREC_FP32 && precision != MWG_PREC_BF16X3) return view;
+    if (!view.frag[precision] && view.n_mat > 0) {
+        const size_t bytes = mwg_fragment_bytes(view.kp1, view.n_mat, precision);
+        if (bytes == 0) throw Error("no matrix-core operand table for this sweep precision");
+        frag[precision].reserve(bytes);
+        device::check(mwg_build_fragments(&view, precision, frag[precision].get()), "operand table");
+        view.frag[precision] = frag[precision].get();
+    }
+    return view;
+}
+
+} // namespace detail
 } // namespace mrcpp

@@ -44,8 +44,9 @@ public:
         ptr_ = static_cast<T *>(p);
         cap_ = nc;
     }
+    // enqueued without waiting; src may be reused on return
     void upload(const T *src, size_t n, size_t at = 0) {
-        if (n) check(mwg_upload(ptr_ + at, src, n * sizeof(T)), "upload");
+        if (n) check(mwg_upload_async(ptr_ + at, src, n * sizeof(T)), "upload");
     }
     void download(T *dst, size_t n, size_t at = 0) const {
         if (n) check(mwg_download(dst, ptr_ + at, n * sizeof(T)), "download");
@@ -72,7 +73,10 @@ struct DeviceBandTables {
     device::DeviceArray<long long> matStart;
     device::DeviceArray<double> mats, matNorm, pairBound, pairBoundW, termNorm;
     device::DeviceArray<int> bandSizeMax;
+    device::DeviceArray<unsigned char> frag[3]; // MFMA operand tables, built on first use
     mwg_band_tables view{};
+    // view with the operand table of this sweep precision built
+    const mwg_band_tables &viewFor(int precision);
 };
 } // namespace detail
 

@@ -74,8 +74,14 @@ struct ApplyStats {
 };
 const ApplyStats &last_apply_stats();
 
-// Precision of the band sweeps (FP64 is the drop-in default).
-enum class SweepPrecision { FP64 = 0 };
+// Precision of the band sweeps (FP64 is the drop-in default).  FP32 and
+// BF16x3 run the sweep products on the matrix cores in lower precision with
+// the band sum accumulated in FP64; the screening, operator construction and
+// transforms stay FP64.  FP64Vector runs the FP64 sweeps on the vector ALUs.
+enum class SweepPrecision { FP64 = 0, FP32 = 1, BF16x3 = 2, FP64Vector = 3 };
+// Sweep precision used by apply() on this thread.
+void set_sweep_precision(SweepPrecision p);
+SweepPrecision sweep_precision();
 
 // apply(prec, out, oper, inp, maxIter, absPrec) (src/treebuilders/apply.cpp:68-93),
 // the band sum running on the MI355X.

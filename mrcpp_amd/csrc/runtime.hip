@@ -1,5 +1,6 @@
 // Device runtime of the mwg_* C ABI: device selection, the pooled allocator,
 // transfers and per-kernel timing with HIP events.
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -60,6 +61,13 @@ std::map<void *, size_t> &block_size() {
     static std::map<void *, size_t> m;
     return m;
 }
+
+// Pinned staging ring of asynchronous uploads.  Space is reused after the
+// null stream has drained (any synchronising call, or when the ring is full).
+constexpr size_t kStageBytes = 64ull << 20;
+constexpr size_t kStageMaxCopy = 16ull << 20;
+unsigned char *g_stage = nullptr;
+size_t g_stage_used = 0;
 
 } // namespace
 
@@ -173,12 +181,15 @@ mwg_status mwg_profile(int enable) {
     MWG_HIP(hipDeviceSynchronize());
     recycle_events();
     g_profile = enable != 0;
-    if (g_profile) // enough events for a few thousand launches up front
+    if (g_profile) { // enough events for a few thousand launches, each recorded once so first use costs nothing
         while (event_pool().size() < 8192) {
             hipEvent_t e{};
             MWG_HIP(hipEventCreate(&e));
+            MWG_HIP(hipEventRecord(e, 0));
             event_pool().push_back(e);
         }
+        MWG_HIP(hipDeviceSynchronize());
+    }
     return MWG_OK;
 }
 
@@ -215,10 +226,28 @@ mwg_status mwg_upload(void *dst, const void *src, size_t bytes) {
     return MWG_OK;
 }
 
+mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes) {
+    if (!bytes) return MWG_OK;
+    if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
+    if (bytes > kStageMaxCopy) return mwg_upload(dst, src, bytes);
+    if (!g_stage) MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_stage), kStageBytes, hipHostMallocDefault));
+    const size_t at = (g_stage_used + 255) & ~size_t(255);
+    if (at + bytes > kStageBytes) {
+        MWG_HIP(hipStreamSynchronize(0));
+        g_stage_used = 0;
+        return mwg_upload_async(dst, src, bytes);
+    }
+    std::memcpy(g_stage + at, src, bytes);
+    MWG_HIP(hipMemcpyAsync(dst, g_stage + at, bytes, hipMemcpyHostToDevice, 0));
+    g_stage_used = at + bytes;
+    return MWG_OK;
+}
+
 mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
     if (!bytes) return MWG_OK;
     if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
     MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    g_stage_used = 0; // the null stream has drained
     return MWG_OK;
 }
 
@@ -238,6 +267,7 @@ mwg_status mwg_zero(void *dst, size_t bytes) {
 
 mwg_status mwg_synchronize(void) {
     MWG_HIP(hipDeviceSynchronize());
+    g_stage_used = 0;
     return MWG_OK;
 }
 

@@ -150,6 +150,8 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(mwg_level l
     if (threadIdx.x == 0) blk_count[blk] = total;
 }
 
+__global__ void k_screen_reset(int *count) { count[1] = 0; }
+
 // Place the survivors of every block at their scanned offset.
 __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(mwg_level lvl, mwg_ftree f, mwg_band_tables op, int nb, const int *cand_pos,
                                                                   const int *cand_code, const int *blk_off, mwg_pairs out) {
@@ -163,9 +165,11 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(mwg_level lvl
         const Box bx = band_box(f, d, op.band_max[d], gl);
         const int p = cand_pos[(size_t)blk * kScreenThreads + threadIdx.x];
         const int q = base + threadIdx.x;
+        const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
+        if (code < 0) atomicAdd(out.count + 1, 1);
         if (q < out.capacity) {
             out.pair_g[q] = g;
-            out.pair_f[q] = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
+            out.pair_f[q] = code;
             out.pair_l[3 * q + 0] = bx.s[0] + p % bx.n[0];
             out.pair_l[3 * q + 1] = bx.s[1] + (p / bx.n[0]) % bx.n[1];
             out.pair_l[3 * q + 2] = bx.s[2] + p / (bx.n[0] * bx.n[1]);
@@ -198,7 +202,7 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, 
     if (lvl->o_depth > mwgkey::kMaxDepth) return fail(MWG_ERR_UNSUPPORTED, "level deeper than the device key range");
     if (lvl->band_w < 0) return fail(MWG_ERR_ARG, "negative band width");
     if (lvl->count <= 0) {
-        MWG_HIP(hipMemset(out->count, 0, sizeof(int)));
+        MWG_HIP(hipMemset(out->count, 0, 2 * sizeof(int)));
         return MWG_OK;
     }
     const long long W = lvl->band_w;
@@ -210,6 +214,8 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, 
     if (g_cand_code.ensure(sizeof(int) * (size_t)nblk * kScreenThreads) != MWG_OK) return MWG_ERR_NOMEM;
     if (g_blk.ensure(sizeof(int) * ((size_t)nblk + 1)) != MWG_OK) return MWG_ERR_NOMEM;
     ProfileScope prof_(MWG_KERNEL_SCREEN);
+    hipLaunchKernelGGL(k_screen_reset, dim3(1), dim3(1), 0, 0, out->count);
+    MWG_LAUNCHED();
     hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, *lvl, *f, *op, (int)nb, g_cand_pos.as<int>(),
                        g_cand_code.as<int>(), g_blk.as<int>());
     MWG_LAUNCHED();
