@@ -136,6 +136,103 @@ def precision_error(mr, w, mra, f, P, precision):
     return {"rel_l2": rel, "same_node_set": same, "nodes_fp64": int(len(i0)), "nodes": int(len(i1))}
 
 
+def setup_batch(mr, W):
+    """The 64-tree batch: uneven Gaussian sums, precisions 1e-5..1e-7, half
+    Poisson and half Helmholtz (one operator per distinct (mu, prec))."""
+    specs = W.batch_sources()
+    mra = mr.MultiResolutionAnalysis((-32, 32), 7)
+    ops, trees = {}, []
+    for sp in specs:
+        f = mr.FunctionTree(mra)
+        g = mr.GaussExp.from_sources(sp["sources"])
+        mr.build_grid(f, g)
+        mr.project(sp["prec"], f, g)
+        key = (sp["mu"], sp["prec"])
+        if key not in ops:
+            ops[key] = mr.PoissonOperator(mra, sp["prec"]) if sp["mu"] is None else mr.HelmholtzOperator(mra, sp["mu"], sp["prec"])
+        trees.append((sp["prec"], ops[key], f))
+    return mra, trees, specs
+
+
+def run_batch_bench(args):
+    """Chosen form (one apply_batch of all trees at --precision) against the
+    plain form (one FP64 apply per tree), same trees; one step = all trees."""
+    import numpy as np
+    import mrcpp_amd as mr
+    from mrcpp_amd import workloads as W
+    mra, trees, specs = setup_batch(mr, W)
+    n = len(trees)
+
+    def plain_step():
+        mr.set_sweep_precision("fp64")
+        outs, sts = [], []
+        for prec, P, f in trees:
+            g = mr.FunctionTree(mra)
+            sts.append(mr.apply(prec, g, P, f))
+            outs.append(g)
+        mr.device_synchronize()
+        return outs, sts
+
+    def chosen_step():
+        mr.set_sweep_precision(args.precision)
+        outs = [mr.FunctionTree(mra) for _ in trees]
+        sts = mr.apply_batch([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)])
+        mr.device_synchronize()
+        return outs, sts
+
+    def timed(step):
+        for _ in range(max(args.warmup, 1)):
+            step()
+        times = []
+        for _ in range(args.steps):
+            t0 = time.perf_counter()
+            outs, sts = step()
+            times.append(time.perf_counter() - t0)
+        return outs, sts, times
+
+    p_outs, p_sts, p_times = timed(plain_step)
+    mr.profile(True)
+    c_outs, c_sts, c_times = timed(chosen_step)
+    prof = mr.profile_read()
+    mr.profile(False)
+    flop = sum(st["n_applied"] for st in p_sts) * flops_per_tuple(7)
+    c_s = float(np.mean(c_times))
+    p_s = float(np.mean(p_times))
+    # error of the chosen form against the plain FP64 trees, and node-set identity
+    num = den = 0.0
+    same = 0
+    for a, b in zip(p_outs, c_outs):
+        ai, ac, _ = a.export()
+        bi, bc, _ = b.export()
+        if np.array_equal(ai, bi):
+            same += 1
+            num += float(np.sum((bc - ac) ** 2))
+            den += float(np.sum(ac ** 2))
+    band_ms, band_launches = prof["band"]
+    band_ms_per_step = band_ms / (args.steps + max(args.warmup, 1))
+    c_flop = sum(st["n_applied"] for st in c_sts) * flops_per_tuple(7)
+    res = {
+        "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64" if args.precision in ("fp64", "fp64_vector") else args.precision + " sweeps, f64 band sum",
+        "precision": args.precision, "data": "synthetic seeded Gaussian sources",
+        "config": {"workload": "batch64_k7", "trees": n, "order": 7, "box": [-32, 32], "precs": sorted({sp["prec"] for sp in specs}),
+                   "n_gaussians": [len(sp["sources"]) for sp in specs], "operators": "Poisson (even) / Helmholtz mu~U[0.5,3] (odd)",
+                   "n_applied": int(sum(st["n_applied"] for st in p_sts)), "parallelism": "single GPU"},
+        "plain_fp64_sequential": {"ms_per_step": round(p_s * 1e3, 4), "s_per_tree": p_s / n, "gflops": round(flop / p_s / 1e9, 3)},
+        "speedup_vs_plain": round(p_s / c_s, 3),
+        "error_vs_plain_fp64": {"rel_l2_same_trees": (num / den) ** 0.5 if den > 0 else None, "same_node_set_trees": same, "trees": n,
+                                "applied_chosen": int(sum(st["n_applied"] for st in c_sts))},
+        "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(c_flop / (band_ms_per_step / 1e3) / 1e12, 4),
+                     "peak": FP64_PEAK_TFLOPS if args.precision in ("fp64", "fp64_vector") else 157.3, "unit": "TFLOP/s",
+                     "launches_per_step": band_launches / (args.steps + max(args.warmup, 1))},
+        "kernel_ms_per_step": {k: round(v[0] / (args.steps + max(args.warmup, 1)), 4) for k, v in prof.items()},
+        "step_times_s": [round(t, 6) for t in c_times],
+    }
+    res["roofline"]["frac"] = round(res["roofline"]["achieved"] / res["roofline"]["peak"], 5)
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +246,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe()
+        return
+    if args.workload == "batch64_k7":
+        run_batch_bench(args)
         return
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "

@@ -70,6 +70,42 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
 int mrcpp_set_sweep_precision(int precision);
 int mrcpp_get_sweep_precision(void);
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats);
+/* n applications at once (no reference counterpart; same result per tree as
+ * mrcpp_apply): arrays of n arguments, max_iter / abs_prec may be NULL
+ * (-1 / 0), stats (may be NULL) receives 16 doubles per task as above, the
+ * seconds entry being the whole batch's wall time. */
+int mrcpp_apply_batch(int n, const double *prec, void *const *out, void *const *oper, void *const *inp, const int *max_iter,
+                      const int *abs_prec, double *stats);
+
+/* Refinement predictor (no reference counterpart): a 12-64-64-1 perceptron
+ * on the device predicting which output nodes the adaptive apply refines.
+ * Parameters are MWG_MLP_PARAMS (5057) floats; see include/mwg.h. */
+int mrcpp_predictor_create(void **pred);
+int mrcpp_predictor_destroy(void *pred);
+int mrcpp_predictor_num_params(void);
+int mrcpp_predictor_set_params(void *pred, const float *params, int n);
+int mrcpp_predictor_get_params(void *pred, float *params, int n);
+/* refine a node when its predicted probability exceeds threshold */
+int mrcpp_predictor_set_threshold(void *pred, double threshold);
+/* prob[i] for features[n][12] (host arrays) */
+int mrcpp_predictor_predict(void *pred, const float *features, int n, float *prob);
+/* Adam on minibatches of the weighted cross-entropy, on the device; losses
+ * (may be NULL) receives the mean loss after every epoch */
+int mrcpp_predictor_train(void *pred, const float *features, const float *labels, int n, int epochs, int batch, double lr,
+                          double pos_weight, unsigned long long seed, float *losses);
+/* mrcpp_apply_batch with the predictor growing every output grid after the
+ * first step (predictor may be NULL: plain).  centers: the source centres of
+ * all tasks back to back (3 doubles each), ncenters[i] of task i (both may
+ * be NULL).  collect != 0 gathers training samples (features, refined) of
+ * every output node beyond the first step, read with mrcpp_samples_*.
+ * stats: 16 doubles per task as mrcpp_apply, entries 12-14 = nodes
+ * predicted, predicted nodes removed by the wavelet test, nodes computed in
+ * fallback passes. */
+int mrcpp_apply_batch_predicted(int n, const double *prec, void *const *out, void *const *oper, void *const *inp, void *predictor,
+                                const double *centers, const int *ncenters, int collect, double *stats);
+int mrcpp_samples_count(void);
+/* copies the samples of the last collecting call; returns their number, -1 if cap is too small */
+int mrcpp_samples_fetch(float *features, float *labels, int cap);
 /* nodes per breadth-first level of the last apply; returns the number of levels */
 int mrcpp_last_apply_levels(int *level_nodes, int cap);
 

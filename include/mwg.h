@@ -127,19 +127,33 @@ typedef struct {
     const unsigned char *is_gen;         /* [nodes] */
 } mwg_ftree;
 
-/* One breadth-first level of output nodes, all at the same scale.
- * Output coefficients go to g_coef + g_off[i]; norms to g_norms + 9 i. */
+/* One run of output nodes of a level that share a tree, a scale and an
+ * operator: nodes [first, first + count) of the level.  Trees of different
+ * sizes, operators and precisions are batched by giving each its segments. */
 typedef struct {
-    int kp1;             /* k+1 of the output coefficients */
-    int count;
-    int scale;
+    int first, count;
+    int scale;           /* scale of the nodes */
     int o_depth;         /* scale - operator root scale */
-    double g_thr;        /* screening threshold (<= 0 disables screening) */
-    int band_w;          /* band half-width at this depth: band_max[o_depth] of the tables */
-    const int *l;        /* [count][3] translations */
-    double *g_coef;
-    const long long *g_off; /* [count] */
-    double *g_norms;     /* [count][9] */
+    int band_w;          /* band half-width at o_depth (band_max[o_depth] of op), -1 for an empty band */
+    double g_thr;        /* screening threshold of the tree (<= 0 disables screening) */
+    mwg_ftree f;         /* the tree's input */
+    mwg_band_tables op;  /* its operator's tables */
+    double *g_coef;      /* the tree's output coefficient pool */
+    unsigned long long *n_applied; /* device counter the segment's applied (f, ft, gt, term) tuples are added to */
+} mwg_segment;
+
+/* One breadth-first level: count output nodes, ordered by segment.  segs is
+ * HOST memory (the library copies it per call); the arrays are device
+ * memory.  Output coefficients go to segs[s].g_coef + g_off[i] for the
+ * segment s of node i; norms to g_norms + 9 i. */
+typedef struct {
+    int kp1;             /* k+1 of every tree in the level */
+    int count;
+    int nseg;
+    const mwg_segment *segs; /* host [nseg], first/count partition [0, count) in order */
+    const int *l;            /* [count][3] translations */
+    const long long *g_off;  /* [count] */
+    double *g_norms;         /* [count][9] */
 } mwg_level;
 
 /* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the
@@ -148,7 +162,8 @@ typedef struct {
  * (MWTree::getNode -> MWNode::retrieveNode, src/trees/MWNode.cpp:1097-1120). */
 typedef struct {
     int capacity;
-    int *count;          /* device int[2]: pairs written/needed, pairs whose band node is missing */
+    int *count;          /* device int[3]: pairs written/needed, pairs whose band node is missing,
+                            nonzero if a segment's band_w is not its tables' band width */
     int *pair_g;         /* [capacity] output-node index in the level */
     int *pair_f;         /* [capacity] */
     int *pair_l;         /* [capacity][3] band node translation */
@@ -164,9 +179,8 @@ typedef struct {
  * components; neither can reject a tuple the exact test would apply.  The
  * pairs of one output node are contiguous and in band order (x fastest);
  * *count receives the number of pairs (pairs beyond capacity are counted but
- * not written: grow the arrays and screen again), or -1 when lvl->band_w is
- * not the tables' band width. */
-mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out);
+ * not written: grow the arrays and screen again). */
+mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
 
 /* Generate scaling-only band nodes below existing parents: node ids[i] gets
  * the scaling block of child child_idx[i] of node parent[i], written at
@@ -182,11 +196,11 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
  * tensorApplyOperComp, src/treebuilders/ConvolutionCalculator.cpp:224-382;
  * MWNode::calcNorms, src/trees/MWNode.cpp:609-655).  Every pair_f must be a
  * node id (missing band nodes generated first).  Adds the number of applied
- * (f, ft, gt, term) tuples to *n_applied.  Summation order is fixed by the
- * pair order, so results are reproducible.  precision: MWG_PREC_*; the
- * table op->frag[precision] must be built. */
-mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs,
-                          int npairs, unsigned long long *n_applied, int precision);
+ * (f, ft, gt, term) tuples to each segment's n_applied.  Summation order is fixed by the
+ * pair order, so results are reproducible.  precision: MWG_PREC_*; every
+ * segment's op.frag[precision] must be built.  pair_f of a pair refers to the
+ * input tree of its output node's segment. */
+mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_pairs *pairs, int npairs, int precision);
 
 /* Add each parent's reconstructed s/d into the scaling block of its 8 children
  * (MWTree::mwTransform(TopDown, false) -> tree_utils::mw_transform,
@@ -209,6 +223,23 @@ mwg_status mwg_node_norms(int kp1, const double *coef, const long long *off, dou
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;
  * returns a device pointer (MWFilter::getSubFilter, src/core/MWFilter.cpp:100-133). */
 mwg_status mwg_filters_upload(int kp1, const double *host_filters, double **dev_filters);
+
+/* Refinement predictor (no reference counterpart): a 12 -> 64 -> 64 -> 1
+ * ReLU perceptron in FP32 that predicts, per output node, whether the
+ * adaptive apply refines it.  params: MWG_MLP_PARAMS floats laid out
+ * W1[12][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3.  features:
+ * [n][12] floats; labels: [n] floats in {0, 1}.  All device memory. */
+#define MWG_MLP_FEATURES 12
+#define MWG_MLP_HIDDEN 64
+#define MWG_MLP_PARAMS (MWG_MLP_FEATURES * 64 + 64 + 64 * 64 + 64 + 64 + 1)
+/* prob[i] = sigmoid(logit of sample i) */
+mwg_status mwg_mlp_forward(const float *params, const float *features, int n, float *prob);
+/* grad[0..PARAMS) = gradient of the mean loss
+ * pos_weight * y * softplus(-z) + (1 - y) * softplus(z); grad[PARAMS] = the
+ * mean loss.  grad holds MWG_MLP_PARAMS + 1 floats.  Deterministic. */
+mwg_status mwg_mlp_gradient(const float *params, const float *features, const float *labels, int n, float pos_weight, float *grad);
+/* One Adam step (step counts from 1) on params with first/second moments m, v. */
+mwg_status mwg_mlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr, float beta1, float beta2, float eps);
 
 #ifdef __cplusplus
 }

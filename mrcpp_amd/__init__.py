@@ -56,6 +56,21 @@ SIGNATURES = {
     "mrcpp_kernel_terms": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _dp, _dp, ctypes.c_int]),
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "mrcpp_apply_batch": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]),
+    "mrcpp_apply_batch_predicted": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
+                                                   _dp, ctypes.POINTER(ctypes.c_int), ctypes.c_int, _dp]),
+    "mrcpp_predictor_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "mrcpp_predictor_destroy": (ctypes.c_int, [_vp]),
+    "mrcpp_predictor_num_params": (ctypes.c_int, []),
+    "mrcpp_predictor_set_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "mrcpp_predictor_get_params": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
+    "mrcpp_predictor_set_threshold": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mrcpp_predictor_predict": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.POINTER(ctypes.c_float)]),
+    "mrcpp_predictor_train": (ctypes.c_int, [_vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float)]),
+    "mrcpp_samples_count": (ctypes.c_int, []),
+    "mrcpp_samples_fetch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
     "mrcpp_set_sweep_precision": (ctypes.c_int, [ctypes.c_int]),
     "mrcpp_get_sweep_precision": (ctypes.c_int, []),
     "mrcpp_last_apply_levels": (ctypes.c_int, [_ip, ctypes.c_int]),
@@ -66,6 +81,14 @@ SIGNATURES = {
     "mwg_profile": (ctypes.c_int, [ctypes.c_int]),
     "mwg_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "mwg_release_pool": (ctypes.c_int, []),
+    "mwg_init": (ctypes.c_int, [ctypes.c_int]),
+    "mwg_malloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "mwg_free": (ctypes.c_int, [_vp]),
+    "mwg_upload": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "mwg_download": (ctypes.c_int, [_vp, _vp, ctypes.c_size_t]),
+    "mwg_mlp_forward": (ctypes.c_int, [_vp, _vp, ctypes.c_int, _vp]),
+    "mwg_mlp_gradient": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int, ctypes.c_float, _vp]),
+    "mwg_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
 }
 
 KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups")
@@ -315,6 +338,121 @@ def set_sweep_precision(name):
 def sweep_precision():
     code = lib().mrcpp_get_sweep_precision()
     return {v: k for k, v in SWEEP_PRECISIONS.items()}[code]
+
+
+_fp = ctypes.POINTER(ctypes.c_float)
+PREDICTOR_FEATURES = 12
+
+
+class RefinePredictor:
+    """12-64-64-1 perceptron (FP32, on the device) predicting which output
+    nodes the adaptive apply refines; apply(..., predictor=P) grows the output
+    grid with it after the first step, the wavelet test staying the judge."""
+
+    def __init__(self, params=None, threshold=None):
+        h = _vp()
+        _check(lib().mrcpp_predictor_create(ctypes.byref(h)))
+        self._h = h
+        if params is not None:
+            self.params = params
+        if threshold is not None:
+            self.threshold = threshold
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.mrcpp_predictor_destroy(self._h)
+            self._h = None
+
+    @property
+    def params(self):
+        p = np.zeros(lib().mrcpp_predictor_num_params(), dtype=np.float32)
+        _check(lib().mrcpp_predictor_get_params(self._h, _p(p, _fp), len(p)))
+        return p
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().mrcpp_predictor_set_params(self._h, _p(p, _fp), len(p)))
+
+    @property
+    def threshold(self):
+        return getattr(self, "_thr", 0.35)
+
+    @threshold.setter
+    def threshold(self, t):
+        _check(lib().mrcpp_predictor_set_threshold(self._h, float(t)))
+        self._thr = float(t)
+
+    def predict(self, features):
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PREDICTOR_FEATURES)
+        out = np.zeros(len(x), dtype=np.float32)
+        _check(lib().mrcpp_predictor_predict(self._h, _p(x, _fp), len(x), _p(out, _fp)))
+        return out
+
+    def train(self, features, labels, epochs=20, batch=4096, lr=2e-3, pos_weight=3.0, seed=1):
+        """Adam on the device; returns the loss after every epoch."""
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PREDICTOR_FEATURES)
+        y = np.ascontiguousarray(labels, dtype=np.float32)
+        losses = np.zeros(epochs, dtype=np.float32)
+        _check(lib().mrcpp_predictor_train(self._h, _p(x, _fp), _p(y, _fp), len(y), int(epochs), int(batch), float(lr), float(pos_weight),
+                                           int(seed), _p(losses, _fp)))
+        return losses
+
+
+def _batch_args(tasks):
+    n = len(tasks)
+    prec = np.array([float(t[0]) for t in tasks], dtype=np.float64)
+    outs = (_vp * n)(*[t[1]._h for t in tasks])
+    opers = (_vp * n)(*[t[2]._h for t in tasks])
+    inps = (_vp * n)(*[t[3]._h for t in tasks])
+    return n, prec, outs, opers, inps
+
+
+def apply_predicted(tasks, predictor=None, centers=None, collect=False):
+    """apply_batch with a refinement predictor (None: plain) and per-task
+    source centres (a list of [k, 3] arrays or None).  collect=True also
+    returns the training samples (features [N, 12], labels [N]) of every
+    output node beyond the first step.  Statistics include 'n_predicted',
+    'n_cropped' and 'n_fallback'."""
+    n, prec, outs, opers, inps = _batch_args(tasks)
+    cs = [np.zeros((0, 3)) if centers is None or centers[i] is None else np.asarray(centers[i], dtype=np.float64).reshape(-1, 3)
+          for i in range(n)]
+    cflat = np.ascontiguousarray(np.concatenate(cs) if n else np.zeros((0, 3)), dtype=np.float64)
+    ncen = np.array([len(c) for c in cs], dtype=np.int32)
+    st = np.zeros(16 * max(n, 1))
+    _check(lib().mrcpp_apply_batch_predicted(n, _p(prec), outs, opers, inps, predictor._h if predictor is not None else None,
+                                             _p(cflat), _p(ncen, _ip), int(bool(collect)), _p(st)))
+    stats = []
+    for i in range(n):
+        d = {k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])}
+        d.update(n_predicted=st[16 * i + 12], n_cropped=st[16 * i + 13], n_fallback=st[16 * i + 14])
+        stats.append(d)
+    if not collect:
+        return stats
+    m = lib().mrcpp_samples_count()
+    x = np.zeros((m, PREDICTOR_FEATURES), dtype=np.float32)
+    y = np.zeros(m, dtype=np.float32)
+    if lib().mrcpp_samples_fetch(_p(x, _fp), _p(y, _fp), m) != m:
+        raise MRCPPError("sample fetch failed")
+    return stats, x, y
+
+
+def apply_batch(tasks):
+    """Many applications at once: tasks is a sequence of (prec, out, oper, inp)
+    or (prec, out, oper, inp, maxIter, absPrec).  Every breadth-first step of
+    all trees shares one screen and one band launch; each output tree equals
+    what apply() alone produces.  Returns one statistics dict per task
+    ('seconds' is the whole batch's wall time)."""
+    n = len(tasks)
+    prec = np.array([float(t[0]) for t in tasks], dtype=np.float64)
+    outs = (_vp * n)(*[t[1]._h for t in tasks])
+    opers = (_vp * n)(*[t[2]._h for t in tasks])
+    inps = (_vp * n)(*[t[3]._h for t in tasks])
+    mi = np.array([int(t[4]) if len(t) > 4 else -1 for t in tasks], dtype=np.int32)
+    ap = np.array([int(bool(t[5])) if len(t) > 5 else 0 for t in tasks], dtype=np.int32)
+    st = np.zeros(16 * max(n, 1))
+    _check(lib().mrcpp_apply_batch(n, _p(prec), outs, opers, inps, _p(mi, _ip), _p(ap, _ip), _p(st)))
+    return [{k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])} for i in range(n)]
 
 
 def apply(prec, out, oper, inp, maxIter=-1, absPrec=False):

@@ -2,20 +2,28 @@
 // sum, generated band nodes and final multiwavelet transforms on the MI355X.
 // The host keeps the tree topology and the split decisions
 // (src/treebuilders/apply.cpp:68-93, src/treebuilders/TreeBuilder.cpp:38-87).
+//
+// apply_batch() runs many applications together: each breadth-first step
+// gathers the work nodes of every unfinished tree into one level of
+// (tree, scale) segments, so one screen and one band launch serve all trees.
+// A tree's result does not depend on the other trees of the batch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 
 #include "function_tree.h"
 #include "keys.h"
+#include "predictor.h"
 
 namespace mrcpp {
 
 namespace {
 
 thread_local ApplyStats g_stats;
+thread_local std::vector<ApplyStats> g_batch_stats;
 thread_local SweepPrecision g_sweep = SweepPrecision::FP64;
 
 using clk = std::chrono::steady_clock;
@@ -34,40 +42,55 @@ Scratch &scratch() {
     return s;
 }
 
-// Device bytes per batch of output nodes for the screen candidates and the
-// per-pair group lists.
+// Device bytes per level batch for the screen candidates and the per-pair
+// group lists.
 constexpr double kBatchBudget = 16.0e9;
 
-struct LevelContext {
-    const MultiResolutionAnalysis<3> &mra;
-    ConvolutionOperator<3> &oper;
-    const detail::BandTables &ht;
-    const mwg_band_tables &bt;
-    device::TreeMirror &fm;
-    detail::NodeStore<3> *fstore;
-    FunctionTree<3> &out;
-    const double *filters;
+// One tree of a batch and its state across breadth-first steps.
+enum class Mode { Plain, Replay };
+
+struct Job {
+    ApplyTask task;
+    const MultiResolutionAnalysis<3> *mra = nullptr;
+    const detail::BandTables *ht = nullptr;
+    const mwg_band_tables *bt = nullptr;
+    device::TreeMirror *fm = nullptr;
+    detail::NodeStore<3> *fstore = nullptr;
+    detail::NodeStore<3> *g = nullptr;
+    const double *filters = nullptr;
+    std::vector<int> work;
+    double sNorm = 0.0, wNorm = 0.0, gthr = -1.0;
+    int iter = 0;
+    ApplyStats stats;
+    // refinement predictor
+    Mode mode = Mode::Plain;
+    PredictorContext ctx;
+    std::vector<int> replayWork; // nodes of the breadth-first step being judged (computed)
+};
+
+struct Shape {
     int kp1, n3, ncoef;
 };
 
-// Extend the input mirror with the generated nodes some candidate pairs need
-// and patch those pairs to point at them.
-void generate_band_nodes(LevelContext &c, std::vector<int> &pairF, const std::vector<int> &pairL, int scale) {
-    device::TreeMirror &fm = c.fm;
+// Extend a tree's input mirror with the generated nodes some pairs need and
+// patch those pairs (pf[i] < 0, band translation pl, target scale sc[i]).
+void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const std::vector<int> &pl, const std::vector<int> &sc) {
+    device::TreeMirror &fm = *job.fm;
     struct NewNode {
         int id, parent, child, scale;
     };
     std::vector<NewNode> made;
-    for (size_t p = 0; p < pairF.size(); p++) {
-        if (pairF[p] >= 0) continue;
-        const int anc = -pairF[p] - 1;
-        const std::array<int, 3> l = {pairL[3 * p], pairL[3 * p + 1], pairL[3 * p + 2]};
+    for (size_t p = 0; p < pf.size(); p++) {
+        if (pf[p] >= 0) continue;
+        const int anc = -pf[p] - 1;
+        const int scale = sc[p];
+        const std::array<int, 3> l = {pl[3 * p], pl[3 * p + 1], pl[3 * p + 2]};
         int prev = anc;
-        const int prevScale = (anc < fm.nReal) ? c.fstore->scale(anc) : fm.genScale[anc - fm.nReal];
+        const int prevScale = (anc < fm.nReal) ? job.fstore->scale(anc) : fm.genScale[anc - fm.nReal];
         for (int s = prevScale + 1; s <= scale; s++) {
             std::array<int, 3> ls;
             for (int d = 0; d < 3; d++) ls[d] = l[d] >> (scale - s);
-            const uint64_t key = device::key_of(c.mra, s, ls);
+            const uint64_t key = device::key_of(*job.mra, s, ls);
             int id = fm.table.find(key);
             if (id < 0) {
                 id = fm.nReal + fm.nGen;
@@ -82,62 +105,91 @@ void generate_band_nodes(LevelContext &c, std::vector<int> &pairF, const std::ve
             }
             prev = id;
         }
-        pairF[p] = prev;
+        pf[p] = prev;
     }
     if (made.empty()) return;
     // device storage for the new nodes
     const int total = fm.nReal + fm.nGen;
     const int firstNew = total - (int)made.size();
     fm.hostOff.resize(total);
-    for (const auto &m : made) fm.hostOff[m.id] = fm.genBase + (long long)(m.id - fm.nReal) * c.n3;
+    for (const auto &m : made) fm.hostOff[m.id] = fm.genBase + (long long)(m.id - fm.nReal) * sh.n3;
     fm.coefOff.reserve(total, firstNew);
     fm.coefOff.upload(fm.hostOff.data() + firstNew, made.size(), firstNew);
     fm.norms.reserve((size_t)total * 9, (size_t)firstNew * 9);
     fm.isGen.reserve(total, firstNew);
     std::vector<unsigned char> ones(made.size(), 1);
     fm.isGen.upload(ones.data(), ones.size(), firstNew);
-    fm.coef.reserve((size_t)fm.genBase + (size_t)fm.nGen * c.n3, (size_t)fm.genBase + (size_t)(fm.nGen - made.size()) * c.n3);
+    fm.coef.reserve((size_t)fm.genBase + (size_t)fm.nGen * sh.n3, (size_t)fm.genBase + (size_t)(fm.nGen - made.size()) * sh.n3);
     fm.uploadTable();
     // one launch per scale, coarse to fine
     std::stable_sort(made.begin(), made.end(), [](const NewNode &a, const NewNode &b) { return a.scale < b.scale; });
+    std::vector<int> ids, par, ch;
+    for (const auto &m : made) {
+        ids.push_back(m.id);
+        par.push_back(m.parent);
+        ch.push_back(m.child);
+    }
     Scratch &S = scratch();
+    S.genIds.reserve(ids.size());
+    S.genParent.reserve(ids.size());
+    S.genChild.reserve(ids.size());
+    S.genIds.upload(ids.data(), ids.size());
+    S.genParent.upload(par.data(), par.size());
+    S.genChild.upload(ch.data(), ch.size());
+    const mwg_ftree fv = fm.view(*job.mra);
     size_t i = 0;
-    const mwg_ftree fv = fm.view(c.mra);
     while (i < made.size()) {
         size_t j = i;
         while (j < made.size() && made[j].scale == made[i].scale) j++;
-        std::vector<int> ids, par, ch;
-        for (size_t q = i; q < j; q++) {
-            ids.push_back(made[q].id);
-            par.push_back(made[q].parent);
-            ch.push_back(made[q].child);
-        }
-        S.genIds.reserve(ids.size());
-        S.genParent.reserve(ids.size());
-        S.genChild.reserve(ids.size());
-        S.genIds.upload(ids.data(), ids.size());
-        S.genParent.upload(par.data(), par.size());
-        S.genChild.upload(ch.data(), ch.size());
-        device::check(mwg_generate_nodes(&fv, c.filters, S.genIds.get(), S.genParent.get(), S.genChild.get(), (int)ids.size()),
+        device::check(mwg_generate_nodes(&fv, job.filters, S.genIds.get() + i, S.genParent.get() + i, S.genChild.get() + i, (int)(j - i)),
                       "generate band nodes");
         i = j;
     }
-    g_stats.nGenerated += (long long)made.size();
+    job.stats.nGenerated += (long long)made.size();
 }
 
-// One batch of output nodes of one scale: screen, generate, band sum, reduce.
-void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double gthr) {
-    Scratch &S = scratch();
-    detail::NodeStore<3> &g = c.out.topology();
-    const int L = (int)nodes.size();
-    const int o_depth = scale - c.oper.getOperatorRoot();
-    const bool empty_band = o_depth < 0 || o_depth >= c.ht.nDepth || c.ht.bandMax[o_depth] < 0;
+// A run of one level batch: nodes of one tree at one scale.
+struct SegmentSpec {
+    int job, scale;
+    std::vector<int> nodes;
+};
 
-    std::vector<int> lv(3 * (size_t)L);
+mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned long long *counter) {
+    mwg_segment s{};
+    s.first = first;
+    s.count = count;
+    s.scale = scale;
+    s.o_depth = scale - job.task.oper->getOperatorRoot();
+    const detail::BandTables &ht = *job.ht;
+    const bool empty_band = s.o_depth < 0 || s.o_depth >= ht.nDepth || ht.bandMax[s.o_depth] < 0;
+    s.band_w = empty_band ? -1 : ht.bandMax[s.o_depth];
+    s.g_thr = job.gthr;
+    s.f = job.fm->view(*job.mra);
+    s.op = *job.bt;
+    s.g_coef = job.task.out->deviceCoefs().get();
+    s.n_applied = counter;
+    return s;
+}
+
+// Screen, generate, band-sum and reduce one level batch of segments; writes
+// the computed nodes' norms back into their trees.
+void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<SegmentSpec> &specs, unsigned long long *counters) {
+    Scratch &S = scratch();
+    int L = 0;
+    for (const auto &sp : specs) L += (int)sp.nodes.size();
+    std::vector<int> lv(3 * (size_t)L), nodeJob(L);
     std::vector<long long> off(L);
-    for (int i = 0; i < L; i++) {
-        for (int d = 0; d < 3; d++) lv[3 * i + d] = g.transl(nodes[i])[d];
-        off[i] = (long long)nodes[i] * c.ncoef;
+    std::vector<mwg_segment> segs;
+    int at = 0;
+    for (const auto &sp : specs) {
+        Job &job = jobs[sp.job];
+        segs.push_back(make_segment(job, sp.scale, at, (int)sp.nodes.size(), counters + sp.job));
+        for (int u : sp.nodes) {
+            for (int d = 0; d < 3; d++) lv[3 * at + d] = job.g->transl(u)[d];
+            off[at] = (long long)u * sh.ncoef;
+            nodeJob[at] = sp.job;
+            at++;
+        }
     }
     S.l.reserve(3 * (size_t)L);
     S.l.upload(lv.data(), lv.size());
@@ -146,21 +198,17 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
     S.gNorms.reserve(9 * (size_t)L);
     S.gFirst.reserve(L);
     S.gCount.reserve(L);
-    S.count.reserve(2);
+    S.count.reserve(3);
 
     mwg_level lvl{};
-    lvl.kp1 = c.kp1;
+    lvl.kp1 = sh.kp1;
     lvl.count = L;
-    lvl.scale = scale;
-    lvl.o_depth = o_depth;
-    lvl.g_thr = gthr;
-    lvl.band_w = empty_band ? 0 : c.ht.bandMax[o_depth];
+    lvl.nseg = (int)segs.size();
+    lvl.segs = segs.data();
     lvl.l = S.l.get();
-    lvl.g_coef = c.out.deviceCoefs().get();
     lvl.g_off = S.gOff.get();
     lvl.g_norms = S.gNorms.get();
 
-    mwg_ftree fv = c.fm.view(c.mra);
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;
     size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
@@ -177,85 +225,107 @@ void run_batch(LevelContext &c, const std::vector<int> &nodes, int scale, double
         pr.g_npairs = S.gCount.get();
     };
     bind_pairs();
-    if (!empty_band) {
-        auto t0 = clk::now();
-        for (;;) {
-            device::check(mwg_pair_screen(&lvl, &fv, &c.bt, &pr), "pair screen");
-            int cnt[2];
-            S.count.download(cnt, 2);
-            npairs = cnt[0];
-            nmissing = cnt[1];
-            if (npairs < 0) throw Error("pair screen: band width does not match the operator tables");
-            if (npairs <= pr.capacity) break;
-            cap = (size_t)npairs + npairs / 4;
-            bind_pairs();
-        }
-        g_stats.tScreen += since(t0);
-        g_stats.nPairs += npairs;
-    }
-    if (npairs > 0 && nmissing > 0) {
-        auto t0 = clk::now();
-        std::vector<int> pf(npairs), pl(3 * (size_t)npairs);
-        S.pairF.download(pf.data(), npairs);
-        {
-            S.pairL.download(pl.data(), pl.size());
-            generate_band_nodes(c, pf, pl, scale);
-            S.pairF.upload(pf.data(), npairs);
-            fv = c.fm.view(c.mra);
-        }
-        g_stats.tGenerate += since(t0);
-    }
     auto t0 = clk::now();
+    for (;;) {
+        device::check(mwg_pair_screen(&lvl, &pr), "pair screen");
+        int cnt[3];
+        S.count.download(cnt, 3);
+        if (cnt[2]) throw Error("pair screen: band width does not match the operator tables");
+        npairs = cnt[0];
+        nmissing = cnt[1];
+        if (npairs <= pr.capacity) break;
+        cap = (size_t)npairs + npairs / 4;
+        bind_pairs();
+    }
+    const double tScreen = since(t0);
+    double tGenerate = 0.0;
+    if (npairs > 0 && nmissing > 0) {
+        t0 = clk::now();
+        std::vector<int> pf(npairs), pl(3 * (size_t)npairs), pg(npairs);
+        S.pairF.download(pf.data(), npairs);
+        S.pairL.download(pl.data(), pl.size());
+        S.pairG.download(pg.data(), npairs);
+        // per tree: its missing pairs, planned and generated on its mirror
+        std::vector<int> segOf(L);
+        for (size_t s = 0; s < segs.size(); s++)
+            for (int i = 0; i < segs[s].count; i++) segOf[segs[s].first + i] = (int)s;
+        std::map<int, std::vector<int>> byJob;
+        for (int p = 0; p < npairs; p++)
+            if (pf[p] < 0) byJob[nodeJob[pg[p]]].push_back(p);
+        for (auto &kv : byJob) {
+            std::vector<int> f, l3, sc;
+            for (int p : kv.second) {
+                f.push_back(pf[p]);
+                for (int d = 0; d < 3; d++) l3.push_back(pl[3 * p + d]);
+                sc.push_back(segs[segOf[pg[p]]].scale);
+            }
+            generate_band_nodes(jobs[kv.first], sh, f, l3, sc);
+            for (size_t i = 0; i < kv.second.size(); i++) pf[kv.second[i]] = f[i];
+        }
+        S.pairF.upload(pf.data(), npairs);
+        // mirrors may have moved: refresh every segment's input view
+        for (size_t s = 0; s < segs.size(); s++) segs[s].f = jobs[nodeJob[segs[s].first]].fm->view(*jobs[nodeJob[segs[s].first]].mra);
+        tGenerate = since(t0);
+    }
+    t0 = clk::now();
     // band sums, output coefficients and norms (zero when no pair survives)
-    device::check(mwg_band_apply(&lvl, &fv, &c.bt, &pr, npairs, S.nApplied.get(), (int)g_sweep), "band apply");
+    device::check(mwg_band_apply(&lvl, &pr, npairs, (int)g_sweep), "band apply");
     std::vector<double> nrm(9 * (size_t)L);
+    std::vector<int> gcount(L);
     S.gNorms.download(nrm.data(), nrm.size());
-    g_stats.tBand += since(t0);
-    for (int i = 0; i < L; i++) {
-        double *cn = g.norms(nodes[i]);
-        for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)i + t];
-        g.setSquareNorm(nodes[i], nrm[9 * (size_t)i + 8]);
-        g.setHasCoefs(nodes[i], true);
+    if (npairs > 0) S.gCount.download(gcount.data(), L);
+    const double tBand = since(t0);
+    std::set<int> touched;
+    for (const auto &sp : specs) touched.insert(sp.job);
+    for (int j : touched) {
+        jobs[j].stats.tScreen += tScreen;
+        jobs[j].stats.tGenerate += tGenerate;
+        jobs[j].stats.tBand += tBand;
+    }
+    at = 0;
+    for (const auto &sp : specs) {
+        detail::NodeStore<3> &g = *jobs[sp.job].g;
+        for (int u : sp.nodes) {
+            double *cn = g.norms(u);
+            for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)at + t];
+            g.setSquareNorm(u, nrm[9 * (size_t)at + 8]);
+            g.setHasCoefs(u, true);
+            jobs[sp.job].stats.nPairs += gcount[at];
+            at++;
+        }
     }
 }
 
-// One scale of the work vector, in batches that bound the device scratch.
-void run_level(LevelContext &c, const std::vector<int> &nodes, int scale, double gthr) {
-    // screen candidates take 8 bytes per band position; group lists 32 M
-    // bytes per pair, with a generous per-node pair estimate (buffers grow
-    // when a batch needs more)
-    const int o_depth = scale - c.oper.getOperatorRoot();
-    const int W = (o_depth >= 0 && o_depth < c.ht.nDepth) ? std::max(c.ht.bandMax[o_depth], 0) : 0;
+// Device bytes of one output node in a level batch (screen candidates and group lists).
+double node_budget(const Job &job, int scale) {
+    const int o_depth = scale - job.task.oper->getOperatorRoot();
+    const detail::BandTables &ht = *job.ht;
+    const int W = (o_depth >= 0 && o_depth < ht.nDepth) ? std::max(ht.bandMax[o_depth], 0) : 0;
     const double box = std::pow(2.0 * W + 1.0, 3.0);
-    const double per_node = 8.0 * (box + 256.0) + 128.0 * (32.0 * c.ht.M + 24.0);
-    size_t batch = std::max<size_t>(1, (size_t)(kBatchBudget / per_node));
-    for (size_t i = 0; i < nodes.size(); i += batch) {
-        std::vector<int> part(nodes.begin() + i, nodes.begin() + std::min(nodes.size(), i + batch));
-        run_batch(c, part, scale, gthr);
-    }
+    return 8.0 * (box + 256.0) + 128.0 * (32.0 * ht.M + 24.0);
 }
 
-void transform_and_norms(LevelContext &c) {
+void transform_and_norms(Job &job, const Shape &sh) {
     Scratch &S = scratch();
-    detail::NodeStore<3> &g = c.out.topology();
+    detail::NodeStore<3> &g = *job.g;
+    FunctionTree<3> &out = *job.task.out;
     auto tab = detail::real_nodes_by_depth(g);
     auto launch = [&](const std::vector<int> &parents, bool up) {
         if (parents.empty()) return;
         std::vector<long long> po, co;
         for (int u : parents) {
-            po.push_back((long long)u * c.ncoef);
-            co.push_back((long long)g.child0(u) * c.ncoef);
+            po.push_back((long long)u * sh.ncoef);
+            co.push_back((long long)g.child0(u) * sh.ncoef);
         }
         S.pOff.reserve(po.size());
         S.cOff.reserve(co.size());
         S.pOff.upload(po.data(), po.size());
         S.cOff.upload(co.data(), co.size());
-        double *coef = c.out.deviceCoefs().get();
+        double *coef = out.deviceCoefs().get();
         if (up)
-            device::check(mwg_compress(c.kp1, c.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), c.ncoef), "compress");
+            device::check(mwg_compress(sh.kp1, job.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), sh.ncoef), "compress");
         else
-            device::check(mwg_reconstruct_add(c.kp1, c.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), c.ncoef),
-                          "reconstruct");
+            device::check(mwg_reconstruct_add(sh.kp1, job.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), sh.ncoef), "reconstruct");
     };
     auto branches = [&](const std::vector<int> &v) {
         std::vector<int> b;
@@ -270,11 +340,11 @@ void transform_and_norms(LevelContext &c) {
     // component norms of every node
     std::vector<int> all = detail::real_nodes(g);
     std::vector<long long> off;
-    for (int u : all) off.push_back((long long)u * c.ncoef);
+    for (int u : all) off.push_back((long long)u * sh.ncoef);
     S.pOff.reserve(off.size());
     S.pOff.upload(off.data(), off.size());
     S.gNorms.reserve(9 * off.size());
-    device::check(mwg_node_norms(c.kp1, c.out.deviceCoefs().get(), S.pOff.get(), S.gNorms.get(), (int)off.size()), "norms");
+    device::check(mwg_node_norms(sh.kp1, out.deviceCoefs().get(), S.pOff.get(), S.gNorms.get(), (int)off.size()), "norms");
     std::vector<double> nrm(9 * off.size());
     S.gNorms.download(nrm.data(), nrm.size());
     for (size_t i = 0; i < all.size(); i++) {
@@ -297,86 +367,286 @@ void drop_generated(device::TreeMirror &fm, const MultiResolutionAnalysis<3> &mr
     fm.uploadTable();
 }
 
+// TreeBuilder::build bookkeeping after a step: approximate tree norm and
+// WaveletAdaptor splits; returns the next work vector.
+void finish_step(Job &job) {
+    detail::NodeStore<3> &g = *job.g;
+    const MultiResolutionAnalysis<3> &mra = *job.mra;
+    if (job.iter == 0) {
+        job.sNorm = 0.0;
+        for (int u : job.work)
+            if (g.depth(u) >= 0) job.sNorm += g.scalingNorm(u);
+    }
+    for (int u : job.work)
+        if (g.depth(u) >= 0) job.wNorm += g.waveletNorm(u);
+    g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
+    if (job.iter >= job.task.maxIter && job.task.maxIter >= 0) job.work.clear();
+    std::vector<int> next;
+    for (int u : job.work) {
+        if (g.hasChildren(u)) continue;
+        if (g.scale(u) + 2 > mra.getMaxScale()) continue;
+        if (detail::wavelet_split(job.task.prec, g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u))) {
+            const int c0 = g.createChildren(u, false);
+            for (int k = 0; k < 8; k++) next.push_back(c0 + k);
+        }
+    }
+    job.work.swap(next);
+    job.iter++;
+}
+
+// Predictor inputs after the first step: the first-step nodes' wavelet norms.
+void first_step_context(Job &job) {
+    PredictorContext &c = job.ctx;
+    c.inp = job.fstore;
+    c.out = job.g;
+    c.bands = job.ht;
+    c.mra = job.mra;
+    c.prec = job.task.prec;
+    c.mu = job.task.oper->kernelMu();
+    c.inpNorm = job.task.inp->getSquareNorm();
+    c.centers = job.task.centers;
+    c.firstStep.assign(job.g->size(), 0);
+    c.firstWavelet.assign(job.g->size(), 0.0);
+    for (int u : job.work) {
+        c.firstStep[u] = 1;
+        c.firstWavelet[u] = job.g->waveletNorm(u);
+    }
+}
+
+// Grow the output grid below the (certain) second-step nodes wherever the
+// predictor expects a refinement; the next pass computes all of them.
+void pregrow(Job &job) {
+    detail::NodeStore<3> &g = *job.g;
+    const int maxScale = job.mra->getMaxScale();
+    job.replayWork = job.work;
+    std::vector<int> all = job.work, cand = job.work;
+    std::vector<float> feat;
+    while (!cand.empty()) {
+        feat.resize(cand.size() * kPredictorFeatures);
+        for (size_t i = 0; i < cand.size(); i++) predictor_features(job.ctx, cand[i], feat.data() + i * kPredictorFeatures);
+        const std::vector<float> prob = job.task.predictor->predict(feat.data(), (int)cand.size());
+        std::vector<int> next;
+        for (size_t i = 0; i < cand.size(); i++) {
+            const int u = cand[i];
+            if (prob[i] <= job.task.predictor->threshold || g.scale(u) + 2 > maxScale) continue;
+            const int c0 = g.createChildren(u, false);
+            for (int k = 0; k < 8; k++) next.push_back(c0 + k);
+        }
+        job.stats.nPredicted += (long long)next.size();
+        all.insert(all.end(), next.begin(), next.end());
+        cand.swap(next);
+    }
+    job.work.swap(all);
+    job.mode = Mode::Replay;
+}
+
+long long subtree_size(const detail::NodeStore<3> &g, int u) {
+    long long n = 0;
+    std::vector<int> st = {u};
+    while (!st.empty()) {
+        const int v = st.back();
+        st.pop_back();
+        if (!g.hasChildren(v)) continue;
+        for (int c = 0; c < 8; c++) {
+            n++;
+            st.push_back(g.child0(v) + c);
+        }
+    }
+    return n;
+}
+
+// The breadth-first steps of TreeBuilder::build replayed on computed nodes:
+// the wavelet test decides every split with the running approximate norm;
+// predicted children it confirms are kept, the others removed; children it
+// wants that were not predicted go to job.work for the next pass.
+void replay(Job &job) {
+    detail::NodeStore<3> &g = *job.g;
+    const int maxScale = job.mra->getMaxScale();
+    job.work.clear();
+    while (!job.replayWork.empty()) {
+        for (int u : job.replayWork)
+            if (g.depth(u) >= 0) job.wNorm += g.waveletNorm(u);
+        g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
+        const bool stop = job.iter >= job.task.maxIter && job.task.maxIter >= 0;
+        std::vector<int> next, pending;
+        for (int u : job.replayWork) {
+            const bool split = !stop && g.scale(u) + 2 <= maxScale &&
+                               detail::wavelet_split(job.task.prec, g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u));
+            if (split) {
+                if (!g.hasChildren(u)) {
+                    const int c0 = g.createChildren(u, false);
+                    for (int k = 0; k < 8; k++) pending.push_back(c0 + k);
+                }
+                for (int k = 0; k < 8; k++) next.push_back(g.child0(u) + k);
+            } else if (g.hasChildren(u)) {
+                job.stats.nCropped += subtree_size(g, u);
+                g.detachChildren(u);
+            }
+        }
+        job.iter++;
+        job.replayWork.swap(next);
+        if (!pending.empty()) {
+            job.stats.nFallback += (long long)pending.size();
+            job.work.swap(pending);
+            return;
+        }
+    }
+    job.mode = Mode::Plain;
+}
+
+// After a pass computed job.work: the next work vector of the job.
+void advance(Job &job) {
+    if (job.mode == Mode::Replay) {
+        replay(job);
+        return;
+    }
+    const bool first = job.iter == 0;
+    const bool learn = first && (job.task.predictor || job.task.samples);
+    if (learn) first_step_context(job);
+    finish_step(job);
+    if (learn) job.ctx.treeNorm = job.g->treeSquareNorm;
+    if (first && job.task.predictor && !job.work.empty()) pregrow(job);
+}
+
+// Training samples of a finished apply: every output node beyond the first
+// step, labelled by whether it was refined.
+void collect_samples(Job &job) {
+    if (!job.task.samples || job.ctx.out == nullptr) return;
+    RefineSamples &rs = *job.task.samples;
+    const detail::NodeStore<3> &g = *job.g;
+    float f[kPredictorFeatures];
+    for (int u : detail::real_nodes(g)) {
+        if (u < (int)job.ctx.firstStep.size() && job.ctx.firstStep[u]) continue;
+        predictor_features(job.ctx, u, f);
+        rs.features.insert(rs.features.end(), f, f + kPredictorFeatures);
+        rs.labels.push_back(g.hasRealChildren(u) ? 1.0f : 0.0f);
+    }
+}
+
 } // namespace
 
 const ApplyStats &last_apply_stats() { return g_stats; }
+const std::vector<ApplyStats> &last_batch_stats() { return g_batch_stats; }
 
 void set_sweep_precision(SweepPrecision p) { g_sweep = p; }
 SweepPrecision sweep_precision() { return g_sweep; }
 
-void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter, bool absPrec) {
-    if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
+void apply_batch(const std::vector<ApplyTask> &tasks) {
     const auto t_start = clk::now();
-    g_stats = ApplyStats{};
+    g_batch_stats.clear();
+    if (tasks.empty()) return;
     device::ensure_init();
-    const MultiResolutionAnalysis<3> &mra = out.getMRA();
-    const int kp1 = mra.getKp1();
+    std::vector<Job> jobs(tasks.size());
+    std::set<const FunctionTree<3> *> outs;
+    const int kp1 = tasks[0].out->getMRA().getKp1();
+    for (size_t i = 0; i < tasks.size(); i++) {
+        const ApplyTask &t = tasks[i];
+        if (!t.out || !t.oper || !t.inp) throw Error("apply: null tree or operator");
+        if (t.out->getMRA() != t.inp->getMRA()) throw Error("Incompatible MRA");
+        if (t.out->getMRA().getKp1() != kp1) throw Error("apply_batch: every tree needs the same Legendre order");
+        if (t.out == t.inp || !outs.insert(t.out).second) throw Error("apply_batch: each output tree may appear once and differ from its input");
+    }
     if (kp1 != 6 && kp1 != 8 && kp1 != 10 && kp1 != 4) throw Error("apply supports Legendre orders 3, 5, 7 and 9");
-    detail::DeviceBandTables *dbt = oper.deviceTables(prec);
-    const detail::BandTables &ht = oper.bandTables(prec);
-    device::TreeMirror &fm = inp.inputMirror();
-    detail::NodeStore<3> &g = out.nodes();
-    LevelContext c{mra, oper, ht, dbt->viewFor((int)g_sweep), fm, &inp.topology(), out, device::filters(mra.getOrder()), kp1, kp1 * kp1 * kp1, 8 * kp1 * kp1 * kp1};
-
+    const Shape sh{kp1, kp1 * kp1 * kp1, 8 * kp1 * kp1 * kp1};
+    for (size_t i = 0; i < tasks.size(); i++) {
+        Job &job = jobs[i];
+        job.task = tasks[i];
+        job.mra = &job.task.out->getMRA();
+        job.ht = &job.task.oper->bandTables(job.task.prec);
+        job.bt = &job.task.oper->deviceTables(job.task.prec)->viewFor((int)g_sweep);
+        job.fm = &job.task.inp->inputMirror();
+        job.fstore = &job.task.inp->topology();
+        job.g = &job.task.out->nodes();
+        job.filters = device::filters(job.mra->getOrder());
+        // output pool sized for an output tree as large as the input (grown on demand)
+        job.task.out->deviceCoefs().reserve((size_t)std::max(job.g->size(), job.fm->nReal) * sh.ncoef);
+        job.work = detail::real_nodes(*job.g); // ConvolutionCalculator::getInitialWorkVector
+    }
     Scratch &S = scratch();
-    S.nApplied.reserve(1);
-    S.nApplied.zero(1);
+    S.nApplied.reserve(jobs.size());
+    S.nApplied.zero(jobs.size());
 
-    // device output storage for the nodes that exist now
-    // output pool sized for an output tree as large as the input (grown on demand)
-    out.deviceCoefs().reserve((size_t)std::max(g.size(), fm.nReal) * c.ncoef);
-
-    const int M = oper.size();
-    const int maxScale = mra.getMaxScale();
-    std::vector<int> work = detail::real_nodes(g); // ConvolutionCalculator::getInitialWorkVector
-    double sNorm = 0.0, wNorm = 0.0;
-    for (int iter = 0; !work.empty(); iter++) {
-        g_stats.levelNodes.push_back((int)work.size());
-        double gthr = g.treeSquareNorm;
-        if (gthr > 0.0) gthr = prec * 1.0 * std::sqrt(gthr / (double)M);
-        out.deviceCoefs().reserve((size_t)g.size() * c.ncoef, (size_t)g.size() * c.ncoef);
-        std::map<int, std::vector<int>> byScale;
-        for (int u : work) byScale[g.scale(u)].push_back(u);
-        for (auto &kv : byScale) run_level(c, kv.second, kv.first, gthr);
-        auto th = clk::now();
-        // TreeBuilder::build norm bookkeeping and WaveletAdaptor splits
-        if (iter == 0) {
-            sNorm = 0.0;
-            for (int u : work)
-                if (g.depth(u) >= 0) sNorm += g.scalingNorm(u);
+    for (;;) {
+        std::vector<int> active;
+        for (size_t i = 0; i < jobs.size(); i++)
+            if (!jobs[i].work.empty()) active.push_back((int)i);
+        if (active.empty()) break;
+        // segments: (tree, scale) runs of this step's work, split into level batches by budget
+        std::vector<SegmentSpec> specs;
+        for (int j : active) {
+            Job &job = jobs[j];
+            job.stats.levelNodes.push_back((int)job.work.size());
+            double gthr = job.g->treeSquareNorm;
+            if (gthr > 0.0) gthr = job.task.prec * 1.0 * std::sqrt(gthr / (double)job.task.oper->size());
+            job.gthr = gthr;
+            job.task.out->deviceCoefs().reserve((size_t)job.g->size() * sh.ncoef, (size_t)job.g->size() * sh.ncoef);
+            std::map<int, std::vector<int>> byScale;
+            for (int u : job.work) byScale[job.g->scale(u)].push_back(u);
+            for (auto &kv : byScale) specs.push_back({j, kv.first, std::move(kv.second)});
         }
-        for (int u : work)
-            if (g.depth(u) >= 0) wNorm += g.waveletNorm(u);
-        g.treeSquareNorm = (sNorm < 0.0 || wNorm < 0.0) ? -1.0 : sNorm + wNorm;
-        if (iter >= maxIter && maxIter >= 0) work.clear();
-        std::vector<int> next;
-        for (int u : work) {
-            if (g.hasChildren(u)) continue;
-            if (g.scale(u) + 2 > maxScale) continue;
-            if (detail::wavelet_split(prec, g.treeSquareNorm, absPrec, g.scale(u), g.waveletNorm(u))) {
-                const int c0 = g.createChildren(u, false);
-                for (int k = 0; k < 8; k++) next.push_back(c0 + k);
+        std::vector<SegmentSpec> batch;
+        double used = 0.0;
+        auto flush = [&]() {
+            if (!batch.empty()) run_level_batch(jobs, sh, batch, S.nApplied.get());
+            batch.clear();
+            used = 0.0;
+        };
+        for (auto &sp : specs) {
+            const double per = node_budget(jobs[sp.job], sp.scale);
+            size_t i = 0;
+            while (i < sp.nodes.size()) {
+                size_t room = (size_t)std::max(1.0, (kBatchBudget - used) / per);
+                if (used > 0.0 && room < 64 && sp.nodes.size() - i > room) {
+                    flush();
+                    continue;
+                }
+                const size_t take = std::min(room, sp.nodes.size() - i);
+                batch.push_back({sp.job, sp.scale, std::vector<int>(sp.nodes.begin() + i, sp.nodes.begin() + i + take)});
+                used += per * (double)take;
+                i += take;
+                if (used >= kBatchBudget) flush();
             }
         }
-        work.swap(next);
-        if (!work.empty()) {
-            // grow the device pool, keeping the computed nodes
-            const size_t keep = (size_t)(g.size() - (int)work.size()) * c.ncoef;
-            out.deviceCoefs().reserve((size_t)g.size() * c.ncoef, keep);
+        flush();
+        for (int j : active) {
+            auto th = clk::now();
+            Job &job = jobs[j];
+            advance(job);
+            if (!job.work.empty()) {
+                // grow the device pool, keeping the computed nodes
+                const size_t keep = (size_t)(job.g->size() - (int)job.work.size()) * sh.ncoef;
+                job.task.out->deviceCoefs().reserve((size_t)job.g->size() * sh.ncoef, keep);
+            }
+            job.stats.tHost += since(th);
         }
-        g_stats.tHost += since(th);
     }
-    auto tt = clk::now();
-    transform_and_norms(c);
-    detail::calc_square_norm(g);
-    out.markDeviceResident();
-    unsigned long long na = 0;
-    S.nApplied.download(&na, 1);
-    g_stats.nApplied = (long long)na;
-    g_stats.tTransform = since(tt);
-    drop_generated(fm, mra, inp.topology());
-    g_stats.nOutputNodes = (long long)detail::real_nodes(g).size();
-    g_stats.seconds = since(t_start);
+    std::vector<unsigned long long> na(jobs.size());
+    S.nApplied.download(na.data(), na.size());
+    std::set<device::TreeMirror *> mirrors;
+    for (size_t i = 0; i < jobs.size(); i++) {
+        Job &job = jobs[i];
+        collect_samples(job);
+        auto tt = clk::now();
+        transform_and_norms(job, sh);
+        detail::calc_square_norm(*job.g);
+        job.task.out->markDeviceResident();
+        job.stats.nApplied = (long long)na[i];
+        job.stats.tTransform = since(tt);
+        job.stats.nOutputNodes = (long long)detail::real_nodes(*job.g).size();
+    }
+    for (auto &job : jobs)
+        if (mirrors.insert(job.fm).second) drop_generated(*job.fm, *job.mra, *job.fstore);
+    const double seconds = since(t_start);
+    for (auto &job : jobs) {
+        job.stats.seconds = seconds;
+        g_batch_stats.push_back(job.stats);
+    }
+}
+
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter, bool absPrec) {
+    if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
+    apply_batch({ApplyTask{prec, &out, &oper, &inp, maxIter, absPrec}});
+    g_stats = g_batch_stats[0];
 }
 
 } // namespace mrcpp

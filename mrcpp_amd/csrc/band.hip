@@ -35,30 +35,33 @@ __device__ inline unsigned group_code(int term, int ft, unsigned mask) { return 
 // ---------------------------------------------------------------------------
 // 1. exact screening, one wave per pair
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_group_screen(mwg_level lvl, mwg_ftree f, mwg_band_tables op, mwg_pairs pr, unsigned *codes, int stride,
-                                                     int *gcount, unsigned long long *n_applied) {
+__global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, mwg_pairs pr, unsigned *codes,
+                                                     int stride, int *gcount) {
     __shared__ unsigned char smask[8 * kMaxTerms];
     const int lane = threadIdx.x;
     const int p = blockIdx.x;
     const int g = pr.pair_g[p];
     const int fid = pr.pair_f[p];
-    const int d = lvl.o_depth;
-    const int M = op.M;
+    const_as4<mwg_segment> &S = ro(segs)[ro(node_seg)[g]];
+    const int d = S.o_depth;
+    const int M = S.op.M;
+    const double g_thr = S.g_thr;
     if (fid < 0) { // unresolved band node: no contribution (the host generates every node first)
         if (lane == 0) gcount[p] = 0;
         return;
     }
     int dl[3], maxdl = 0;
     for (int i = 0; i < 3; i++) {
-        dl[i] = pr.pair_l[3 * p + i] - lvl.l[3 * g + i];
+        dl[i] = pr.pair_l[3 * p + i] - l[3 * g + i];
         maxdl = max(maxdl, abs(dl[i]));
     }
     double fn[8];
-    for (int t = 0; t < 8; t++) fn[t] = f.norms[(size_t)fid * 9 + t];
-    const auto wd = ro(op.widths + (size_t)d * M * 5);
-    const auto bsz = ro(op.band_size + (size_t)d * M * 64);
-    const auto ms = ro(op.mat_start + (size_t)d * M);
-    const auto mn = ro(op.mat_norm);
+    const double *fnorms = S.f.norms;
+    for (int t = 0; t < 8; t++) fn[t] = fnorms[(size_t)fid * 9 + t];
+    const auto wd = ro(S.op.widths + (size_t)d * M * 5);
+    const auto bsz = ro(S.op.band_size + (size_t)d * M * 64);
+    const auto ms = ro(S.op.mat_start + (size_t)d * M);
+    const auto mn = ro(S.op.mat_norm);
     unsigned napp = 0;
     for (int c0 = 0; c0 < M; c0 += 64) {
         const int i = c0 + lane;
@@ -89,7 +92,7 @@ __global__ __launch_bounds__(64) void k_group_screen(mwg_level lvl, mwg_ftree f,
                     onorm *= ny;
                     onorm *= nz;
                     const double fthr = bsz[i * 64 + gt * 8 + ft] * fn[ft];
-                    if (onorm * fthr > lvl.g_thr) m |= 1u << gt;
+                    if (onorm * fthr > g_thr) m |= 1u << gt;
                 }
                 m8[ft] = m;
                 napp += __popc(m);
@@ -115,7 +118,7 @@ __global__ __launch_bounds__(64) void k_group_screen(mwg_level lvl, mwg_ftree f,
     for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
     if (lane == 0) {
         gcount[p] = base;
-        if (napp) atomicAdd(n_applied, (unsigned long long)napp);
+        if (napp) atomicAdd(S.n_applied, (unsigned long long)napp);
     }
 }
 
@@ -176,17 +179,15 @@ __global__ __launch_bounds__(256) void k_unit_write(int npairs, mwg_pairs pr, co
 // Read-only views a unit kernel walks (nothing here is written by the band
 // kernels, so loads at wave-uniform addresses are scalar loads).
 struct UnitView {
-    const_as4<int> *lv, *pg, *pf, *pl, *gs, *wd, *unit;
-    const_as4<long long> *ms, *coff;
+    const_as4<mwg_segment> *segs;
+    const_as4<int> *node_seg, *lv, *pg, *pf, *pl, *gs, *unit;
     const_as4<unsigned> *cd;
-    const double *coef;
+    const_as4<long long> *g_off;
     int stride, total;
-    __device__ UnitView(const mwg_level &lvl, const mwg_ftree &f, const mwg_band_tables &op, const mwg_pairs &pr, const unsigned *codes, int stride_,
-                        const int *gstart, const int4 *units, const long long *uscan, int L)
-            : lv(ro(lvl.l)), pg(ro(pr.pair_g)), pf(ro(pr.pair_f)), pl(ro(pr.pair_l)), gs(ro(gstart)),
-              wd(ro(op.widths + (size_t)lvl.o_depth * op.M * 5)), unit(ro(reinterpret_cast<const int *>(units))),
-              ms(ro(op.mat_start + (size_t)lvl.o_depth * op.M)), coff(ro(f.coef_off)), cd(ro(codes)), coef(f.coef), stride(stride_),
-              total(unit_first(ro(uscan)[L])) {}
+    __device__ UnitView(const mwg_segment *segs_, const int *node_seg_, const int *l, const long long *g_off_, const mwg_pairs &pr,
+                        const unsigned *codes, int stride_, const int *gstart, const int4 *units, const long long *uscan, int L)
+            : segs(ro(segs_)), node_seg(ro(node_seg_)), lv(ro(l)), pg(ro(pr.pair_g)), pf(ro(pr.pair_f)), pl(ro(pr.pair_l)), gs(ro(gstart)),
+              unit(ro(reinterpret_cast<const int *>(units))), cd(ro(codes)), g_off(ro(g_off_)), stride(stride_), total(unit_first(ro(uscan)[L])) {}
 };
 
 // Next unit for the workgroup (all threads get the same index), or -1.
@@ -205,11 +206,18 @@ struct Group {
     bool fresh_f; // band node or ft differs from the previous group's
 };
 
+// Walks the groups of one unit; holds the unit's segment tables.
 struct UnitCursor {
     int g, slot, left;
     int gl[3], dl[3];
     int p, k, pend, loaded, curft;
     const double *fsrc;
+    // segment of the unit's output node
+    const_as4<int> *wd;       // widths at the node's depth
+    const_as4<long long> *ms; // matrix starts at the node's depth
+    const double *coef;
+    const_as4<long long> *coff;
+    double *out;              // the node's output slot
 
     __device__ void start(const UnitView &v, int u) {
         const auto ud = v.unit + 4 * (size_t)u;
@@ -219,6 +227,13 @@ struct UnitCursor {
         slot = ud[3];
         g = v.pg[p];
         for (int i = 0; i < 3; i++) gl[i] = v.lv[3 * g + i];
+        const_as4<mwg_segment> &S = v.segs[v.node_seg[g]];
+        const int d = S.o_depth, M = S.op.M;
+        wd = ro(S.op.widths + (size_t)d * M * 5);
+        ms = ro(S.op.mat_start + (size_t)d * M);
+        coef = S.f.coef;
+        coff = ro(S.f.coef_off);
+        out = S.g_coef + v.g_off[g];
         pend = v.gs[p + 1] - v.gs[p];
         loaded = -1;
         curft = -1;
@@ -235,7 +250,7 @@ struct UnitCursor {
         bool fresh = false;
         if (p != loaded) { // a pair with groups always has a band node id
             for (int i = 0; i < 3; i++) dl[i] = v.pl[3 * p + i] - gl[i];
-            fsrc = v.coef + v.coff[v.pf[p]];
+            fsrc = coef + coff[v.pf[p]];
             loaded = p;
             fresh = true;
         }
@@ -248,6 +263,8 @@ struct UnitCursor {
         curft = grp.ft;
         return true;
     }
+    // the unit's result: the output node itself (first unit) or a partial slot
+    __device__ double *result(double *partial, int ncoef) const { return slot < 0 ? out : partial + (size_t)slot * ncoef; }
 };
 
 // x / y needs of a gt mask: bit gx of xneed, bit gx + 2 gy of yneed.
@@ -277,9 +294,9 @@ template <int N1> struct UnitShape {
 };
 
 template <int N1>
-__global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl, mwg_ftree f, mwg_band_tables op, mwg_pairs pr, const unsigned *codes,
-                                                                  int stride, const int *gstart, const int4 *units, const long long *uscan, int L,
-                                                                  double *partial, int *counter) {
+__global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segment *segs, const int *node_seg, const int *l, const long long *g_off,
+                                                                  mwg_pairs pr, const unsigned *codes, int stride, const int *gstart, const int4 *units,
+                                                                  const long long *uscan, int L, double *partial, int *counter) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1;
     constexpr int P = UnitShape<N1>::P;
     __shared__ double sF[N1 * P];
@@ -289,12 +306,12 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
 
     const int tid = threadIdx.x;
     const bool owner = tid < N2;
-    const UnitView v(lvl, f, op, pr, codes, stride, gstart, units, uscan, L);
-    const auto mats = ro(op.mats);
+    const UnitView v(segs, node_seg, l, g_off, pr, codes, stride, gstart, units, uscan, L);
 
     for (int u; (u = take_unit(counter, &s_unit, v.total)) >= 0;) {
         UnitCursor cur;
         cur.start(v, u);
+        const auto mats = ro(v.segs[v.node_seg[cur.g]].op.mats);
         double acc[8][N1];
 #pragma unroll
         for (int t = 0; t < 8; t++)
@@ -310,8 +327,8 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
                 for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = cur.fsrc[(size_t)ft * N3 + e];
                 __syncthreads();
             }
-            const int w4 = v.wd[grp.term * 5 + 4];
-            const long long mbase = v.ms[grp.term];
+            const int w4 = cur.wd[grp.term * 5 + 4];
+            const long long mbase = cur.ms[grp.term];
             unsigned xneed, yneed;
             sweep_needs(gm, xneed, yneed);
 #pragma unroll
@@ -375,7 +392,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(mwg_level lvl,
                 }
             }
         }
-        double *dst = (cur.slot < 0) ? lvl.g_coef + ro(lvl.g_off)[cur.g] : partial + (size_t)cur.slot * 8 * N3;
+        double *dst = cur.result(partial, 8 * N3);
         if (owner) {
 #pragma unroll
             for (int t = 0; t < 8; t++)
@@ -472,9 +489,9 @@ __device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename Band
 }
 
 template <int N1, int PREC>
-__global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, mwg_band_tables op, mwg_pairs pr, const unsigned *codes, int stride,
-                                                   const int *gstart, const int4 *units, const long long *uscan, int L, double *partial, int *counter,
-                                                   const typename BandTypes<PREC>::B *frag) {
+__global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, const int *node_seg, const int *l, const long long *g_off, mwg_pairs pr,
+                                                   const unsigned *codes, int stride, const int *gstart, const int4 *units, const long long *uscan, int L,
+                                                   double *partial, int *counter) {
     using S = SweepShape<N1>;
     using G = BandGeom<N1, PREC>;
     using LT = typename BandTypes<PREC>::L;
@@ -490,7 +507,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, m
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int gxw = w & 1, gyw = w >> 1;
     const int li = lane & 15;
-    const UnitView v(lvl, f, op, pr, codes, stride, gstart, units, uscan, L);
+    const UnitView v(segs, node_seg, l, g_off, pr, codes, stride, gstart, units, uscan, L);
 
     for (int i = tid; i < BLK; i += 256) sF[i] = LT(0);
     for (int i = tid; i < 2 * BLK; i += 256) sT1[i] = LT(0);
@@ -499,6 +516,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, m
     for (int u; (u = take_unit(counter, &s_unit, v.total)) >= 0;) {
         UnitCursor cur;
         cur.start(v, u);
+        const BT *frag = static_cast<const BT *>(v.segs[v.node_seg[cur.g]].op.frag[PREC]);
         dvec4 acc[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
@@ -512,7 +530,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, m
             unsigned xneed, yneed;
             sweep_needs(gm, xneed, yneed);
             // operand fragments: matrix block b4 = mat_start/4 + w + dl of each direction
-            const long long b40 = v.ms[grp.term] / 4 + v.wd[grp.term * 5 + 4];
+            const long long b40 = cur.ms[grp.term] / 4 + cur.wd[grp.term * 5 + 4];
             BT bx[NT][KSTEPS], by[NT][KSTEPS], bz[NT][KSTEPS];
 #pragma unroll
             for (int nt = 0; nt < NT; nt++) {
@@ -583,7 +601,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(mwg_level lvl, mwg_ftree f, m
             }
         }
         // this wave's blocks gt = gx + 2 gy + 4 gz of the unit result
-        double *dst = (cur.slot < 0) ? lvl.g_coef + ro(lvl.g_off)[cur.g] : partial + (size_t)cur.slot * 8 * N3;
+        double *dst = cur.result(partial, 8 * N3);
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -645,7 +663,14 @@ template <int N1, int PREC> __global__ __launch_bounds__(256) void k_build_frag(
 // 4. per output node and chunk of 256 coefficients: first unit + partial
 //    slots in unit order; then the component norms per node
 // ---------------------------------------------------------------------------
-template <int N1> __global__ __launch_bounds__(256) void k_reduce_units(mwg_level lvl, const long long *uscan, const double *partial) {
+// Output slot of node g (its segment's pool + g_off).
+__device__ inline double *node_out(const mwg_segment *segs, const int *node_seg, const long long *g_off, int g) {
+    return ro(segs)[ro(node_seg)[g]].g_coef + g_off[g];
+}
+
+template <int N1>
+__global__ __launch_bounds__(256) void k_reduce_units(const mwg_segment *segs, const int *node_seg, const long long *g_off, const long long *uscan,
+                                                      const double *partial) {
     constexpr int N3 = N1 * N1 * N1;
     const int g = blockIdx.x;
     const int o = blockIdx.y * 256 + threadIdx.x;
@@ -653,7 +678,7 @@ template <int N1> __global__ __launch_bounds__(256) void k_reduce_units(mwg_leve
     const int Ug = unit_first(v);
     const int nu = unit_first(uscan[g + 1]) - Ug;
     if (nu == 1 || o >= 8 * N3) return; // a single unit wrote the node already
-    double *dst = lvl.g_coef + lvl.g_off[g];
+    double *dst = node_out(segs, node_seg, g_off, g);
     if (nu == 0) {
         dst[o] = 0.0;
         return;
@@ -665,19 +690,21 @@ template <int N1> __global__ __launch_bounds__(256) void k_reduce_units(mwg_leve
     dst[o] = acc;
 }
 
-template <int N1> __global__ __launch_bounds__(kThreads) void k_level_norms(mwg_level lvl) {
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_level_norms(const mwg_segment *segs, const int *node_seg, const long long *g_off, double *g_norms) {
     __shared__ double red[8 * (kThreads / 64)];
     const int g = blockIdx.x;
-    block_norms<N1>(lvl.g_coef + lvl.g_off[g], lvl.g_norms + (size_t)g * 9, red);
+    block_norms<N1>(node_out(segs, node_seg, g_off, g), g_norms + (size_t)g * 9, red);
 }
 
 // A level without candidate pairs: zero coefficients and norms.
-template <int N1> __global__ __launch_bounds__(kThreads) void k_zero_level(mwg_level lvl) {
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs, const int *node_seg, const long long *g_off, double *g_norms) {
     constexpr int N3 = N1 * N1 * N1;
     const int g = blockIdx.x;
-    double *dst = lvl.g_coef + lvl.g_off[g];
+    double *dst = node_out(segs, node_seg, g_off, g);
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = 0.0;
-    if (threadIdx.x < 9) lvl.g_norms[(size_t)g * 9 + threadIdx.x] = 0.0;
+    if (threadIdx.x < 9) g_norms[(size_t)g * 9 + threadIdx.x] = 0.0;
 }
 
 struct BandWork {
@@ -705,26 +732,30 @@ template <int N1, int PREC> int mfma_grid() {
     static const int grid = resident_grid(k_band_mfma<N1, PREC>, 256);
     return grid;
 }
-template <int N1, int PREC> void launch_mfma(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, BandWork &w,
-                                             int stride, int L) {
-    hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, *lvl, *f, *op, *pr, w.codes.as<unsigned>(), stride,
-                       w.gstart.as<int>(), w.units.as<int4>(), w.uscan.as<long long>(), L, w.partial.as<double>(), w.counter.as<int>(),
-                       static_cast<const typename BandTypes<PREC>::B *>(op->frag[PREC]));
+template <int N1, int PREC>
+void launch_mfma(const mwg_level *lvl, const LevelDevice &dev, const mwg_pairs *pr, BandWork &w, int stride) {
+    hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->g_off, *pr,
+                       w.codes.as<unsigned>(), stride, w.gstart.as<int>(), w.units.as<int4>(), w.uscan.as<long long>(), lvl->count,
+                       w.partial.as<double>(), w.counter.as<int>());
 }
 
 template <int N1> struct BandL {
-    static mwg_status run(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pr, int npairs,
-                          unsigned long long *n_applied, int precision) {
+    static mwg_status run(const mwg_level *lvl, const mwg_pairs *pr, int npairs, int precision) {
         constexpr int N3 = N1 * N1 * N1;
         const int L = lvl->count;
+        LevelDevice dev;
+        mwg_status st = upload_level(lvl, dev);
+        if (st != MWG_OK) return st;
         if (npairs == 0) {
             ProfileScope prof_(MWG_KERNEL_REDUCE);
-            hipLaunchKernelGGL(k_zero_level<N1>, dim3(L), dim3(kThreads), 0, 0, *lvl);
+            hipLaunchKernelGGL(k_zero_level<N1>, dim3(L), dim3(kThreads), 0, 0, dev.segs, dev.node_seg, lvl->g_off, lvl->g_norms);
             MWG_LAUNCHED();
             return MWG_OK;
         }
         BandWork &w = work();
-        const int stride = 8 * op->M;
+        int maxM = 1;
+        for (int s = 0; s < lvl->nseg; s++) maxM = std::max(maxM, lvl->segs[s].op.M);
+        const int stride = 8 * maxM; // group-list stride per pair
         const size_t slot_bytes = sizeof(double) * 8 * N3;
         const int slots = (int)std::max<double>(256.0, kPartialBudget / (double)slot_bytes);
         if ((double)npairs * stride >= 2147483647.0) return fail(MWG_ERR_UNSUPPORTED, "too many groups in one level batch");
@@ -738,7 +769,8 @@ template <int N1> struct BandL {
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
-            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, *lvl, *f, *op, *pr, w.codes.as<unsigned>(), stride, gstart, n_applied);
+            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, dev.segs, dev.node_seg, lvl->l, *pr, w.codes.as<unsigned>(), stride,
+                               gstart);
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
@@ -752,22 +784,23 @@ template <int N1> struct BandL {
         {
             ProfileScope prof_(MWG_KERNEL_BAND);
             if (precision == MWG_PREC_FP64)
-                launch_mfma<N1, MWG_PREC_FP64>(lvl, f, op, pr, w, stride, L);
+                launch_mfma<N1, MWG_PREC_FP64>(lvl, dev, pr, w, stride);
             else if (precision == MWG_PREC_FP32)
-                launch_mfma<N1, MWG_PREC_FP32>(lvl, f, op, pr, w, stride, L);
+                launch_mfma<N1, MWG_PREC_FP32>(lvl, dev, pr, w, stride);
             else if (precision == MWG_PREC_BF16X3)
-                launch_mfma<N1, MWG_PREC_BF16X3>(lvl, f, op, pr, w, stride, L);
+                launch_mfma<N1, MWG_PREC_BF16X3>(lvl, dev, pr, w, stride);
             else
-                hipLaunchKernelGGL(k_band_units<N1>, dim3(unit_grid<N1>()), dim3(UnitShape<N1>::NT), 0, 0, *lvl, *f, *op, *pr,
-                                   w.codes.as<unsigned>(), stride, gstart, w.units.as<int4>(), uscan, L, w.partial.as<double>(),
+                hipLaunchKernelGGL(k_band_units<N1>, dim3(unit_grid<N1>()), dim3(UnitShape<N1>::NT), 0, 0, dev.segs, dev.node_seg, lvl->l,
+                                   lvl->g_off, *pr, w.codes.as<unsigned>(), stride, gstart, w.units.as<int4>(), uscan, L, w.partial.as<double>(),
                                    w.counter.as<int>());
             MWG_LAUNCHED();
         }
         {
             ProfileScope prof_(MWG_KERNEL_REDUCE);
-            hipLaunchKernelGGL(k_reduce_units<N1>, dim3(L, (8 * N3 + 255) / 256), dim3(256), 0, 0, *lvl, uscan, w.partial.as<double>());
+            hipLaunchKernelGGL(k_reduce_units<N1>, dim3(L, (8 * N3 + 255) / 256), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->g_off, uscan,
+                               w.partial.as<double>());
             MWG_LAUNCHED();
-            hipLaunchKernelGGL(k_level_norms<N1>, dim3(L), dim3(kThreads), 0, 0, *lvl);
+            hipLaunchKernelGGL(k_level_norms<N1>, dim3(L), dim3(kThreads), 0, 0, dev.segs, dev.node_seg, lvl->g_off, lvl->g_norms);
             MWG_LAUNCHED();
         }
         return MWG_OK;
@@ -833,16 +866,18 @@ extern "C" mwg_status mwg_build_fragments(const mwg_band_tables *op, int precisi
     return dispatch<FragL>(op->kp1, op, precision, dst);
 }
 
-extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, const mwg_pairs *pairs, int npairs,
-                                     unsigned long long *n_applied, int precision) {
+extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_pairs *pairs, int npairs, int precision) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
-    if (!lvl || !f || !op || !pairs || !n_applied) return fail(MWG_ERR_ARG, "null argument");
+    if (!pairs) return fail(MWG_ERR_ARG, "null argument");
     if (npairs < 0) return fail(MWG_ERR_ARG, "negative pair count");
-    if (lvl->count <= 0) return MWG_OK;
+    const mwg_status st = check_level(lvl);
+    if (st != MWG_OK) return st;
+    if (lvl->count == 0) return MWG_OK;
     if (precision < MWG_PREC_FP64 || precision > kPrecValu) return fail(MWG_ERR_UNSUPPORTED, "unsupported sweep precision");
-    if (npairs > 0 && precision != kPrecValu && !op->frag[precision]) return fail(MWG_ERR_ARG, "operand table of this precision not built");
-    if (op->M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 512");
-    if (op->kp1 != f->kp1 || lvl->kp1 != f->kp1) return fail(MWG_ERR_ARG, "operator, level and tree orders differ");
+    if (npairs > 0 && precision != kPrecValu)
+        for (int s = 0; s < lvl->nseg; s++)
+            if (lvl->segs[s].count > 0 && lvl->segs[s].band_w >= 0 && !lvl->segs[s].op.frag[precision])
+                return fail(MWG_ERR_ARG, "operand table of this precision not built");
     if (npairs > pairs->capacity) return fail(MWG_ERR_ARG, "more pairs than the pair arrays hold");
-    return dispatch<BandL>(f->kp1, lvl, f, op, pairs, npairs, n_applied, precision);
+    return dispatch<BandL>(lvl->kp1, lvl, pairs, npairs, precision);
 }

@@ -5,6 +5,7 @@
 #include <string>
 
 #include "function_tree.h"
+#include "predictor.h"
 
 using namespace mrcpp;
 
@@ -170,16 +171,136 @@ int mrcpp_set_sweep_precision(int precision) {
 
 int mrcpp_get_sweep_precision(void) { return (int)sweep_precision(); }
 
+namespace {
+void pack_stats(const ApplyStats &s, double *out) {
+    double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,
+                    (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tReduce, s.tTransform, s.tHost, 0, 0, 0, 0};
+    std::memcpy(out, v, sizeof(v));
+}
+} // namespace
+
+int mrcpp_apply_batch(int n, const double *prec, void *const *out, void *const *oper, void *const *inp, const int *max_iter, const int *abs_prec,
+                      double *stats) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!prec || !out || !oper || !inp))) throw Error("apply_batch: null argument");
+        std::vector<ApplyTask> tasks(n);
+        for (int i = 0; i < n; i++)
+            tasks[i] = ApplyTask{prec[i], static_cast<FunctionTree<3> *>(out[i]), static_cast<ConvolutionOperator<3> *>(oper[i]),
+                                 static_cast<FunctionTree<3> *>(inp[i]), max_iter ? max_iter[i] : -1, abs_prec ? abs_prec[i] != 0 : false};
+        apply_batch(tasks);
+        if (stats)
+            for (int i = 0; i < n; i++) pack_stats(last_batch_stats()[i], stats + 16 * (size_t)i);
+    });
+}
+
+namespace {
+thread_local RefineSamples g_samples;
+
+std::vector<std::vector<std::array<double, 3>>> unpack_centers(int n, const double *centers, const int *ncenters) {
+    std::vector<std::vector<std::array<double, 3>>> out(n);
+    if (!centers || !ncenters) return out;
+    size_t at = 0;
+    for (int i = 0; i < n; i++) {
+        if (ncenters[i] < 0) throw Error("negative centre count");
+        for (int c = 0; c < ncenters[i]; c++, at++) out[i].push_back({centers[3 * at], centers[3 * at + 1], centers[3 * at + 2]});
+    }
+    return out;
+}
+} // namespace
+
+int mrcpp_predictor_create(void **pred) {
+    return guarded([&] {
+        if (!pred) throw Error("null argument");
+        *pred = new RefinePredictor();
+    });
+}
+
+int mrcpp_predictor_destroy(void *pred) {
+    return guarded([&] { delete static_cast<RefinePredictor *>(pred); });
+}
+
+int mrcpp_predictor_num_params(void) { return MWG_MLP_PARAMS; }
+
+int mrcpp_predictor_set_params(void *pred, const float *params, int n) {
+    return guarded([&] {
+        if (!pred || !params) throw Error("null argument");
+        static_cast<RefinePredictor *>(pred)->setParams(std::vector<float>(params, params + n));
+    });
+}
+
+int mrcpp_predictor_get_params(void *pred, float *params, int n) {
+    return guarded([&] {
+        if (!pred || !params || n != MWG_MLP_PARAMS) throw Error("bad parameter buffer");
+        const auto p = static_cast<RefinePredictor *>(pred)->params();
+        std::memcpy(params, p.data(), sizeof(float) * p.size());
+    });
+}
+
+int mrcpp_predictor_set_threshold(void *pred, double threshold) {
+    return guarded([&] {
+        if (!pred) throw Error("null argument");
+        static_cast<RefinePredictor *>(pred)->threshold = (float)threshold;
+    });
+}
+
+int mrcpp_predictor_predict(void *pred, const float *features, int n, float *prob) {
+    return guarded([&] {
+        if (!pred || (n > 0 && (!features || !prob))) throw Error("null argument");
+        const auto p = static_cast<RefinePredictor *>(pred)->predict(features, n);
+        if (n > 0) std::memcpy(prob, p.data(), sizeof(float) * n);
+    });
+}
+
+int mrcpp_predictor_train(void *pred, const float *features, const float *labels, int n, int epochs, int batch, double lr, double pos_weight,
+                          unsigned long long seed, float *losses) {
+    return guarded([&] {
+        if (!pred || (n > 0 && (!features || !labels))) throw Error("null argument");
+        const auto l = static_cast<RefinePredictor *>(pred)->train(features, labels, n, epochs, batch, (float)lr, (float)pos_weight, seed);
+        if (losses) std::memcpy(losses, l.data(), sizeof(float) * l.size());
+    });
+}
+
+int mrcpp_apply_batch_predicted(int n, const double *prec, void *const *out, void *const *oper, void *const *inp, void *predictor,
+                                const double *centers, const int *ncenters, int collect, double *stats) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!prec || !out || !oper || !inp))) throw Error("apply_batch: null argument");
+        const auto cs = unpack_centers(n, centers, ncenters);
+        g_samples = RefineSamples{};
+        std::vector<ApplyTask> tasks(n);
+        for (int i = 0; i < n; i++) {
+            tasks[i] = ApplyTask{prec[i], static_cast<FunctionTree<3> *>(out[i]), static_cast<ConvolutionOperator<3> *>(oper[i]),
+                                 static_cast<FunctionTree<3> *>(inp[i])};
+            tasks[i].predictor = static_cast<const RefinePredictor *>(predictor);
+            tasks[i].centers = &cs[i];
+            if (collect) tasks[i].samples = &g_samples;
+        }
+        apply_batch(tasks);
+        if (stats)
+            for (int i = 0; i < n; i++) {
+                const ApplyStats &st = last_batch_stats()[i];
+                pack_stats(st, stats + 16 * (size_t)i);
+                stats[16 * (size_t)i + 12] = (double)st.nPredicted;
+                stats[16 * (size_t)i + 13] = (double)st.nCropped;
+                stats[16 * (size_t)i + 14] = (double)st.nFallback;
+            }
+    });
+}
+
+int mrcpp_samples_count(void) { return (int)g_samples.labels.size(); }
+
+int mrcpp_samples_fetch(float *features, float *labels, int cap) {
+    const int n = (int)g_samples.labels.size();
+    if (!features || !labels || cap < n) return -1;
+    std::memcpy(features, g_samples.features.data(), sizeof(float) * g_samples.features.size());
+    std::memcpy(labels, g_samples.labels.data(), sizeof(float) * n);
+    return n;
+}
+
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats) {
     return guarded([&] {
         apply(prec, *static_cast<FunctionTree<3> *>(out), *static_cast<ConvolutionOperator<3> *>(oper), *static_cast<FunctionTree<3> *>(inp),
               max_iter, abs_prec != 0);
-        if (stats) {
-            const ApplyStats &s = last_apply_stats();
-            double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,
-                            (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tReduce, s.tTransform, s.tHost, 0, 0, 0, 0};
-            std::memcpy(stats, v, sizeof(v));
-        }
+        if (stats) pack_stats(last_apply_stats(), stats);
     });
 }
 

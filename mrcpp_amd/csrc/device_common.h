@@ -44,6 +44,15 @@ struct Buffer {
 // Compute units of the current device.
 int device_cus();
 
+// A level's segments in device memory and the segment of every output node
+// (valid until the next upload_level call).
+struct LevelDevice {
+    const mwg_segment *segs = nullptr;
+    const int *node_seg = nullptr;
+};
+mwg_status check_level(const mwg_level *lvl);
+mwg_status upload_level(const mwg_level *lvl, LevelDevice &out);
+
 // Exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in and
 // out may alias.  Enqueued on the null stream.
 mwg_status scan_exclusive(const int *in, int *out, int n);
@@ -65,6 +74,16 @@ mwg_status scan_exclusive(const long long *in, long long *out, int n);
 // runs: loads through it at wave-uniform addresses become scalar loads.
 template <typename T> using const_as4 = const __attribute__((address_space(4))) T;
 template <typename T> __device__ inline const_as4<T> *ro(const T *p) { return (const_as4<T> *)p; }
+// Copy of a read-only struct into registers (dword by dword: scalar loads).
+template <typename T> __device__ inline T ld(const_as4<T> &x) {
+    static_assert(sizeof(T) % 4 == 0, "dword-sized struct");
+    T out;
+    const auto src = (const_as4<unsigned> *)&x;
+    unsigned *dst = reinterpret_cast<unsigned *>(&out);
+#pragma unroll
+    for (size_t i = 0; i < sizeof(T) / 4; i++) dst[i] = src[i];
+    return out;
+}
 
 __device__ inline int f_lookup(const mwg_ftree &f, int depth, int x, int y, int z) {
     const int ox = x - f.corner[0] * (1 << depth);

@@ -69,6 +69,9 @@ struct ApplyStats {
     long long nGenerated = 0;  // generated band nodes
     long long nOutputNodes = 0;
     std::vector<int> levelNodes;
+    long long nPredicted = 0;  // output nodes created by the refinement predictor
+    long long nCropped = 0;    // predicted nodes the wavelet test rejected (removed)
+    long long nFallback = 0;   // nodes the predictor missed, computed in extra passes
     double seconds = 0.0;      // host wall time of apply()
     double tScreen = 0.0, tGenerate = 0.0, tBand = 0.0, tReduce = 0.0, tTransform = 0.0, tHost = 0.0;
 };
@@ -87,5 +90,34 @@ SweepPrecision sweep_precision();
 // the band sum running on the MI355X.
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter = -1,
            bool absPrec = false);
+
+class RefinePredictor;
+struct RefineSamples;
+
+// One application of a batch: the arguments of apply(), and optionally a
+// refinement predictor that grows the output grid after the first step (the
+// wavelet test stays the judge: predicted nodes it rejects are removed,
+// nodes it adds that were not predicted are computed in further passes),
+// source centres for the predictor's features, and a sink for training
+// samples (every output node beyond the first step: features, refined or not).
+struct ApplyTask {
+    double prec;
+    FunctionTree<3> *out;
+    ConvolutionOperator<3> *oper;
+    FunctionTree<3> *inp;
+    int maxIter = -1;
+    bool absPrec = false;
+    const RefinePredictor *predictor = nullptr;
+    const std::vector<std::array<double, 3>> *centers = nullptr;
+    RefineSamples *samples = nullptr;
+};
+// Many applications at once (no reference counterpart): every breadth-first
+// step of every unfinished tree shares one screen and one band-sum launch.
+// Each output tree equals what apply() alone produces.  All trees must have
+// the same Legendre order; output trees must be distinct.
+void apply_batch(const std::vector<ApplyTask> &tasks);
+// Per-task statistics of the last apply_batch on this thread (seconds is the
+// whole batch's wall time).
+const std::vector<ApplyStats> &last_batch_stats();
 
 } // namespace mrcpp

@@ -332,6 +332,7 @@ PoissonOperator::PoissonOperator(const MultiResolutionAnalysis<3> &mra, double p
 
 HelmholtzOperator::HelmholtzOperator(const MultiResolutionAnalysis<3> &mra, double mu, double prec)
         : ConvolutionOperator<3>(mra) {
+    mu_ = mu;
     const double k_prec = prec / 10.0;
     initialize(helmholtz_kernel(mu, k_prec, mra.calcMinDistance(k_prec), mra.calcMaxDistance()), prec);
 }

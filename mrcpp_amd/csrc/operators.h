@@ -74,6 +74,8 @@ public:
     int getOperatorRoot() const { return mra_.getRootScale(); }
     double getBuildPrec() const { return build_prec_; }
     const std::vector<KernelTerm> &kernel() const { return kernel_; }
+    // screening parameter of the kernel (Helmholtz mu; 0 for Poisson)
+    double kernelMu() const { return mu_; }
     const detail::OperatorTerm &term(int i) const { return *terms_[i]; }
     double buildSeconds() const { return build_seconds_; }
 
@@ -91,6 +93,7 @@ public:
 protected:
     ConvolutionOperator(const MultiResolutionAnalysis<3> &mra);
     void initialize(const std::vector<KernelTerm> &kernel, double prec);
+    double mu_ = 0.0;
 
 private:
     MultiResolutionAnalysis<3> mra_;

@@ -34,22 +34,24 @@ __device__ inline Box band_box(const mwg_ftree &f, int d, int W, const int *gl) 
     return b;
 }
 
-// Whether band position (x, y, z) can contribute to output node gl; *code is
-// the band node id, or -(ancestor id + 1) when the node is missing.
-__device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f, const mwg_band_tables &op, const int *gl, int x, int y, int z,
-                                       int *code) {
-    const int d = lvl.o_depth;
-    const double *pb = op.pair_bound + (size_t)d * (2 * op.wmax + 1) + op.wmax;
+// Whether band position (x, y, z) can contribute to output node gl of segment
+// S; *code is the band node id, or -(ancestor id + 1) when the node is missing.
+__device__ inline bool screen_position(const_as4<mwg_segment> &S, const int *gl, int x, int y, int z, int *code) {
+    const int d = S.o_depth;
+    const double g_thr = S.g_thr;
+    const int wmax = S.op.wmax;
+    const double *pb = S.op.pair_bound + (size_t)d * (2 * wmax + 1) + wmax;
     double a = pb[x - gl[0]] * pb[y - gl[1]] * pb[z - gl[2]];
     if (d > 0) {
         // below the root scale every tuple has a wavelet component (C, B or A)
         // in at least one direction
-        const double *pw = op.pair_bound_w + (size_t)d * (2 * op.wmax + 1) + op.wmax;
+        const double *pw = S.op.pair_bound_w + (size_t)d * (2 * wmax + 1) + wmax;
         const double mx = pb[x - gl[0]], my = pb[y - gl[1]], mz = pb[z - gl[2]];
         a = fmax(pw[x - gl[0]] * my * mz, fmax(mx * pw[y - gl[1]] * mz, mx * my * pw[z - gl[2]]));
     }
     // a zero bound cannot pass a non-negative threshold: skip the lookup
-    if (a == 0.0 && lvl.g_thr >= 0.0) return false;
+    if (a == 0.0 && g_thr >= 0.0) return false;
+    const mwg_ftree f = ld(S.f);
     const int id = f_lookup(f, d, x, y, z);
     double maxf;
     double fnv[8]; // per-component norms of the band node (or their bound)
@@ -79,26 +81,27 @@ __device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f,
         for (int t = 1; t < 8; t++) fnv[t] = 0.0;
         *code = -(anc + 1);
     }
-    if (!(a * maxf * kBoundSlack > lvl.g_thr)) return false;
+    if (!(a * maxf * kBoundSlack > g_thr)) return false;
     // Per-term bound, resolved per f component ft: the direction-d factor of
     // a tuple is the norm of component 2*gbit_d + ftbit_d, so over all gt it is
     // at most max(n_d(ftbit_d), n_d(2+ftbit_d)); at depth > 0 the (ft,gt) =
     // (0,0) tuple is skipped, so some direction carries the gt bit.
-    const int M = op.M;
+    const int M = S.op.M;
     const int dl3[3] = {x - gl[0], y - gl[1], z - gl[2]};
     const int maxdl = max(abs(dl3[0]), max(abs(dl3[1]), abs(dl3[2])));
-    const int *wd = op.widths + (size_t)d * M * 5;
-    const long long *ms = op.mat_start + (size_t)d * M;
-    const int *bsm = op.band_size_max + (size_t)d * M;
+    const int *wd = S.op.widths + (size_t)d * M * 5;
+    const long long *ms = S.op.mat_start + (size_t)d * M;
+    const int *bsm = S.op.band_size_max + (size_t)d * M;
+    const double *term_norm = S.op.term_norm, *mat_norm = S.op.mat_norm;
     for (int i = 0; i < M; i++) {
         const int *wi = wd + i * 5;
         const int w = wi[4];
         if (maxdl > w) continue;
         // cheap per-term bound first: largest component norm per direction
-        const double *tn = op.term_norm + ms[i] / 4;
+        const double *tn = term_norm + ms[i] / 4;
         const double bs = (double)bsm[i];
-        if (!(bs * tn[dl3[0] + w] * tn[dl3[1] + w] * tn[dl3[2] + w] * maxf * kBoundSlack > lvl.g_thr)) continue;
-        const double *mn = op.mat_norm + ms[i];
+        if (!(bs * tn[dl3[0] + w] * tn[dl3[1] + w] * tn[dl3[2] + w] * maxf * kBoundSlack > g_thr)) continue;
+        const double *mn = mat_norm + ms[i];
         double n[3][4];
         for (int dim = 0; dim < 3; dim++)
             for (int c = 0; c < 4; c++) n[dim][c] = (abs(dl3[dim]) <= wi[c]) ? mn[(dl3[dim] + w) * 4 + c] : 0.0;
@@ -116,78 +119,121 @@ __device__ inline bool screen_position(const mwg_level &lvl, const mwg_ftree &f,
                 prod = fmax(g1[0] * gm[1] * gm[2], fmax(gm[0] * g1[1] * gm[2], gm[0] * gm[1] * g1[2]));
             else
                 prod = gm[0] * gm[1] * gm[2];
-            if (bs * prod * fnv[ft] * kBoundSlack > lvl.g_thr) return true;
+            if (bs * prod * fnv[ft] * kBoundSlack > g_thr) return true;
         }
     }
     return false;
 }
 
-// One thread per band position; block-local compaction in position order.
-__global__ __launch_bounds__(kScreenThreads) void k_screen_positions(mwg_level lvl, mwg_ftree f, mwg_band_tables op, int nb, int *cand_pos,
-                                                                    int *cand_code, int *blk_count) {
-    __shared__ int tmp[kScreenThreads / 64];
+// Screen blocks per output node of a segment (one for an empty band, so its
+// nodes still get their zero pair counts).
+__host__ __device__ inline int screen_blocks(int band_w) {
+    if (band_w < 0) return 1;
+    const long long box = (2ll * band_w + 1) * (2ll * band_w + 1) * (2ll * band_w + 1);
+    return (int)((box + kScreenThreads - 1) / kScreenThreads);
+}
+
+// Segment, output node and block of a screen workgroup (seg_blk: first block
+// of every segment, nseg + 1 entries).
+struct ScreenBlock {
+    int s, g, b;
+};
+__device__ inline ScreenBlock locate(const int *seg_blk, int nseg, const_as4<mwg_segment> *segs) {
     const int blk = blockIdx.x;
-    const int g = blk / nb, b = blk - g * nb;
-    const int d = lvl.o_depth;
-    const int gl[3] = {lvl.l[3 * g], lvl.l[3 * g + 1], lvl.l[3 * g + 2]};
-    const Box bx = band_box(f, d, op.band_max[d], gl);
+    int lo = 0, hi = nseg - 1; // last segment whose first block <= blk
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) / 2;
+        if (seg_blk[mid] <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    const int nb = screen_blocks(segs[lo].band_w);
+    const int rel = blk - seg_blk[lo];
+    return {lo, segs[lo].first + rel / nb, rel % nb};
+}
+
+__device__ inline Box segment_box(const_as4<mwg_segment> &S, const int *gl) {
+    if (S.band_w < 0) {
+        Box b;
+        for (int i = 0; i < 3; i++) {
+            b.s[i] = gl[i];
+            b.n[i] = 0;
+        }
+        return b;
+    }
+    const mwg_ftree f = ld(S.f);
+    return band_box(f, S.o_depth, S.op.band_max[S.o_depth], gl);
+}
+
+// One thread per band position; block-local compaction in position order.
+__global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_segment *segs_, int nseg, const int *seg_blk, const int *l, int *cand_pos,
+                                                                    int *cand_code, int *blk_count, int *count) {
+    __shared__ int tmp[kScreenThreads / 64];
+    const auto segs = ro(segs_);
+    const ScreenBlock sb = locate(seg_blk, nseg, segs);
+    const_as4<mwg_segment> &S = segs[sb.s];
+    const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
+    const Box bx = segment_box(S, gl);
     const int P = bx.n[0] * bx.n[1] * bx.n[2];
-    const int p = b * kScreenThreads + threadIdx.x;
+    const int p = sb.b * kScreenThreads + threadIdx.x;
+    if (sb.b == 0 && threadIdx.x == 0 && S.band_w >= 0 && sb.g == S.first) {
+        // the block count covers the box only if band_w is the table's band width
+        if (S.op.band_max[S.o_depth] != S.band_w) count[2] = 1;
+    }
     int code = 0;
     bool ok = false;
     if (p < P) {
         const int x = bx.s[0] + p % bx.n[0];
         const int y = bx.s[1] + (p / bx.n[0]) % bx.n[1];
         const int z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
-        ok = screen_position(lvl, f, op, gl, x, y, z, &code);
+        ok = screen_position(S, gl, x, y, z, &code);
     }
     int excl;
     const int total = block_exclusive_scan<int, kScreenThreads>(ok ? 1 : 0, tmp, excl);
     if (ok) {
-        cand_pos[(size_t)blk * kScreenThreads + excl] = p;
-        cand_code[(size_t)blk * kScreenThreads + excl] = code;
+        cand_pos[(size_t)blockIdx.x * kScreenThreads + excl] = p;
+        cand_code[(size_t)blockIdx.x * kScreenThreads + excl] = code;
     }
-    if (threadIdx.x == 0) blk_count[blk] = total;
+    if (threadIdx.x == 0) blk_count[blockIdx.x] = total;
 }
 
-__global__ void k_screen_reset(int *count) { count[1] = 0; }
+__global__ void k_screen_reset(int *count) {
+    count[1] = 0;
+    count[2] = 0;
+}
 
 // Place the survivors of every block at their scanned offset.
-__global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(mwg_level lvl, mwg_ftree f, mwg_band_tables op, int nb, const int *cand_pos,
+__global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(const mwg_segment *segs_, int nseg, const int *seg_blk, const int *l, const int *cand_pos,
                                                                   const int *cand_code, const int *blk_off, mwg_pairs out) {
+    const auto segs = ro(segs_);
+    const ScreenBlock sb = locate(seg_blk, nseg, segs);
     const int blk = blockIdx.x;
-    const int g = blk / nb, b = blk - g * nb;
-    const int d = lvl.o_depth;
     const int base = blk_off[blk];
     const int cnt = blk_off[blk + 1] - base;
     if (threadIdx.x < cnt) {
-        const int gl[3] = {lvl.l[3 * g], lvl.l[3 * g + 1], lvl.l[3 * g + 2]};
-        const Box bx = band_box(f, d, op.band_max[d], gl);
+        const_as4<mwg_segment> &S = segs[sb.s];
+        const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
+        const Box bx = segment_box(S, gl);
         const int p = cand_pos[(size_t)blk * kScreenThreads + threadIdx.x];
         const int q = base + threadIdx.x;
         const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
         if (code < 0) atomicAdd(out.count + 1, 1);
         if (q < out.capacity) {
-            out.pair_g[q] = g;
+            out.pair_g[q] = sb.g;
             out.pair_f[q] = code;
             out.pair_l[3 * q + 0] = bx.s[0] + p % bx.n[0];
             out.pair_l[3 * q + 1] = bx.s[1] + (p / bx.n[0]) % bx.n[1];
             out.pair_l[3 * q + 2] = bx.s[2] + p / (bx.n[0] * bx.n[1]);
         }
     }
-    if (b == 0 && threadIdx.x == 0) {
-        out.g_first[g] = base;
-        out.g_npairs[g] = blk_off[(g + 1) * nb] - base;
+    if (sb.b == 0 && threadIdx.x == 0) {
+        const int nb = screen_blocks(segs[sb.s].band_w);
+        out.g_first[sb.g] = base;
+        out.g_npairs[sb.g] = blk_off[blk + nb] - base;
     }
-    if (blk == 0 && threadIdx.x == 0) {
-        // the grid covers the box only if the host's band width is the table's
-        const int W = op.band_max[d];
-        const long long need = (long long)(2 * W + 1) * (2 * W + 1) * (2 * W + 1);
-        *out.count = (need <= (long long)nb * kScreenThreads) ? blk_off[gridDim.x] : -1;
-    }
+    if (blk == 0 && threadIdx.x == 0) *out.count = blk_off[gridDim.x];
 }
 
-Buffer g_cand_pos, g_cand_code, g_blk;
+Buffer g_cand_pos, g_cand_code, g_blk, g_seg_blk;
 
 } // namespace
 
@@ -195,34 +241,39 @@ Buffer g_cand_pos, g_cand_code, g_blk;
 
 using namespace mwgdev;
 
-extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, const mwg_ftree *f, const mwg_band_tables *op, mwg_pairs *out) {
+extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
-    if (!lvl || !f || !op || !out || !out->count) return fail(MWG_ERR_ARG, "null argument");
-    if (lvl->o_depth < 0 || lvl->o_depth >= op->n_depth) return fail(MWG_ERR_ARG, "level depth outside the band tables");
-    if (lvl->o_depth > mwgkey::kMaxDepth) return fail(MWG_ERR_UNSUPPORTED, "level deeper than the device key range");
-    if (lvl->band_w < 0) return fail(MWG_ERR_ARG, "negative band width");
-    if (lvl->count <= 0) {
-        MWG_HIP(hipMemset(out->count, 0, 2 * sizeof(int)));
+    if (!out || !out->count) return fail(MWG_ERR_ARG, "null argument");
+    mwg_status st = check_level(lvl);
+    if (st != MWG_OK) return st;
+    if (lvl->count == 0) {
+        MWG_HIP(hipMemset(out->count, 0, 3 * sizeof(int)));
         return MWG_OK;
     }
-    const long long W = lvl->band_w;
-    const long long box = (2 * W + 1) * (2 * W + 1) * (2 * W + 1);
-    const long long nb = (box + kScreenThreads - 1) / kScreenThreads;
-    const long long nblk = nb * lvl->count;
-    if (nblk >= (1ll << 31) / kScreenThreads) return fail(MWG_ERR_UNSUPPORTED, "level too large for one screen launch");
+    std::vector<int> seg_blk(lvl->nseg + 1, 0);
+    long long nblk = 0;
+    for (int s = 0; s < lvl->nseg; s++) {
+        seg_blk[s] = (int)nblk;
+        nblk += (long long)screen_blocks(lvl->segs[s].band_w) * lvl->segs[s].count;
+        if (nblk >= (1ll << 31) / kScreenThreads) return fail(MWG_ERR_UNSUPPORTED, "level too large for one screen launch");
+    }
+    seg_blk[lvl->nseg] = (int)nblk;
+    LevelDevice dev;
+    if ((st = upload_level(lvl, dev)) != MWG_OK) return st;
     if (g_cand_pos.ensure(sizeof(int) * (size_t)nblk * kScreenThreads) != MWG_OK) return MWG_ERR_NOMEM;
     if (g_cand_code.ensure(sizeof(int) * (size_t)nblk * kScreenThreads) != MWG_OK) return MWG_ERR_NOMEM;
     if (g_blk.ensure(sizeof(int) * ((size_t)nblk + 1)) != MWG_OK) return MWG_ERR_NOMEM;
+    if (g_seg_blk.ensure(sizeof(int) * seg_blk.size()) != MWG_OK) return MWG_ERR_NOMEM;
+    if ((st = mwg_upload_async(g_seg_blk.p, seg_blk.data(), sizeof(int) * seg_blk.size())) != MWG_OK) return st;
     ProfileScope prof_(MWG_KERNEL_SCREEN);
     hipLaunchKernelGGL(k_screen_reset, dim3(1), dim3(1), 0, 0, out->count);
     MWG_LAUNCHED();
-    hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, *lvl, *f, *op, (int)nb, g_cand_pos.as<int>(),
-                       g_cand_code.as<int>(), g_blk.as<int>());
+    hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
+                       g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
     MWG_LAUNCHED();
-    const mwg_status s = scan_exclusive(g_blk.as<int>(), g_blk.as<int>(), (int)nblk);
-    if (s != MWG_OK) return s;
-    hipLaunchKernelGGL(k_screen_scatter, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, *lvl, *f, *op, (int)nb, g_cand_pos.as<int>(),
-                       g_cand_code.as<int>(), g_blk.as<int>(), *out);
+    if ((st = scan_exclusive(g_blk.as<int>(), g_blk.as<int>(), (int)nblk)) != MWG_OK) return st;
+    hipLaunchKernelGGL(k_screen_scatter, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
+                       g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), *out);
     MWG_LAUNCHED();
     return MWG_OK;
 }

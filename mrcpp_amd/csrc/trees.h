@@ -272,6 +272,26 @@ public:
         return first;
     }
 
+    // Remove the subtree below u (its nodes leave the index and the depth
+    // counts; their slots stay allocated and unreachable).
+    void detachChildren(int u) {
+        if (!hasChildren(u)) return;
+        std::vector<int> stack = {u};
+        while (!stack.empty()) {
+            const int v = stack.back();
+            stack.pop_back();
+            if (!hasChildren(v)) continue;
+            for (int c = 0; c < TD; c++) {
+                const int w = child0_[v] + c;
+                index_.erase({n_[w], l_[w]});
+                if (!gen_[w]) nodes_at_depth_[n_[w] - rootScale()]--;
+                stack.push_back(w);
+            }
+            if (v != u) child0_[v] = -1;
+        }
+        child0_[u] = -1;
+    }
+
     // Drop every generated child block (FunctionTree::deleteGenerated).  Ids of
     // generated nodes are recycled only by truncation, so generated nodes are
     // expected at the tail (they are created after all real nodes of a build).

@@ -91,3 +91,17 @@ def batch_sources(seed=77, n_trees=64):
         mu = None if t % 2 == 0 else rng.uniform(0.5, 3.0)
         out.append(dict(sources=src, order=7, box=32, prec=prec, mu=mu))
     return out
+
+
+def predictor_sources(seed=2024, n_trees=250):
+    """Trees for the refinement predictor: 4..32 Gaussians, centres
+    U[-6,6]^3, exponents log-uniform in [1,1000], order 7, prec 1e-6; even
+    index Poisson, odd index Helmholtz with mu ~ U[0.5,3]."""
+    rng = SplitMix64(seed)
+    out = []
+    for t in range(n_trees):
+        n = rng.randint(4, 32)
+        src = random_sources(rng.next_u64(), n, 6.0)
+        mu = None if t % 2 == 0 else rng.uniform(0.5, 3.0)
+        out.append(dict(sources=src, order=7, box=32, prec=1e-6, mu=mu))
+    return out

@@ -1,0 +1,130 @@
+"""Refinement predictor on the MI355X.
+
+1. The device kernels of the 12-64-64-1 perceptron (forward, gradient, Adam)
+   against the float64 CPU restatement oracle/mlp.py (itself pinned by finite
+   differences): FP32 tolerances stated in each check.
+2. Trained on refine labels emitted by the plain adaptive apply, the
+   predictor grows the output grid after the first step; on held-out trees
+   the result has the plain apply's node set, coefficients within 10*prec of
+   it, and needs fewer device passes -- the gain of the trained network over
+   the method without it.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+from mrcpp_amd import workloads as W
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+class Dev:
+    """Device buffers through the library's C ABI."""
+
+    def __init__(self, mr):
+        self.L = mr.lib()
+        assert self.L.mwg_init(0) == 0
+        self.bufs = []
+
+    def put(self, a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert self.L.mwg_malloc(ctypes.byref(p), max(a.nbytes, 4)) == 0
+        assert self.L.mwg_upload(p, a.ctypes.data, a.nbytes) == 0
+        self.bufs.append(p)
+        return p
+
+    def get(self, p, n, dtype=np.float32):
+        out = np.zeros(n, dtype=dtype)
+        assert self.L.mwg_download(out.ctypes.data, p, out.nbytes) == 0
+        return out
+
+    def free(self):
+        for p in self.bufs:
+            self.L.mwg_free(p)
+
+
+def test_mlp_kernels_match_oracle(mr):
+    rng = np.random.default_rng(11)
+    n = 1531
+    p = (rng.normal(0.0, 0.25, mlp.N_PARAMS)).astype(np.float32)
+    X = rng.normal(0.0, 1.0, (n, mlp.F)).astype(np.float32)
+    Y = (rng.random(n) > 0.7).astype(np.float32)
+    d = Dev(mr)
+    dp, dx, dy = d.put(p), d.put(X), d.put(Y)
+    dprob, dgrad = d.put(np.zeros(n, np.float32)), d.put(np.zeros(mlp.N_PARAMS + 1, np.float32))
+    assert d.L.mwg_mlp_forward(dp, dx, n, dprob) == 0
+    prob = d.get(dprob, n)
+    z, _ = mlp.forward(p, X)
+    assert np.allclose(prob, mlp.sigmoid(z), atol=2e-6, rtol=1e-5)
+    assert d.L.mwg_mlp_gradient(dp, dx, dy, n, ctypes.c_float(2.5), dgrad) == 0
+    g = d.get(dgrad, mlp.N_PARAMS + 1)
+    ref = mlp.gradient(p, X, Y, 2.5)
+    err = np.abs(g[:-1] - ref).max() / np.abs(ref).max()
+    _say("gradient max error / max |grad|", err, "loss", g[-1], mlp.loss(p, X, Y, 2.5))
+    assert err <= 1e-4
+    assert g[-1] == pytest.approx(mlp.loss(p, X, Y, 2.5), rel=1e-5)
+    # determinism of the ordered reduction
+    assert d.L.mwg_mlp_gradient(dp, dx, dy, n, ctypes.c_float(2.5), dgrad) == 0
+    assert np.array_equal(d.get(dgrad, mlp.N_PARAMS + 1), g)
+    # one Adam step
+    dm, dv = d.put(np.zeros(mlp.N_PARAMS, np.float32)), d.put(np.zeros(mlp.N_PARAMS, np.float32))
+    assert d.L.mwg_mlp_adam(dp, dgrad, dm, dv, 1, ctypes.c_float(1e-3), ctypes.c_float(0.9), ctypes.c_float(0.999), ctypes.c_float(1e-8)) == 0
+    q = d.get(dp, mlp.N_PARAMS)
+    qref, _, _ = mlp.adam(p.astype(np.float64), g[:-1].astype(np.float64), 0.0, 0.0, 1, 1e-3)
+    assert np.allclose(q, qref, atol=1e-6)
+    d.free()
+
+
+def make_tree(mr, mra, seed, prec, mu, order):
+    rng = np.random.default_rng(seed)
+    src = W.random_sources(seed, int(rng.integers(2, 9)), center_range=4.0, beta_range=(2.0, 500.0))
+    f = mr.FunctionTree(mra)
+    gx = mr.GaussExp.from_sources(src)
+    mr.build_grid(f, gx)
+    mr.project(prec, f, gx)
+    P = mr.PoissonOperator(mra, prec) if mu is None else mr.HelmholtzOperator(mra, mu, prec)
+    return src, f, P
+
+
+def test_trained_predictor_saves_passes(mr):
+    order, box = 5, 20
+    mra = mr.MultiResolutionAnalysis((-box, box), order)
+    specs = [(100 + i, [1e-4, 1e-5][i % 2], None if i % 2 == 0 else 0.5 + 0.25 * i) for i in range(16)]
+    trees = [make_tree(mr, mra, s, prec, mu, order) for s, prec, mu in specs]
+    train, held = list(range(12)), list(range(12, 16))
+    # training samples from the plain adaptive apply
+    outs = [mr.FunctionTree(mra) for _ in train]
+    _, X, Y = mr.apply_predicted([(specs[i][1], o, trees[i][2], trees[i][1]) for i, o in zip(train, outs)],
+                                 centers=[trees[i][0].centers for i in train], collect=True)
+    _say("samples", len(Y), "refined fraction", float(Y.mean()))
+    pred = mr.RefinePredictor()
+    losses = pred.train(X, Y, epochs=40, batch=2048, lr=3e-3, pos_weight=3.0, seed=3)
+    _say("losses", losses[0], losses[-1])
+    assert losses[-1] < 0.6 * losses[0]
+    pred.threshold = 0.35
+    plain_passes = pred_passes = 0
+    for i in held:
+        src, f, P = trees[i]
+        prec = specs[i][1]
+        g0 = mr.FunctionTree(mra)
+        st0 = mr.apply_predicted([(prec, g0, P, f)])[0]
+        g1 = mr.FunctionTree(mra)
+        st1 = mr.apply_predicted([(prec, g1, P, f)], predictor=pred, centers=[src.centers])[0]
+        i0, c0, _ = g0.export()
+        i1, c1, _ = g1.export()
+        _say("tree", i, "passes plain", st0["n_levels"], "predicted", st1["n_levels"], "nodes", len(i0), len(i1), "predicted",
+             st1["n_predicted"], "cropped", st1["n_cropped"], "fallback", st1["n_fallback"])
+        assert np.array_equal(i0, i1), "node sets differ"
+        assert np.linalg.norm(c1 - c0) / np.linalg.norm(c0) <= 10 * prec
+        plain_passes += st0["n_levels"]
+        pred_passes += st1["n_levels"]
+    _say("device passes: plain", plain_passes, "with predictor", pred_passes)
+    assert pred_passes < plain_passes
