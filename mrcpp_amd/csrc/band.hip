@@ -204,6 +204,8 @@ struct Group {
     int term, ft;
     unsigned mask;
     bool fresh_f; // band node or ft differs from the previous group's
+    int dl[3];    // band node translation minus output translation
+    const double *fsrc;
 };
 
 // Walks the groups of one unit; holds the unit's segment tables.
@@ -261,6 +263,8 @@ struct UnitCursor {
         grp.mask = code >> 24;
         grp.fresh_f = fresh || grp.ft != curft;
         curft = grp.ft;
+        for (int i = 0; i < 3; i++) grp.dl[i] = dl[i];
+        grp.fsrc = fsrc;
         return true;
     }
     // the unit's result: the output node itself (first unit) or a partial slot
@@ -324,7 +328,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
             const unsigned gm = grp.mask;
             if (grp.fresh_f) {
                 __syncthreads();
-                for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = cur.fsrc[(size_t)ft * N3 + e];
+                for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = grp.fsrc[(size_t)ft * N3 + e];
                 __syncthreads();
             }
             const int w4 = cur.wd[grp.term * 5 + 4];
@@ -335,7 +339,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
             for (int gx = 0; gx < 2; gx++) {
                 if (!(xneed >> gx & 1)) continue;
                 // x sweep: T1[i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox(i0, j0)
-                const auto Ox = mats + (size_t)(mbase + (long long)(cur.dl[0] + w4) * 4 + 2 * gx + (ft & 1)) * N2;
+                const auto Ox = mats + (size_t)(mbase + (long long)(grp.dl[0] + w4) * 4 + 2 * gx + (ft & 1)) * N2;
                 if (owner) {
                     double x[N1];
 #pragma unroll
@@ -354,7 +358,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
                 for (int gy = 0; gy < 2; gy++) {
                     if (!(yneed >> (gx + 2 * gy) & 1)) continue;
                     // y sweep: T2[i2][j0 + N1 j1] = sum_i1 T1[i1][i2 + N1 j0] Oy(i1, j1)
-                    const auto Oy = mats + (size_t)(mbase + (long long)(cur.dl[1] + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
+                    const auto Oy = mats + (size_t)(mbase + (long long)(grp.dl[1] + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
                     if (owner) {
                         double y[N1];
 #pragma unroll
@@ -378,7 +382,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
                         for (int gz = 0; gz < 2; gz++) {
                             const int gt = gx + 2 * gy + 4 * gz;
                             if (!(gm >> gt & 1)) continue;
-                            const auto Oz = mats + (size_t)(mbase + (long long)(cur.dl[2] + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
+                            const auto Oz = mats + (size_t)(mbase + (long long)(grp.dl[2] + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
 #pragma unroll
                             for (int j = 0; j < N1; j++) {
                                 double s = 0.0;
@@ -523,32 +527,42 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
 #pragma unroll
             for (int nt = 0; nt < NT; nt++) acc[mt][nt] = dvec4{0.0, 0.0, 0.0, 0.0};
 
-        Group grp;
-        while (cur.next(v, grp)) {
-            const int ft = grp.ft;
-            const unsigned gm = grp.mask;
-            unsigned xneed, yneed;
-            sweep_needs(gm, xneed, yneed);
-            // operand fragments: matrix block b4 = mat_start/4 + w + dl of each direction
-            const long long b40 = cur.ms[grp.term] / 4 + cur.wd[grp.term * 5 + 4];
-            BT bx[NT][KSTEPS], by[NT][KSTEPS], bz[NT][KSTEPS];
+        // operand fragments of a group: matrix block b4 = mat_start/4 + w + dl of
+        // each direction; the z columns of a gz outside this wave's mask zeroed
+        auto fetch = [&](const Group &q, BT (&fx)[NT][KSTEPS], BT (&fy)[NT][KSTEPS], BT (&fz)[NT][KSTEPS]) {
+            const long long b40 = cur.ms[q.term] / 4 + cur.wd[q.term * 5 + 4];
 #pragma unroll
             for (int nt = 0; nt < NT; nt++) {
                 const int nn = 16 * nt + li;
                 const int gz = nn / N1;
-                const bool keep = nn < 2 * N1 && ((gm >> (gxw + 2 * gyw + 4 * gz)) & 1);
+                const bool keep = nn < 2 * N1 && ((q.mask >> (gxw + 2 * gyw + 4 * gz)) & 1);
 #pragma unroll
                 for (int ks = 0; ks < KSTEPS; ks++) {
                     const size_t o = ((size_t)ks * NT + nt) * 64 + lane;
-                    bx[nt][ks] = frag[((b40 + cur.dl[0]) * 2 + (ft & 1)) * FRAG + o];
-                    by[nt][ks] = frag[((b40 + cur.dl[1]) * 2 + ((ft >> 1) & 1)) * FRAG + o];
-                    const BT zv = frag[((b40 + cur.dl[2]) * 2 + ((ft >> 2) & 1)) * FRAG + o];
-                    bz[nt][ks] = keep ? zv : BT{};
+                    fx[nt][ks] = frag[((b40 + q.dl[0]) * 2 + (q.ft & 1)) * FRAG + o];
+                    fy[nt][ks] = frag[((b40 + q.dl[1]) * 2 + ((q.ft >> 1) & 1)) * FRAG + o];
+                    const BT zv = frag[((b40 + q.dl[2]) * 2 + ((q.ft >> 2) & 1)) * FRAG + o];
+                    fz[nt][ks] = keep ? zv : BT{};
                 }
             }
+        };
+        // software pipeline: the next group's code is read and its fragments
+        // requested before the current group's sweeps
+        Group grp, nxt;
+        BT bx[NT][KSTEPS], by[NT][KSTEPS], bz[NT][KSTEPS];
+        BT nx[NT][KSTEPS], ny[NT][KSTEPS], nz[NT][KSTEPS];
+        bool have = cur.next(v, grp);
+        if (have) fetch(grp, bx, by, bz);
+        while (have) {
+            const bool have_next = cur.next(v, nxt);
+            if (have_next) fetch(nxt, nx, ny, nz);
+            const int ft = grp.ft;
+            const unsigned gm = grp.mask;
+            unsigned xneed, yneed;
+            sweep_needs(gm, xneed, yneed);
             if (grp.fresh_f) {
                 // F[i0][i1 + N1 i2] for element e = i0 + N1 (i1 + N1 i2)
-                for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)cur.fsrc[(size_t)ft * N3 + e];
+                for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)grp.fsrc[(size_t)ft * N3 + e];
                 __syncthreads();
             }
             // x sweep: T1[gx][i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox_gx(i0, j0)
@@ -599,6 +613,16 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
                         }
                     }
             }
+            if (!have_next) break;
+            grp = nxt;
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) {
+                    bx[nt][ks] = nx[nt][ks];
+                    by[nt][ks] = ny[nt][ks];
+                    bz[nt][ks] = nz[nt][ks];
+                }
         }
         // this wave's blocks gt = gx + 2 gy + 4 gz of the unit result
         double *dst = cur.result(partial, 8 * N3);
