@@ -48,9 +48,9 @@ def copy_probe():
     print("copied %d bytes x3" % COPY_PROBE_BYTES, flush=True)
 
 
-def measured_traffic(workload):
+def measured_traffic(workload, precision="fp64"):
     """HBM bytes per band-kernel launch from the committed counter summary
-    (lines 'workload: <name>' and 'traffic_bytes_per_launch: <int>')."""
+    (lines 'workload: <name>' and 'traffic_bytes_per_launch_<precision>: <int>')."""
     path = os.path.join(HERE, "profiles", "%s_%s_traffic.md" % (BAND_KERNEL, workload))
     if not os.path.exists(path):
         return None, None
@@ -59,9 +59,10 @@ def measured_traffic(workload):
         if ":" in line and not line.startswith(("#", "`", " ")):
             k, v = line.split(":", 1)
             kv[k.strip()] = v.strip()
-    if kv.get("workload") != workload or "traffic_bytes_per_launch" not in kv:
+    key = "traffic_bytes_per_launch_" + precision
+    if kv.get("workload") != workload or key not in kv:
         return None, None
-    return int(float(kv["traffic_bytes_per_launch"])), os.path.relpath(path, HERE)
+    return int(float(kv[key])), os.path.relpath(path, HERE)
 
 
 def flops_per_tuple(order):
@@ -173,10 +174,13 @@ def run_batch_bench(args):
         mr.device_synchronize()
         return outs, sts
 
+    pred = None if args.no_predictor else load_predictor(mr, 7)
+    centers = [np.asarray(sp["sources"].centers) for sp in specs]
+
     def chosen_step():
         mr.set_sweep_precision(args.precision)
         outs = [mr.FunctionTree(mra) for _ in trees]
-        sts = mr.apply_batch([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)])
+        sts = mr.apply_predicted([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)], predictor=pred, centers=centers)
         mr.device_synchronize()
         return outs, sts
 
@@ -215,7 +219,8 @@ def run_batch_bench(args):
         "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64" if args.precision in ("fp64", "fp64_vector") else args.precision + " sweeps, f64 band sum",
-        "precision": args.precision, "data": "synthetic seeded Gaussian sources",
+        "precision": args.precision, "predictor": pred is not None, "data": "synthetic seeded Gaussian sources",
+        "passes_chosen": int(max(st["n_levels"] for st in c_sts)), "passes_plain": int(max(st["n_levels"] for st in p_sts)),
         "config": {"workload": "batch64_k7", "trees": n, "order": 7, "box": [-32, 32], "precs": sorted({sp["prec"] for sp in specs}),
                    "n_gaussians": [len(sp["sources"]) for sp in specs], "operators": "Poisson (even) / Helmholtz mu~U[0.5,3] (odd)",
                    "n_applied": int(sum(st["n_applied"] for st in p_sts)), "parallelism": "single GPU"},
@@ -233,6 +238,52 @@ def run_batch_bench(args):
     print(json.dumps(res))
 
 
+PEAKS = {"fp64": FP64_PEAK_TFLOPS, "fp64_vector": FP64_PEAK_TFLOPS, "fp32": 157.3, "bf16x3": 2500.0}
+DTYPES = {"fp64": "f64", "fp64_vector": "f64", "fp32": "f32 sweeps, f64 band sum", "bf16x3": "bf16x3 sweeps, f64 band sum"}
+PREDICTOR_PARAMS = os.path.join(HERE, "mrcpp_amd", "share", "predictor_k7.npy")
+
+
+def load_predictor(mr, order):
+    """The trained refinement predictor (order 7 only), or None."""
+    import numpy as np
+    if order != 7 or not os.path.exists(PREDICTOR_PARAMS):
+        return None
+    return mr.RefinePredictor(params=np.load(PREDICTOR_PARAMS), threshold=0.35)
+
+
+def time_form(mr, run_one, steps, warmup):
+    """warmup untimed runs, then steps timed runs with per-kernel HIP-event timing."""
+    for _ in range(max(warmup, 1)):
+        run_one()
+    mr.device_synchronize()
+    mr.profile(True)
+    mr.device_synchronize()
+    times, stats = [], []
+    t0 = time.perf_counter()
+    out = None
+    for _ in range(steps):
+        ts = time.perf_counter()
+        out, st = run_one()
+        mr.device_synchronize()
+        times.append(time.perf_counter() - ts)
+        stats.append(st)
+    mr.device_synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = mr.profile_read()
+    mr.profile(False)
+    return dict(s=elapsed / steps, times=times, stats=stats, prof=prof, out=out)
+
+
+def roofline(prof, flop, steps, precision, workload):
+    band_ms, launches = prof["band"]
+    achieved = flop / (band_ms / steps / 1e3) / 1e12  # algorithmic TFLOP/s while the band kernel runs
+    peak = PEAKS[precision]
+    traffic, src = measured_traffic(workload, precision)
+    return {"bound": "mfma", "kernel": BAND_KERNEL, "precision": precision, "achieved": round(achieved, 4), "peak": peak,
+            "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": traffic, "traffic_source": src,
+            "launches_per_step": launches / steps, "avg_launch_ms": band_ms / max(launches, 1), "flop_per_launch": flop / max(launches / steps, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,90 +292,84 @@ def main():
     ap.add_argument("--workload", default="gauss16_k7")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp64_vector"],
-                    help="band sweep precision (fp64 = the drop-in path)")
+    ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3", "fp64_vector"],
+                    help="band sweep precision of the chosen form (the plain form is always fp64)")
+    ap.add_argument("--no-predictor", action="store_true", help="chosen form without the refinement predictor")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X: the apply path runs on a single GPU "
+              "(multi-GPU sharding is described in DESIGN.md, not launched); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         copy_probe()
         return
     if args.workload == "batch64_k7":
         run_batch_bench(args)
         return
-    if args.gpus != 1:
-        print("bench.py measures one MI355X: the apply path runs on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md, not launched); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
 
+    import numpy as np
     mr, w, src, mra, f, P = setup(args.workload)
-    order = w["order"]
-    mr.set_sweep_precision(args.precision)
-    for _ in range(max(args.warmup, 1)):
-        out = mr.FunctionTree(mra)
-        st = mr.apply(w["prec"], out, P, f)
-        del out
-    mr.device_synchronize()
+    order, prec = w["order"], w["prec"]
+    pred = None if args.no_predictor else load_predictor(mr, order)
 
-    # timed region: K full applies
-    times, stats = [], []
-    mr.profile(True)
-    mr.device_synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        out = mr.FunctionTree(mra)
-        st = mr.apply(w["prec"], out, P, f)
-        mr.device_synchronize()
-        times.append(time.perf_counter() - ts)
-        stats.append(st)
-        del out
-    mr.device_synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = mr.profile_read()
-    mr.profile(False)
+    def plain_one():
+        mr.set_sweep_precision("fp64")
+        g = mr.FunctionTree(mra)
+        return g, mr.apply(prec, g, P, f)
 
-    n_applied = stats[-1]["n_applied"]
-    flop = n_applied * flops_per_tuple(order)
-    s_per_tree = elapsed / args.steps
-    gflops = flop / s_per_tree / 1e9
-    band_ms, band_launches = prof["band"]
-    band_ms_per_apply = band_ms / args.steps
-    band_achieved = flop / (band_ms_per_apply / 1e3) / 1e12  # TFLOP/s while the band kernel runs
-    traffic, traffic_src = measured_traffic(args.workload)
+    def chosen_one():
+        mr.set_sweep_precision(args.precision)
+        g = mr.FunctionTree(mra)
+        st = mr.apply_predicted([(prec, g, P, f)], predictor=pred, centers=[src.centers])[0]
+        return g, st
+
+    plain = time_form(mr, plain_one, args.steps, args.warmup)
+    chosen = time_form(mr, chosen_one, args.steps, args.warmup)
+    mr.set_sweep_precision("fp64")
+    pst, cst = plain["stats"][-1], chosen["stats"][-1]
+    n_applied = pst["n_applied"]
+    flop = n_applied * flops_per_tuple(order)  # algorithmic work: tuples under the plain screening
+    i0, c0, _ = plain["out"].export()
+    i1, c1, _ = chosen["out"].export()
+    same = bool(np.array_equal(i0, i1))
+    rel = float(np.linalg.norm(c1 - c0) / np.linalg.norm(c0)) if same else None
     res = {
         "metric": METRIC,
-        "value": round(gflops, 3),
+        "value": round(flop / chosen["s"] / 1e9, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(s_per_tree * 1e3, 4),
-        "s_per_tree": s_per_tree,
+        "ms_per_step": round(chosen["s"] * 1e3, 4),
+        "s_per_tree": chosen["s"],
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp64_vector": "f64", "fp32": "f32 sweeps, f64 band sum", "bf16x3": "bf16x3 sweeps, f64 band sum"}[args.precision],
-        "precision": args.precision,
+        "dtype": DTYPES[args.precision],
         "data": "synthetic seeded Gaussian sources",
-        "config": {"workload": args.workload, "order": order, "prec": w["prec"], "box": [-w["box"], w["box"]],
+        "config": {"workload": args.workload, "order": order, "prec": prec, "box": [-w["box"], w["box"]],
                    "n_gaussians": len(src), "operator": "Poisson" if w["mu"] is None else "Helmholtz mu=%g" % w["mu"],
                    "separation_rank": P.size(), "n_applied": int(n_applied),
-                   "n_candidate_pairs": int(stats[-1]["n_pairs"]), "n_generated_nodes": int(stats[-1]["n_generated"]),
-                   "n_output_nodes": int(stats[-1]["n_output_nodes"]), "levels": int(stats[-1]["n_levels"]),
-                   "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(band_achieved, 4),
-                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(band_achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": traffic, "traffic_source": traffic_src, "launches_per_step": band_launches / args.steps,
-                     "avg_launch_ms": band_ms / max(band_launches, 1),
-                     "flop_per_launch": flop / max(band_launches / args.steps, 1)},
-        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in prof.items()},
-        "host_breakdown_s": {k: stats[-1][k] for k in ("t_screen", "t_generate", "t_band", "t_reduce", "t_transform", "t_host")},
-        "step_times_s": [round(t, 6) for t in times],
+                   "n_candidate_pairs": int(pst["n_pairs"]), "n_generated_nodes": int(pst["n_generated"]),
+                   "n_output_nodes": int(pst["n_output_nodes"]), "parallelism": "single GPU"},
+        "chosen": {"form": "%s sweeps%s" % (args.precision, " + refinement predictor" if pred is not None else ""),
+                   "passes": int(cst["n_levels"]), "n_predicted": int(cst.get("n_predicted", 0)),
+                   "n_cropped": int(cst.get("n_cropped", 0)), "n_fallback": int(cst.get("n_fallback", 0)),
+                   "n_applied": int(cst["n_applied"]), "step_times_s": [round(t, 6) for t in chosen["times"]],
+                   "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in chosen["prof"].items()}},
+        "plain_fp64": {"ms_per_step": round(plain["s"] * 1e3, 4), "s_per_tree": plain["s"], "gflops": round(flop / plain["s"] / 1e9, 3),
+                       "passes": int(pst["n_levels"]), "step_times_s": [round(t, 6) for t in plain["times"]],
+                       "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in plain["prof"].items()},
+                       "host_breakdown_s": {k: pst[k] for k in ("t_screen", "t_generate", "t_band", "t_transform", "t_host")}},
+        "speedup_vs_plain": round(plain["s"] / chosen["s"], 3),
+        "error_vs_plain_fp64": {"same_node_set": same, "rel_l2": rel, "nodes_plain": int(len(i0)), "nodes_chosen": int(len(i1))},
+        "roofline": roofline(chosen["prof"], flop, args.steps, args.precision, args.workload),
+        "roofline_plain_fp64": roofline(plain["prof"], flop, args.steps, "fp64", args.workload),
     }
-    if args.precision not in ("fp64", "fp64_vector"):
-        res["error_vs_fp64"] = precision_error(mr, w, mra, f, P, args.precision)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
-        res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / s_per_tree
+        res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]
+        res["plain_gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]
     print(json.dumps(res))
 
 

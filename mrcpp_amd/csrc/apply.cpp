@@ -66,7 +66,14 @@ struct Job {
     Mode mode = Mode::Plain;
     PredictorContext ctx;
     std::vector<int> replayWork; // nodes of the breadth-first step being judged (computed)
+    std::vector<char> speculative; // computed ahead of its step (with an earlier step's threshold)
+    bool edgeChecked = false;      // knife-edge nodes of the current replay step recomputed
 };
+
+// Relative distance from the split threshold within which a node computed
+// ahead of its step is recomputed with its own step's screening threshold
+// before the split is decided.
+constexpr double kKnifeEdge = 1e-7;
 
 struct Shape {
     int kp1, n3, ncoef;
@@ -418,20 +425,27 @@ void first_step_context(Job &job) {
 void pregrow(Job &job) {
     detail::NodeStore<3> &g = *job.g;
     const int maxScale = job.mra->getMaxScale();
+    // predicted nodes stay within the input tree's depth (and the device key
+    // range) and a budget of four times the input's nodes; anything deeper the
+    // wavelet test asks for is computed in fallback passes
+    const int maxDepth = std::min(job.fstore->getDepth() - 1, mwgkey::kMaxDepth - 1);
+    const long long budget = 4ll * job.fm->nReal;
     job.replayWork = job.work;
     std::vector<int> all = job.work, cand = job.work;
     std::vector<float> feat;
-    while (!cand.empty()) {
+    while (!cand.empty() && job.stats.nPredicted < budget) {
         feat.resize(cand.size() * kPredictorFeatures);
         for (size_t i = 0; i < cand.size(); i++) predictor_features(job.ctx, cand[i], feat.data() + i * kPredictorFeatures);
         const std::vector<float> prob = job.task.predictor->predict(feat.data(), (int)cand.size());
         std::vector<int> next;
         for (size_t i = 0; i < cand.size(); i++) {
             const int u = cand[i];
-            if (prob[i] <= job.task.predictor->threshold || g.scale(u) + 2 > maxScale) continue;
+            if (prob[i] <= job.task.predictor->threshold || g.scale(u) + 2 > maxScale || g.depth(u) + 1 > maxDepth) continue;
             const int c0 = g.createChildren(u, false);
             for (int k = 0; k < 8; k++) next.push_back(c0 + k);
         }
+        job.speculative.resize(g.size(), 0);
+        for (int v : next) job.speculative[v] = 1;
         job.stats.nPredicted += (long long)next.size();
         all.insert(all.end(), next.begin(), next.end());
         cand.swap(next);
@@ -464,6 +478,28 @@ void replay(Job &job) {
     const int maxScale = job.mra->getMaxScale();
     job.work.clear();
     while (!job.replayWork.empty()) {
+        if (!job.edgeChecked) {
+            // nodes computed ahead whose split decision sits on the threshold:
+            // recompute them with this step's screening threshold first
+            double w = job.wNorm;
+            for (int u : job.replayWork)
+                if (g.depth(u) >= 0) w += g.waveletNorm(u);
+            const double norm = (job.sNorm < 0.0 || w < 0.0) ? -1.0 : job.sNorm + w;
+            std::vector<int> edge;
+            for (int u : job.replayWork) {
+                if (u >= (int)job.speculative.size() || !job.speculative[u]) continue;
+                const double thr = detail::wavelet_threshold(job.task.prec, norm, job.task.absPrec, g.scale(u));
+                if (std::fabs(std::sqrt(g.waveletNorm(u)) / thr - 1.0) < kKnifeEdge) edge.push_back(u);
+            }
+            job.edgeChecked = true;
+            if (!edge.empty()) {
+                for (int u : edge) job.speculative[u] = 0;
+                job.stats.nFallback += (long long)edge.size();
+                job.work.swap(edge);
+                return;
+            }
+        }
+        job.edgeChecked = false;
         for (int u : job.replayWork)
             if (g.depth(u) >= 0) job.wNorm += g.waveletNorm(u);
         g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
@@ -613,9 +649,10 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
             Job &job = jobs[j];
             advance(job);
             if (!job.work.empty()) {
-                // grow the device pool, keeping the computed nodes
-                const size_t keep = (size_t)(job.g->size() - (int)job.work.size()) * sh.ncoef;
-                job.task.out->deviceCoefs().reserve((size_t)job.g->size() * sh.ncoef, keep);
+                // grow the device pool, keeping every slot (recomputed nodes
+                // need not be the newest)
+                const size_t n = (size_t)job.g->size() * sh.ncoef;
+                job.task.out->deviceCoefs().reserve(n, std::min(n, job.task.out->deviceCoefs().capacity()));
             }
             job.stats.tHost += since(th);
         }

@@ -44,6 +44,7 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--threshold", type=float, default=0.35)
     ap.add_argument("--out", default=os.path.join(HERE, "share", "predictor_k7.npy"))
+    ap.add_argument("--evaluate", default=None, help="skip training: evaluate these parameters on the held-out trees")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(HERE))
     import mrcpp_amd as mr
@@ -53,7 +54,7 @@ def main(argv=None):
     mra = mr.MultiResolutionAnalysis((-32, 32), 7)
     t0 = time.time()
     xs, ys = [], []
-    for c in range(0, args.train, args.chunk):
+    for c in range(0, 0 if args.evaluate else args.train, args.chunk):
         trees = build_trees(mr, mra, specs[c:min(args.train, c + args.chunk)])
         outs = [mr.FunctionTree(mra) for _ in trees]
         _, x, y = mr.apply_predicted([(t[0], o, t[1], t[2]) for t, o in zip(trees, outs)], centers=[t[3] for t in trees], collect=True)
@@ -61,19 +62,28 @@ def main(argv=None):
         ys.append(y)
         print("training trees %d..%d: %d samples" % (c, c + len(trees) - 1, len(y)), file=sys.stderr, flush=True)
         del trees, outs
-    X, Y = np.concatenate(xs), np.concatenate(ys)
-    t_data = time.time() - t0
     pred = mr.RefinePredictor()
-    t0 = time.time()
-    losses = pred.train(X, Y, epochs=args.epochs, batch=4096, lr=2e-3, pos_weight=3.0, seed=7)
-    t_train = time.time() - t0
+    if args.evaluate:
+        pred.params = np.load(args.evaluate)
+        rep = dict(params=args.evaluate)
+    else:
+        X, Y = np.concatenate(xs), np.concatenate(ys)
+        t_data = time.time() - t0
+        t0 = time.time()
+        losses = pred.train(X, Y, epochs=args.epochs, batch=4096, lr=2e-3, pos_weight=3.0, seed=7)
+        t_train = time.time() - t0
+        os.makedirs(os.path.dirname(args.out), exist_ok=True)
+        np.save(args.out, pred.params)
+        rep = dict(samples=int(len(Y)), refined_fraction=float(Y.mean()), losses=[float(v) for v in losses], seconds_data=t_data,
+                   seconds_train=t_train)
     pred.threshold = args.threshold
-    os.makedirs(os.path.dirname(args.out), exist_ok=True)
-    np.save(args.out, pred.params)
+    rep["threshold"] = args.threshold
     # held-out trees, batched: plain against predicted
-    rep = dict(samples=int(len(Y)), refined_fraction=float(Y.mean()), losses=[float(v) for v in losses], seconds_data=t_data,
-               seconds_train=t_train, threshold=args.threshold)
     held = build_trees(mr, mra, specs[args.train:args.train + args.held])
+    # warm-up: operator tables and input mirrors are built on first use
+    warm = [mr.FunctionTree(mra) for _ in held]
+    mr.apply_predicted([(t[0], o, t[1], t[2]) for t, o in zip(held, warm)], predictor=pred, centers=[t[3] for t in held])
+    del warm
     res = {}
     for name, p in (("plain", None), ("predicted", pred)):
         mr.device_synchronize()
