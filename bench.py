@@ -174,13 +174,13 @@ def run_batch_bench(args):
         mr.device_synchronize()
         return outs, sts
 
-    pred = None if args.no_predictor else load_predictor(mr, 7)
+    pred = load_predictor(mr, 7)
     centers = [np.asarray(sp["sources"].centers) for sp in specs]
 
-    def chosen_step():
+    def chosen_step(predictor=None):
         mr.set_sweep_precision(args.precision)
         outs = [mr.FunctionTree(mra) for _ in trees]
-        sts = mr.apply_predicted([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)], predictor=pred, centers=centers)
+        sts = mr.apply_predicted([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)], predictor=predictor, centers=centers)
         mr.device_synchronize()
         return outs, sts
 
@@ -195,6 +195,12 @@ def run_batch_bench(args):
         return outs, sts, times
 
     p_outs, p_sts, p_times = timed(plain_step)
+    with_pred = None
+    if pred is not None and not args.no_predictor:
+        w_outs, w_sts, w_times = timed(lambda: chosen_step(pred))
+        with_pred = {"ms_per_step": round(float(np.mean(w_times)) * 1e3, 4), "passes": int(max(st["n_levels"] for st in w_sts)),
+                     "n_applied": int(sum(st["n_applied"] for st in w_sts)),
+                     "same_node_set_trees": int(sum(np.array_equal(a.export(False)[0], b.export(False)[0]) for a, b in zip(p_outs, w_outs)))}
     mr.profile(True)
     c_outs, c_sts, c_times = timed(chosen_step)
     prof = mr.profile_read()
@@ -219,7 +225,8 @@ def run_batch_bench(args):
         "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64" if args.precision in ("fp64", "fp64_vector") else args.precision + " sweeps, f64 band sum",
-        "precision": args.precision, "predictor": pred is not None, "data": "synthetic seeded Gaussian sources",
+        "precision": args.precision, "form": "one apply_batch of all trees, %s sweeps" % args.precision,
+        "with_refinement_predictor": with_pred, "data": "synthetic seeded Gaussian sources",
         "passes_chosen": int(max(st["n_levels"] for st in c_sts)), "passes_plain": int(max(st["n_levels"] for st in p_sts)),
         "config": {"workload": "batch64_k7", "trees": n, "order": 7, "box": [-32, 32], "precs": sorted({sp["prec"] for sp in specs}),
                    "n_gaussians": [len(sp["sources"]) for sp in specs], "operators": "Poisson (even) / Helmholtz mu~U[0.5,3] (odd)",
@@ -248,27 +255,34 @@ def load_predictor(mr, order):
     import numpy as np
     if order != 7 or not os.path.exists(PREDICTOR_PARAMS):
         return None
-    return mr.RefinePredictor(params=np.load(PREDICTOR_PARAMS), threshold=0.35)
+    return mr.RefinePredictor(params=np.load(PREDICTOR_PARAMS), threshold=0.5)
 
 
 def time_form(mr, run_one, steps, warmup):
-    """warmup untimed runs, then steps timed runs with per-kernel HIP-event timing."""
+    """warmup untimed runs, then steps timed runs with per-kernel HIP-event
+    timing (Python's cyclic garbage collector paused while timing)."""
+    import gc
     for _ in range(max(warmup, 1)):
         run_one()
     mr.device_synchronize()
     mr.profile(True)
     mr.device_synchronize()
     times, stats = [], []
-    t0 = time.perf_counter()
     out = None
-    for _ in range(steps):
-        ts = time.perf_counter()
-        out, st = run_one()
+    gc.collect()
+    gc.disable()
+    try:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ts = time.perf_counter()
+            out, st = run_one()
+            mr.device_synchronize()
+            times.append(time.perf_counter() - ts)
+            stats.append(st)
         mr.device_synchronize()
-        times.append(time.perf_counter() - ts)
-        stats.append(st)
-    mr.device_synchronize()
-    elapsed = time.perf_counter() - t0
+        elapsed = time.perf_counter() - t0
+    finally:
+        gc.enable()
     prof = mr.profile_read()
     mr.profile(False)
     return dict(s=elapsed / steps, times=times, stats=stats, prof=prof, out=out)
@@ -287,8 +301,8 @@ def roofline(prof, flop, steps, precision, workload):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="gauss16_k7")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-probe", action="store_true")

@@ -347,7 +347,7 @@ PREDICTOR_FEATURES = 12
 class RefinePredictor:
     """12-64-64-1 perceptron (FP32, on the device) predicting which output
     nodes the adaptive apply refines; apply(..., predictor=P) grows the output
-    grid with it after the first step, the wavelet test staying the judge."""
+    grid with it after the second step, the wavelet test staying the judge."""
 
     def __init__(self, params=None, threshold=None):
         h = _vp()
@@ -376,7 +376,7 @@ class RefinePredictor:
 
     @property
     def threshold(self):
-        return getattr(self, "_thr", 0.35)
+        return getattr(self, "_thr", 0.5)
 
     @threshold.setter
     def threshold(self, t):
@@ -412,7 +412,7 @@ def apply_predicted(tasks, predictor=None, centers=None, collect=False):
     """apply_batch with a refinement predictor (None: plain) and per-task
     source centres (a list of [k, 3] arrays or None).  collect=True also
     returns the training samples (features [N, 12], labels [N]) of every
-    output node beyond the first step.  Statistics include 'n_predicted',
+    output node beyond the second step.  Statistics include 'n_predicted',
     'n_cropped' and 'n_fallback'."""
     n, prec, outs, opers, inps = _batch_args(tasks)
     cs = [np.zeros((0, 3)) if centers is None or centers[i] is None else np.asarray(centers[i], dtype=np.float64).reshape(-1, 3)

@@ -401,27 +401,34 @@ void finish_step(Job &job) {
     job.iter++;
 }
 
-// Predictor inputs after the first step: the first-step nodes' wavelet norms.
-void first_step_context(Job &job) {
+// Breadth-first steps computed exactly before the predictor grows the grid
+// (steps 0 .. kAnchorStep): every predicted node then has a computed parent.
+constexpr int kAnchorStep = 1;
+
+// Predictor inputs: the wavelet norms of the nodes computed in the exact
+// steps (called after each of them, before the split).
+void record_exact_step(Job &job) {
     PredictorContext &c = job.ctx;
-    c.inp = job.fstore;
-    c.out = job.g;
-    c.bands = job.ht;
-    c.mra = job.mra;
-    c.prec = job.task.prec;
-    c.mu = job.task.oper->kernelMu();
-    c.inpNorm = job.task.inp->getSquareNorm();
-    c.centers = job.task.centers;
-    c.firstStep.assign(job.g->size(), 0);
-    c.firstWavelet.assign(job.g->size(), 0.0);
+    if (job.iter == 0) {
+        c.inp = job.fstore;
+        c.out = job.g;
+        c.bands = job.ht;
+        c.mra = job.mra;
+        c.prec = job.task.prec;
+        c.mu = job.task.oper->kernelMu();
+        c.inpNorm = job.task.inp->getSquareNorm();
+        c.centers = job.task.centers;
+    }
+    c.firstStep.resize(job.g->size(), 0);
+    c.firstWavelet.resize(job.g->size(), 0.0);
     for (int u : job.work) {
         c.firstStep[u] = 1;
         c.firstWavelet[u] = job.g->waveletNorm(u);
     }
 }
 
-// Grow the output grid below the (certain) second-step nodes wherever the
-// predictor expects a refinement; the next pass computes all of them.
+// Grow the output grid below the (certain) nodes of the step after the anchor
+// wherever the predictor expects a refinement; the next pass computes all of them.
 void pregrow(Job &job) {
     detail::NodeStore<3> &g = *job.g;
     const int maxScale = job.mra->getMaxScale();
@@ -536,16 +543,16 @@ void advance(Job &job) {
         replay(job);
         return;
     }
-    const bool first = job.iter == 0;
-    const bool learn = first && (job.task.predictor || job.task.samples);
-    if (learn) first_step_context(job);
+    const bool learn = (job.task.predictor || job.task.samples) && job.iter <= kAnchorStep;
+    const bool anchor = learn && job.iter == kAnchorStep;
+    if (learn) record_exact_step(job);
     finish_step(job);
-    if (learn) job.ctx.treeNorm = job.g->treeSquareNorm;
-    if (first && job.task.predictor && !job.work.empty()) pregrow(job);
+    if (anchor) job.ctx.treeNorm = job.g->treeSquareNorm;
+    if (anchor && job.task.predictor && !job.work.empty()) pregrow(job);
 }
 
-// Training samples of a finished apply: every output node beyond the first
-// step, labelled by whether it was refined.
+// Training samples of a finished apply: every output node beyond the exact
+// steps, labelled by whether it was refined.
 void collect_samples(Job &job) {
     if (!job.task.samples || job.ctx.out == nullptr) return;
     RefineSamples &rs = *job.task.samples;

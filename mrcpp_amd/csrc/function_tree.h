@@ -95,11 +95,11 @@ class RefinePredictor;
 struct RefineSamples;
 
 // One application of a batch: the arguments of apply(), and optionally a
-// refinement predictor that grows the output grid after the first step (the
+// refinement predictor that grows the output grid after the second step (the
 // wavelet test stays the judge: predicted nodes it rejects are removed,
 // nodes it adds that were not predicted are computed in further passes),
 // source centres for the predictor's features, and a sink for training
-// samples (every output node beyond the first step: features, refined or not).
+// samples (every output node beyond the second step: features, refined or not).
 struct ApplyTask {
     double prec;
     FunctionTree<3> *out;

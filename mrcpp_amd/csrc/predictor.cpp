@@ -69,7 +69,7 @@ void predictor_features(const PredictorContext &c, int u, float *out) {
     out[2] = v >= 0 ? lg(c.inp->scalingNorm(v)) : -3.0f;
     out[3] = (v >= 0 && gap == 0) ? lg(c.inp->waveletNorm(v)) : -3.0f;
     out[4] = (float)gap / 10.0f;
-    // nearest first-step output ancestor
+    // nearest output ancestor computed in an exact step
     int a = u, up = 0;
     while (a >= 0 && !(a < (int)c.firstStep.size() && c.firstStep[a])) {
         a = g.parent(a);

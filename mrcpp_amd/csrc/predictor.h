@@ -16,8 +16,8 @@ namespace mrcpp {
 constexpr int kPredictorFeatures = MWG_MLP_FEATURES;
 
 // What the features of a node see: the input tree, the operator at the apply
-// precision, the source centres (optional), and the output after the first
-// breadth-first step (its nodes' wavelet norms and the approximate tree norm).
+// precision, the source centres (optional), and the output after the exact
+// breadth-first steps (their nodes' wavelet norms and the approximate tree norm).
 struct PredictorContext {
     const detail::NodeStore<3> *inp = nullptr;
     const detail::NodeStore<3> *out = nullptr;
@@ -25,9 +25,9 @@ struct PredictorContext {
     const MultiResolutionAnalysis<3> *mra = nullptr;
     double prec = 0.0, mu = 0.0;
     double inpNorm = 0.0;      // square norm of the input tree
-    double treeNorm = 0.0;     // approximate output square norm after the first step
+    double treeNorm = 0.0;     // approximate output square norm after the exact steps
     const std::vector<std::array<double, 3>> *centers = nullptr;
-    std::vector<char> firstStep; // output node computed in the first step
+    std::vector<char> firstStep; // output node computed in an exact step
     std::vector<double> firstWavelet; // its wavelet norm (by output node id)
 };
 
@@ -37,9 +37,9 @@ struct PredictorContext {
 //  2 log10 of the input scaling norm at the node, or at its nearest existing input ancestor
 //  3 log10 of the input wavelet norm at the node (-30 when the node is not in the input)
 //  4 levels between the node and that input ancestor
-//  5 log10 of the wavelet norm of the nearest first-step output ancestor
+//  5 log10 of the wavelet norm of the nearest computed output ancestor (the parent)
 //  6 levels between the node and that ancestor
-//  7 log10 of the wavelet split threshold at the node's scale (approximate tree norm after the first step)
+//  7 log10 of the wavelet split threshold at the node's scale (approximate tree norm after the exact steps)
 //  8 log10 of the apply precision
 //  9 Helmholtz mu (0 for Poisson)
 // 10 log10 of the largest operator pair bound at the node's depth
@@ -62,7 +62,7 @@ public:
     std::vector<float> train(const float *features, const float *labels, int n, int epochs, int batch, float lr, float posWeight,
                              uint64_t seed);
 
-    float threshold = 0.35f; // refine when p > threshold (recall over precision: a miss costs a pass)
+    float threshold = 0.5f; // refine when p > threshold
 
 private:
     float *params_ = nullptr, *m_ = nullptr, *v_ = nullptr, *grad_ = nullptr;

@@ -42,7 +42,7 @@ def main(argv=None):
     ap.add_argument("--held", type=int, default=50)
     ap.add_argument("--chunk", type=int, default=16)
     ap.add_argument("--epochs", type=int, default=30)
-    ap.add_argument("--threshold", type=float, default=0.35)
+    ap.add_argument("--threshold", type=float, default=0.5)
     ap.add_argument("--out", default=os.path.join(HERE, "share", "predictor_k7.npy"))
     ap.add_argument("--evaluate", default=None, help="skip training: evaluate these parameters on the held-out trees")
     args = ap.parse_args(argv)
