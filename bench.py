@@ -374,7 +374,8 @@ def main():
         "plain_fp64": {"ms_per_step": round(plain["s"] * 1e3, 4), "s_per_tree": plain["s"], "gflops": round(flop / plain["s"] / 1e9, 3),
                        "passes": int(pst["n_levels"]), "step_times_s": [round(t, 6) for t in plain["times"]],
                        "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in plain["prof"].items()},
-                       "host_breakdown_s": {k: pst[k] for k in ("t_screen", "t_generate", "t_band", "t_transform", "t_host")}},
+                       "host_breakdown_s": {k: pst[k] for k in ("seconds", "t_setup", "t_screen", "t_generate", "t_band", "t_transform",
+                                                                "t_host", "t_cleanup")}},
         "speedup_vs_plain": round(plain["s"] / chosen["s"], 3),
         "error_vs_plain_fp64": {"same_node_set": same, "rel_l2": rel, "nodes_plain": int(len(i0)), "nodes_chosen": int(len(i1))},
         "roofline": roofline(chosen["prof"], flop, args.steps, args.precision, args.workload),

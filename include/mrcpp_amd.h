@@ -63,7 +63,7 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
 /* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
  * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
  * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
- * t_reduce, t_transform, t_host */
+ * t_setup, t_transform, t_host, t_cleanup */
 /* Sweep precision of later mrcpp_apply calls on this thread: 0 FP64 (default,
  * matrix cores), 1 FP32, 2 BF16x3, 3 FP64 on the vector ALUs.  No reference
  * counterpart (the reference computes in FP64 only). */

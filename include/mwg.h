@@ -36,7 +36,9 @@ const char *mwg_last_error(void);
 int mwg_device_count(void);
 
 /* Device memory helpers (bytes).  Freed blocks are kept in a per-size pool
- * and reused by later allocations; mwg_release_pool returns them. */
+ * and reused by later allocations; mwg_release_pool returns them.  mwg_free
+ * waits for the device first, so work enqueued before it has finished with
+ * the block. */
 mwg_status mwg_malloc(void **ptr, size_t bytes);
 mwg_status mwg_free(void *ptr);
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
@@ -139,7 +141,7 @@ typedef struct {
     mwg_ftree f;         /* the tree's input */
     mwg_band_tables op;  /* its operator's tables */
     double *g_coef;      /* the tree's output coefficient pool */
-    unsigned long long *n_applied; /* device counter the segment's applied (f, ft, gt, term) tuples are added to */
+    unsigned long long *counters; /* device [2] the segment adds to: applied (f, ft, gt, term) tuples, candidate pairs */
 } mwg_segment;
 
 /* One breadth-first level: count output nodes, ordered by segment.  segs is
@@ -169,6 +171,9 @@ typedef struct {
     int *pair_l;         /* [capacity][3] band node translation */
     int *g_first;        /* [level count] first pair of each output node */
     int *g_npairs;       /* [level count] */
+    int *missing;        /* [missing_cap][6]: pair, code, node, translation of each pair whose band
+                            node is missing, in no particular order (count[1] of them) */
+    int missing_cap;
 } mwg_pairs;
 
 /* Screen the band of every output node of a level and emit candidate pairs
@@ -181,6 +186,9 @@ typedef struct {
  * *count receives the number of pairs (pairs beyond capacity are counted but
  * not written: grow the arrays and screen again). */
 mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
+
+/* pair_f[pair[i]] = node[i] for count patches (band nodes generated on the host's plan). */
+mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, int count);
 
 /* Generate scaling-only band nodes below existing parents: node ids[i] gets
  * the scaling block of child child_idx[i] of node parent[i], written at
@@ -196,7 +204,7 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
  * tensorApplyOperComp, src/treebuilders/ConvolutionCalculator.cpp:224-382;
  * MWNode::calcNorms, src/trees/MWNode.cpp:609-655).  Every pair_f must be a
  * node id (missing band nodes generated first).  Adds the number of applied
- * (f, ft, gt, term) tuples to each segment's n_applied.  Summation order is fixed by the
+ * (f, ft, gt, term) tuples to each segment's counters[0].  Summation order is fixed by the
  * pair order, so results are reproducible.  precision: MWG_PREC_*; every
  * segment's op.frag[precision] must be built.  pair_f of a pair refers to the
  * input tree of its output node's segment. */

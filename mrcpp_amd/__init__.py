@@ -299,7 +299,7 @@ def dot(bra, ket):
 
 
 STAT_KEYS = ("n_applied", "n_pairs", "n_generated", "n_output_nodes", "seconds", "n_levels", "t_screen", "t_generate",
-             "t_band", "t_reduce", "t_transform", "t_host")
+             "t_band", "t_setup", "t_transform", "t_host", "t_cleanup")
 
 
 def device_synchronize():

@@ -31,10 +31,10 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild;
+    device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild, missing, patch;
     device::DeviceArray<long long> gOff, pOff, cOff;
     device::DeviceArray<double> gNorms;
-    device::DeviceArray<unsigned long long> nApplied;
+    device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
 
 Scratch &scratch() {
@@ -161,7 +161,7 @@ struct SegmentSpec {
     std::vector<int> nodes;
 };
 
-mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned long long *counter) {
+mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned long long *counters) {
     mwg_segment s{};
     s.first = first;
     s.count = count;
@@ -174,7 +174,7 @@ mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned lon
     s.f = job.fm->view(*job.mra);
     s.op = *job.bt;
     s.g_coef = job.task.out->deviceCoefs().get();
-    s.n_applied = counter;
+    s.counters = counters;
     return s;
 }
 
@@ -190,7 +190,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     int at = 0;
     for (const auto &sp : specs) {
         Job &job = jobs[sp.job];
-        segs.push_back(make_segment(job, sp.scale, at, (int)sp.nodes.size(), counters + sp.job));
+        segs.push_back(make_segment(job, sp.scale, at, (int)sp.nodes.size(), counters + 2 * sp.job));
         for (int u : sp.nodes) {
             for (int d = 0; d < 3; d++) lv[3 * at + d] = job.g->transl(u)[d];
             off[at] = (long long)u * sh.ncoef;
@@ -219,10 +219,12 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;
     size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
+    size_t mcap = std::max<size_t>(S.missing.capacity() / 6, 1024);
     auto bind_pairs = [&]() {
         S.pairG.reserve(cap);
         S.pairF.reserve(cap);
         S.pairL.reserve(3 * cap);
+        S.missing.reserve(6 * mcap);
         pr.capacity = (int)std::min<size_t>(S.pairF.capacity(), S.pairL.capacity() / 3);
         pr.count = S.count.get();
         pr.pair_g = S.pairG.get();
@@ -230,6 +232,8 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         pr.pair_l = S.pairL.get();
         pr.g_first = S.gFirst.get();
         pr.g_npairs = S.gCount.get();
+        pr.missing = S.missing.get();
+        pr.missing_cap = (int)(S.missing.capacity() / 6);
     };
     bind_pairs();
     auto t0 = clk::now();
@@ -240,36 +244,45 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         if (cnt[2]) throw Error("pair screen: band width does not match the operator tables");
         npairs = cnt[0];
         nmissing = cnt[1];
-        if (npairs <= pr.capacity) break;
-        cap = (size_t)npairs + npairs / 4;
+        if (npairs <= pr.capacity && nmissing <= pr.missing_cap) break;
+        cap = std::max(cap, (size_t)npairs + npairs / 4);
+        mcap = std::max(mcap, (size_t)nmissing + nmissing / 4);
         bind_pairs();
     }
     const double tScreen = since(t0);
     double tGenerate = 0.0;
     if (npairs > 0 && nmissing > 0) {
         t0 = clk::now();
-        std::vector<int> pf(npairs), pl(3 * (size_t)npairs), pg(npairs);
-        S.pairF.download(pf.data(), npairs);
-        S.pairL.download(pl.data(), pl.size());
-        S.pairG.download(pg.data(), npairs);
-        // per tree: its missing pairs, planned and generated on its mirror
+        // the pairs whose band node is missing: (pair, ancestor code, node, translation)
+        std::vector<int> rec(6 * (size_t)nmissing);
+        S.missing.download(rec.data(), rec.size());
+        std::vector<int> order(nmissing);
+        for (int i = 0; i < nmissing; i++) order[i] = i;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return rec[6 * a] < rec[6 * b]; }); // pair order: deterministic plan
         std::vector<int> segOf(L);
         for (size_t s = 0; s < segs.size(); s++)
             for (int i = 0; i < segs[s].count; i++) segOf[segs[s].first + i] = (int)s;
+        // per tree: its missing pairs, planned and generated on its mirror
         std::map<int, std::vector<int>> byJob;
-        for (int p = 0; p < npairs; p++)
-            if (pf[p] < 0) byJob[nodeJob[pg[p]]].push_back(p);
+        for (int i : order) byJob[nodeJob[rec[6 * i + 2]]].push_back(i);
+        std::vector<int> pp, pn; // patched pairs and their band node ids
         for (auto &kv : byJob) {
             std::vector<int> f, l3, sc;
-            for (int p : kv.second) {
-                f.push_back(pf[p]);
-                for (int d = 0; d < 3; d++) l3.push_back(pl[3 * p + d]);
-                sc.push_back(segs[segOf[pg[p]]].scale);
+            for (int i : kv.second) {
+                f.push_back(rec[6 * i + 1]);
+                for (int d = 0; d < 3; d++) l3.push_back(rec[6 * i + 3 + d]);
+                sc.push_back(segs[segOf[rec[6 * i + 2]]].scale);
             }
             generate_band_nodes(jobs[kv.first], sh, f, l3, sc);
-            for (size_t i = 0; i < kv.second.size(); i++) pf[kv.second[i]] = f[i];
+            for (size_t k = 0; k < kv.second.size(); k++) {
+                pp.push_back(rec[6 * kv.second[k]]);
+                pn.push_back(f[k]);
+            }
         }
-        S.pairF.upload(pf.data(), npairs);
+        pp.insert(pp.end(), pn.begin(), pn.end()); // one upload: [pairs | nodes]
+        S.patch.reserve(pp.size());
+        S.patch.upload(pp.data(), pp.size());
+        device::check(mwg_patch_pairs(&pr, S.patch.get(), S.patch.get() + nmissing, nmissing), "patch pairs");
         // mirrors may have moved: refresh every segment's input view
         for (size_t s = 0; s < segs.size(); s++) segs[s].f = jobs[nodeJob[segs[s].first]].fm->view(*jobs[nodeJob[segs[s].first]].mra);
         tGenerate = since(t0);
@@ -278,9 +291,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     // band sums, output coefficients and norms (zero when no pair survives)
     device::check(mwg_band_apply(&lvl, &pr, npairs, (int)g_sweep), "band apply");
     std::vector<double> nrm(9 * (size_t)L);
-    std::vector<int> gcount(L);
     S.gNorms.download(nrm.data(), nrm.size());
-    if (npairs > 0) S.gCount.download(gcount.data(), L);
     const double tBand = since(t0);
     std::set<int> touched;
     for (const auto &sp : specs) touched.insert(sp.job);
@@ -297,7 +308,6 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
             for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)at + t];
             g.setSquareNorm(u, nrm[9 * (size_t)at + 8]);
             g.setHasCoefs(u, true);
-            jobs[sp.job].stats.nPairs += gcount[at];
             at++;
         }
     }
@@ -316,43 +326,48 @@ void transform_and_norms(Job &job, const Shape &sh) {
     Scratch &S = scratch();
     detail::NodeStore<3> &g = *job.g;
     FunctionTree<3> &out = *job.task.out;
-    auto tab = detail::real_nodes_by_depth(g);
-    auto launch = [&](const std::vector<int> &parents, bool up) {
+    const auto tab = detail::real_nodes_by_depth(g);
+    // parents with children per depth: TopDown (MWTree::mwTransformDown,
+    // overwrite=false) from the root, then BottomUp (MWTree::mwTransformUp)
+    std::vector<std::vector<int>> br(tab.size());
+    for (size_t d = 0; d < tab.size(); d++)
+        for (int u : tab[d])
+            if (g.hasRealChildren(u)) br[d].push_back(u);
+    struct Pass {
+        size_t at, n;
+        bool up;
+    };
+    std::vector<Pass> passes;
+    std::vector<long long> po, co;
+    auto add = [&](const std::vector<int> &parents, bool up) {
         if (parents.empty()) return;
-        std::vector<long long> po, co;
+        passes.push_back({po.size(), parents.size(), up});
         for (int u : parents) {
             po.push_back((long long)u * sh.ncoef);
             co.push_back((long long)g.child0(u) * sh.ncoef);
         }
-        S.pOff.reserve(po.size());
-        S.cOff.reserve(co.size());
-        S.pOff.upload(po.data(), po.size());
-        S.cOff.upload(co.data(), co.size());
-        double *coef = out.deviceCoefs().get();
-        if (up)
-            device::check(mwg_compress(sh.kp1, job.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), sh.ncoef), "compress");
+    };
+    for (size_t d = 0; d < br.size(); d++) add(br[d], false);
+    for (int d = (int)br.size() - 2; d >= 0; d--) add(br[d], true);
+    // then the component norms of every node: offsets appended to po
+    const std::vector<int> all = detail::real_nodes(g);
+    const size_t normAt = po.size();
+    for (int u : all) po.push_back((long long)u * sh.ncoef);
+    S.pOff.reserve(po.size());
+    S.cOff.reserve(std::max<size_t>(co.size(), 1));
+    S.pOff.upload(po.data(), po.size());
+    S.cOff.upload(co.data(), co.size());
+    double *coef = out.deviceCoefs().get();
+    for (const Pass &p : passes) {
+        if (p.up)
+            device::check(mwg_compress(sh.kp1, job.filters, coef, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef), "compress");
         else
-            device::check(mwg_reconstruct_add(sh.kp1, job.filters, coef, S.pOff.get(), S.cOff.get(), (int)po.size(), sh.ncoef), "reconstruct");
-    };
-    auto branches = [&](const std::vector<int> &v) {
-        std::vector<int> b;
-        for (int u : v)
-            if (g.hasRealChildren(u)) b.push_back(u);
-        return b;
-    };
-    // TopDown: add coarse-scale contributions (MWTree::mwTransformDown, overwrite=false)
-    for (size_t d = 0; d < tab.size(); d++) launch(branches(tab[d]), false);
-    // BottomUp: recompress (MWTree::mwTransformUp)
-    for (int d = (int)tab.size() - 2; d >= 0; d--) launch(branches(tab[d]), true);
-    // component norms of every node
-    std::vector<int> all = detail::real_nodes(g);
-    std::vector<long long> off;
-    for (int u : all) off.push_back((long long)u * sh.ncoef);
-    S.pOff.reserve(off.size());
-    S.pOff.upload(off.data(), off.size());
-    S.gNorms.reserve(9 * off.size());
-    device::check(mwg_node_norms(sh.kp1, out.deviceCoefs().get(), S.pOff.get(), S.gNorms.get(), (int)off.size()), "norms");
-    std::vector<double> nrm(9 * off.size());
+            device::check(mwg_reconstruct_add(sh.kp1, job.filters, coef, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef),
+                          "reconstruct");
+    }
+    S.gNorms.reserve(9 * all.size());
+    device::check(mwg_node_norms(sh.kp1, coef, S.pOff.get() + normAt, S.gNorms.get(), (int)all.size()), "norms");
+    std::vector<double> nrm(9 * all.size());
     S.gNorms.download(nrm.data(), nrm.size());
     for (size_t i = 0; i < all.size(); i++) {
         double *cn = g.norms(all[i]);
@@ -606,8 +621,9 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         job.work = detail::real_nodes(*job.g); // ConvolutionCalculator::getInitialWorkVector
     }
     Scratch &S = scratch();
-    S.nApplied.reserve(jobs.size());
-    S.nApplied.zero(jobs.size());
+    S.counters.reserve(2 * jobs.size());
+    S.counters.zero(2 * jobs.size());
+    const double tSetup = since(t_start);
 
     for (;;) {
         std::vector<int> active;
@@ -630,7 +646,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         std::vector<SegmentSpec> batch;
         double used = 0.0;
         auto flush = [&]() {
-            if (!batch.empty()) run_level_batch(jobs, sh, batch, S.nApplied.get());
+            if (!batch.empty()) run_level_batch(jobs, sh, batch, S.counters.get());
             batch.clear();
             used = 0.0;
         };
@@ -664,8 +680,8 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
             job.stats.tHost += since(th);
         }
     }
-    std::vector<unsigned long long> na(jobs.size());
-    S.nApplied.download(na.data(), na.size());
+    std::vector<unsigned long long> na(2 * jobs.size());
+    S.counters.download(na.data(), na.size());
     std::set<device::TreeMirror *> mirrors;
     for (size_t i = 0; i < jobs.size(); i++) {
         Job &job = jobs[i];
@@ -674,15 +690,20 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         transform_and_norms(job, sh);
         detail::calc_square_norm(*job.g);
         job.task.out->markDeviceResident();
-        job.stats.nApplied = (long long)na[i];
+        job.stats.nApplied = (long long)na[2 * i];
+        job.stats.nPairs = (long long)na[2 * i + 1];
         job.stats.tTransform = since(tt);
         job.stats.nOutputNodes = (long long)detail::real_nodes(*job.g).size();
     }
+    const auto tc = clk::now();
     for (auto &job : jobs)
         if (mirrors.insert(job.fm).second) drop_generated(*job.fm, *job.mra, *job.fstore);
+    const double tCleanup = since(tc);
     const double seconds = since(t_start);
     for (auto &job : jobs) {
         job.stats.seconds = seconds;
+        job.stats.tSetup = tSetup;
+        job.stats.tCleanup = tCleanup;
         g_batch_stats.push_back(job.stats);
     }
 }

@@ -118,7 +118,7 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
     for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
     if (lane == 0) {
         gcount[p] = base;
-        if (napp) atomicAdd(S.n_applied, (unsigned long long)napp);
+        if (napp) atomicAdd(S.counters, (unsigned long long)napp);
     }
 }
 

@@ -174,7 +174,7 @@ int mrcpp_get_sweep_precision(void) { return (int)sweep_precision(); }
 namespace {
 void pack_stats(const ApplyStats &s, double *out) {
     double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,
-                    (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tReduce, s.tTransform, s.tHost, 0, 0, 0, 0};
+                    (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tSetup, s.tTransform, s.tHost, s.tCleanup, 0, 0, 0};
     std::memcpy(out, v, sizeof(v));
 }
 } // namespace

@@ -74,6 +74,8 @@ struct ApplyStats {
     long long nFallback = 0;   // nodes the predictor missed, computed in extra passes
     double seconds = 0.0;      // host wall time of apply()
     double tScreen = 0.0, tGenerate = 0.0, tBand = 0.0, tReduce = 0.0, tTransform = 0.0, tHost = 0.0;
+    double tSetup = 0.0;   // tables, mirrors and output pool before the first step
+    double tCleanup = 0.0; // generated band nodes dropped, statistics
 };
 const ApplyStats &last_apply_stats();
 

@@ -33,7 +33,7 @@ mwg_status check_level(const mwg_level *lvl) {
         next += S.count;
         if (S.f.kp1 != lvl->kp1 || S.op.kp1 != lvl->kp1) return fail(MWG_ERR_ARG, "operator, level and tree orders differ");
         if (S.op.M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 512");
-        if (!S.g_coef || !S.n_applied) return fail(MWG_ERR_ARG, "segment without output pool or counter");
+        if (!S.g_coef || !S.counters) return fail(MWG_ERR_ARG, "segment without output pool or counters");
         if (S.band_w >= 0) {
             if (S.o_depth < 0 || S.o_depth >= S.op.n_depth) return fail(MWG_ERR_ARG, "segment depth outside its band tables");
             if (S.o_depth > mwgkey::kMaxDepth) return fail(MWG_ERR_UNSUPPORTED, "level deeper than the device key range");

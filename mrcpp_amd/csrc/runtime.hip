@@ -168,6 +168,10 @@ mwg_status mwg_malloc(void **ptr, size_t bytes) {
 
 mwg_status mwg_free(void *ptr) {
     if (!ptr) return MWG_OK;
+    // the block is handed out again only after the work enqueued before this
+    // free (kernels, asynchronous copies) has finished with it
+    MWG_HIP(hipDeviceSynchronize());
+    g_stage_used = 0;
     std::lock_guard<std::mutex> lk(pool_mutex());
     auto it = block_size().find(ptr);
     if (it == block_size().end()) return fail(MWG_ERR_ARG, "mwg_free of an unknown pointer");

@@ -175,9 +175,9 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
     const Box bx = segment_box(S, gl);
     const int P = bx.n[0] * bx.n[1] * bx.n[2];
     const int p = sb.b * kScreenThreads + threadIdx.x;
-    if (sb.b == 0 && threadIdx.x == 0 && S.band_w >= 0 && sb.g == S.first) {
-        // the block count covers the box only if band_w is the table's band width
-        if (S.op.band_max[S.o_depth] != S.band_w) count[2] = 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { // counters the scatter adds to
+        count[1] = 0;
+        count[2] = 0;
     }
     int code = 0;
     bool ok = false;
@@ -196,11 +196,6 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
     if (threadIdx.x == 0) blk_count[blockIdx.x] = total;
 }
 
-__global__ void k_screen_reset(int *count) {
-    count[1] = 0;
-    count[2] = 0;
-}
-
 // Place the survivors of every block at their scanned offset.
 __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(const mwg_segment *segs_, int nseg, const int *seg_blk, const int *l, const int *cand_pos,
                                                                   const int *cand_code, const int *blk_off, mwg_pairs out) {
@@ -216,19 +211,36 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(const mwg_seg
         const int p = cand_pos[(size_t)blk * kScreenThreads + threadIdx.x];
         const int q = base + threadIdx.x;
         const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
-        if (code < 0) atomicAdd(out.count + 1, 1);
+        const int x = bx.s[0] + p % bx.n[0], y = bx.s[1] + (p / bx.n[0]) % bx.n[1], z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
+        if (code < 0) {
+            const int m = atomicAdd(out.count + 1, 1);
+            if (m < out.missing_cap) {
+                int *rec = out.missing + 6 * (size_t)m;
+                rec[0] = q;
+                rec[1] = code;
+                rec[2] = sb.g;
+                rec[3] = x;
+                rec[4] = y;
+                rec[5] = z;
+            }
+        }
         if (q < out.capacity) {
             out.pair_g[q] = sb.g;
             out.pair_f[q] = code;
-            out.pair_l[3 * q + 0] = bx.s[0] + p % bx.n[0];
-            out.pair_l[3 * q + 1] = bx.s[1] + (p / bx.n[0]) % bx.n[1];
-            out.pair_l[3 * q + 2] = bx.s[2] + p / (bx.n[0] * bx.n[1]);
+            out.pair_l[3 * q + 0] = x;
+            out.pair_l[3 * q + 1] = y;
+            out.pair_l[3 * q + 2] = z;
         }
     }
     if (sb.b == 0 && threadIdx.x == 0) {
-        const int nb = screen_blocks(segs[sb.s].band_w);
+        const_as4<mwg_segment> &S = segs[sb.s];
+        const int nb = screen_blocks(S.band_w);
+        const int np = blk_off[blk + nb] - base;
         out.g_first[sb.g] = base;
-        out.g_npairs[sb.g] = blk_off[blk + nb] - base;
+        out.g_npairs[sb.g] = np;
+        if (np) atomicAdd(S.counters + 1, (unsigned long long)np);
+        // the block count covers the box only if band_w is the table's band width
+        if (sb.g == S.first && S.band_w >= 0 && S.op.band_max[S.o_depth] != S.band_w) out.count[2] = 1;
     }
     if (blk == 0 && threadIdx.x == 0) *out.count = blk_off[gridDim.x];
 }
@@ -240,6 +252,20 @@ Buffer g_cand_pos, g_cand_code, g_blk, g_seg_blk;
 } // namespace mwgdev
 
 using namespace mwgdev;
+
+__global__ __launch_bounds__(256) void k_patch_pairs(int *pair_f, const int *pair, const int *node, int count) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < count) pair_f[pair[i]] = node[i];
+}
+
+extern "C" mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!pairs || !pair || !node) return fail(MWG_ERR_ARG, "null argument");
+    hipLaunchKernelGGL(k_patch_pairs, dim3((count + 255) / 256), dim3(256), 0, 0, pairs->pair_f, pair, node, count);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
 
 extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
@@ -266,8 +292,6 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     if (g_seg_blk.ensure(sizeof(int) * seg_blk.size()) != MWG_OK) return MWG_ERR_NOMEM;
     if ((st = mwg_upload_async(g_seg_blk.p, seg_blk.data(), sizeof(int) * seg_blk.size())) != MWG_OK) return st;
     ProfileScope prof_(MWG_KERNEL_SCREEN);
-    hipLaunchKernelGGL(k_screen_reset, dim3(1), dim3(1), 0, 0, out->count);
-    MWG_LAUNCHED();
     hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
                        g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
     MWG_LAUNCHED();
