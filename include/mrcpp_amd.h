@@ -60,16 +60,22 @@ int mrcpp_operator_node(void *oper, int term, int depth, int l, int comp, double
  * src/operators/HelmholtzKernel.cpp:47-80); returns the number of terms */
 int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_max, double *alpha, double *coef, int cap);
 
-/* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
- * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
- * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
- * t_setup, t_transform, t_host, t_cleanup */
 /* Sweep precision of later mrcpp_apply calls on this thread: 0 FP64 (default,
  * matrix cores), 1 FP32, 2 BF16x3, 3 FP64 on the vector ALUs.  No reference
  * counterpart (the reference computes in FP64 only). */
 int mrcpp_set_sweep_precision(int precision);
 int mrcpp_get_sweep_precision(void);
+/* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
+ * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
+ * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
+ * t_setup, t_transform, t_host, t_cleanup */
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats);
+/* apply(prec, out, oper, inp, precTrees, max_iter, abs_prec) (src/treebuilders/apply.h:43,
+ * src/treebuilders/apply.cpp:214-251): prec_trees holds n_prec tree handles
+ * (n_prec 0: every node's precision factor is 1); the output tree may not be
+ * one of them.  stats as mrcpp_apply. */
+int mrcpp_apply_prec(double prec, void *out, void *oper, void *inp, void *const *prec_trees, int n_prec, int max_iter, int abs_prec,
+                     double *stats);
 /* n applications at once (no reference counterpart; same result per tree as
  * mrcpp_apply): arrays of n arguments, max_iter / abs_prec may be NULL
  * (-1 / 0), stats (may be NULL) receives 16 doubles per task as above, the

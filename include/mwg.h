@@ -156,6 +156,9 @@ typedef struct {
     const int *l;            /* [count][3] translations */
     const long long *g_off;  /* [count] */
     double *g_norms;         /* [count][9] */
+    const double *node_thr;  /* [count] screening threshold of each node, or NULL for its segment's g_thr
+                                (prec * precFunc(idx) * sqrt(|g|^2 / M) of apply with precTrees,
+                                src/treebuilders/ConvolutionCalculator.cpp:241-246) */
 } mwg_level;
 
 /* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the

@@ -56,6 +56,7 @@ SIGNATURES = {
     "mrcpp_kernel_terms": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                           _dp, _dp, ctypes.c_int]),
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
+    "mrcpp_apply_prec": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "mrcpp_apply_batch": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]),
     "mrcpp_apply_batch_predicted": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
@@ -455,10 +456,24 @@ def apply_batch(tasks):
     return [{k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])} for i in range(n)]
 
 
-def apply(prec, out, oper, inp, maxIter=-1, absPrec=False):
-    """apply(prec, out, oper, inp, maxIter, absPrec); returns the run statistics."""
+def apply(prec, out, oper, inp, maxIter=-1, absPrec=False, precTrees=None):
+    """apply(prec, out, oper, inp, maxIter, absPrec), or with precision trees
+    apply(prec, out, oper, inp, precTrees, maxIter, absPrec) as the reference's
+    overload (src/treebuilders/apply.h:43): a list in the fifth place is
+    precTrees.  Returns the run statistics."""
+    if isinstance(maxIter, (list, tuple)):
+        # positional form of the overload: (precTrees, maxIter, absPrec) in places 5-7
+        trees = list(maxIter)
+        maxIter = -1 if absPrec is False else int(absPrec)
+        absPrec = bool(precTrees) if precTrees is not None else False
+        precTrees = trees
     st = np.zeros(16)
-    _check(lib().mrcpp_apply(float(prec), out._h, oper._h, inp._h, int(maxIter), int(bool(absPrec)), _p(st)))
+    if precTrees is None:
+        _check(lib().mrcpp_apply(float(prec), out._h, oper._h, inp._h, int(maxIter), int(bool(absPrec)), _p(st)))
+    else:
+        pts = (_vp * max(1, len(precTrees)))(*[t._h for t in precTrees])
+        _check(lib().mrcpp_apply_prec(float(prec), out._h, oper._h, inp._h, pts, len(precTrees), int(maxIter), int(bool(absPrec)),
+                                      _p(st)))
     stats = {k: float(v) for k, v in zip(STAT_KEYS, st)}
     lv = np.zeros(128, dtype=np.int32)
     n = lib().mrcpp_last_apply_levels(_p(lv, _ip), 128)

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <set>
 
 #include "function_tree.h"
@@ -33,7 +34,7 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 struct Scratch {
     device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild, missing, patch;
     device::DeviceArray<long long> gOff, pOff, cOff;
-    device::DeviceArray<double> gNorms;
+    device::DeviceArray<double> gNorms, nodeThr;
     device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
 
@@ -45,6 +46,105 @@ Scratch &scratch() {
 // Device bytes per level batch for the screen candidates and the per-pair
 // group lists.
 constexpr double kBatchBudget = 16.0e9;
+
+// Local precision factors of apply with precision trees
+// (src/treebuilders/apply.cpp:214-251): precFunc(idx) = 1 / max over the trees
+// of sqrt(getMaxSquareNorm) of the tree's node at idx, 1 without trees.
+// getMaxSquareNorm (src/trees/MWNode.h:84) is the largest scaled square norm
+// 2^(3n) |c|^2 of the node and its real descendants (MWTree::makeMaxSquareNorms,
+// src/trees/MWTree.cpp:536-543; MWNode::setMaxSquareNorm,
+// src/trees/MWNode.cpp:1257-1268).  A node below a tree's leaves is generated
+// from its nearest real ancestor (MWTree::getNode) and contributes its own
+// scaled square norm; generated nodes live in a private cache, so the trees
+// themselves are not changed.
+class PrecFactors {
+public:
+    explicit PrecFactors(const std::vector<FunctionTree<3> *> &trees) {
+        for (FunctionTree<3> *t : trees) {
+            Src src;
+            src.s = &t->nodes();
+            const detail::NodeStore<3> &s = *src.s;
+            src.maxSq.assign(s.size(), -1.0);
+            const std::vector<int> order = detail::real_nodes(s); // parents before children
+            for (auto it = order.rbegin(); it != order.rend(); ++it) {
+                const int u = *it;
+                double m = scaled(s.scale(u), s.squareNorm(u));
+                if (s.hasRealChildren(u))
+                    for (int c = 0; c < 8; c++) m = std::max(m, src.maxSq[s.child0(u) + c]);
+                src.maxSq[u] = m;
+            }
+            src_.push_back(std::move(src));
+        }
+    }
+
+    double operator()(int n, const std::array<int, 3> &l) {
+        double maxNorm = src_.empty() ? 1.0 : 0.0;
+        for (Src &src : src_) maxNorm = std::max(maxNorm, std::sqrt(maxSquareNorm(src, n, l)));
+        return 1.0 / maxNorm;
+    }
+
+private:
+    using Key = std::pair<int, std::array<int, 3>>;
+    struct Gen {
+        double sq;
+        std::vector<double> block; // scaling block
+    };
+    struct Src {
+        const detail::NodeStore<3> *s = nullptr;
+        std::vector<double> maxSq;
+        std::map<Key, Gen> gen;
+    };
+    std::vector<Src> src_;
+
+    static double scaled(int n, double sq) { return std::pow(2.0, 3 * n) * sq; }
+
+    static double maxSquareNorm(Src &src, int n, const std::array<int, 3> &l) {
+        const detail::NodeStore<3> &s = *src.s;
+        const int u = s.findReal(n, l);
+        if (u >= 0) return src.maxSq[u] > 0.0 ? src.maxSq[u] : scaled(n, s.squareNorm(u));
+        auto hit = src.gen.find({n, l});
+        if (hit != src.gen.end()) return scaled(n, hit->second.sq);
+        // nearest real or generated ancestor, then generate down to (n, l)
+        int an = n, real = -1;
+        std::array<int, 3> al = l;
+        const Gen *gen = nullptr;
+        for (;;) {
+            if (an <= s.rootScale()) throw Error("precision tree: node outside the world box");
+            an--;
+            for (int d = 0; d < 3; d++) al[d] >>= 1;
+            if ((real = s.findReal(an, al)) >= 0) break;
+            auto g = src.gen.find({an, al});
+            if (g != src.gen.end()) {
+                gen = &g->second;
+                break;
+            }
+        }
+        const int kd = s.kp1_d();
+        std::vector<double> kids((size_t)8 * kd), parent;
+        if (gen) parent = gen->block;
+        for (int sc = an + 1; sc <= n; sc++) {
+            const bool scalingOnly = real < 0 || sc > an + 1;
+            detail::children_scaling_blocks(s, (sc == an + 1 && real >= 0) ? s.coefs(real) : parent.data(), scalingOnly, kids.data());
+            std::array<int, 3> pl;
+            for (int d = 0; d < 3; d++) pl[d] = l[d] >> (n - sc + 1);
+            for (int c = 0; c < 8; c++) {
+                std::array<int, 3> cl;
+                for (int d = 0; d < 3; d++) cl[d] = 2 * pl[d] + ((c >> d) & 1);
+                Gen g;
+                g.block.assign(kids.begin() + (size_t)c * kd, kids.begin() + (size_t)(c + 1) * kd);
+                double ss = 0.0;
+                for (double x : g.block) ss += x * x;
+                const double cn = std::sqrt(ss); // MWNode::calcNorms of a generated node: the scaling component only
+                g.sq = cn * cn;
+                src.gen.emplace(Key{sc, cl}, std::move(g));
+            }
+            std::array<int, 3> ol;
+            for (int d = 0; d < 3; d++) ol[d] = l[d] >> (n - sc);
+            parent = src.gen.at({sc, ol}).block;
+        }
+        return scaled(n, src.gen.at({n, l}).sq);
+    }
+};
 
 // One tree of a batch and its state across breadth-first steps.
 enum class Mode { Plain, Replay };
@@ -68,7 +168,30 @@ struct Job {
     std::vector<int> replayWork; // nodes of the breadth-first step being judged (computed)
     std::vector<char> speculative; // computed ahead of its step (with an earlier step's threshold)
     bool edgeChecked = false;      // knife-edge nodes of the current replay step recomputed
+    // precision trees: precFunc of every output node (by id, -1 until needed)
+    std::unique_ptr<PrecFactors> precFunc;
+    std::vector<double> precFac;
 };
+
+// Precision of output node u (WaveletAdaptor::splitNode, src/treebuilders/WaveletAdaptor.h:51-54).
+double node_prec(const Job &job, int u) { return job.precFunc ? job.task.prec * job.precFac[u] : job.task.prec; }
+
+// Screening threshold of output node u (ConvolutionCalculator::calcNode,
+// src/treebuilders/ConvolutionCalculator.cpp:241-246).
+double node_threshold(const Job &job, int u) {
+    double t = job.g->treeSquareNorm;
+    if (t > 0.0) t = job.task.prec * job.precFac[u] * std::sqrt(t / (double)job.task.oper->size());
+    return t;
+}
+
+// precFunc of the work nodes not evaluated yet.
+void eval_prec_factors(Job &job) {
+    if (!job.precFunc) return;
+    const detail::NodeStore<3> &g = *job.g;
+    if (job.precFac.size() < (size_t)g.size()) job.precFac.resize(g.size(), -1.0);
+    for (int u : job.work)
+        if (job.precFac[u] < 0.0) job.precFac[u] = (*job.precFunc)(g.scale(u), g.transl(u));
+}
 
 // Relative distance from the split threshold within which a node computed
 // ahead of its step is recomputed with its own step's screening threshold
@@ -215,6 +338,17 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.l = S.l.get();
     lvl.g_off = S.gOff.get();
     lvl.g_norms = S.gNorms.get();
+    bool perNode = false;
+    for (const auto &sp : specs) perNode = perNode || jobs[sp.job].precFunc != nullptr;
+    if (perNode) {
+        std::vector<double> thr;
+        thr.reserve(L);
+        for (const auto &sp : specs)
+            for (int u : sp.nodes) thr.push_back(jobs[sp.job].precFunc ? node_threshold(jobs[sp.job], u) : jobs[sp.job].gthr);
+        S.nodeThr.reserve(L);
+        S.nodeThr.upload(thr.data(), thr.size());
+        lvl.node_thr = S.nodeThr.get();
+    }
 
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;
@@ -407,7 +541,7 @@ void finish_step(Job &job) {
     for (int u : job.work) {
         if (g.hasChildren(u)) continue;
         if (g.scale(u) + 2 > mra.getMaxScale()) continue;
-        if (detail::wavelet_split(job.task.prec, g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u))) {
+        if (detail::wavelet_split(node_prec(job, u), g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u))) {
             const int c0 = g.createChildren(u, false);
             for (int k = 0; k < 8; k++) next.push_back(c0 + k);
         }
@@ -510,7 +644,7 @@ void replay(Job &job) {
             std::vector<int> edge;
             for (int u : job.replayWork) {
                 if (u >= (int)job.speculative.size() || !job.speculative[u]) continue;
-                const double thr = detail::wavelet_threshold(job.task.prec, norm, job.task.absPrec, g.scale(u));
+                const double thr = detail::wavelet_threshold(node_prec(job, u), norm, job.task.absPrec, g.scale(u));
                 if (std::fabs(std::sqrt(g.waveletNorm(u)) / thr - 1.0) < kKnifeEdge) edge.push_back(u);
             }
             job.edgeChecked = true;
@@ -529,7 +663,7 @@ void replay(Job &job) {
         std::vector<int> next, pending;
         for (int u : job.replayWork) {
             const bool split = !stop && g.scale(u) + 2 <= maxScale &&
-                               detail::wavelet_split(job.task.prec, g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u));
+                               detail::wavelet_split(node_prec(job, u), g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u));
             if (split) {
                 if (!g.hasChildren(u)) {
                     const int c0 = g.createChildren(u, false);
@@ -603,7 +737,16 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         if (t.out->getMRA() != t.inp->getMRA()) throw Error("Incompatible MRA");
         if (t.out->getMRA().getKp1() != kp1) throw Error("apply_batch: every tree needs the same Legendre order");
         if (t.out == t.inp || !outs.insert(t.out).second) throw Error("apply_batch: each output tree may appear once and differ from its input");
+        if (t.precTrees)
+            for (const FunctionTree<3> *p : *t.precTrees) {
+                if (!p) throw Error("apply: null precision tree");
+                if (p->getMRA() != t.out->getMRA()) throw Error("Incompatible MRA");
+            }
     }
+    for (const ApplyTask &t : tasks)
+        if (t.precTrees)
+            for (const FunctionTree<3> *p : *t.precTrees)
+                if (outs.count(p)) throw Error("apply: an output tree cannot be a precision tree");
     if (kp1 != 6 && kp1 != 8 && kp1 != 10 && kp1 != 4) throw Error("apply supports Legendre orders 3, 5, 7 and 9");
     const Shape sh{kp1, kp1 * kp1 * kp1, 8 * kp1 * kp1 * kp1};
     for (size_t i = 0; i < tasks.size(); i++) {
@@ -619,6 +762,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         // output pool sized for an output tree as large as the input (grown on demand)
         job.task.out->deviceCoefs().reserve((size_t)std::max(job.g->size(), job.fm->nReal) * sh.ncoef);
         job.work = detail::real_nodes(*job.g); // ConvolutionCalculator::getInitialWorkVector
+        if (job.task.precTrees) job.precFunc = std::make_unique<PrecFactors>(*job.task.precTrees);
     }
     Scratch &S = scratch();
     S.counters.reserve(2 * jobs.size());
@@ -635,6 +779,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         for (int j : active) {
             Job &job = jobs[j];
             job.stats.levelNodes.push_back((int)job.work.size());
+            eval_prec_factors(job);
             double gthr = job.g->treeSquareNorm;
             if (gthr > 0.0) gthr = job.task.prec * 1.0 * std::sqrt(gthr / (double)job.task.oper->size());
             job.gthr = gthr;
@@ -711,6 +856,15 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter, bool absPrec) {
     if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
     apply_batch({ApplyTask{prec, &out, &oper, &inp, maxIter, absPrec}});
+    g_stats = g_batch_stats[0];
+}
+
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, const std::vector<FunctionTree<3> *> &precTrees,
+           int maxIter, bool absPrec) {
+    if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
+    ApplyTask t{prec, &out, &oper, &inp, maxIter, absPrec};
+    t.precTrees = &precTrees;
+    apply_batch({t});
     g_stats = g_batch_stats[0];
 }
 

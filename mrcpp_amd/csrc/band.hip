@@ -35,8 +35,8 @@ __device__ inline unsigned group_code(int term, int ft, unsigned mask) { return 
 // ---------------------------------------------------------------------------
 // 1. exact screening, one wave per pair
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, mwg_pairs pr, unsigned *codes,
-                                                     int stride, int *gcount) {
+__global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr, mwg_pairs pr,
+                                                     unsigned *codes, int stride, int *gcount) {
     __shared__ unsigned char smask[8 * kMaxTerms];
     const int lane = threadIdx.x;
     const int p = blockIdx.x;
@@ -45,7 +45,7 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
     const_as4<mwg_segment> &S = ro(segs)[ro(node_seg)[g]];
     const int d = S.o_depth;
     const int M = S.op.M;
-    const double g_thr = S.g_thr;
+    const double g_thr = node_thr ? node_thr[g] : S.g_thr;
     if (fid < 0) { // unresolved band node: no contribution (the host generates every node first)
         if (lane == 0) gcount[p] = 0;
         return;
@@ -793,8 +793,8 @@ template <int N1> struct BandL {
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
-            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, dev.segs, dev.node_seg, lvl->l, *pr, w.codes.as<unsigned>(), stride,
-                               gstart);
+            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
+                               w.codes.as<unsigned>(), stride, gstart);
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;

@@ -304,6 +304,17 @@ int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int
     });
 }
 
+int mrcpp_apply_prec(double prec, void *out, void *oper, void *inp, void *const *prec_trees, int n_prec, int max_iter, int abs_prec, double *stats) {
+    return guarded([&] {
+        if (n_prec < 0 || (n_prec > 0 && !prec_trees)) throw Error("apply: bad precision tree list");
+        std::vector<FunctionTree<3> *> pt;
+        for (int i = 0; i < n_prec; i++) pt.push_back(static_cast<FunctionTree<3> *>(prec_trees[i]));
+        apply(prec, *static_cast<FunctionTree<3> *>(out), *static_cast<ConvolutionOperator<3> *>(oper), *static_cast<FunctionTree<3> *>(inp), pt,
+              max_iter, abs_prec != 0);
+        if (stats) pack_stats(last_apply_stats(), stats);
+    });
+}
+
 int mrcpp_last_apply_levels(int *level_nodes, int cap) {
     const ApplyStats &s = last_apply_stats();
     for (size_t i = 0; i < s.levelNodes.size() && (int)i < cap; i++) level_nodes[i] = s.levelNodes[i];

@@ -92,6 +92,12 @@ SweepPrecision sweep_precision();
 // the band sum running on the MI355X.
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter = -1,
            bool absPrec = false);
+// apply(prec, out, oper, inp, precTrees, maxIter, absPrec)
+// (src/treebuilders/apply.h:43, src/treebuilders/apply.cpp:214-251): the
+// precision of output node idx is prec / max over precTrees of the largest
+// norm at or below idx (scaled to unit volume).
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp,
+           const std::vector<FunctionTree<3> *> &precTrees, int maxIter = -1, bool absPrec = false);
 
 class RefinePredictor;
 struct RefineSamples;
@@ -112,6 +118,8 @@ struct ApplyTask {
     const RefinePredictor *predictor = nullptr;
     const std::vector<std::array<double, 3>> *centers = nullptr;
     RefineSamples *samples = nullptr;
+    // precision trees (nullptr: the apply without them)
+    const std::vector<FunctionTree<3> *> *precTrees = nullptr;
 };
 // Many applications at once (no reference counterpart): every breadth-first
 // step of every unfinished tree shares one screen and one band-sum launch.

@@ -36,9 +36,8 @@ __device__ inline Box band_box(const mwg_ftree &f, int d, int W, const int *gl) 
 
 // Whether band position (x, y, z) can contribute to output node gl of segment
 // S; *code is the band node id, or -(ancestor id + 1) when the node is missing.
-__device__ inline bool screen_position(const_as4<mwg_segment> &S, const int *gl, int x, int y, int z, int *code) {
+__device__ inline bool screen_position(const_as4<mwg_segment> &S, double g_thr, const int *gl, int x, int y, int z, int *code) {
     const int d = S.o_depth;
-    const double g_thr = S.g_thr;
     const int wmax = S.op.wmax;
     const double *pb = S.op.pair_bound + (size_t)d * (2 * wmax + 1) + wmax;
     double a = pb[x - gl[0]] * pb[y - gl[1]] * pb[z - gl[2]];
@@ -165,8 +164,8 @@ __device__ inline Box segment_box(const_as4<mwg_segment> &S, const int *gl) {
 }
 
 // One thread per band position; block-local compaction in position order.
-__global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_segment *segs_, int nseg, const int *seg_blk, const int *l, int *cand_pos,
-                                                                    int *cand_code, int *blk_count, int *count) {
+__global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_segment *segs_, int nseg, const int *seg_blk, const int *l,
+                                                                    const double *node_thr, int *cand_pos, int *cand_code, int *blk_count, int *count) {
     __shared__ int tmp[kScreenThreads / 64];
     const auto segs = ro(segs_);
     const ScreenBlock sb = locate(seg_blk, nseg, segs);
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
         const int x = bx.s[0] + p % bx.n[0];
         const int y = bx.s[1] + (p / bx.n[0]) % bx.n[1];
         const int z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
-        ok = screen_position(S, gl, x, y, z, &code);
+        ok = screen_position(S, node_thr ? node_thr[sb.g] : S.g_thr, gl, x, y, z, &code);
     }
     int excl;
     const int total = block_exclusive_scan<int, kScreenThreads>(ok ? 1 : 0, tmp, excl);
@@ -293,7 +292,7 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     if ((st = mwg_upload_async(g_seg_blk.p, seg_blk.data(), sizeof(int) * seg_blk.size())) != MWG_OK) return st;
     ProfileScope prof_(MWG_KERNEL_SCREEN);
     hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
-                       g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
+                       lvl->node_thr, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
     MWG_LAUNCHED();
     if ((st = scan_exclusive(g_blk.as<int>(), g_blk.as<int>(), (int)nblk)) != MWG_OK) return st;
     hipLaunchKernelGGL(k_screen_scatter, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,

@@ -476,6 +476,34 @@ template <int D> void reconstruct_into_children(NodeStore<D> &s, int u, bool ove
     }
 }
 
+// Scaling blocks of the 2^D children of a node from coefficients held outside
+// any store: all 2^D blocks of in, or its scaling block only (a generated
+// node), written to out[TD][kd] (tree_utils::mw_transform,
+// src/utils/tree_utils.cpp:113-216, with the passes of reconstruct_into_children).
+template <int D> void children_scaling_blocks(const NodeStore<D> &s, const double *in, bool scaling_only, double *out) {
+    constexpr int TD = NodeStore<D>::TD;
+    const int kd = s.kp1_d(), m = kd / s.kp1();
+    const auto &fb = s.mra().filter();
+    std::vector<double> a((size_t)TD * kd, 0.0), b((size_t)TD * kd, 0.0);
+    const double *src = in;
+    double *bufs[2] = {a.data(), b.data()};
+    for (int dim = 0; dim < D; dim++) {
+        const int bit = 1 << dim;
+        const int nin = scaling_only ? (1 << dim) : TD;
+        double *dst = (dim == D - 1) ? out : bufs[dim & 1];
+        for (int gt = 0; gt < TD; gt++) {
+            bool acc = false;
+            for (int ft = 0; ft < nin; ft++) {
+                if ((gt | bit) != (ft | bit)) continue;
+                const int f = 2 * ((gt >> dim) & 1) + ((ft >> dim) & 1);
+                core::filter_pass(dst + (size_t)gt * kd, src + (size_t)ft * kd, fb.sub(f, Reconstruction), m, acc);
+                acc = true;
+            }
+        }
+        src = dst;
+    }
+}
+
 // MWNode::giveChildrenCoefs (src/trees/MWNode.cpp:312-335)
 template <int D> void give_children_coefs(NodeStore<D> &s, int u, bool overwrite) {
     const int c0 = s.child0(u);

@@ -448,6 +448,7 @@ template <int D> struct Node {
     std::vector<double> coefs;
     double cn[TD];
     double sq = -1.0;
+    double maxSq = -1.0; // MWNode::maxSquareNorm (src/trees/MWNode.h:180)
     omp_lock_t lock;
 
     Node() {
@@ -977,15 +978,52 @@ template <int D> static bool split_check(const Node<D> &nd, double prec, double 
     return split;
 }
 
+// Local precision factor of a node (the precFunc of apply with precTrees,
+// src/treebuilders/apply.cpp:222-229): 1 / the largest sqrt(getMaxSquareNorm)
+// of the node's index over the precision trees, generating nodes below their
+// leaves (MWTree::getNode).  getMaxSquareNorm (src/trees/MWNode.h:84) falls
+// back to the node's own scaled square norm 2^(D n) |c|^2 (:212) when no
+// positive maximum was set.
+template <int D> static double scaled_square_norm(const Node<D> &nd) { return std::pow(2.0, D * nd.n) * nd.sq; }
+template <int D> static double max_square_norm(const Node<D> &nd) { return nd.maxSq > 0.0 ? nd.maxSq : scaled_square_norm(nd); }
+
+// MWNode::setMaxSquareNorm (src/trees/MWNode.cpp:1257-1268) from every root
+// (MWTree::makeMaxSquareNorms, src/trees/MWTree.cpp:536-543).
+template <int D> static void set_max_square_norm(Node<D> &nd) {
+    nd.maxSq = scaled_square_norm(nd);
+    if (nd.isEnd()) return;
+    for (int i = 0; i < Node<D>::TD; i++) {
+        Node<D> &c = nd.child(i);
+        set_max_square_norm(c);
+        nd.maxSq = std::max(nd.maxSq, c.maxSq);
+    }
+}
+template <int D> static void make_max_square_norms(Tree<D> &t) {
+    for (auto &r : t.roots) set_max_square_norm(*r);
+}
+
+template <int D> struct PrecFunc {
+    std::vector<Tree<D> *> trees;
+    double operator()(int n, const typename Node<D>::Index &l) const {
+        double maxNorm = trees.empty() ? 1.0 : 0.0;
+        for (Tree<D> *t : trees) maxNorm = std::max(maxNorm, std::sqrt(max_square_norm(get_node(*t, n, l))));
+        return 1.0 / maxNorm;
+    }
+};
+
 // WaveletAdaptor (src/treebuilders/WaveletAdaptor.h:36-54)
 template <int D> struct WaveletAdaptor : Adaptor<D> {
     double prec;
     bool absPrec;
+    const PrecFunc<D> *precFunc = nullptr; // setPrecFunction (:43); 1.0 when unset (:49)
     WaveletAdaptor(double p, int ms, bool ap)
             : Adaptor<D>(ms)
             , prec(p)
             , absPrec(ap) {}
-    bool split(const Node<D> &nd) const override { return split_check(nd, prec, 1.0, absPrec); }
+    bool split(const Node<D> &nd) const override {
+        const double precFac = precFunc ? (*precFunc)(nd.n, nd.l) : 1.0;
+        return split_check(nd, prec * precFac, 1.0, absPrec);
+    }
 };
 
 // ProjectionCalculator::calcNode (src/treebuilders/ProjectionCalculator.cpp:34-51)
@@ -1345,6 +1383,7 @@ struct ConvolutionCalculator : Calculator<3> {
     Operator *oper;
     Tree<3> *fTree;
     ApplyStats *stats;
+    const PrecFunc<3> *precFunc = nullptr; // setPrecFunction (ConvolutionCalculator.h:43)
     // bandSizes per term: MaxDepth x 65 (:105-139)
     std::vector<std::vector<int>> bandSizes;
     int kp1, kp1_2, kp1_d, kp1_dm1;
@@ -1422,7 +1461,8 @@ struct ConvolutionCalculator : Calculator<3> {
         double gThrs = g.tree->squareNorm;
         if (gThrs > 0.0) {
             double nTerms = (double)oper->terms.size();
-            gThrs = prec * 1.0 * std::sqrt(gThrs / nTerms);
+            const double precFac = precFunc ? (*precFunc)(g.n, g.l) : 1.0;
+            gThrs = prec * precFac * std::sqrt(gThrs / nTerms);
         }
         std::vector<double> scr(2 * kp1_d);
         long long applied = 0, genApplied = 0, nPairs = 0, nGroups = 0, nXs = 0, nYs = 0;
@@ -1504,6 +1544,28 @@ static void apply(double prec, Tree<3> &out, Operator &oper, Tree<3> &inp, int m
     calc_band_widths(oper, prec);
     WaveletAdaptor<3> adaptor(prec, out.mra->maxScale(), absPrec);
     ConvolutionCalculator calc(prec, oper, inp, &st);
+    BuildStats bst;
+    build(out, calc, adaptor, maxIter, &bst);
+    st.level_nodes = bst.level_nodes;
+    mw_transform_down(out, false);
+    mw_transform_up(out);
+    calc_square_norm(out);
+    delete_generated(inp);
+    st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// apply with precision trees (src/treebuilders/apply.cpp:214-251): the
+// screening threshold and the split test of node idx use prec * precFunc(idx).
+static void apply_prec(double prec, Tree<3> &out, Operator &oper, Tree<3> &inp, std::vector<Tree<3> *> &precTrees, int maxIter, bool absPrec,
+                       ApplyStats &st) {
+    auto t0 = std::chrono::steady_clock::now();
+    calc_band_widths(oper, prec);
+    for (Tree<3> *t : precTrees) make_max_square_norms(*t);
+    PrecFunc<3> pf{precTrees};
+    WaveletAdaptor<3> adaptor(prec, out.mra->maxScale(), absPrec);
+    adaptor.precFunc = &pf;
+    ConvolutionCalculator calc(prec, oper, inp, &st);
+    calc.precFunc = &pf;
     BuildStats bst;
     build(out, calc, adaptor, maxIter, &bst);
     st.level_nodes = bst.level_nodes;
@@ -1779,10 +1841,19 @@ int orc_operator_node(void *op, int term, int depth, int l, int comp, double *ma
 
 // apply; stats = [N_applied, N_gen_applied, N_g, seconds, n_levels, band pairs,
 // pairs with work, (g,f,ft,term) groups, shared x-sweeps, shared y-sweeps]
-int orc_apply(double prec, void *outp, void *op, void *inp, int maxIter, int absPrec, double *stats, int *level_nodes, int max_levels) {
+// precTrees: n_prec tree handles, or n_prec < 0 for the plain apply
+int orc_apply(double prec, void *outp, void *op, void *inp, int maxIter, int absPrec, double *stats, int *level_nodes, int max_levels,
+              void **prec_trees, int n_prec) {
     try {
         ApplyStats st;
-        apply(prec, *static_cast<OrcTree *>(outp)->t, *static_cast<Operator *>(op), *static_cast<OrcTree *>(inp)->t, maxIter, absPrec != 0, st);
+        if (n_prec < 0) {
+            apply(prec, *static_cast<OrcTree *>(outp)->t, *static_cast<Operator *>(op), *static_cast<OrcTree *>(inp)->t, maxIter, absPrec != 0, st);
+        } else {
+            std::vector<Tree<3> *> pt;
+            for (int i = 0; i < n_prec; i++) pt.push_back(static_cast<OrcTree *>(prec_trees[i])->t.get());
+            apply_prec(prec, *static_cast<OrcTree *>(outp)->t, *static_cast<Operator *>(op), *static_cast<OrcTree *>(inp)->t, pt, maxIter,
+                       absPrec != 0, st);
+        }
         stats[0] = (double)st.fCount.load();
         stats[1] = (double)st.genCount.load();
         stats[2] = (double)st.gCount.load();
@@ -1798,6 +1869,22 @@ int orc_apply(double prec, void *outp, void *op, void *inp, int maxIter, int abs
     } catch (std::exception &e) {
         g_last_error = e.what();
         return -1;
+    }
+}
+
+// precFunc(n, l) over the given trees after makeMaxSquareNorms (the local
+// precision factor of apply with precTrees); generates nodes in the trees
+double orc_prec_factor(void **trees, int n_trees, int n, const int *l) {
+    try {
+        PrecFunc<3> pf;
+        for (int i = 0; i < n_trees; i++) {
+            pf.trees.push_back(static_cast<OrcTree *>(trees[i])->t.get());
+            make_max_square_norms(*pf.trees.back());
+        }
+        return pf(n, {l[0], l[1], l[2]});
+    } catch (std::exception &e) {
+        g_last_error = e.what();
+        return -1.0;
     }
 }
 

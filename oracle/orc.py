@@ -69,8 +69,11 @@ class Oracle:
             L.orc_operator_bandwidths.argtypes = [ctypes.c_void_p, ctypes.c_double, _ip, ctypes.c_int]
             L.orc_operator_node.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
             L.orc_apply.argtypes = [ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
-                                    ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int]
+                                    ctypes.c_int, ctypes.c_int, _dp, _ip, ctypes.c_int,
+                                    ctypes.POINTER(ctypes.c_void_p), ctypes.c_int]
             L.orc_set_threads.argtypes = [ctypes.c_int]
+            L.orc_prec_factor.restype = ctypes.c_double
+            L.orc_prec_factor.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, _ip]
             L.orc_set_filter_dir(FILTER_DIR.encode())
             cls._lib = L
         return cls._lib
@@ -211,12 +214,14 @@ class OracleOperator:
         return mat.reshape(kp1, kp1, order="F"), nrm.value
 
 
-def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False):
-    """Run the oracle apply; returns a stats dict."""
+def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False, prec_trees=None):
+    """Run the oracle apply (with precision trees when prec_trees is a list,
+    src/treebuilders/apply.cpp:214-251); returns a stats dict."""
     stats = np.zeros(10)
     levels = np.zeros(64, dtype=np.int32)
+    pts = (ctypes.c_void_p * max(1, len(prec_trees or ())))(*[t.h for t in (prec_trees or ())])
     rc = Oracle.lib().orc_apply(float(prec), out.h, oper.h, inp.h, int(max_iter), int(abs_prec),
-                                _ptr(stats), _ptr(levels, _ip), 64)
+                                _ptr(stats), _ptr(levels, _ip), 64, pts, -1 if prec_trees is None else len(prec_trees))
     if rc != 0:
         raise RuntimeError(Oracle.error())
     nlev = int(stats[4])
@@ -224,3 +229,14 @@ def apply(prec, out, oper, inp, max_iter=-1, abs_prec=False):
             "seconds": float(stats[3]), "level_nodes": levels[:nlev].tolist(),
             "n_band_pairs": int(stats[5]), "n_pairs": int(stats[6]), "n_groups": int(stats[7]),
             "n_xsweeps": int(stats[8]), "n_ysweeps": int(stats[9])}
+
+
+def prec_factor(trees, n, l):
+    """precFunc of apply with precision trees at node (n, l)
+    (src/treebuilders/apply.cpp:222-229)."""
+    pts = (ctypes.c_void_p * max(1, len(trees)))(*[t.h for t in trees])
+    la = np.array(l, dtype=np.int32)
+    v = Oracle.lib().orc_prec_factor(pts, len(trees), int(n), _ptr(la, _ip))
+    if v < 0:
+        raise RuntimeError(Oracle.error())
+    return v
