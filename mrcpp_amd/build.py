@@ -26,6 +26,10 @@ HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip",
 CXXFLAGS = ["-O3", "-std=c++17", "-fPIC", "-fopenmp", "-march=x86-64-v3", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-Wno-unused-result"]
+# The band sweeps keep their loop-carried accumulators in VGPRs: the matrix
+# instructions then read and write them in place instead of the compiler
+# copying them into AGPRs and back around every group.
+HIP_FILE_FLAGS = {"band.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 
 def _run(cmd):
@@ -46,7 +50,7 @@ def build(verbose=False, jobs=8):
         objs.append(o)
     for s in HIP_SOURCES:
         o = os.path.join(OBJ, s + ".o")
-        cmds.append([HIPCC] + HIPFLAGS + ["-c", os.path.join(SRC, s), "-o", o])
+        cmds.append([HIPCC] + HIPFLAGS + HIP_FILE_FLAGS.get(s, []) + ["-c", os.path.join(SRC, s), "-o", o])
         objs.append(o)
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         for r in ex.map(_run, cmds):

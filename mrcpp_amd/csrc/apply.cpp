@@ -453,7 +453,7 @@ double node_budget(const Job &job, int scale) {
     const detail::BandTables &ht = *job.ht;
     const int W = (o_depth >= 0 && o_depth < ht.nDepth) ? std::max(ht.bandMax[o_depth], 0) : 0;
     const double box = std::pow(2.0 * W + 1.0, 3.0);
-    return 8.0 * (box + 256.0) + 128.0 * (32.0 * ht.M + 24.0);
+    return 8.0 * (box + 256.0) + 128.0 * (96.0 * ht.M + 48.0); // screen candidates; per pair: group codes, level-order records, pair record
 }
 
 void transform_and_norms(Job &job, const Shape &sh) {

@@ -27,16 +27,27 @@ namespace mwgdev {
 namespace {
 
 constexpr int kGroupMinPerUnit = 16;
+constexpr int kPrecValuBand = 3; // FP64 sweeps on the vector ALUs (comparison path)
 constexpr int kMaxUnitsPerNode = 512;
 constexpr double kPartialBudget = 1024.0 * 1024.0 * 1024.0; // bytes of partial slots
 
-__device__ inline unsigned group_code(int term, int ft, unsigned mask) { return (unsigned)term | ((unsigned)ft << 16) | (mask << 24); }
+// Group code: bits 0-19 the operator matrix block of displacement 0 of the
+// group's term at the node's depth (mat_start / 4 + width), 20-22 the f
+// component, 23-30 the gt mask, 31 set when the group's band block differs
+// from the previous group's of the pair (a new band node or f component).
+__device__ inline unsigned group_code(long long b40, int ft, unsigned mask, bool fresh) {
+    return (unsigned)b40 | ((unsigned)ft << 20) | (mask << 23) | ((unsigned)fresh << 31);
+}
+__device__ inline int code_block(unsigned c) { return (int)(c & 0xfffffu); }
+__device__ inline int code_ft(unsigned c) { return (int)((c >> 20) & 7u); }
+__device__ inline unsigned code_mask(unsigned c) { return (c >> 23) & 0xffu; }
+__device__ inline bool code_fresh(unsigned c) { return (c >> 31) != 0u; }
 
 // ---------------------------------------------------------------------------
 // 1. exact screening, one wave per pair
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr, mwg_pairs pr,
-                                                     unsigned *codes, int stride, int *gcount) {
+                                                     unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo) {
     __shared__ unsigned char smask[8 * kMaxTerms];
     const int lane = threadIdx.x;
     const int p = blockIdx.x;
@@ -54,6 +65,10 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
     for (int i = 0; i < 3; i++) {
         dl[i] = pr.pair_l[3 * p + i] - l[3 * g + i];
         maxdl = max(maxdl, abs(dl[i]));
+    }
+    if (lane == 0) { // the pair's record for the sweeps
+        pdl[p] = make_int4(dl[0], dl[1], dl[2], fid);
+        pfo[p] = S.f.coef_off[fid];
     }
     double fn[8];
     const double *fnorms = S.f.norms;
@@ -107,14 +122,21 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
     int base = 0;
     unsigned *dst = codes + (size_t)p * stride;
     const unsigned long long below = (1ull << lane) - 1ull;
-    for (int ft = 0; ft < 8; ft++)
+    for (int ft = 0; ft < 8; ft++) {
+        bool first = true; // the first group of f component ft starts a new band block
         for (int c0 = 0; c0 < M; c0 += 64) {
             const int i = c0 + lane;
             const unsigned m = (i < M) ? smask[ft * M + i] : 0u;
             const unsigned long long bal = __ballot(m != 0);
-            if (m) dst[base + __popcll(bal & below)] = group_code(i, ft, m);
+            if (m) {
+                const int rank = __popcll(bal & below);
+                const long long b40 = ms[i] / 4 + wd[i * 5 + 4];
+                dst[base + rank] = group_code(b40, ft, m, first && rank == 0);
+            }
+            if (bal) first = false;
             base += __popcll(bal);
         }
+    }
     for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
     if (lane == 0) {
         gcount[p] = base;
@@ -168,6 +190,13 @@ __global__ __launch_bounds__(256) void k_unit_write(int npairs, mwg_pairs pr, co
         units[Ug + j] = make_int4(p, j * G - a, min(G, T - j * G), j == 0 ? -1 : Sg + j - 1);
 }
 
+// Group codes in level order (pair by pair, gstart numbering) with their pair.
+__global__ __launch_bounds__(64) void k_group_compact(const unsigned *codes, int stride, const int *gstart, int2 *grec) {
+    const int p = blockIdx.x;
+    const int base = gstart[p], cnt = gstart[p + 1] - base;
+    for (int k = threadIdx.x; k < cnt; k += 64) grec[base + k] = make_int2((int)codes[(size_t)p * stride + k], p);
+}
+
 // ---------------------------------------------------------------------------
 // 3. persistent unit kernels
 //
@@ -201,7 +230,7 @@ __device__ inline int take_unit(int *counter, int *s_unit, int total) {
 
 // One group of a unit: term, f component, gt mask, and the band node it acts on.
 struct Group {
-    int term, ft;
+    int b40, ft;  // operator matrix block of displacement 0, f component
     unsigned mask;
     bool fresh_f; // band node or ft differs from the previous group's
     int dl[3];    // band node translation minus output translation
@@ -258,9 +287,9 @@ struct UnitCursor {
         }
         const unsigned code = v.cd[(size_t)p * v.stride + k];
         k++;
-        grp.term = (int)(code & 0xffffu);
-        grp.ft = (int)((code >> 16) & 7u);
-        grp.mask = code >> 24;
+        grp.b40 = code_block(code);
+        grp.ft = code_ft(code);
+        grp.mask = code_mask(code);
         grp.fresh_f = fresh || grp.ft != curft;
         curft = grp.ft;
         for (int i = 0; i < 3; i++) grp.dl[i] = dl[i];
@@ -331,15 +360,14 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
                 for (int e = tid; e < N3; e += UnitShape<N1>::NT) sF[(e % N1) * P + e / N1] = grp.fsrc[(size_t)ft * N3 + e];
                 __syncthreads();
             }
-            const int w4 = cur.wd[grp.term * 5 + 4];
-            const long long mbase = cur.ms[grp.term];
+            const long long b4 = grp.b40;
             unsigned xneed, yneed;
             sweep_needs(gm, xneed, yneed);
 #pragma unroll
             for (int gx = 0; gx < 2; gx++) {
                 if (!(xneed >> gx & 1)) continue;
                 // x sweep: T1[i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox(i0, j0)
-                const auto Ox = mats + (size_t)(mbase + (long long)(grp.dl[0] + w4) * 4 + 2 * gx + (ft & 1)) * N2;
+                const auto Ox = mats + (size_t)((b4 + grp.dl[0]) * 4 + 2 * gx + (ft & 1)) * N2;
                 if (owner) {
                     double x[N1];
 #pragma unroll
@@ -358,7 +386,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
                 for (int gy = 0; gy < 2; gy++) {
                     if (!(yneed >> (gx + 2 * gy) & 1)) continue;
                     // y sweep: T2[i2][j0 + N1 j1] = sum_i1 T1[i1][i2 + N1 j0] Oy(i1, j1)
-                    const auto Oy = mats + (size_t)(mbase + (long long)(grp.dl[1] + w4) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
+                    const auto Oy = mats + (size_t)((b4 + grp.dl[1]) * 4 + 2 * gy + ((ft >> 1) & 1)) * N2;
                     if (owner) {
                         double y[N1];
 #pragma unroll
@@ -382,7 +410,7 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
                         for (int gz = 0; gz < 2; gz++) {
                             const int gt = gx + 2 * gy + 4 * gz;
                             if (!(gm >> gt & 1)) continue;
-                            const auto Oz = mats + (size_t)(mbase + (long long)(grp.dl[2] + w4) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
+                            const auto Oz = mats + (size_t)((b4 + grp.dl[2]) * 4 + 2 * gz + ((ft >> 2) & 1)) * N2;
 #pragma unroll
                             for (int j = 0; j < N1; j++) {
                                 double s = 0.0;
@@ -492,10 +520,18 @@ __device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename Band
     return c;
 }
 
+// An empty use of a loaded value: the compiler waits for its load here.
+__device__ inline void settled(double x) { asm volatile("" ::"v"(x)); }
+__device__ inline void settled(float x) { asm volatile("" ::"v"(x)); }
+__device__ inline void settled(const uint4 &x) {
+    const unsigned a = x.x, b = x.y, c = x.z, d = x.w;
+    asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d));
+}
+
 template <int N1, int PREC>
-__global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, const int *node_seg, const int *l, const long long *g_off, mwg_pairs pr,
-                                                   const unsigned *codes, int stride, const int *gstart, const int4 *units, const long long *uscan, int L,
-                                                   double *partial, int *counter) {
+__global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
+                                                   const int4 *pdl, const long long *pfo, const int *gstart, const int4 *units,
+                                                   const long long *uscan, int L, double *partial, int *counter) {
     using S = SweepShape<N1>;
     using G = BandGeom<N1, PREC>;
     using LT = typename BandTypes<PREC>::L;
@@ -503,72 +539,137 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
     using MM = typename BandTypes<PREC>::MM;
     constexpr int N2 = S::N2, N3 = S::N3, MT = S::MT, NT = S::NT, KSTEPS = G::KSTEPS, BLK = G::BLK;
     constexpr int FRAG = KSTEPS * NT * 64; // operand elements per (matrix block, f bit)
+    constexpr int FPT = (N3 + 255) / 256;  // band block elements per thread
     __shared__ __align__(16) LT sF[BLK];
     __shared__ __align__(16) LT sT1[2 * BLK];
     __shared__ __align__(16) LT sT2[4 * BLK];
     __shared__ int s_unit;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const int gxw = w & 1, gyw = w >> 1;
     const int li = lane & 15;
-    const UnitView v(segs, node_seg, l, g_off, pr, codes, stride, gstart, units, uscan, L);
+    const auto useg = ro(segs);
+    const auto unode = ro(node_seg);
+    const auto ud4 = ro(reinterpret_cast<const int *>(units));
+    const int total = unit_first(ro(uscan)[L]);
 
     for (int i = tid; i < BLK; i += 256) sF[i] = LT(0);
     for (int i = tid; i < 2 * BLK; i += 256) sT1[i] = LT(0);
     for (int i = tid; i < 4 * BLK; i += 256) sT2[i] = LT(0);
 
-    for (int u; (u = take_unit(counter, &s_unit, v.total)) >= 0;) {
-        UnitCursor cur;
-        cur.start(v, u);
-        const BT *frag = static_cast<const BT *>(v.segs[v.node_seg[cur.g]].op.frag[PREC]);
+    for (int u; (u = take_unit(counter, &s_unit, total)) >= 0;) {
+        // unit: groups [gi0, gi0 + ng) of one output node, contiguous in level order
+        const int p0 = ud4[4 * (size_t)u], ng = ud4[4 * (size_t)u + 2], slot = ud4[4 * (size_t)u + 3];
+        // this wave's (gx, gy) for the unit, rotated from unit to unit so that
+        // every SIMD of a CU serves all four z sweeps (how often each is needed
+        // depends on the gt masks)
+        const int role = (w + u) & 3;
+        const int gxw = role & 1, gyw = role >> 1;
+        const int gi0 = ro(gstart)[p0] + ud4[4 * (size_t)u + 1];
+        const int g = ro(pr.pair_g)[p0];
+        const_as4<mwg_segment> &SG = useg[unode[g]];
+        const BT *frag = static_cast<const BT *>(SG.op.frag[PREC]);
+        const double *coef = SG.f.coef;
+        double *dst = slot < 0 ? SG.g_coef + ro(g_off)[g] : partial + (size_t)slot * 8 * N3;
         dvec4 acc[MT][NT];
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
             for (int nt = 0; nt < NT; nt++) acc[mt][nt] = dvec4{0.0, 0.0, 0.0, 0.0};
 
-        // operand fragments of a group: matrix block b4 = mat_start/4 + w + dl of
-        // each direction; the z columns of a gz outside this wave's mask zeroed
-        auto fetch = [&](const Group &q, BT (&fx)[NT][KSTEPS], BT (&fy)[NT][KSTEPS], BT (&fz)[NT][KSTEPS]) {
-            const long long b40 = cur.ms[q.term] / 4 + cur.wd[q.term * 5 + 4];
-#pragma unroll
-            for (int nt = 0; nt < NT; nt++) {
-                const int nn = 16 * nt + li;
-                const int gz = nn / N1;
-                const bool keep = nn < 2 * N1 && ((q.mask >> (gxw + 2 * gyw + 4 * gz)) & 1);
-#pragma unroll
-                for (int ks = 0; ks < KSTEPS; ks++) {
-                    const size_t o = ((size_t)ks * NT + nt) * 64 + lane;
-                    fx[nt][ks] = frag[((b40 + q.dl[0]) * 2 + (q.ft & 1)) * FRAG + o];
-                    fy[nt][ks] = frag[((b40 + q.dl[1]) * 2 + ((q.ft >> 1) & 1)) * FRAG + o];
-                    const BT zv = frag[((b40 + q.dl[2]) * 2 + ((q.ft >> 2) & 1)) * FRAG + o];
-                    fz[nt][ks] = keep ? zv : BT{};
-                }
+        // The unit's group records, 64 to a chunk, lane j holding group 64 c + j:
+        // its code and its pair's translation difference and band node offset.
+        // Groups are read from registers (no memory access per group); chunk
+        // c + 1 is requested when group 64 c becomes current.
+        struct Chunk {
+            int code, d0, d1, d2, flo, fhi;
+        };
+        auto load_chunk = [&](int c, Chunk &ch) {
+            const int i = 64 * c + lane;
+            if (i < ng) {
+                const int2 r = grec[gi0 + i];
+                const int4 d = pdl[r.y];
+                const long long fo = pfo[r.y];
+                ch = Chunk{r.x, d.x, d.y, d.z, (int)(unsigned)(fo & 0xffffffffll), (int)(fo >> 32)};
             }
         };
-        // software pipeline: the next group's code is read and its fragments
-        // requested before the current group's sweeps
-        Group grp, nxt;
-        BT bx[NT][KSTEPS], by[NT][KSTEPS], bz[NT][KSTEPS];
-        BT nx[NT][KSTEPS], ny[NT][KSTEPS], nz[NT][KSTEPS];
-        bool have = cur.next(v, grp);
-        if (have) fetch(grp, bx, by, bz);
-        while (have) {
-            const bool have_next = cur.next(v, nxt);
-            if (have_next) fetch(nxt, nx, ny, nz);
-            const int ft = grp.ft;
-            const unsigned gm = grp.mask;
+        auto take = [&](const Chunk &ch, int i, Group &q) {
+            const int j = i & 63;
+            const unsigned code = (unsigned)__builtin_amdgcn_readlane(ch.code, j);
+            q.b40 = code_block(code);
+            q.ft = code_ft(code);
+            q.mask = code_mask(code);
+            q.fresh_f = code_fresh(code);
+            q.dl[0] = __builtin_amdgcn_readlane(ch.d0, j);
+            q.dl[1] = __builtin_amdgcn_readlane(ch.d1, j);
+            q.dl[2] = __builtin_amdgcn_readlane(ch.d2, j);
+            const long long fo = ((long long)__builtin_amdgcn_readlane(ch.fhi, j) << 32) | (unsigned)__builtin_amdgcn_readlane(ch.flo, j);
+            q.fsrc = coef + fo + (long long)q.ft * N3;
+        };
+        // operand fragments of a group: matrix block b40 + dl of each direction
+        // (nothing here may use the loaded values: the loads of the next group
+        // must stay in flight during the current group's sweeps)
+        auto fetch = [&](const Group &q, BT (&fx)[NT][KSTEPS], BT (&fy)[NT][KSTEPS], BT (&fz)[NT][KSTEPS]) {
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) {
+                    const int o = (ks * NT + nt) * 64 + lane;
+                    fx[nt][ks] = frag[(size_t)((q.b40 + q.dl[0]) * 2 + (q.ft & 1)) * FRAG + o];
+                    fy[nt][ks] = frag[(size_t)((q.b40 + q.dl[1]) * 2 + ((q.ft >> 1) & 1)) * FRAG + o];
+                    fz[nt][ks] = frag[(size_t)((q.b40 + q.dl[2]) * 2 + ((q.ft >> 2) & 1)) * FRAG + o];
+                }
+        };
+
+        // software pipeline: the next group's fragments are requested and, when
+        // it starts a new band block, that block loaded into registers before
+        // the current group's sweeps; the block goes to LDS between the current
+        // group's x and y sweeps (F is free then, and the barrier after the y
+        // sweep publishes it to the next x sweep)
+        Chunk ca{}, cb{};
+        load_chunk(0, ca);
+        if (ng > 64) load_chunk(1, cb);
+        using Frags = BT[NT][KSTEPS];
+        Group ga, gb;
+        Frags ax, ay, az, bx, by, bz; // operand fragments of two groups, used in turn
+        double fpre[FPT];
+        take(ca, 0, ga);
+        fetch(ga, ax, ay, az);
+        // F[i0][i1 + N1 i2] for element e = i0 + N1 (i1 + N1 i2)
+        for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)ga.fsrc[e];
+        __syncthreads();
+        // group i with its fragments in (fx, fy, fz); group i + 1 is read into nq
+        // and its fragments requested into (nfx, nfy, nfz).  Two buffer sets
+        // used in turn keep every load in flight until its group runs.
+        auto step = [&](int i, Group &cur, Frags &fx, Frags &fy, Frags &fz, Group &nq, Frags &nfx, Frags &nfy, Frags &nfz) -> bool {
+            // this group's fragments (requested one group ago) complete before
+            // anything new is requested, so no wait below queues behind the
+            // loads for the next group
+#pragma unroll
+            for (int nt = 0; nt < NT; nt++)
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) {
+                    settled(fx[nt][ks]);
+                    settled(fy[nt][ks]);
+                    settled(fz[nt][ks]);
+                }
+            // Every group requests the same loads in the same order (the last
+            // one its own data again, unused), so the waits below are exact.
+            const bool have_next = i + 1 < ng;
+            if (have_next && ((i + 1) & 63) == 0) take(cb, i + 1, nq);
+            else take(ca, have_next ? i + 1 : i, nq);
+            // the band block first: it is written to LDS within this group,
+            // so its loads must not queue behind the operand loads
+            const bool next_f = have_next && nq.fresh_f;
+#pragma unroll
+            for (int k = 0; k < FPT; k++) fpre[k] = nq.fsrc[min(tid + 256 * k, N3 - 1)];
+            fetch(nq, nfx, nfy, nfz);
+            const unsigned gm = cur.mask;
             unsigned xneed, yneed;
             sweep_needs(gm, xneed, yneed);
-            if (grp.fresh_f) {
-                // F[i0][i1 + N1 i2] for element e = i0 + N1 (i1 + N1 i2)
-                for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)grp.fsrc[(size_t)ft * N3 + e];
-                __syncthreads();
-            }
             // x sweep: T1[gx][i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox_gx(i0, j0)
             for (int t = w; t < MT * NT; t += 4) {
                 const int mt = t / NT, nt = t % NT;
-                const auto c = sweep_tile<N1, PREC>(sF, bx[nt], mt, lane);
+                const auto c = sweep_tile<N1, PREC>(sF, fx[nt], mt, lane);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int line = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
@@ -579,12 +680,19 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
                 }
             }
             __syncthreads();
+            if (next_f) {
+#pragma unroll
+                for (int k = 0; k < FPT; k++) {
+                    const int e = tid + 256 * k;
+                    if (e < N3) sF[G::at(e % N1, e / N1)] = (LT)fpre[k];
+                }
+            }
             // y sweep: T2[gx][gy][i2][j0 + N1 j1] = sum_i1 T1[gx][i1][i2 + N1 j0] Oy_gy(i1, j1)
             for (int t = w; t < 2 * MT * NT; t += 4) {
                 const int gx = t / (MT * NT), tt = t % (MT * NT);
                 if (!(xneed >> gx & 1)) continue;
                 const int mt = tt / NT, nt = tt % NT;
-                const auto c = sweep_tile<N1, PREC>(sT1 + gx * BLK, by[nt], mt, lane);
+                const auto c = sweep_tile<N1, PREC>(sT1 + gx * BLK, fy[nt], mt, lane);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int m = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
@@ -598,6 +706,15 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
             // z sweep of this wave's (gx, gy): G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[gx][gy][i2][j0 + N1 j1] Oz_gz(i2, j2)
             if (yneed >> (gxw + 2 * gyw) & 1) {
                 const LT *src = sT2 + (gxw * 2 + gyw) * BLK;
+                // the columns of a gz outside the mask zeroed
+                BT zb[NT][KSTEPS];
+#pragma unroll
+                for (int nt = 0; nt < NT; nt++) {
+                    const int nn = 16 * nt + li;
+                    const bool keep = nn < 2 * N1 && ((gm >> (gxw + 2 * gyw + 4 * (nn / N1))) & 1);
+#pragma unroll
+                    for (int ks = 0; ks < KSTEPS; ks++) zb[nt][ks] = keep ? fz[nt][ks] : BT{};
+                }
 #pragma unroll
                 for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -605,27 +722,25 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
                         if constexpr (PREC == MWG_PREC_FP64) {
 #pragma unroll
                             for (int ks = 0; ks < KSTEPS; ks++)
-                                acc[mt][nt] = MM::mma(src[G::at(4 * ks + (lane >> 4), 16 * mt + li)], bz[nt][ks], acc[mt][nt]);
+                                acc[mt][nt] = MM::mma(src[G::at(4 * ks + (lane >> 4), 16 * mt + li)], zb[nt][ks], acc[mt][nt]);
                         } else {
-                            const auto c = sweep_tile<N1, PREC>(src, bz[nt], mt, lane);
+                            const auto c = sweep_tile<N1, PREC>(src, zb[nt], mt, lane);
 #pragma unroll
                             for (int r = 0; r < 4; r++) acc[mt][nt][r] += (double)c[r];
                         }
                     }
             }
-            if (!have_next) break;
-            grp = nxt;
-#pragma unroll
-            for (int nt = 0; nt < NT; nt++)
-#pragma unroll
-                for (int ks = 0; ks < KSTEPS; ks++) {
-                    bx[nt][ks] = nx[nt][ks];
-                    by[nt][ks] = ny[nt][ks];
-                    bz[nt][ks] = nz[nt][ks];
-                }
+            if (have_next && ((i + 1) & 63) == 0) { // group i + 1 opens chunk (i + 1) / 64
+                ca = cb;
+                if (i + 1 + 64 < ng) load_chunk((i + 1) / 64 + 1, cb);
+            }
+            return have_next;
+        };
+        for (int i = 0;; i += 2) {
+            if (!step(i, ga, ax, ay, az, gb, bx, by, bz)) break;
+            if (!step(i + 1, gb, bx, by, bz, ga, ax, ay, az)) break;
         }
         // this wave's blocks gt = gx + 2 gy + 4 gz of the unit result
-        double *dst = cur.result(partial, 8 * N3);
 #pragma unroll
         for (int mt = 0; mt < MT; mt++)
 #pragma unroll
@@ -732,7 +847,7 @@ __global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs
 }
 
 struct BandWork {
-    Buffer codes, gstart, uscan, units, partial, counter;
+    Buffer codes, gstart, uscan, units, partial, counter, grec, pdl, pfo;
 };
 BandWork &work() {
     static BandWork w;
@@ -758,9 +873,9 @@ template <int N1, int PREC> int mfma_grid() {
 }
 template <int N1, int PREC>
 void launch_mfma(const mwg_level *lvl, const LevelDevice &dev, const mwg_pairs *pr, BandWork &w, int stride) {
-    hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->g_off, *pr,
-                       w.codes.as<unsigned>(), stride, w.gstart.as<int>(), w.units.as<int4>(), w.uscan.as<long long>(), lvl->count,
-                       w.partial.as<double>(), w.counter.as<int>());
+    hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->g_off, *pr,
+                       w.grec.as<int2>(), w.pdl.as<int4>(), w.pfo.as<long long>(), w.gstart.as<int>(), w.units.as<int4>(),
+                       w.uscan.as<long long>(), lvl->count, w.partial.as<double>(), w.counter.as<int>());
 }
 
 template <int N1> struct BandL {
@@ -789,15 +904,23 @@ template <int N1> struct BandL {
         if (w.units.ensure(sizeof(int4) * ((size_t)slots + L)) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.partial.ensure(slot_bytes * (size_t)slots) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.counter.ensure(sizeof(int)) != MWG_OK) return MWG_ERR_NOMEM;
+        const bool mfma = precision != kPrecValuBand;
+        if (mfma && w.grec.ensure(sizeof(int2) * (size_t)npairs * stride) != MWG_OK) return MWG_ERR_NOMEM;
+        if (w.pdl.ensure(sizeof(int4) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
+        if (w.pfo.ensure(sizeof(long long) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         int *gstart = w.gstart.as<int>();
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
             hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
-                               w.codes.as<unsigned>(), stride, gstart);
+                               w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>());
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
+            if (mfma) {
+                hipLaunchKernelGGL(k_group_compact, dim3(npairs), dim3(64), 0, 0, w.codes.as<unsigned>(), stride, gstart, w.grec.as<int2>());
+                MWG_LAUNCHED();
+            }
             hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>());
             MWG_LAUNCHED();
             s = scan_exclusive(uscan, uscan, L);
@@ -863,7 +986,7 @@ template <int N1> size_t frag_bytes(long long n_mat, int precision) {
 using namespace mwgdev;
 
 // Precision 3 selects the FP64 sweeps on the vector ALUs (comparison path).
-constexpr int kPrecValu = 3;
+constexpr int kPrecValu = kPrecValuBand;
 
 extern "C" size_t mwg_fragment_bytes(int kp1, long long n_mat, int precision) {
     if (n_mat < 0) return 0;
