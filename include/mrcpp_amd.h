@@ -104,7 +104,7 @@ int mrcpp_predictor_train(void *pred, const float *features, const float *labels
  * all tasks back to back (3 doubles each), ncenters[i] of task i (both may
  * be NULL).  collect != 0 gathers training samples (features, refined) of
  * every output node beyond the first step, read with mrcpp_samples_*.
- * stats: 16 doubles per task as mrcpp_apply, entries 12-14 = nodes
+ * stats: 16 doubles per task as mrcpp_apply, entries 13-15 = nodes
  * predicted, predicted nodes removed by the wavelet test, nodes computed in
  * fallback passes. */
 int mrcpp_apply_batch_predicted(int n, const double *prec, void *const *out, void *const *oper, void *const *inp, void *predictor,

@@ -426,7 +426,7 @@ def apply_predicted(tasks, predictor=None, centers=None, collect=False):
     stats = []
     for i in range(n):
         d = {k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])}
-        d.update(n_predicted=st[16 * i + 12], n_cropped=st[16 * i + 13], n_fallback=st[16 * i + 14])
+        d.update(n_predicted=st[16 * i + 13], n_cropped=st[16 * i + 14], n_fallback=st[16 * i + 15])
         stats.append(d)
     if not collect:
         return stats

@@ -32,9 +32,10 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> l, pairG, pairF, pairL, gFirst, gCount, count, genIds, genParent, genChild, missing, patch;
-    device::DeviceArray<long long> gOff, pOff, cOff;
-    device::DeviceArray<double> gNorms, nodeThr;
+    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, count, gen, missing, patch;
+    device::DeviceArray<long long> pOff, cOff;
+    device::DeviceArray<double> gNorms;
+    device::DeviceArray<long long> lvl; // a level's inputs in one upload: g_off | node thresholds | translations
     device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
 
@@ -253,26 +254,24 @@ void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const 
     fm.uploadTable();
     // one launch per scale, coarse to fine
     std::stable_sort(made.begin(), made.end(), [](const NewNode &a, const NewNode &b) { return a.scale < b.scale; });
-    std::vector<int> ids, par, ch;
-    for (const auto &m : made) {
-        ids.push_back(m.id);
-        par.push_back(m.parent);
-        ch.push_back(m.child);
+    // one upload: ids [n] | parents [n] | child indices [n]
+    const size_t n = made.size();
+    std::vector<int> gen(3 * n);
+    for (size_t k = 0; k < n; k++) {
+        gen[k] = made[k].id;
+        gen[n + k] = made[k].parent;
+        gen[2 * n + k] = made[k].child;
     }
     Scratch &S = scratch();
-    S.genIds.reserve(ids.size());
-    S.genParent.reserve(ids.size());
-    S.genChild.reserve(ids.size());
-    S.genIds.upload(ids.data(), ids.size());
-    S.genParent.upload(par.data(), par.size());
-    S.genChild.upload(ch.data(), ch.size());
+    S.gen.reserve(gen.size());
+    S.gen.upload(gen.data(), gen.size());
+    const int *ids = S.gen.get(), *par = ids + n, *ch = ids + 2 * n;
     const mwg_ftree fv = fm.view(*job.mra);
     size_t i = 0;
-    while (i < made.size()) {
+    while (i < n) {
         size_t j = i;
-        while (j < made.size() && made[j].scale == made[i].scale) j++;
-        device::check(mwg_generate_nodes(&fv, job.filters, S.genIds.get() + i, S.genParent.get() + i, S.genChild.get() + i, (int)(j - i)),
-                      "generate band nodes");
+        while (j < n && made[j].scale == made[i].scale) j++;
+        device::check(mwg_generate_nodes(&fv, job.filters, ids + i, par + i, ch + i, (int)(j - i)), "generate band nodes");
         i = j;
     }
     job.stats.nGenerated += (long long)made.size();
@@ -321,10 +320,21 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
             at++;
         }
     }
-    S.l.reserve(3 * (size_t)L);
-    S.l.upload(lv.data(), lv.size());
-    S.gOff.reserve(L);
-    S.gOff.upload(off.data(), L);
+    bool perNode = false;
+    for (const auto &sp : specs) perNode = perNode || jobs[sp.job].precFunc != nullptr;
+    // one upload: g_off [L] | node thresholds [L] (precision trees only) | translations [3L ints]
+    const size_t nThr = perNode ? (size_t)L : 0, nL8 = (3 * (size_t)L + 1) / 2;
+    std::vector<long long> packed((size_t)L + nThr + nL8, 0);
+    std::memcpy(packed.data(), off.data(), sizeof(long long) * L);
+    if (perNode) {
+        double *thr = reinterpret_cast<double *>(packed.data() + L);
+        size_t i = 0;
+        for (const auto &sp : specs)
+            for (int u : sp.nodes) thr[i++] = jobs[sp.job].precFunc ? node_threshold(jobs[sp.job], u) : jobs[sp.job].gthr;
+    }
+    std::memcpy(packed.data() + L + nThr, lv.data(), sizeof(int) * lv.size());
+    S.lvl.reserve(packed.size());
+    S.lvl.upload(packed.data(), packed.size());
     S.gNorms.reserve(9 * (size_t)L);
     S.gFirst.reserve(L);
     S.gCount.reserve(L);
@@ -335,20 +345,10 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.count = L;
     lvl.nseg = (int)segs.size();
     lvl.segs = segs.data();
-    lvl.l = S.l.get();
-    lvl.g_off = S.gOff.get();
+    lvl.g_off = S.lvl.get();
+    lvl.node_thr = perNode ? reinterpret_cast<const double *>(S.lvl.get() + L) : nullptr;
+    lvl.l = reinterpret_cast<const int *>(S.lvl.get() + L + nThr);
     lvl.g_norms = S.gNorms.get();
-    bool perNode = false;
-    for (const auto &sp : specs) perNode = perNode || jobs[sp.job].precFunc != nullptr;
-    if (perNode) {
-        std::vector<double> thr;
-        thr.reserve(L);
-        for (const auto &sp : specs)
-            for (int u : sp.nodes) thr.push_back(jobs[sp.job].precFunc ? node_threshold(jobs[sp.job], u) : jobs[sp.job].gthr);
-        S.nodeThr.reserve(L);
-        S.nodeThr.upload(thr.data(), thr.size());
-        lvl.node_thr = S.nodeThr.get();
-    }
 
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;

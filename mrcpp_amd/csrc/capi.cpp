@@ -279,9 +279,9 @@ int mrcpp_apply_batch_predicted(int n, const double *prec, void *const *out, voi
             for (int i = 0; i < n; i++) {
                 const ApplyStats &st = last_batch_stats()[i];
                 pack_stats(st, stats + 16 * (size_t)i);
-                stats[16 * (size_t)i + 12] = (double)st.nPredicted;
-                stats[16 * (size_t)i + 13] = (double)st.nCropped;
-                stats[16 * (size_t)i + 14] = (double)st.nFallback;
+                stats[16 * (size_t)i + 13] = (double)st.nPredicted;
+                stats[16 * (size_t)i + 14] = (double)st.nCropped;
+                stats[16 * (size_t)i + 15] = (double)st.nFallback;
             }
     });
 }

@@ -1,6 +1,8 @@
 // Device copy of a level's segments and the segment of every output node.
 #include "device_common.h"
 
+#include <cstring>
+
 namespace mwgdev {
 
 namespace {
@@ -18,6 +20,11 @@ __global__ __launch_bounds__(256) void k_node_seg(const mwg_segment *segs, int n
 }
 
 Buffer g_segs, g_node_seg;
+// the segments and node count of the last upload (a level whose segments did
+// not change since is not uploaded again)
+std::vector<char> g_last_segs;
+int g_last_count = -1;
+void *g_last_segs_dev = nullptr;
 
 } // namespace
 
@@ -46,12 +53,21 @@ mwg_status check_level(const mwg_level *lvl) {
 
 mwg_status upload_level(const mwg_level *lvl, LevelDevice &out) {
     const size_t sb = sizeof(mwg_segment) * (size_t)lvl->nseg;
+    const size_t b0 = g_segs.bytes, b1 = g_node_seg.bytes;
     if (g_segs.ensure(sb) != MWG_OK || g_node_seg.ensure(sizeof(int) * ((size_t)lvl->count + 1)) != MWG_OK) return MWG_ERR_NOMEM;
-    const mwg_status s = mwg_upload_async(g_segs.p, lvl->segs, sb);
-    if (s != MWG_OK) return s;
-    hipLaunchKernelGGL(k_node_seg, dim3((lvl->count + 255) / 256), dim3(256), 0, 0, g_segs.as<mwg_segment>(), lvl->nseg, lvl->count,
-                       g_node_seg.as<int>());
-    MWG_LAUNCHED();
+    const bool grown = g_segs.bytes != b0 || g_node_seg.bytes != b1; // reallocated: contents lost
+    const bool same = !grown && g_last_count == lvl->count && g_last_segs_dev == g_segs.p && g_last_segs.size() == sb &&
+                      std::memcmp(g_last_segs.data(), lvl->segs, sb) == 0;
+    if (!same) {
+        const mwg_status s = mwg_upload_async(g_segs.p, lvl->segs, sb);
+        if (s != MWG_OK) return s;
+        hipLaunchKernelGGL(k_node_seg, dim3((lvl->count + 255) / 256), dim3(256), 0, 0, g_segs.as<mwg_segment>(), lvl->nseg, lvl->count,
+                           g_node_seg.as<int>());
+        MWG_LAUNCHED();
+        g_last_segs.assign(reinterpret_cast<const char *>(lvl->segs), reinterpret_cast<const char *>(lvl->segs) + sb);
+        g_last_count = lvl->count;
+        g_last_segs_dev = g_segs.p;
+    }
     out.segs = g_segs.as<mwg_segment>();
     out.node_seg = g_node_seg.as<int>();
     return MWG_OK;

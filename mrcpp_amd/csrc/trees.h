@@ -15,6 +15,8 @@
 #include <functional>
 #include <memory>
 #include <unordered_map>
+
+#include <sys/mman.h>
 #include <vector>
 
 #include "core.h"
@@ -250,8 +252,13 @@ public:
     double treeSquareNorm = -1.0;
 
     int find(int n, const Index &l) const {
-        auto it = index_.find({n, l});
-        return it == index_.end() ? -1 : it->second;
+        if (slots_.empty()) return -1;
+        const size_t mask = slots_.size() - 1;
+        for (size_t s = slot_of(n, l);; s = (s + 1) & mask) {
+            const int u = slots_[s];
+            if (u < 0) return -1;
+            if (n_[u] == n && l_[u] == l) return u;
+        }
     }
     int findReal(int n, const Index &l) const {
         int u = find(n, l);
@@ -283,7 +290,7 @@ public:
             if (!hasChildren(v)) continue;
             for (int c = 0; c < TD; c++) {
                 const int w = child0_[v] + c;
-                index_.erase({n_[w], l_[w]});
+                unindex(w);
                 if (!gen_[w]) nodes_at_depth_[n_[w] - rootScale()]--;
                 stack.push_back(w);
             }
@@ -303,7 +310,7 @@ public:
             }
         for (int u = 0; u < keep; u++)
             if (child0_[u] >= keep) child0_[u] = -1;
-        for (int u = keep; u < size(); u++) index_.erase({n_[u], l_[u]});
+        for (int u = keep; u < size(); u++) unindex(u);
         n_.resize(keep);
         l_.resize(keep);
         parent_.resize(keep);
@@ -321,11 +328,13 @@ private:
     int add_node(int n, const Index &l, int parent, bool gen) {
         int u = size();
         if ((u >> kChunkBits) >= (int)chunks_.size()) {
-            // zero pages are mapped lazily: trees whose coefficients live on
-            // the device never touch this memory
-            void *p = std::calloc((size_t)(1 << kChunkBits) * ncoef_, sizeof(double));
-            if (!p) throw Error("host allocation failed");
-            chunks_.emplace_back(static_cast<double *>(p));
+            // anonymous pages are zero and mapped lazily: trees whose
+            // coefficients live on the device never touch this memory (a
+            // calloc from a reused heap block would clear it eagerly)
+            const size_t bytes = (size_t)(1 << kChunkBits) * ncoef_ * sizeof(double);
+            void *p = ::mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS, -1, 0);
+            if (p == MAP_FAILED) throw Error("host allocation failed");
+            chunks_.emplace_back(static_cast<double *>(p), MapDeleter{bytes});
         }
         n_.push_back(n);
         l_.push_back(l);
@@ -335,7 +344,7 @@ private:
         has_coefs_.push_back(0);
         for (int t = 0; t < TD; t++) norm_.push_back(-1.0);
         sq_.push_back(-1.0);
-        index_[{n, l}] = u;
+        index(u);
         if (!gen) {
             int dep = n - rootScale();
             if (dep >= (int)nodes_at_depth_.size()) nodes_at_depth_.resize(dep + 1, 0);
@@ -354,11 +363,62 @@ private:
     std::vector<int> parent_, child0_;
     std::vector<uint8_t> gen_, has_coefs_;
     std::vector<double> norm_, sq_;
-    struct FreeDeleter {
-        void operator()(double *p) const { std::free(p); }
+    struct MapDeleter {
+        size_t bytes = 0;
+        void operator()(double *p) const { ::munmap(p, bytes); }
     };
-    std::vector<std::unique_ptr<double, FreeDeleter>> chunks_;
-    std::unordered_map<std::pair<int, Index>, int, KeyHash<D>> index_;
+    std::vector<std::unique_ptr<double, MapDeleter>> chunks_;
+    // Open-addressing index (n, l) -> node id with linear probing: a slot holds
+    // a node id (-1 empty) and the key is that node's own (n, l).
+    std::vector<int> slots_;
+    size_t indexed_ = 0;
+    size_t slot_of(int n, const Index &l) const { return KeyHash<D>{}({n, l}) & (slots_.size() - 1); }
+    void index(int u) {
+        if (2 * (indexed_ + 1) > slots_.size()) {
+            std::vector<int> old(std::max<size_t>(1024, 2 * slots_.size()), -1);
+            old.swap(slots_);
+            indexed_ = 0;
+            for (int v : old)
+                if (v >= 0) place(v);
+        }
+        place(u);
+    }
+    void place(int u) {
+        const size_t mask = slots_.size() - 1;
+        size_t s = slot_of(n_[u], l_[u]);
+        while (slots_[s] >= 0) {
+            if (n_[slots_[s]] == n_[u] && l_[slots_[s]] == l_[u]) { // same index: the newer node replaces it
+                slots_[s] = u;
+                return;
+            }
+            s = (s + 1) & mask;
+        }
+        slots_[s] = u;
+        indexed_++;
+    }
+    // Remove node u's entry (if the index still maps its (n, l) to u), closing
+    // the gap by shifting later entries of the probe run back.
+    void unindex(int u) {
+        if (slots_.empty()) return;
+        const size_t mask = slots_.size() - 1;
+        size_t s = slot_of(n_[u], l_[u]);
+        while (slots_[s] != u) {
+            if (slots_[s] < 0) return;
+            s = (s + 1) & mask;
+        }
+        slots_[s] = -1;
+        indexed_--;
+        for (size_t j = (s + 1) & mask; slots_[j] >= 0; j = (j + 1) & mask) {
+            const size_t h = slot_of(n_[slots_[j]], l_[slots_[j]]);
+            // entry j may move to the hole s unless its home h lies cyclically in (s, j]
+            const bool stays = (s < j) ? (h > s && h <= j) : (h > s || h <= j);
+            if (!stays) {
+                slots_[s] = slots_[j];
+                slots_[j] = -1;
+                s = j;
+            }
+        }
+    }
     std::vector<int> nodes_at_depth_;
 };
 
