@@ -5,15 +5,19 @@
 // Pipeline, all on one stream:
 //  1. k_group_screen: one wave per candidate pair; lane t tests term c0 + t
 //     against every (ft, gt) with the exact norm test and the pair's groups
-//     (ft, term, gt mask) are written ft-major to a fixed-stride list.
+//     (operator block, ft, gt mask) are written ft-major to a fixed-stride
+//     list; the pair's translation difference and band node go to a pair
+//     record.
 //  2. scan of the group counts over pairs, k_unit_count per output node,
 //     scan of the unit counts, k_unit_write per pair: the groups of every
 //     output node are cut into work units of at most G consecutive groups
-//     (G grows with the level so the partial results fit a fixed budget).
-//  3. k_band_units: persistent workgroups take units from an atomic counter;
-//     a unit accumulates its groups' 3-sweep tensor products in registers
-//     and writes one 8-block result, the node's first unit straight into the
-//     output coefficients, later units into partial slots.
+//     (G grows with the level so the partial results fit a fixed budget), and
+//     the group codes are copied into level order.
+//  3. k_band_mfma (matrix cores) or k_band_units (vector ALUs): persistent
+//     workgroups take units from an atomic counter; a unit accumulates its
+//     groups' 3-sweep tensor products in registers and writes one 8-block
+//     result, the node's first unit straight into the output coefficients,
+//     later units into partial slots.
 //  4. k_reduce_units: per output node and chunk of coefficients, the partial
 //     slots are added in unit order; k_level_norms computes the component
 //     norms (MWNode::calcNorms, src/trees/MWNode.cpp:609-655).
@@ -171,30 +175,29 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
 __device__ inline int unit_first(long long v) { return (int)(v & 0xffffffffll); }
 __device__ inline int nonzero_before(long long v) { return (int)(v >> 32); }
 
-// Descriptors {pair, first group in the pair, groups, partial slot or -1} of
-// the units that start inside each pair.
-__global__ __launch_bounds__(256) void k_unit_write(int npairs, mwg_pairs pr, const int *gstart, const long long *uscan, int slots, int4 *units) {
-    const int p = blockIdx.x * 256 + threadIdx.x;
-    if (p >= npairs) return;
+// One wave per pair: the descriptors {pair, first group in the pair, groups,
+// partial slot or -1} of the units that start inside the pair and, when grec
+// is given, the pair's group codes in level order (gstart numbering) with the
+// pair index.
+__global__ __launch_bounds__(64) void k_unit_write(int npairs, mwg_pairs pr, const int *gstart, const long long *uscan, int slots, int4 *units,
+                                                   const unsigned *codes, int stride, int2 *grec) {
+    const int p = blockIdx.x;
+    const int base = gstart[p], cnt = gstart[p + 1] - base;
+    if (cnt == 0) return;
+    if (grec)
+        for (int k = threadIdx.x; k < cnt; k += 64) grec[base + k] = make_int2((int)codes[(size_t)p * stride + k], p);
+    if (threadIdx.x != 0) return;
     const int g = pr.pair_g[p];
     const int gf = pr.g_first[g], np = pr.g_npairs[g];
     const int B = gstart[gf];
     const int T = gstart[gf + np] - B;
     const int G = units_group_size(gstart[npairs], slots, T);
-    const int a = gstart[p] - B, e = gstart[p + 1] - B;
-    if (a == e) return;
+    const int a = base - B, e = a + cnt;
     const long long v = uscan[g];
     const int Ug = unit_first(v);
     const int Sg = Ug - nonzero_before(v);
     for (int j = (a + G - 1) / G; j * G < e; j++)
         units[Ug + j] = make_int4(p, j * G - a, min(G, T - j * G), j == 0 ? -1 : Sg + j - 1);
-}
-
-// Group codes in level order (pair by pair, gstart numbering) with their pair.
-__global__ __launch_bounds__(64) void k_group_compact(const unsigned *codes, int stride, const int *gstart, int2 *grec) {
-    const int p = blockIdx.x;
-    const int base = gstart[p], cnt = gstart[p + 1] - base;
-    for (int k = threadIdx.x; k < cnt; k += 64) grec[base + k] = make_int2((int)codes[(size_t)p * stride + k], p);
 }
 
 // ---------------------------------------------------------------------------
@@ -439,9 +442,9 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
 //
 // Four waves per workgroup.  Per group (ft, term, gt mask): the x sweep
 // F^T [Ox_0 | Ox_1] (tiles shared out over the waves), the y sweep for each
-// needed gx, then wave w = gx + 2 gy runs the z sweep of its own (gx, gy)
-// into its accumulators, the columns of a gz outside the mask zeroed in the
-// operand.  LDS holds F, T1 = [gx][i1][i2 + N1 j0], T2 = [gx][gy][i2][j0 + N1 j1].
+// needed gx, then each wave runs the z sweep of its own (gx, gy) (rotated
+// from unit to unit) into its accumulators, the columns of a gz outside the
+// mask zeroed in the operand.  LDS holds F, T1 = [gx][i1][i2 + N1 j0], T2 = [gx][gy][i2][j0 + N1 j1].
 //
 // PREC selects the operand types:
 //   FP64: 16x16x4 f64, LDS [contracted index][row], accumulate in place;
@@ -917,15 +920,12 @@ template <int N1> struct BandL {
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
-            if (mfma) {
-                hipLaunchKernelGGL(k_group_compact, dim3(npairs), dim3(64), 0, 0, w.codes.as<unsigned>(), stride, gstart, w.grec.as<int2>());
-                MWG_LAUNCHED();
-            }
             hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>());
             MWG_LAUNCHED();
             s = scan_exclusive(uscan, uscan, L);
             if (s != MWG_OK) return s;
-            hipLaunchKernelGGL(k_unit_write, dim3((npairs + 255) / 256), dim3(256), 0, 0, npairs, *pr, gstart, uscan, slots, w.units.as<int4>());
+            hipLaunchKernelGGL(k_unit_write, dim3(npairs), dim3(64), 0, 0, npairs, *pr, gstart, uscan, slots, w.units.as<int4>(), w.codes.as<unsigned>(),
+                               stride, mfma ? w.grec.as<int2>() : nullptr);
             MWG_LAUNCHED();
         }
         {

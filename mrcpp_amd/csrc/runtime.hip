@@ -67,6 +67,10 @@ std::map<void *, size_t> &block_size() {
 constexpr size_t kStageBytes = 64ull << 20;
 constexpr size_t kStageMaxCopy = 16ull << 20;
 unsigned char *g_stage = nullptr;
+// pinned landing buffer of downloads up to kStageMaxCopy: the copy is queued
+// on the stream behind the kernels (a pageable destination makes the runtime
+// drain the stream before it starts the copy)
+unsigned char *g_landing = nullptr;
 size_t g_stage_used = 0;
 
 } // namespace
@@ -250,7 +254,14 @@ mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes) {
 mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
     if (!bytes) return MWG_OK;
     if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
-    MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (bytes <= kStageMaxCopy) {
+        if (!g_landing) MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_landing), kStageMaxCopy, hipHostMallocDefault));
+        MWG_HIP(hipMemcpyAsync(g_landing, src, bytes, hipMemcpyDeviceToHost, 0));
+        MWG_HIP(hipStreamSynchronize(0));
+        std::memcpy(dst, g_landing, bytes);
+    } else {
+        MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
     g_stage_used = 0; // the null stream has drained
     return MWG_OK;
 }

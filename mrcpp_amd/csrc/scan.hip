@@ -39,6 +39,27 @@ template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_add
     if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = boff[nblocks];
 }
 
+// The whole scan in one workgroup for n <= kSingleMax: thread t owns the
+// contiguous items [t c, t c + c); two passes over its items (sum, then
+// write), so in and out may alias.
+constexpr int kSingleThreads = 1024;
+constexpr int kSingleMax = 64 * kSingleThreads;
+template <typename T> __global__ __launch_bounds__(kSingleThreads) void k_scan_single(const T *in, T *out, int n) {
+    __shared__ T tmp[kSingleThreads / 64];
+    const int c = (n + kSingleThreads - 1) / kSingleThreads;
+    const int a = min(n, (int)threadIdx.x * c), b = min(n, a + c);
+    T s = 0;
+    for (int i = a; i < b; i++) s += in[i];
+    T excl;
+    const T total = block_exclusive_scan<T, kSingleThreads>(s, tmp, excl);
+    for (int i = a; i < b; i++) {
+        const T v = in[i];
+        out[i] = excl;
+        excl += v;
+    }
+    if (threadIdx.x == 0) out[n] = total;
+}
+
 template <typename T> Buffer &scan_temp(int level) {
     static Buffer b[8];
     return b[level];
@@ -46,6 +67,11 @@ template <typename T> Buffer &scan_temp(int level) {
 
 template <typename T> mwg_status scan_rec(const T *in, T *out, int n, int level) {
     const int nb = (n + kScanItems - 1) / kScanItems;
+    if (nb > 1 && n <= kSingleMax) {
+        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kSingleThreads), 0, 0, in, out, n);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
     if (nb <= 1) {
         hipLaunchKernelGGL(k_scan_block<T>, dim3(1), dim3(kScanThreads), 0, 0, in, out, out + n, n);
         MWG_LAUNCHED();
