@@ -308,7 +308,8 @@ def main():
     ap.add_argument("--copy-probe", action="store_true")
     ap.add_argument("--precision", default="fp32", choices=["fp64", "fp32", "bf16x3", "fp64_vector"],
                     help="band sweep precision of the chosen form (the plain form is always fp64)")
-    ap.add_argument("--no-predictor", action="store_true", help="chosen form without the refinement predictor")
+    ap.add_argument("--predictor", action="store_true", help="chosen form with the refinement predictor")
+    ap.add_argument("--no-predictor", action="store_true", help="do not time the refinement-predictor variant beside the chosen form")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
@@ -324,21 +325,29 @@ def main():
     import numpy as np
     mr, w, src, mra, f, P = setup(args.workload)
     order, prec = w["order"], w["prec"]
-    pred = None if args.no_predictor else load_predictor(mr, order)
+    # the chosen form: FP32 sweeps (the fastest form within 1e-6 of plain
+    # FP64); the refinement predictor is timed beside it (or chosen with --predictor)
+    trained = load_predictor(mr, order)
+    pred = trained if args.predictor else None
 
     def plain_one():
         mr.set_sweep_precision("fp64")
         g = mr.FunctionTree(mra)
         return g, mr.apply(prec, g, P, f)
 
-    def chosen_one():
-        mr.set_sweep_precision(args.precision)
-        g = mr.FunctionTree(mra)
-        st = mr.apply_predicted([(prec, g, P, f)], predictor=pred, centers=[src.centers])[0]
-        return g, st
+    def form_one(predictor):
+        def one():
+            mr.set_sweep_precision(args.precision)
+            g = mr.FunctionTree(mra)
+            st = mr.apply_predicted([(prec, g, P, f)], predictor=predictor, centers=[src.centers])[0]
+            return g, st
+        return one
 
     plain = time_form(mr, plain_one, args.steps, args.warmup)
-    chosen = time_form(mr, chosen_one, args.steps, args.warmup)
+    chosen = time_form(mr, form_one(pred), args.steps, args.warmup)
+    variant = None
+    if not args.predictor and not args.no_predictor:
+        variant = time_form(mr, form_one(trained), args.steps, args.warmup)
     mr.set_sweep_precision("fp64")
     pst, cst = plain["stats"][-1], chosen["stats"][-1]
     n_applied = pst["n_applied"]
@@ -381,6 +390,18 @@ def main():
         "roofline": roofline(chosen["prof"], flop, args.steps, args.precision, args.workload),
         "roofline_plain_fp64": roofline(plain["prof"], flop, args.steps, "fp64", args.workload),
     }
+    if variant is not None:
+        vst = variant["stats"][-1]
+        i2, c2, _ = variant["out"].export()
+        vsame = bool(np.array_equal(i0, i2))
+        res["with_refinement_predictor"] = {
+            "form": "%s sweeps + refinement predictor" % args.precision, "ms_per_step": round(variant["s"] * 1e3, 4),
+            "speedup_vs_plain": round(plain["s"] / variant["s"], 3), "passes": int(vst["n_levels"]),
+            "n_predicted": int(vst.get("n_predicted", 0)), "n_cropped": int(vst.get("n_cropped", 0)),
+            "n_fallback": int(vst.get("n_fallback", 0)), "same_node_set": vsame,
+            "rel_l2": float(np.linalg.norm(c2 - c0) / np.linalg.norm(c0)) if vsame else None,
+            "step_times_s": [round(t, 6) for t in variant["times"]],
+            "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]
