@@ -88,6 +88,31 @@ def setup(workload):
     return mr, w, src, mra, f, P
 
 
+def time_projection(mr, w, src, mra, reps=3):
+    """build_grid + project of the workload's input on the GPU (median of reps)
+    and on the host cores (one run), with the node counts."""
+    import time
+    g = mr.GaussExp.from_sources(src)
+    out = {}
+    for target, n in (("device", reps), ("host", 1)):
+        mr.set_projection_target(target)
+        ts, nodes = [], 0
+        for _ in range(n):
+            f = mr.FunctionTree(mra)
+            mr.device_synchronize()
+            t0 = time.perf_counter()
+            mr.build_grid(f, g)
+            mr.project(w["prec"], f, g)
+            mr.device_synchronize()
+            ts.append(time.perf_counter() - t0)
+            nodes = f.getNNodes()
+        out[target] = {"s": sorted(ts)[len(ts) // 2], "nodes": int(nodes)}
+    mr.set_projection_target("auto")
+    return {"gpu_s": round(out["device"]["s"], 6), "host_s": round(out["host"]["s"], 4), "nodes": out["device"]["nodes"],
+            "nodes_host": out["host"]["nodes"], "speedup": round(out["host"]["s"] / out["device"]["s"], 1),
+            "what": "build_grid + project of the input tree (ProjectionCalculator levels on the GPU vs the host cores)"}
+
+
 def cpu_baseline(workload, w, src):
     """The CPU oracle's apply on the same workload (one full tree), timed on
     this host's cores."""
@@ -402,6 +427,7 @@ def main():
             "rel_l2": float(np.linalg.norm(c2 - c0) / np.linalg.norm(c0)) if vsame else None,
             "step_times_s": [round(t, 6) for t in variant["times"]],
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
+    res["projection"] = time_projection(mr, w, src, mra)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]

@@ -34,6 +34,9 @@ void mrcpp_tree_destroy(void *tree);
  * project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102) */
 int mrcpp_tree_build_grid_gaussians(void *tree, int n, const double *coefs, const double *alphas, const double *centers);
 int mrcpp_tree_project_gaussians(void *tree, double prec, int n, const double *coefs, const double *alphas, const double *centers);
+/* the same with project()'s maxIter and absPrec (src/treebuilders/project.cpp:85) */
+int mrcpp_tree_project_gaussians_ex(void *tree, double prec, int n, const double *coefs, const double *alphas, const double *centers, int max_iter,
+                                    int abs_prec);
 long long mrcpp_tree_num_nodes(void *tree);
 double mrcpp_tree_square_norm(void *tree);
 /* real nodes depth first: idx[5 i] = scale, lx, ly, lz, is_end;
@@ -112,6 +115,12 @@ int mrcpp_apply_batch_predicted(int n, const double *prec, void *const *out, voi
 int mrcpp_samples_count(void);
 /* copies the samples of the last collecting call; returns their number, -1 if cap is too small */
 int mrcpp_samples_fetch(float *features, float *labels, int cap);
+/* Where later projections (mrcpp_tree_project_gaussians) run on this thread:
+ * 0 the GPU when present (default), 1 the host cores, 2 the GPU.  No reference
+ * counterpart (ProjectionCalculator runs on OpenMP threads, src/treebuilders/ProjectionCalculator.cpp:34-51). */
+int mrcpp_set_projection_target(int target);
+/* 1 if the last projection on this thread ran on the GPU */
+int mrcpp_last_projection_on_device(void);
 /* nodes per breadth-first level of the last apply; returns the number of levels */
 int mrcpp_last_apply_levels(int *level_nodes, int cap);
 

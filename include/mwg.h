@@ -64,7 +64,8 @@ enum {
     MWG_KERNEL_COMPRESS = 5,
     MWG_KERNEL_NORMS = 6,
     MWG_KERNEL_GROUPS = 7, /* exact group screen and work-unit construction */
-    MWG_KERNEL_KINDS = 8
+    MWG_KERNEL_PROJECT = 8, /* projection of analytic functions onto tree nodes */
+    MWG_KERNEL_KINDS = 9
 };
 mwg_status mwg_profile(int enable);
 mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds);
@@ -229,6 +230,33 @@ mwg_status mwg_compress(int kp1, const double *filters, double *coef, const long
 
 /* Component norms of count nodes at coef + off[i], written to norms[9 i]. */
 mwg_status mwg_node_norms(int kp1, const double *coef, const long long *off, double *norms, int count);
+
+/* Projection of a sum of Gaussians, sum_g coef_g exp(-sum_d alpha_gd (r_d - pos_gd)^2),
+ * onto count nodes of one tree level (ProjectionCalculator::calcNode,
+ * src/treebuilders/ProjectionCalculator.cpp:34-51: function values at the
+ * Gauss-Legendre points of the node's 8 children, MWNode::getExpandedChildPts
+ * src/trees/MWNode.cpp:854-925; value -> coefficient map, MWNode::cvTransform
+ * :448-490; compression to the node's scaling and wavelet blocks,
+ * MWNode::mwTransform; MWNode::calcNorms :609-616).  Node i has scale[i] and
+ * translation l[3 i ..]; its 8 (k+1)^3 coefficients go to coef + off[i] and
+ * its 8 component norms and square norm to norms[9 i].  All arrays are device
+ * memory; gauss holds ngauss terms. */
+typedef struct {
+    double coef;
+    double alpha[3];
+    double pos[3];
+} mwg_gauss;
+typedef struct {
+    int kp1;
+    double sf[3];            /* world box scaling factors */
+    const double *nodes;     /* [kp1] quadrature nodes on [0, 1] */
+    const double *vcmap;     /* [kp1 * kp1] value -> coefficient map, column-major */
+    const double *filters;   /* mwg_filters_upload */
+    const mwg_gauss *gauss;
+    int ngauss;
+} mwg_projection;
+mwg_status mwg_project_nodes(const mwg_projection *pj, const int *scale, const int *l, const long long *off, double *coef, double *norms,
+                             int count);
 
 /* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;

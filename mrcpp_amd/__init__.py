@@ -40,6 +40,7 @@ SIGNATURES = {
     "mrcpp_tree_destroy": (None, [_vp]),
     "mrcpp_tree_build_grid_gaussians": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, _dp]),
     "mrcpp_tree_project_gaussians": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]),
+    "mrcpp_tree_project_gaussians_ex": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int]),
     "mrcpp_tree_num_nodes": (ctypes.c_longlong, [_vp]),
     "mrcpp_tree_square_norm": (ctypes.c_double, [_vp]),
     "mrcpp_tree_export": (ctypes.c_int, [_vp, _ip, _dp, _dp]),
@@ -57,6 +58,8 @@ SIGNATURES = {
                                           _dp, _dp, ctypes.c_int]),
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
     "mrcpp_apply_prec": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "mrcpp_set_projection_target": (ctypes.c_int, [ctypes.c_int]),
+    "mrcpp_last_projection_on_device": (ctypes.c_int, []),
     "mrcpp_apply_batch": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]),
     "mrcpp_apply_batch_predicted": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
@@ -92,7 +95,7 @@ SIGNATURES = {
     "mwg_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
 }
 
-KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups")
+KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups", "project")
 
 
 class MRCPPError(RuntimeError):
@@ -290,9 +293,24 @@ def build_grid(out, gexp):
     _check(lib().mrcpp_tree_build_grid_gaussians(out._h, len(c), _p(c), _p(a), _p(r)))
 
 
-def project(prec, out, gexp):
+def project(prec, out, gexp, maxIter=-1, absPrec=False):
+    """project(prec, tree, GaussExp, maxIter, absPrec): on the GPU when one is
+    present (set_projection_target), else on the host cores."""
     c, a, r = gexp.arrays()
-    _check(lib().mrcpp_tree_project_gaussians(out._h, float(prec), len(c), _p(c), _p(a), _p(r)))
+    _check(lib().mrcpp_tree_project_gaussians_ex(out._h, float(prec), len(c), _p(c), _p(a), _p(r), int(maxIter), int(bool(absPrec))))
+
+
+_PROJ_TARGETS = {"auto": 0, "host": 1, "device": 2}
+
+
+def set_projection_target(target):
+    """Where later project() calls run: "auto" (the GPU when present), "host" or "device"."""
+    _check(lib().mrcpp_set_projection_target(_PROJ_TARGETS[target]))
+
+
+def last_projection_on_device():
+    """Whether the last project() ran on the GPU."""
+    return bool(lib().mrcpp_last_projection_on_device())
 
 
 def dot(bra, ket):

@@ -39,9 +39,11 @@ struct Scratch {
     device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
 
+// kept until the process ends (never destroyed: no device call runs from a
+// static destructor)
 Scratch &scratch() {
-    static Scratch s;
-    return s;
+    static Scratch *s = new Scratch();
+    return *s;
 }
 
 // Device bytes per level batch for the screen candidates and the per-pair

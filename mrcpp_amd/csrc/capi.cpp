@@ -64,6 +64,11 @@ int mrcpp_tree_project_gaussians(void *tree, double prec, int n, const double *c
     return guarded([&] { project(prec, *static_cast<FunctionTree<3> *>(tree), gauss_sum(n, coefs, alphas, centers)); });
 }
 
+int mrcpp_tree_project_gaussians_ex(void *tree, double prec, int n, const double *coefs, const double *alphas, const double *centers, int max_iter,
+                                    int abs_prec) {
+    return guarded([&] { project(prec, *static_cast<FunctionTree<3> *>(tree), gauss_sum(n, coefs, alphas, centers), max_iter, abs_prec != 0); });
+}
+
 long long mrcpp_tree_num_nodes(void *tree) { return static_cast<FunctionTree<3> *>(tree)->getNNodes(); }
 
 double mrcpp_tree_square_norm(void *tree) { return static_cast<FunctionTree<3> *>(tree)->getSquareNorm(); }
@@ -314,6 +319,15 @@ int mrcpp_apply_prec(double prec, void *out, void *oper, void *inp, void *const 
         if (stats) pack_stats(last_apply_stats(), stats);
     });
 }
+
+int mrcpp_set_projection_target(int target) {
+    return guarded([&] {
+        if (target < 0 || target > 2) throw Error("projection target: 0 auto, 1 host, 2 device");
+        set_projection_target(static_cast<ProjectionTarget>(target));
+    });
+}
+
+int mrcpp_last_projection_on_device(void) { return last_projection_on_device() ? 1 : 0; }
 
 int mrcpp_last_apply_levels(int *level_nodes, int cap) {
     const ApplyStats &s = last_apply_stats();

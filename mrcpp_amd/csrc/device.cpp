@@ -14,12 +14,18 @@ void check(mwg_status s, const char *what) {
     if (s != MWG_OK) throw Error(std::string(what) + ": " + mwg_last_error());
 }
 
-void ensure_init() {
+namespace {
+mwg_status init_status() {
     static std::once_flag once;
     static mwg_status st = MWG_OK;
     std::call_once(once, [] { st = mwg_init(0); });
-    check(st, "device initialisation");
+    return st;
 }
+} // namespace
+
+void ensure_init() { check(init_status(), "device initialisation"); }
+
+bool available() { return init_status() == MWG_OK; }
 
 detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
     ensure_init();

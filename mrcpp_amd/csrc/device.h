@@ -15,6 +15,8 @@ namespace mrcpp {
 namespace device {
 
 void ensure_init();
+// whether a GPU is present and the device layer initialised
+bool available();
 void check(mwg_status s, const char *what);
 
 template <typename T> class DeviceArray {

@@ -1,6 +1,10 @@
 // FunctionTree<3> host methods, input mirroring and the host-side builders.
 #include "function_tree.h"
 
+#include <map>
+#include <memory>
+#include <mutex>
+
 #include <cmath>
 #include <cstring>
 
@@ -92,7 +96,10 @@ double FunctionTree<3>::evalf(const Coord<3> &r) {
 device::TreeMirror &FunctionTree<3>::inputMirror() {
     if (mirror_ && mirror_->valid) return *mirror_;
     device::ensure_init();
-    detail::NodeStore<3> &s = nodes();
+    // coefficients already in this tree's device pool (slot = node id, the
+    // mirror's layout) are copied on the device
+    const bool onDevice = host_stale_;
+    detail::NodeStore<3> &s = onDevice ? *store_ : nodes();
     auto m = std::make_unique<device::TreeMirror>();
     m->kp1 = s.kp1();
     m->ncoef = s.ncoef();
@@ -105,17 +112,21 @@ device::TreeMirror &FunctionTree<3>::inputMirror() {
     m->table.reset(4 * N + 1024);
     for (int u = 0; u < N; u++) m->table.insert(device::key_of(mra_, s.scale(u), s.transl(u)), u);
     const size_t nc = m->ncoef;
-    std::vector<double> stage((size_t)N * nc);
     std::vector<double> nrm((size_t)N * 9);
     m->hostOff.resize(N);
     for (int u = 0; u < N; u++) {
-        std::memcpy(stage.data() + (size_t)u * nc, s.coefs(u), sizeof(double) * nc);
         for (int t = 0; t < 8; t++) nrm[(size_t)u * 9 + t] = s.compNorm(u, t);
         nrm[(size_t)u * 9 + 8] = s.squareNorm(u);
         m->hostOff[u] = (long long)u * nc;
     }
-    m->coef.reserve(stage.size() + 1024 * (size_t)s.kp1_d());
-    m->coef.upload(stage.data(), stage.size());
+    m->coef.reserve((size_t)N * nc + 1024 * (size_t)s.kp1_d());
+    if (onDevice) {
+        device::check(mwg_copy(m->coef.get(), dev_coef_.get(), sizeof(double) * (size_t)N * nc), "mirror copy");
+    } else {
+        std::vector<double> stage((size_t)N * nc);
+        for (int u = 0; u < N; u++) std::memcpy(stage.data() + (size_t)u * nc, s.coefs(u), sizeof(double) * nc);
+        m->coef.upload(stage.data(), stage.size());
+    }
     m->norms.reserve(nrm.size() + 9 * 1024);
     m->norms.upload(nrm.data(), nrm.size());
     m->coefOff.reserve(N + 1024);
@@ -134,7 +145,147 @@ void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp) {
     out.invalidate();
 }
 
+namespace {
+
+thread_local ProjectionTarget g_proj_target = ProjectionTarget::Auto;
+thread_local bool g_proj_device = false;
+
+// Basis quadrature nodes and value -> coefficient map of an order on the
+// device, kept until the process ends (never destroyed: no device call runs
+// from a static destructor).
+struct BasisTables {
+    device::DeviceArray<double> nodes, vcmap;
+};
+const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra) {
+    static std::map<int, BasisTables *> memo;
+    static std::mutex m;
+    std::lock_guard<std::mutex> lk(m);
+    auto &slot = memo[mra.getOrder()];
+    if (!slot) {
+        slot = new BasisTables();
+        const auto &b = mra.basis();
+        slot->nodes.reserve(b.nodes().size());
+        slot->nodes.upload(b.nodes().data(), b.nodes().size());
+        slot->vcmap.reserve(b.vcMap().a.size());
+        slot->vcmap.upload(b.vcMap().a.data(), b.vcMap().a.size());
+    }
+    return *slot;
+}
+
+// project() with every level's nodes computed on the device (ProjectionCalculator
+// per level, then MWTree::mwTransform(BottomUp) and calcSquareNorm,
+// src/treebuilders/project.cpp:85-102); coefficients stay in the tree's device pool.
+void project_device(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter, bool absPrec) {
+    device::ensure_init();
+    detail::NodeStore<3> &s = out.topology();
+    const auto &mra = out.getMRA();
+    const int nc = s.ncoef();
+    const BasisTables &bt = basis_tables(mra);
+    const double *filt = device::filters(mra.getOrder());
+    std::vector<mwg_gauss> gs(inp.size());
+    for (int i = 0; i < inp.size(); i++) {
+        const auto &g = inp.getFunc(i);
+        gs[i].coef = g.getCoef();
+        for (int d = 0; d < 3; d++) {
+            gs[i].alpha[d] = g.getExp()[d];
+            gs[i].pos[d] = g.getPos()[d];
+        }
+    }
+    device::DeviceArray<mwg_gauss> dg;
+    dg.reserve(std::max<size_t>(gs.size(), 1));
+    dg.upload(gs.data(), gs.size());
+    mwg_projection pj{};
+    pj.kp1 = s.kp1();
+    for (int d = 0; d < 3; d++) pj.sf[d] = mra.getWorldBox().getScalingFactor(d);
+    pj.nodes = bt.nodes.get();
+    pj.vcmap = bt.vcmap.get();
+    pj.filters = filt;
+    pj.gauss = dg.get();
+    pj.ngauss = (int)gs.size();
+    device::DeviceArray<double> &pool = out.deviceCoefs();
+    device::DeviceArray<long long> lvl; // per level: offsets [n] | scale and translation ints [4n]
+    device::DeviceArray<double> nrm;
+    pool.reserve((size_t)s.size() * nc);
+    auto set_norms = [&](const std::vector<int> &v, const std::vector<double> &h) {
+        for (size_t i = 0; i < v.size(); i++) {
+            double *cn = s.norms(v[i]);
+            for (int t = 0; t < 8; t++) cn[t] = h[9 * i + t];
+            s.setSquareNorm(v[i], h[9 * i + 8]);
+            s.setHasCoefs(v[i], true);
+        }
+    };
+    detail::BuildHooks<3> hk;
+    hk.initial = [](detail::NodeStore<3> &st) { return detail::end_nodes(st); };
+    hk.compute = [&](detail::NodeStore<3> &st, const std::vector<int> &v) {
+        const size_t n = v.size();
+        if (!n) return;
+        pool.reserve((size_t)st.size() * nc, pool.capacity());
+        std::vector<long long> packed(n + (4 * n + 1) / 2, 0);
+        int *ints = reinterpret_cast<int *>(packed.data() + n);
+        for (size_t i = 0; i < n; i++) {
+            packed[i] = (long long)v[i] * nc;
+            ints[i] = st.scale(v[i]);
+            for (int d = 0; d < 3; d++) ints[n + 3 * i + d] = st.transl(v[i])[d];
+        }
+        lvl.reserve(packed.size());
+        lvl.upload(packed.data(), packed.size());
+        nrm.reserve(9 * n);
+        const int *dints = reinterpret_cast<const int *>(lvl.get() + n);
+        device::check(mwg_project_nodes(&pj, dints, dints + n, lvl.get(), pool.get(), nrm.get(), (int)n), "project nodes");
+        std::vector<double> h(9 * n);
+        nrm.download(h.data(), h.size());
+        set_norms(v, h);
+    };
+    hk.split = [prec, absPrec](const detail::NodeStore<3> &st, int u) {
+        return detail::wavelet_split(prec, st.treeSquareNorm, absPrec, st.scale(u), st.waveletNorm(u));
+    };
+    detail::build_tree(s, hk, maxIter);
+    // BottomUp (MWTree::mwTransformUp): parents from the deepest level up, then their norms
+    const auto tab = detail::real_nodes_by_depth(s);
+    std::vector<int> parents;
+    device::DeviceArray<long long> offs;
+    for (int d = (int)tab.size() - 2; d >= 0; d--) {
+        std::vector<int> v;
+        for (int u : tab[d])
+            if (s.hasRealChildren(u)) v.push_back(u);
+        if (v.empty()) continue;
+        std::vector<long long> o(2 * v.size());
+        for (size_t i = 0; i < v.size(); i++) {
+            o[i] = (long long)v[i] * nc;
+            o[v.size() + i] = (long long)s.child0(v[i]) * nc;
+        }
+        offs.reserve(o.size());
+        offs.upload(o.data(), o.size());
+        device::check(mwg_compress(s.kp1(), filt, pool.get(), offs.get(), offs.get() + v.size(), (int)v.size(), nc), "compress");
+        parents.insert(parents.end(), v.begin(), v.end());
+    }
+    if (!parents.empty()) {
+        std::vector<long long> o(parents.size());
+        for (size_t i = 0; i < parents.size(); i++) o[i] = (long long)parents[i] * nc;
+        offs.reserve(o.size());
+        offs.upload(o.data(), o.size());
+        nrm.reserve(9 * parents.size());
+        device::check(mwg_node_norms(s.kp1(), pool.get(), offs.get(), nrm.get(), (int)parents.size()), "node norms");
+        std::vector<double> h(9 * parents.size());
+        nrm.download(h.data(), h.size());
+        set_norms(parents, h);
+    }
+    detail::calc_square_norm(s);
+    out.markDeviceResident();
+}
+
+} // namespace
+
+void set_projection_target(ProjectionTarget t) { g_proj_target = t; }
+bool last_projection_on_device() { return g_proj_device; }
+
 void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter, bool absPrec) {
+    const bool dev = g_proj_target == ProjectionTarget::Device || (g_proj_target == ProjectionTarget::Auto && device::available());
+    g_proj_device = dev;
+    if (dev) {
+        project_device(prec, out, inp, maxIter, absPrec);
+        return;
+    }
     project_nodes(out.nodes(), prec, inp, maxIter, absPrec);
     out.invalidate();
 }

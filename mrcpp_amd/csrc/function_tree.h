@@ -57,8 +57,16 @@ private:
 
 // build_grid(tree, GaussExp) (src/treebuilders/grid.cpp:106-123)
 void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp);
-// project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102)
+// project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102).  Each
+// breadth-first level is computed on the MI355X when one is present (the
+// function values, value -> coefficient maps, compression and norms; the
+// host keeps the topology and the split test), else on the host cores.
 void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter = -1, bool absPrec = false);
+enum class ProjectionTarget { Auto = 0, Host = 1, Device = 2 };
+// Where later project() calls on this thread run (Auto: the GPU if present).
+void set_projection_target(ProjectionTarget t);
+// Whether the last project() on this thread ran on the GPU.
+bool last_projection_on_device();
 // dot(bra, ket) (src/treebuilders/multiply.cpp:286-318)
 double dot(FunctionTree<3> &bra, FunctionTree<3> &ket);
 

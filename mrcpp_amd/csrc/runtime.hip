@@ -6,6 +6,8 @@
 
 #include "device_common.h"
 
+#include <cstdlib>
+
 namespace mwgdev {
 
 namespace {
@@ -67,6 +69,7 @@ std::map<void *, size_t> &block_size() {
 constexpr size_t kStageBytes = 64ull << 20;
 constexpr size_t kStageMaxCopy = 16ull << 20;
 unsigned char *g_stage = nullptr;
+bool g_exiting = false; // set by an exit handler
 // pinned landing buffer of downloads up to kStageMaxCopy: the copy is queued
 // on the stream behind the kernels (a pageable destination makes the runtime
 // drain the stream before it starts the copy)
@@ -138,6 +141,10 @@ mwg_status mwg_init(int device) {
     MWG_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(MWG_ERR_ARG, "no such device");
     MWG_HIP(hipSetDevice(device));
+    // registered after the runtime's own exit handlers, so it runs before
+    // them: device memory held by static objects is not freed at exit
+    static const bool registered = std::atexit([] { g_exiting = true; }) == 0;
+    (void)registered;
     g_ready = true;
     return MWG_OK;
 }
@@ -172,6 +179,7 @@ mwg_status mwg_malloc(void **ptr, size_t bytes) {
 
 mwg_status mwg_free(void *ptr) {
     if (!ptr) return MWG_OK;
+    if (g_exiting) return MWG_OK; // process exit: the runtime may be gone, the memory goes with the process
     // the block is handed out again only after the work enqueued before this
     // free (kernels, asynchronous copies) has finished with it
     MWG_HIP(hipDeviceSynchronize());
