@@ -113,6 +113,21 @@ def time_projection(mr, w, src, mra, reps=3):
             "what": "build_grid + project of the input tree (ProjectionCalculator levels on the GPU vs the host cores)"}
 
 
+def time_operator_build(mr, w, mra):
+    """Construction of the workload's operator with its 2D operator trees formed
+    on the GPU (second of two builds) and on the host cores; the trees are
+    bit-identical."""
+    out = {}
+    for target, n in (("device", 2), ("host", 1)):
+        mr.set_operator_target(target)
+        for _ in range(n):
+            P = mr.PoissonOperator(mra, w["prec"]) if w["mu"] is None else mr.HelmholtzOperator(mra, w["mu"], w["prec"])
+        out[target] = P.build_seconds()
+    mr.set_operator_target("auto")
+    return {"gpu_s": round(out["device"], 5), "host_s": round(out["host"], 5), "speedup": round(out["host"] / out["device"], 1),
+            "terms": P.size(), "what": "operator construction (kernel expansion, 1D kernel trees, 2D operator trees, transforms, caches)"}
+
+
 def cpu_baseline(workload, w, src):
     """The CPU oracle's apply on the same workload (one full tree), timed on
     this host's cores."""
@@ -428,6 +443,7 @@ def main():
             "step_times_s": [round(t, 6) for t in variant["times"]],
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
     res["projection"] = time_projection(mr, w, src, mra)
+    res["operator_build"] = time_operator_build(mr, w, mra)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]

@@ -53,6 +53,13 @@ int mrcpp_operator_create(void *mra, int kind, double mu, double prec, void **op
 void mrcpp_operator_destroy(void *oper);
 int mrcpp_operator_size(void *oper);
 double mrcpp_operator_build_seconds(void *oper);
+/* Where later operator constructions on this thread form their 2D operator
+ * trees: 0 default (the host cores), 1 the host cores, 2 the GPU (one launch
+ * per refinement level for all terms); the trees are bit-identical either
+ * way.  No reference counterpart (CrossCorrelationCalculator runs on OpenMP threads). */
+int mrcpp_set_operator_target(int target);
+/* 1 if the operator's trees were formed on the GPU */
+int mrcpp_operator_built_on_device(void *oper);
 int mrcpp_operator_kernel(void *oper, double *alpha, double *coef);
 /* band widths at an apply precision: out[(term * max_depth + depth) * 5 + c], c = T,C,B,A,max
  * (MWOperator::calcBandWidths, src/operators/MWOperator.cpp:78-108) */

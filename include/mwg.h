@@ -258,6 +258,21 @@ typedef struct {
 mwg_status mwg_project_nodes(const mwg_projection *pj, const int *scale, const int *l, const long long *off, double *coef, double *norms,
                              int count);
 
+/* count operator-tree nodes of one refinement level, any mix of separated
+ * terms (CrossCorrelationCalculator::calcNode / applyCcc,
+ * src/treebuilders/CrossCorrelationCalculator.cpp:36-96; MWNode::mwTransform
+ * compression; OperatorNode::calcComponentNorm, src/trees/OperatorNode.cpp:56-80).
+ * kernel_blocks holds per node, for each child c (x bit c & 1, y bit c >> 1),
+ * the scaling blocks of the two 1D kernel nodes at translations l1 - l0 - 1 and
+ * l1 - l0 (8 x 2 kp1 doubles); fac[i] = sqrt(sf) 2^-(n+1)/2, thr[i] the norm
+ * threshold max(MachinePrec, prec / (8 2^depth)).  left / right: the
+ * cross-correlation tables (kp1^2 rows of 2 kp1, row-major).  Writes 4 kp1^2
+ * coefficients per node to coef and 4 component norms + square norm to
+ * norms[5 i].  Results are bit-identical to the host formation (no fused
+ * multiply-add, the host's summation order). */
+mwg_status mwg_operator_nodes(int kp1, const double *left, const double *right, const double *filters, const double *kernel_blocks,
+                              const double *fac, const double *thr, double *coef, double *norms, int count);
+
 /* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;
  * returns a device pointer (MWFilter::getSubFilter, src/core/MWFilter.cpp:100-133). */

@@ -59,6 +59,8 @@ SIGNATURES = {
     "mrcpp_apply": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.c_int, ctypes.c_int, _dp]),
     "mrcpp_apply_prec": (ctypes.c_int, [ctypes.c_double, _vp, _vp, _vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "mrcpp_set_projection_target": (ctypes.c_int, [ctypes.c_int]),
+    "mrcpp_set_operator_target": (ctypes.c_int, [ctypes.c_int]),
+    "mrcpp_operator_built_on_device": (ctypes.c_int, [_vp]),
     "mrcpp_last_projection_on_device": (ctypes.c_int, []),
     "mrcpp_apply_batch": (ctypes.c_int, [ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp),
                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), _dp]),
@@ -245,6 +247,10 @@ class ConvolutionOperator:
     def build_seconds(self):
         return float(lib().mrcpp_operator_build_seconds(self._h))
 
+    def built_on_device(self):
+        """Whether the 2D operator trees were formed on the GPU."""
+        return bool(lib().mrcpp_operator_built_on_device(self._h))
+
     def kernel(self):
         m = self.size()
         a, c = np.zeros(m), np.zeros(m)
@@ -306,6 +312,12 @@ _PROJ_TARGETS = {"auto": 0, "host": 1, "device": 2}
 def set_projection_target(target):
     """Where later project() calls run: "auto" (the GPU when present), "host" or "device"."""
     _check(lib().mrcpp_set_projection_target(_PROJ_TARGETS[target]))
+
+
+def set_operator_target(target):
+    """Where later operator constructions form their 2D operator trees: "auto"
+    (the host cores), "host" or "device" (bit-identical results)."""
+    _check(lib().mrcpp_set_operator_target(_PROJ_TARGETS[target]))
 
 
 def last_projection_on_device():

@@ -19,7 +19,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MRCPP_AMD_ARCH", "gfx950")
 
 HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "apply.cpp", "predictor.cpp", "capi.cpp"]
-HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip", "transform.hip", "project.hip", "mlp.hip"]
+HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip", "transform.hip", "project.hip", "opbuild.hip", "mlp.hip"]
 
 # IEEE evaluation order on the host (no FMA contraction) keeps host results
 # reproducible across compilers and identical to the CPU checker.
@@ -29,7 +29,9 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "
 # The band sweeps keep their loop-carried accumulators in VGPRs: the matrix
 # instructions then read and write them in place instead of the compiler
 # copying them into AGPRs and back around every group.
-HIP_FILE_FLAGS = {"band.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
+HIP_FILE_FLAGS = {"band.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
+                  # operator nodes are formed bit-identically to the host construction
+                  "opbuild.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd):

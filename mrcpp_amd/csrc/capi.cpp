@@ -120,6 +120,15 @@ int mrcpp_operator_size(void *oper) { return static_cast<ConvolutionOperator<3> 
 
 double mrcpp_operator_build_seconds(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->buildSeconds(); }
 
+int mrcpp_operator_built_on_device(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->builtOnDevice() ? 1 : 0; }
+
+int mrcpp_set_operator_target(int target) {
+    return guarded([&] {
+        if (target < 0 || target > 2) throw Error("operator target: 0 auto, 1 host, 2 device");
+        set_operator_target(static_cast<OperatorTarget>(target));
+    });
+}
+
 int mrcpp_operator_kernel(void *oper, double *alpha, double *coef) {
     return guarded([&] {
         const auto &k = static_cast<ConvolutionOperator<3> *>(oper)->kernel();

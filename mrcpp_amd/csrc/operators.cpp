@@ -3,10 +3,19 @@
 
 #include <chrono>
 #include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
 
 #include "device.h"
 
 namespace mrcpp {
+
+namespace {
+thread_local OperatorTarget g_operator_target = OperatorTarget::Auto;
+} // namespace
+
+void set_operator_target(OperatorTarget t) { g_operator_target = t; }
 
 std::vector<KernelTerm> poisson_kernel(double epsilon, double r_min, double r_max) {
     // Trapezoidal rule on the sinh-substituted integral representation of
@@ -113,39 +122,62 @@ std::vector<int> max_translations(const NodeStore<2> &t) {
     return mt;
 }
 
-// One Gaussian term: 1D kernel projection, 2D tree by cross-correlation,
-// bottom-up transform and translation cache (ConvolutionOperator::initialize,
-// src/operators/ConvolutionOperator.cpp:78-108).
-std::unique_ptr<detail::OperatorTerm> build_term(const MultiResolutionAnalysis<3> &mra, const MultiResolutionAnalysis<1> &kmra,
-                                                  const KernelTerm &term, double k_prec, double o_prec, int reach) {
+// The 1D kernel tree of one Gaussian term, projected at the kernel order
+// (ConvolutionOperator::initialize, src/operators/ConvolutionOperator.cpp:78-108).
+std::unique_ptr<NodeStore<1>> kernel_tree(const MultiResolutionAnalysis<1> &kmra, const KernelTerm &term, double k_prec) {
     GaussFunc<1> kf(term.alpha, std::copysign(std::pow(std::abs(term.coef), 1.0 / 3), term.coef));
-    NodeStore<1> kt(kmra);
+    auto kt = std::make_unique<NodeStore<1>>(kmra);
     GaussExp<1> ge;
     ge.append(kf);
-    build_grid_nodes(kt, ge);
-    project_nodes(kt, k_prec, kf);
+    build_grid_nodes(*kt, ge);
+    project_nodes(*kt, k_prec, kf);
+    return kt;
+}
 
+// An operator term with its 2D tree at the root boxes.
+std::unique_ptr<detail::OperatorTerm> term_shell(const MultiResolutionAnalysis<3> &mra, double o_prec, int reach) {
     auto term_out = std::make_unique<detail::OperatorTerm>();
     const double sf = mra.getWorldBox().getScalingFactor(0);
     term_out->mra = std::make_unique<MultiResolutionAnalysis<2>>(
         BoundingBox<2>(mra.getRootScale(), {0, 0}, {reach, reach}, {sf, sf}), LegendreBasis(mra.getOrder()));
     term_out->tree = std::make_unique<NodeStore<2>>(*term_out->mra, true, o_prec);
-    NodeStore<2> &ot = *term_out->tree;
+    return term_out;
+}
 
+// OperatorAdaptor::splitNode (src/treebuilders/OperatorAdaptor.h:38-48)
+bool operator_split(const NodeStore<2> &st, int u) {
+    const auto &l = st.transl(u);
+    if (!(l[0] == 0 || l[1] == 0)) return false;
+    for (int c = 1; c < 4; c++)
+        if (st.compNorm(u, c) > 0.0) return true;
+    return false;
+}
+
+void finish_term(detail::OperatorTerm &term);
+
+// One Gaussian term: 1D kernel projection, 2D tree by cross-correlation,
+// bottom-up transform and translation cache (ConvolutionOperator::initialize,
+// src/operators/ConvolutionOperator.cpp:78-108).
+std::unique_ptr<detail::OperatorTerm> build_term(const MultiResolutionAnalysis<3> &mra, const MultiResolutionAnalysis<1> &kmra,
+                                                  const KernelTerm &term, double k_prec, double o_prec, int reach) {
+    std::unique_ptr<NodeStore<1>> kt = kernel_tree(kmra, term, k_prec);
+    auto term_out = term_shell(mra, o_prec, reach);
+    NodeStore<2> &ot = *term_out->tree;
     detail::BuildHooks<2> h;
     h.initial = [](NodeStore<2> &st) { return detail::end_nodes(st); };
     h.compute = [&kt](NodeStore<2> &st, const std::vector<int> &v) {
-        for (int u : v) cross_correlate_node(st, u, kt);
+        for (int u : v) cross_correlate_node(st, u, *kt);
     };
-    // OperatorAdaptor::splitNode (src/treebuilders/OperatorAdaptor.h:38-48)
-    h.split = [](const NodeStore<2> &st, int u) {
-        const auto &l = st.transl(u);
-        if (!(l[0] == 0 || l[1] == 0)) return false;
-        for (int c = 1; c < 4; c++)
-            if (st.compNorm(u, c) > 0.0) return true;
-        return false;
-    };
+    h.split = operator_split;
     detail::build_tree(ot, h, -1);
+    finish_term(*term_out);
+    return term_out;
+}
+
+// Bottom-up transform, tree norm and translation cache of a built term.
+void finish_term(detail::OperatorTerm &term) {
+    auto *term_out = &term;
+    NodeStore<2> &ot = *term.tree;
     detail::transform_up(ot);
     detail::calc_square_norm(ot);
 
@@ -160,7 +192,127 @@ std::unique_ptr<detail::OperatorTerm> build_term(const MultiResolutionAnalysis<3
         for (int i = nt; i >= 0; i--) row[nt - i] = detail::get_or_generate(ot, scale, {0, i});
         for (int i = 1; i <= nt; i++) row[nt + i] = detail::get_or_generate(ot, scale, {i, 0});
     }
-    return term_out;
+}
+
+// Device tables of the cross-correlation for an order (left, right), kept
+// until the process ends (never destroyed: no device call from a static destructor).
+struct CccTables {
+    device::DeviceArray<double> left, right;
+};
+const CccTables &ccc_tables(int order) {
+    static std::map<int, CccTables *> memo;
+    static std::mutex m;
+    std::lock_guard<std::mutex> lk(m);
+    auto &slot = memo[order];
+    if (!slot) {
+        slot = new CccTables();
+        const auto &cc = core::cross_correlation(order);
+        const size_t n = (size_t)(order + 1) * (order + 1) * 2 * (order + 1);
+        slot->left.reserve(n);
+        slot->left.upload(cc.left(), n);
+        slot->right.reserve(n);
+        slot->right.upload(cc.right(), n);
+    }
+    return *slot;
+}
+
+// The 2D trees of all terms built together, one refinement level of every
+// term per device launch (TreeBuilder::build per term with
+// CrossCorrelationCalculator and OperatorAdaptor).  The host keeps each
+// term's topology, approximate norm and split test; the device forms the
+// nodes bit-identically to cross_correlate_node.
+void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &terms, std::vector<std::unique_ptr<NodeStore<1>>> &kts,
+                        const MultiResolutionAnalysis<3> &mra, double o_prec) {
+    const int M = (int)terms.size();
+    const int K = mra.getOrder() + 1, kd = K * K, K2 = 2 * K;
+    const CccTables &cc = ccc_tables(mra.getOrder());
+    const double *filt = device::filters(mra.getOrder());
+    const double sf = mra.getWorldBox().getScalingFactor(0);
+    struct State {
+        std::vector<int> work;
+        double sNorm = 0.0, wNorm = 0.0;
+        int iter = 0;
+    };
+    std::vector<State> st(M);
+    for (int i = 0; i < M; i++) st[i].work = detail::end_nodes(*terms[i]->tree);
+    device::DeviceArray<double> din, dfac, dthr, dcoef, dnrm;
+    for (;;) {
+        std::vector<size_t> base(M + 1, 0);
+        for (int i = 0; i < M; i++) base[i + 1] = base[i] + st[i].work.size();
+        const size_t n = base[M];
+        if (n == 0) break;
+        std::vector<double> in(n * 8 * K2), fac(n), thr(n);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < M; i++) {
+            NodeStore<2> &ot = *terms[i]->tree;
+            NodeStore<1> &kt = *kts[i];
+            for (size_t j = 0; j < st[i].work.size(); j++) {
+                const int u = st[i].work[j];
+                const size_t k = base[i] + j;
+                const int scale = ot.scale(u) + 1;
+                const auto &l = ot.transl(u);
+                for (int c = 0; c < 4; c++) {
+                    const int l0 = 2 * l[0] + (c & 1), l1 = 2 * l[1] + ((c >> 1) & 1);
+                    const int a = detail::get_or_generate(kt, scale, {l1 - l0 - 1});
+                    const int b = detail::get_or_generate(kt, scale, {l1 - l0});
+                    std::memcpy(in.data() + (k * 8 + 2 * c) * K2, kt.coefs(a), sizeof(double) * K2);
+                    std::memcpy(in.data() + (k * 8 + 2 * c + 1) * K2, kt.coefs(b), sizeof(double) * K2);
+                }
+                const double two_n = std::pow(2.0, -scale / 2.0);
+                fac[k] = std::sqrt(sf) * two_n;
+                thr[k] = std::max(MachinePrec, ot.normPrec() / (8.0 * (1 << ot.depth(u))));
+            }
+        }
+        din.reserve(in.size());
+        din.upload(in.data(), in.size());
+        dfac.reserve(n);
+        dfac.upload(fac.data(), n);
+        dthr.reserve(n);
+        dthr.upload(thr.data(), n);
+        dcoef.reserve(n * 4 * kd);
+        dnrm.reserve(n * 5);
+        device::check(mwg_operator_nodes(K, cc.left.get(), cc.right.get(), filt, din.get(), dfac.get(), dthr.get(), dcoef.get(), dnrm.get(), (int)n),
+                      "operator nodes");
+        std::vector<double> coef(n * 4 * kd), nrm(n * 5);
+        dcoef.download(coef.data(), coef.size());
+        dnrm.download(nrm.data(), nrm.size());
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < M; i++) {
+            NodeStore<2> &ot = *terms[i]->tree;
+            State &S = st[i];
+            if (S.work.empty()) continue;
+            for (size_t j = 0; j < S.work.size(); j++) {
+                const int u = S.work[j];
+                const size_t k = base[i] + j;
+                std::memcpy(ot.coefs(u), coef.data() + k * 4 * kd, sizeof(double) * 4 * kd);
+                double *cn = ot.norms(u);
+                for (int t = 0; t < 4; t++) cn[t] = nrm[k * 5 + t];
+                ot.setSquareNorm(u, nrm[k * 5 + 4]);
+                ot.setHasCoefs(u, true);
+            }
+            // TreeBuilder::build bookkeeping (src/treebuilders/TreeBuilder.cpp:38-87)
+            if (S.iter == 0) {
+                S.sNorm = 0.0;
+                for (int u : S.work)
+                    if (ot.depth(u) >= 0) S.sNorm += ot.scalingNorm(u);
+            }
+            for (int u : S.work)
+                if (ot.depth(u) >= 0) S.wNorm += ot.waveletNorm(u);
+            ot.treeSquareNorm = (S.sNorm < 0.0 || S.wNorm < 0.0) ? -1.0 : S.sNorm + S.wNorm;
+            std::vector<int> next;
+            const int maxScale = ot.mra().getMaxScale();
+            for (int u : S.work) {
+                if (ot.hasChildren(u)) continue;
+                if (ot.scale(u) + 2 > maxScale) continue;
+                if (operator_split(ot, u)) {
+                    const int c0 = ot.createChildren(u, false);
+                    for (int c = 0; c < 4; c++) next.push_back(c0 + c);
+                }
+            }
+            S.work.swap(next);
+            S.iter++;
+        }
+    }
 }
 
 // OperatorTree::calcBandWidth (src/trees/OperatorTree.cpp:109-134)
@@ -214,8 +366,24 @@ void ConvolutionOperator<3>::initialize(const std::vector<KernelTerm> &kernel, d
     core::filters(2 * mra_.getOrder() + 1);
     terms_.resize(kernel.size());
     const int M = (int)kernel.size();
+    // Auto builds on the host cores: the device path pays one host round
+    // trip per refinement level (about 25 for these kernels), so it only wins
+    // with few host cores
+    built_on_device_ = g_operator_target == OperatorTarget::Device;
+    if (built_on_device_) {
+        std::vector<std::unique_ptr<NodeStore<1>>> kts(M);
 #pragma omp parallel for schedule(dynamic, 1)
-    for (int i = 0; i < M; i++) terms_[i] = build_term(mra_, kmra, kernel[i], k_prec, o_prec, reach);
+        for (int i = 0; i < M; i++) {
+            kts[i] = kernel_tree(kmra, kernel[i], k_prec);
+            terms_[i] = term_shell(mra_, o_prec, reach);
+        }
+        build_trees_device(terms_, kts, mra_, o_prec);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < M; i++) finish_term(*terms_[i]);
+    } else {
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < M; i++) terms_[i] = build_term(mra_, kmra, kernel[i], k_prec, o_prec, reach);
+    }
     build_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 

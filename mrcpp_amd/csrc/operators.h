@@ -61,6 +61,12 @@ struct DeviceBandTables; // owned by the device layer
 
 template <int D> class ConvolutionOperator;
 
+// Where later operator constructions on this thread build the 2D operator
+// trees: Auto (the host cores), Host, Device (one launch per refinement
+// level for all terms).  The trees are bit-identical either way.
+enum class OperatorTarget { Auto = 0, Host = 1, Device = 2 };
+void set_operator_target(OperatorTarget t);
+
 // Isotropic separated 3D convolution operator (src/operators/ConvolutionOperator.h:62-84).
 template <> class ConvolutionOperator<3> {
 public:
@@ -78,6 +84,7 @@ public:
     double kernelMu() const { return mu_; }
     const detail::OperatorTerm &term(int i) const { return *terms_[i]; }
     double buildSeconds() const { return build_seconds_; }
+    bool builtOnDevice() const { return built_on_device_; }
 
     // MWOperator::calcBandWidths / getMaxBandWidth (src/operators/MWOperator.cpp:62-108)
     void calcBandWidths(double prec);
@@ -102,6 +109,7 @@ private:
     std::vector<int> band_max_;
     double build_prec_ = 0.0;
     double build_seconds_ = 0.0;
+    bool built_on_device_ = false;
     double table_prec_ = -1.0;
     std::unique_ptr<detail::BandTables> tables_;
     detail::DeviceBandTables *dev_tables_ = nullptr;
