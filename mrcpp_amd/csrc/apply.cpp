@@ -32,7 +32,7 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, count, gen, missing, patch;
+    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, gen, missing, patch;
     device::DeviceArray<long long> pOff, cOff;
     device::DeviceArray<double> gNorms;
     device::DeviceArray<long long> lvl; // a level's inputs in one upload: g_off | node thresholds | translations
@@ -340,7 +340,6 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     S.gNorms.reserve(9 * (size_t)L);
     S.gFirst.reserve(L);
     S.gCount.reserve(L);
-    S.count.reserve(3);
 
     mwg_level lvl{};
     lvl.kp1 = sh.kp1;
@@ -355,31 +354,35 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;
     size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
-    size_t mcap = std::max<size_t>(S.missing.capacity() / 6, 1024);
+    // S.missing: the screen's counters (4 ints) then the missing-node records
+    // (6 ints each), so one download returns the counts and the first records
+    constexpr size_t kHead = 4, kEagerRecords = 2048;
+    size_t mcap = std::max<size_t>(S.missing.capacity() > kHead ? (S.missing.capacity() - kHead) / 6 : 0, 1024);
     auto bind_pairs = [&]() {
         S.pairG.reserve(cap);
         S.pairF.reserve(cap);
         S.pairL.reserve(3 * cap);
-        S.missing.reserve(6 * mcap);
+        S.missing.reserve(kHead + 6 * mcap);
         pr.capacity = (int)std::min<size_t>(S.pairF.capacity(), S.pairL.capacity() / 3);
-        pr.count = S.count.get();
+        pr.count = S.missing.get();
         pr.pair_g = S.pairG.get();
         pr.pair_f = S.pairF.get();
         pr.pair_l = S.pairL.get();
         pr.g_first = S.gFirst.get();
         pr.g_npairs = S.gCount.get();
-        pr.missing = S.missing.get();
-        pr.missing_cap = (int)(S.missing.capacity() / 6);
+        pr.missing = S.missing.get() + kHead;
+        pr.missing_cap = (int)((S.missing.capacity() - kHead) / 6);
     };
     bind_pairs();
     auto t0 = clk::now();
+    std::vector<int> head;
     for (;;) {
         device::check(mwg_pair_screen(&lvl, &pr), "pair screen");
-        int cnt[3];
-        S.count.download(cnt, 3);
-        if (cnt[2]) throw Error("pair screen: band width does not match the operator tables");
-        npairs = cnt[0];
-        nmissing = cnt[1];
+        head.resize(kHead + 6 * std::min<size_t>(kEagerRecords, (size_t)pr.missing_cap));
+        S.missing.download(head.data(), head.size());
+        if (head[2]) throw Error("pair screen: band width does not match the operator tables");
+        npairs = head[0];
+        nmissing = head[1];
         if (npairs <= pr.capacity && nmissing <= pr.missing_cap) break;
         cap = std::max(cap, (size_t)npairs + npairs / 4);
         mcap = std::max(mcap, (size_t)nmissing + nmissing / 4);
@@ -391,7 +394,8 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         t0 = clk::now();
         // the pairs whose band node is missing: (pair, ancestor code, node, translation)
         std::vector<int> rec(6 * (size_t)nmissing);
-        S.missing.download(rec.data(), rec.size());
+        if ((size_t)nmissing <= kEagerRecords) std::copy(head.begin() + kHead, head.begin() + kHead + rec.size(), rec.begin());
+        else S.missing.download(rec.data(), rec.size(), kHead);
         std::vector<int> order(nmissing);
         for (int i = 0; i < nmissing; i++) order[i] = i;
         std::sort(order.begin(), order.end(), [&](int a, int b) { return rec[6 * a] < rec[6 * b]; }); // pair order: deterministic plan
