@@ -123,9 +123,30 @@ def time_operator_build(mr, w, mra):
         for _ in range(n):
             P = mr.PoissonOperator(mra, w["prec"]) if w["mu"] is None else mr.HelmholtzOperator(mra, w["mu"], w["prec"])
         out[target] = P.build_seconds()
+        if target == "device":
+            phases = P.build_phases()
     mr.set_operator_target("auto")
     return {"gpu_s": round(out["device"], 5), "host_s": round(out["host"], 5), "speedup": round(out["host"] / out["device"], 1),
-            "terms": P.size(), "what": "operator construction (kernel expansion, 1D kernel trees, 2D operator trees, transforms, caches)"}
+            "terms": P.size(), "gpu_phases_s": {k: round(v, 5) for k, v in phases.items()},
+            "what": "operator construction (kernel expansion, 1D kernel trees, 2D operator trees, transforms, caches)"}
+
+
+def time_batch_operator_build(mr, mra, specs):
+    """Construction of the batch's distinct operators: all together with their
+    2D trees formed on the GPU (second of two builds) against one by one on
+    the host cores."""
+    keys = sorted({(sp["mu"], sp["prec"]) for sp in specs}, key=lambda k: (k[0] is not None, k[0] or 0.0, k[1]))
+    out = {}
+    for target, n in (("device", 2), ("host", 1)):
+        for _ in range(n):
+            ops = mr.build_operators(mra, keys, target)
+        out[target] = ops[0].build_seconds()
+        if target == "device":
+            phases = ops[0].build_phases()
+            terms = sum(op.size() for op in ops)
+    return {"gpu_s": round(out["device"], 5), "host_s": round(out["host"], 5), "speedup": round(out["host"] / out["device"], 2),
+            "operators": len(keys), "terms": terms, "gpu_phases_s": {k: round(v, 5) for k, v in phases.items()},
+            "what": "construction of the batch's distinct Poisson/Helmholtz operators (one device pass vs one by one on the host cores)"}
 
 
 def cpu_baseline(workload, w, src):
@@ -282,6 +303,7 @@ def run_batch_bench(args):
         "step_times_s": [round(t, 6) for t in c_times],
     }
     res["roofline"]["frac"] = round(res["roofline"]["achieved"] / res["roofline"]["peak"], 5)
+    res["operator_build"] = time_batch_operator_build(mr, mra, specs)
     print(json.dumps(res))
 
 

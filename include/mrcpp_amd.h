@@ -60,6 +60,18 @@ double mrcpp_operator_build_seconds(void *oper);
 int mrcpp_set_operator_target(int target);
 /* 1 if the operator's trees were formed on the GPU */
 int mrcpp_operator_built_on_device(void *oper);
+/* n operators on one MRA built together: opers[i] is Poisson when mu[i] <= 0,
+ * else Helmholtz(mu[i]), at prec[i]; target 2 forms the 2D trees of all their
+ * terms in one GPU launch per refinement level, 0/1 builds them one by one on
+ * the host cores.  Each equals its single construction bit for bit.  No
+ * reference counterpart (MRCPP builds each operator in its constructor,
+ * src/operators/ConvolutionOperator.cpp:78-108). */
+int mrcpp_operators_create(void *mra, int n, const double *mu, const double *prec, int target, void **opers);
+/* seconds of the construction phases (out[5]): kernel trees and term set-up;
+ * per refinement level the packing of the GPU's inputs, the GPU round trips
+ * and the host's share (a host build counts its whole 2D tree construction
+ * here); bottom-up transforms and translation caches */
+int mrcpp_operator_build_phases(void *oper, double *out);
 int mrcpp_operator_kernel(void *oper, double *alpha, double *coef);
 /* band widths at an apply precision: out[(term * max_depth + depth) * 5 + c], c = T,C,B,A,max
  * (MWOperator::calcBandWidths, src/operators/MWOperator.cpp:78-108) */

@@ -49,6 +49,8 @@ SIGNATURES = {
     "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
     "mrcpp_operator_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
     "mrcpp_operator_destroy": (None, [_vp]),
+    "mrcpp_operators_create": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mrcpp_operator_build_phases": (ctypes.c_int, [_vp, _dp]),
     "mrcpp_operator_size": (ctypes.c_int, [_vp]),
     "mrcpp_operator_build_seconds": (ctypes.c_double, [_vp]),
     "mrcpp_operator_kernel": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -251,6 +253,15 @@ class ConvolutionOperator:
         """Whether the 2D operator trees were formed on the GPU."""
         return bool(lib().mrcpp_operator_built_on_device(self._h))
 
+    def build_phases(self):
+        """Seconds of the construction phases: kernel trees and term set-up;
+        per refinement level the packing of the GPU's inputs, the GPU round
+        trips and the host's share (a host build counts its whole 2D tree
+        construction here); bottom-up transforms and translation caches."""
+        out = np.zeros(5)
+        _check(lib().mrcpp_operator_build_phases(self._h, _p(out)))
+        return dict(zip(("kernel_trees", "pack", "device", "host_levels", "finish"), map(float, out)))
+
     def kernel(self):
         m = self.size()
         a, c = np.zeros(m), np.zeros(m)
@@ -284,6 +295,27 @@ class HelmholtzOperator(ConvolutionOperator):
 
     def __init__(self, mra, mu, prec):
         super().__init__(mra, self.HELMHOLTZ, mu, prec)
+
+
+def build_operators(mra, specs, target="device"):
+    """Several operators on one MRA built together: specs is a list of
+    (mu, prec), mu None or <= 0 for Poisson, else Helmholtz(mu).  With target
+    "device" the 2D trees of all their terms are formed in one GPU launch per
+    refinement level; "host" builds them one by one on the host cores.  Each
+    operator equals its own constructor's, bit for bit."""
+    n = len(specs)
+    mus = np.array([0.0 if mu is None else float(mu) for mu, _ in specs], dtype=np.float64)
+    precs = np.array([float(p) for _, p in specs], dtype=np.float64)
+    hs = (_vp * max(n, 1))()
+    _check(lib().mrcpp_operators_create(mra._h, n, _p(mus), _p(precs), _PROJ_TARGETS[target], hs))
+    out = []
+    for i in range(n):
+        mu = float(mus[i])
+        op = ConvolutionOperator.__new__(PoissonOperator if mu <= 0.0 else HelmholtzOperator)
+        op.mra = mra
+        op._h = _vp(hs[i])
+        out.append(op)
+    return out
 
 
 def kernel_terms(kind, mu, eps, r_min, r_max):

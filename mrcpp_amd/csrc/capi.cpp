@@ -122,6 +122,24 @@ double mrcpp_operator_build_seconds(void *oper) { return static_cast<Convolution
 
 int mrcpp_operator_built_on_device(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->builtOnDevice() ? 1 : 0; }
 
+int mrcpp_operators_create(void *mra, int n, const double *mu, const double *prec, int target, void **opers) {
+    return guarded([&] {
+        if (n < 0 || (n > 0 && (!mu || !prec || !opers))) throw Error("operators: bad arguments");
+        if (target < 0 || target > 2) throw Error("operator target: 0 auto, 1 host, 2 device");
+        std::vector<OperatorSpec> specs(n);
+        for (int i = 0; i < n; i++) specs[i] = {mu[i], prec[i]};
+        auto ops = build_operators(*static_cast<MultiResolutionAnalysis<3> *>(mra), specs, static_cast<OperatorTarget>(target));
+        for (int i = 0; i < n; i++) opers[i] = ops[i].release();
+    });
+}
+
+int mrcpp_operator_build_phases(void *oper, double *out) {
+    return guarded([&] {
+        const auto &ph = static_cast<ConvolutionOperator<3> *>(oper)->buildPhases();
+        for (int i = 0; i < 5; i++) out[i] = ph[i];
+    });
+}
+
 int mrcpp_set_operator_target(int target) {
     return guarded([&] {
         if (target < 0 || target > 2) throw Error("operator target: 0 auto, 1 host, 2 device");

@@ -221,8 +221,14 @@ const CccTables &ccc_tables(int order) {
 // CrossCorrelationCalculator and OperatorAdaptor).  The host keeps each
 // term's topology, approximate norm and split test; the device forms the
 // nodes bit-identically to cross_correlate_node.
-void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &terms, std::vector<std::unique_ptr<NodeStore<1>>> &kts,
-                        const MultiResolutionAnalysis<3> &mra, double o_prec) {
+// Returns the seconds spent packing, on the device round trips and on the
+// host's per-level work.
+std::array<double, 3> build_trees_device(const std::vector<detail::OperatorTerm *> &terms, const std::vector<NodeStore<1> *> &kts,
+                                         const MultiResolutionAnalysis<3> &mra) {
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); };
+    std::array<double, 3> times{};
+    device::ensure_init();
     const int M = (int)terms.size();
     const int K = mra.getOrder() + 1, kd = K * K, K2 = 2 * K;
     const CccTables &cc = ccc_tables(mra.getOrder());
@@ -235,13 +241,18 @@ void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &term
     };
     std::vector<State> st(M);
     for (int i = 0; i < M; i++) st[i].work = detail::end_nodes(*terms[i]->tree);
-    device::DeviceArray<double> din, dfac, dthr, dcoef, dnrm;
+    // per level one upload [kernel blocks (8 x 2K per node) | factors | thresholds]
+    // and one download [coefficients (4 K^2 per node) | norms (5 per node)]
+    device::DeviceArray<double> din, dout;
+    std::vector<double> in, outv;
     for (;;) {
         std::vector<size_t> base(M + 1, 0);
         for (int i = 0; i < M; i++) base[i + 1] = base[i] + st[i].work.size();
         const size_t n = base[M];
         if (n == 0) break;
-        std::vector<double> in(n * 8 * K2), fac(n), thr(n);
+        auto t0 = clk::now();
+        in.resize(n * (8 * K2 + 2));
+        double *fac = in.data() + n * 8 * K2, *thr = fac + n;
 #pragma omp parallel for schedule(dynamic, 1)
         for (int i = 0; i < M; i++) {
             NodeStore<2> &ot = *terms[i]->tree;
@@ -263,19 +274,19 @@ void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &term
                 thr[k] = std::max(MachinePrec, ot.normPrec() / (8.0 * (1 << ot.depth(u))));
             }
         }
+        times[0] += since(t0);
+        t0 = clk::now();
         din.reserve(in.size());
         din.upload(in.data(), in.size());
-        dfac.reserve(n);
-        dfac.upload(fac.data(), n);
-        dthr.reserve(n);
-        dthr.upload(thr.data(), n);
-        dcoef.reserve(n * 4 * kd);
-        dnrm.reserve(n * 5);
-        device::check(mwg_operator_nodes(K, cc.left.get(), cc.right.get(), filt, din.get(), dfac.get(), dthr.get(), dcoef.get(), dnrm.get(), (int)n),
+        dout.reserve(n * (4 * kd + 5));
+        device::check(mwg_operator_nodes(K, cc.left.get(), cc.right.get(), filt, din.get(), din.get() + n * 8 * K2, din.get() + n * (8 * K2 + 1),
+                                         dout.get(), dout.get() + n * 4 * kd, (int)n),
                       "operator nodes");
-        std::vector<double> coef(n * 4 * kd), nrm(n * 5);
-        dcoef.download(coef.data(), coef.size());
-        dnrm.download(nrm.data(), nrm.size());
+        outv.resize(n * (4 * kd + 5));
+        dout.download(outv.data(), outv.size());
+        const double *coef = outv.data(), *nrm = outv.data() + n * 4 * kd;
+        times[1] += since(t0);
+        t0 = clk::now();
 #pragma omp parallel for schedule(dynamic, 1)
         for (int i = 0; i < M; i++) {
             NodeStore<2> &ot = *terms[i]->tree;
@@ -284,7 +295,7 @@ void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &term
             for (size_t j = 0; j < S.work.size(); j++) {
                 const int u = S.work[j];
                 const size_t k = base[i] + j;
-                std::memcpy(ot.coefs(u), coef.data() + k * 4 * kd, sizeof(double) * 4 * kd);
+                std::memcpy(ot.coefs(u), coef + k * 4 * kd, sizeof(double) * 4 * kd);
                 double *cn = ot.norms(u);
                 for (int t = 0; t < 4; t++) cn[t] = nrm[k * 5 + t];
                 ot.setSquareNorm(u, nrm[k * 5 + 4]);
@@ -312,7 +323,9 @@ void build_trees_device(std::vector<std::unique_ptr<detail::OperatorTerm>> &term
             S.work.swap(next);
             S.iter++;
         }
+        times[2] += since(t0);
     }
+    return times;
 }
 
 // OperatorTree::calcBandWidth (src/trees/OperatorTree.cpp:109-134)
@@ -351,40 +364,130 @@ ConvolutionOperator<3>::ConvolutionOperator(const MultiResolutionAnalysis<3> &mr
 
 ConvolutionOperator<3>::~ConvolutionOperator() { device::destroy_band_tables(dev_tables_); }
 
-void ConvolutionOperator<3>::initialize(const std::vector<KernelTerm> &kernel, double prec) {
-    auto t0 = std::chrono::steady_clock::now();
+namespace {
+
+using clock_type = std::chrono::steady_clock;
+double seconds_since(clock_type::time_point t0) { return std::chrono::duration<double>(clock_type::now() - t0).count(); }
+
+// Root-box reach of the kernel MRA: order 2k+1 on [-reach, reach) root boxes
+// (ConvolutionOperator::getKernelMRA, src/operators/ConvolutionOperator.cpp:110-142).
+int kernel_reach(const MultiResolutionAnalysis<3> &mra) {
+    int reach = 0;
+    for (int d = 0; d < 3; d++) reach = std::max(reach, mra.getWorldBox().size(d));
+    return reach;
+}
+
+MultiResolutionAnalysis<1> kernel_mra(const MultiResolutionAnalysis<3> &mra) {
+    const int reach = kernel_reach(mra);
+    const double sf = mra.getWorldBox().getScalingFactor(0);
+    return MultiResolutionAnalysis<1>(BoundingBox<1>(mra.getRootScale(), {-reach}, {2 * reach}, {sf}), LegendreBasis(2 * mra.getOrder() + 1));
+}
+
+// the shared filter and cross-correlation caches, filled before a parallel region
+void prime_tables(const MultiResolutionAnalysis<3> &mra) {
+    core::cross_correlation(mra.getOrder());
+    core::filters(2 * mra.getOrder() + 1);
+}
+
+} // namespace
+
+void ConvolutionOperator<3>::prepare(const std::vector<KernelTerm> &kernel, double prec) {
     kernel_ = kernel;
     build_prec_ = prec;
-    const double o_prec = prec, k_prec = prec / 10.0;
-    // kernel MRA: order 2k+1 on [-reach, reach) root boxes (ConvolutionOperator::getKernelMRA :110-142)
-    int reach = 0;
-    for (int d = 0; d < 3; d++) reach = std::max(reach, mra_.getWorldBox().size(d));
-    const double sf = mra_.getWorldBox().getScalingFactor(0);
-    MultiResolutionAnalysis<1> kmra(BoundingBox<1>(mra_.getRootScale(), {-reach}, {2 * reach}, {sf}), LegendreBasis(2 * mra_.getOrder() + 1));
-    // prime the shared caches before the parallel region
-    core::cross_correlation(mra_.getOrder());
-    core::filters(2 * mra_.getOrder() + 1);
+    terms_.clear();
     terms_.resize(kernel.size());
-    const int M = (int)kernel.size();
-    // Auto builds on the host cores: the device path pays one host round
-    // trip per refinement level (about 25 for these kernels), so it only wins
-    // with few host cores
-    built_on_device_ = g_operator_target == OperatorTarget::Device;
-    if (built_on_device_) {
-        std::vector<std::unique_ptr<NodeStore<1>>> kts(M);
+    band_max_.clear();
+    build_phases_ = {};
+}
+
+void ConvolutionOperator<3>::build_host() {
+    const auto t0 = clock_type::now();
+    const MultiResolutionAnalysis<1> kmra = kernel_mra(mra_);
+    const int reach = kernel_reach(mra_);
+    prime_tables(mra_);
+    const double o_prec = build_prec_, k_prec = build_prec_ / 10.0;
+    const int M = (int)kernel_.size();
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int i = 0; i < M; i++) {
-            kts[i] = kernel_tree(kmra, kernel[i], k_prec);
-            terms_[i] = term_shell(mra_, o_prec, reach);
-        }
-        build_trees_device(terms_, kts, mra_, o_prec);
+    for (int i = 0; i < M; i++) terms_[i] = build_term(mra_, kmra, kernel_[i], k_prec, o_prec, reach);
+    built_on_device_ = false;
+    build_seconds_ = seconds_since(t0);
+    build_phases_ = {0.0, 0.0, 0.0, build_seconds_, 0.0};
+}
+
+// The terms of all operators (one MRA) in one list: kernel trees and term
+// shells on the host cores, the 2D trees one device launch per level for all,
+// then the bottom-up transforms and caches on the host cores.
+void ConvolutionOperator<3>::build_device(const std::vector<ConvolutionOperator<3> *> &ops) {
+    if (ops.empty()) return;
+    const auto t0 = clock_type::now();
+    const MultiResolutionAnalysis<3> &mra = ops[0]->mra_;
+    const MultiResolutionAnalysis<1> kmra = kernel_mra(mra);
+    const int reach = kernel_reach(mra);
+    prime_tables(mra);
+    std::vector<std::pair<ConvolutionOperator<3> *, int>> all;
+    for (auto *op : ops)
+        for (int i = 0; i < (int)op->kernel_.size(); i++) all.push_back({op, i});
+    const int T = (int)all.size();
+    std::vector<std::unique_ptr<NodeStore<1>>> kts(T);
+    std::vector<NodeStore<1> *> kraw(T);
+    std::vector<detail::OperatorTerm *> traw(T);
 #pragma omp parallel for schedule(dynamic, 1)
-        for (int i = 0; i < M; i++) finish_term(*terms_[i]);
-    } else {
-#pragma omp parallel for schedule(dynamic, 1)
-        for (int i = 0; i < M; i++) terms_[i] = build_term(mra_, kmra, kernel[i], k_prec, o_prec, reach);
+    for (int j = 0; j < T; j++) {
+        ConvolutionOperator<3> &op = *all[j].first;
+        const int i = all[j].second;
+        kts[j] = kernel_tree(kmra, op.kernel_[i], op.build_prec_ / 10.0);
+        op.terms_[i] = term_shell(mra, op.build_prec_, reach);
+        kraw[j] = kts[j].get();
+        traw[j] = op.terms_[i].get();
     }
-    build_seconds_ = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    const double tKernel = seconds_since(t0);
+    const std::array<double, 3> tLevels = build_trees_device(traw, kraw, mra);
+    const auto t2 = clock_type::now();
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int j = 0; j < T; j++) finish_term(*traw[j]);
+    const double tFinish = seconds_since(t2), total = seconds_since(t0);
+    for (auto *op : ops) {
+        op->built_on_device_ = true;
+        op->build_seconds_ = total;
+        op->build_phases_ = {tKernel, tLevels[0], tLevels[1], tLevels[2], tFinish};
+    }
+}
+
+void ConvolutionOperator<3>::initialize(const std::vector<KernelTerm> &kernel, double prec) {
+    prepare(kernel, prec);
+    // Auto builds on the host cores: the device path pays one host round trip
+    // per refinement level (about 25 for these kernels), which one operator's
+    // few hundred nodes per level do not amortise
+    if (g_operator_target == OperatorTarget::Device)
+        build_device({this});
+    else
+        build_host();
+}
+
+std::vector<std::unique_ptr<ConvolutionOperator<3>>> build_operators(const MultiResolutionAnalysis<3> &mra, const std::vector<OperatorSpec> &specs,
+                                                                     OperatorTarget target) {
+    std::vector<std::unique_ptr<ConvolutionOperator<3>>> out;
+    std::vector<ConvolutionOperator<3> *> raw;
+    for (const OperatorSpec &sp : specs) {
+        if (!(sp.prec > 0.0)) throw Error("operator precision must be positive");
+        const double k_prec = sp.prec / 10.0;
+        std::unique_ptr<ConvolutionOperator<3>> op(new ConvolutionOperator<3>(mra));
+        op->mu_ = sp.mu > 0.0 ? sp.mu : 0.0;
+        op->prepare(sp.mu > 0.0 ? helmholtz_kernel(sp.mu, k_prec, mra.calcMinDistance(k_prec), mra.calcMaxDistance())
+                                : poisson_kernel(k_prec, mra.calcMinDistance(k_prec), mra.calcMaxDistance()),
+                    sp.prec);
+        raw.push_back(op.get());
+        out.push_back(std::move(op));
+    }
+    if (target == OperatorTarget::Device) {
+        ConvolutionOperator<3>::build_device(raw);
+    } else {
+        const auto t0 = clock_type::now();
+        for (auto *op : raw) op->build_host();
+        const double total = seconds_since(t0);
+        for (auto *op : raw) op->build_seconds_ = total;
+    }
+    return out;
 }
 
 void ConvolutionOperator<3>::calcBandWidths(double prec) {

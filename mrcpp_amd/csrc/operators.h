@@ -67,6 +67,21 @@ template <int D> class ConvolutionOperator;
 enum class OperatorTarget { Auto = 0, Host = 1, Device = 2 };
 void set_operator_target(OperatorTarget t);
 
+// One operator of a batch construction: Poisson when mu <= 0, else Helmholtz(mu).
+struct OperatorSpec {
+    double mu = 0.0;
+    double prec = 1e-6;
+};
+
+// Several operators on one MRA constructed together (no reference
+// counterpart; MRCPP builds each operator in its constructor).  With
+// OperatorTarget::Device the 2D trees of every term of every operator are
+// formed in one device launch per refinement level; Host (or Auto) builds them
+// one after the other on the host cores.  Each operator equals the one its own
+// constructor builds, bit for bit; buildSeconds() of each is the batch's time.
+std::vector<std::unique_ptr<ConvolutionOperator<3>>> build_operators(const MultiResolutionAnalysis<3> &mra, const std::vector<OperatorSpec> &specs,
+                                                                     OperatorTarget target = OperatorTarget::Device);
+
 // Isotropic separated 3D convolution operator (src/operators/ConvolutionOperator.h:62-84).
 template <> class ConvolutionOperator<3> {
 public:
@@ -84,6 +99,12 @@ public:
     double kernelMu() const { return mu_; }
     const detail::OperatorTerm &term(int i) const { return *terms_[i]; }
     double buildSeconds() const { return build_seconds_; }
+    // seconds of the construction's phases: 1D kernel trees and term set-up;
+    // per refinement level the packing of the device's inputs, the device
+    // round trip (upload, launch, download) and the host's share (storing the
+    // nodes, norms, split tests; a host build counts its whole 2D tree
+    // construction here); bottom-up transforms and translation caches
+    const std::array<double, 5> &buildPhases() const { return build_phases_; }
     bool builtOnDevice() const { return built_on_device_; }
 
     // MWOperator::calcBandWidths / getMaxBandWidth (src/operators/MWOperator.cpp:62-108)
@@ -103,12 +124,20 @@ protected:
     double mu_ = 0.0;
 
 private:
+    void prepare(const std::vector<KernelTerm> &kernel, double prec);
+    void build_host();
+    static void build_device(const std::vector<ConvolutionOperator<3> *> &ops);
+
+    friend std::vector<std::unique_ptr<ConvolutionOperator<3>>> build_operators(const MultiResolutionAnalysis<3> &,
+                                                                                const std::vector<OperatorSpec> &, OperatorTarget);
+
     MultiResolutionAnalysis<3> mra_;
     std::vector<KernelTerm> kernel_;
     std::vector<std::unique_ptr<detail::OperatorTerm>> terms_;
     std::vector<int> band_max_;
     double build_prec_ = 0.0;
     double build_seconds_ = 0.0;
+    std::array<double, 5> build_phases_{};
     bool built_on_device_ = false;
     double table_prec_ = -1.0;
     std::unique_ptr<detail::BandTables> tables_;

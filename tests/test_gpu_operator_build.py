@@ -30,14 +30,11 @@ def _build(mr, mra, prec, mu, target):
         mr.set_operator_target("auto")
 
 
-@pytest.mark.parametrize("name,order,box,prec,mu", CASES, ids=[c[0] for c in CASES])
-def test_device_operator_trees_are_bit_identical(mr, name, order, box, prec, mu):
-    mra = mr.MultiResolutionAnalysis((-box, box), order)
-    H = _build(mr, mra, prec, mu, "host")
-    D = _build(mr, mra, prec, mu, "device")
-    assert not H.built_on_device() and D.built_on_device()
-    _say(name, "terms", D.size(), "build s host", H.build_seconds(), "device", D.build_seconds())
+def _same_operator(H, D, prec):
+    """Every cached operator node (matrix and norm) and the band widths at prec
+    identical; returns the number of nodes compared."""
     assert H.size() == D.size()
+    assert np.array_equal(*(op.kernel()[0] for op in (H, D))) and np.array_equal(*(op.kernel()[1] for op in (H, D)))
     bh, bd = H.bandwidths(prec), D.bandwidths(prec)
     assert np.array_equal(bh, bd)
     checked = 0
@@ -55,8 +52,34 @@ def test_device_operator_trees_are_bit_identical(mr, name, order, box, prec, mu)
                         continue
                     assert np.array_equal(mh, md) and nh == nd, (term, depth, l, comp)
                     checked += 1
+    return checked
+
+
+@pytest.mark.parametrize("name,order,box,prec,mu", CASES, ids=[c[0] for c in CASES])
+def test_device_operator_trees_are_bit_identical(mr, name, order, box, prec, mu):
+    mra = mr.MultiResolutionAnalysis((-box, box), order)
+    H = _build(mr, mra, prec, mu, "host")
+    D = _build(mr, mra, prec, mu, "device")
+    assert not H.built_on_device() and D.built_on_device()
+    _say(name, "terms", D.size(), "build s host", H.build_seconds(), "device", D.build_seconds())
+    checked = _same_operator(H, D, prec)
     _say(name, "nodes compared", checked)
     assert checked > 100
+
+
+def test_batched_device_construction(mr):
+    """Poisson and Helmholtz operators of different precisions built in one
+    device pass equal their own host constructions."""
+    mra = mr.MultiResolutionAnalysis((-20, 20), 5)
+    specs = [(None, 1e-5), (1.5, 1e-5), (0.7, 1e-6), (None, 1e-6), (2.9, 1e-4)]
+    D = mr.build_operators(mra, specs, "device")
+    assert len(D) == len(specs) and all(op.built_on_device() for op in D)
+    ph = D[0].build_phases()
+    _say("batched build s", D[0].build_seconds(), ph)
+    for (mu, prec), d in zip(specs, D):
+        h = _build(mr, mra, prec, mu, "host")
+        assert isinstance(d, mr.PoissonOperator if mu is None else mr.HelmholtzOperator)
+        assert _same_operator(h, d, prec) > 50
 
 
 def test_apply_with_device_built_operator(mr):

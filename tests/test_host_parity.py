@@ -69,6 +69,35 @@ def test_operator_bit_identical(mr, orc, box, order, prec, mu):
     assert compared > 1000
 
 
+def test_batched_host_construction_matches_oracle(mr, orc):
+    """build_operators on the host cores: each operator of a mixed batch equals
+    the oracle's construction of the same operator bit for bit."""
+    mra = mr.MultiResolutionAnalysis((-20, 20), 3)
+    omra = orc.OracleMRA((-20, 20), 3)
+    specs = [(None, 1e-4), (1.2, 1e-4), (None, 1e-3)]
+    ops = mr.build_operators(mra, specs, "host")
+    assert [type(op) for op in ops] == [mr.PoissonOperator, mr.HelmholtzOperator, mr.PoissonOperator]
+    for (mu, prec), P in zip(specs, ops):
+        assert not P.built_on_device()
+        OP = orc.OracleOperator(omra, prec, mu=mu)
+        assert OP.size() == P.size()
+        assert np.array_equal(OP.bandwidths(prec), P.bandwidths(prec))
+        compared = 0
+        for i in range(P.size()):
+            for d in range(0, 10):
+                for l in range(-3, 4):
+                    for c in range(4):
+                        a, an = OP.node(i, d, l, c)
+                        b, bn = P.node(i, d, l, c)
+                        assert (a is None) == (b is None)
+                        if a is not None:
+                            assert np.array_equal(a, b) and an == bn
+                            compared += 1
+        assert compared > 200
+    with pytest.raises(mr.MRCPPError):
+        mr.build_operators(mra, [(None, 0.0)], "host")
+
+
 def test_kernel_terms_match(mr, orc):
     for kind, mu in ((0, 0.0), (1, 0.7)):
         a1, c1 = orc.Oracle.kernel_terms(kind, mu, 1e-7, 1e-5, 55.4)
