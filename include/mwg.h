@@ -41,6 +41,11 @@ int mwg_device_count(void);
  * the block. */
 mwg_status mwg_malloc(void **ptr, size_t bytes);
 mwg_status mwg_free(void *ptr);
+/* Page-locked host memory for large transfers: mwg_upload / mwg_download with
+ * it on the host side run at the link's DMA rate.  mwg_host_free waits for the
+ * device first; after process exit has begun it does nothing. */
+mwg_status mwg_host_alloc(void **ptr, size_t bytes);
+mwg_status mwg_host_free(void *ptr);
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
 mwg_status mwg_download(void *dst, const void *src, size_t bytes);
 /* Upload through a pinned staging ring and return without waiting: src may be
@@ -262,16 +267,18 @@ mwg_status mwg_project_nodes(const mwg_projection *pj, const int *scale, const i
  * terms (CrossCorrelationCalculator::calcNode / applyCcc,
  * src/treebuilders/CrossCorrelationCalculator.cpp:36-96; MWNode::mwTransform
  * compression; OperatorNode::calcComponentNorm, src/trees/OperatorNode.cpp:56-80).
- * kernel_blocks holds per node, for each child c (x bit c & 1, y bit c >> 1),
- * the scaling blocks of the two 1D kernel nodes at translations l1 - l0 - 1 and
- * l1 - l0 (8 x 2 kp1 doubles); fac[i] = sqrt(sf) 2^-(n+1)/2, thr[i] the norm
- * threshold max(MachinePrec, prec / (8 2^depth)).  left / right: the
- * cross-correlation tables (kp1^2 rows of 2 kp1, row-major).  Writes 4 kp1^2
- * coefficients per node to coef and 4 component norms + square norm to
- * norms[5 i].  Results are bit-identical to the host formation (no fused
- * multiply-add, the host's summation order). */
+ * kernel_blocks: scaling blocks (2 kp1 doubles each) of the 1D kernel nodes at
+ * the children's scale; node i at translation (l0, l1) with D = 2 (l1 - l0)
+ * reads the four consecutive blocks from slot[i] on, at translations D - 2 ..
+ * D + 1 (child c, x bit c & 1, y bit c >> 1, uses l1' - l0' - 1 and
+ * l1' - l0').  fac[i] = sqrt(sf) 2^-(n+1)/2, thr[i] the norm threshold
+ * max(MachinePrec, prec / (8 2^depth)).  left / right: the cross-correlation
+ * tables (kp1^2 rows of 2 kp1, row-major).  Writes 4 kp1^2 coefficients per
+ * node to coef and 4 component norms + square norm to norms[5 i].  Results
+ * are bit-identical to the host formation (no fused multiply-add, the host's
+ * summation order). */
 mwg_status mwg_operator_nodes(int kp1, const double *left, const double *right, const double *filters, const double *kernel_blocks,
-                              const double *fac, const double *thr, double *coef, double *norms, int count);
+                              const int *slot, const double *fac, const double *thr, double *coef, double *norms, int count);
 
 /* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;

@@ -67,6 +67,37 @@ private:
     size_t cap_ = 0;
 };
 
+// Page-locked host buffer, grown on demand (contents are not kept).
+template <typename T> class PinnedArray {
+public:
+    PinnedArray() = default;
+    ~PinnedArray() { release(); }
+    PinnedArray(const PinnedArray &) = delete;
+    PinnedArray &operator=(const PinnedArray &) = delete;
+
+    T *data() const { return ptr_; }
+    size_t capacity() const { return cap_; }
+    void reserve(size_t n) {
+        if (n <= cap_) return;
+        size_t nc = cap_ ? cap_ : 4096;
+        while (nc < n) nc *= 2;
+        release();
+        void *p = nullptr;
+        check(mwg_host_alloc(&p, nc * sizeof(T)), "pinned host allocation");
+        ptr_ = static_cast<T *>(p);
+        cap_ = nc;
+    }
+    void release() {
+        if (ptr_) mwg_host_free(ptr_);
+        ptr_ = nullptr;
+        cap_ = 0;
+    }
+
+private:
+    T *ptr_ = nullptr;
+    size_t cap_ = 0;
+};
+
 } // namespace device
 
 namespace detail {

@@ -17,17 +17,25 @@ namespace {
 
 constexpr int kOpThreads = 256;
 
-// One workgroup per node.  in: for each child c (x bit = c & 1, y bit = c >> 1)
-// the scaling blocks of the two 1D kernel nodes at translations l1 - l0 - 1 and
-// l1 - l0 (2K values each, K = k + 1), 8 x 2K doubles per node.
+// One workgroup per node.  blocks: 1D kernel scaling blocks (2K values each,
+// K = k + 1); the node's four consecutive blocks from slot[k] hold the
+// translations D - 2 .. D + 1, D = 2 (l1 - l0).  Child c (x bit c & 1, y bit
+// c >> 1) correlates the blocks at l1' - l0' - 1 and l1' - l0', where
+// l1' - l0' = D + (c >> 1) - (c & 1): offsets 1 + (c >> 1) - (c & 1) and one more.
 template <int N1>
-__global__ __launch_bounds__(kOpThreads) void k_cross_correlate(const double *left, const double *right, const double *filt, const double *in,
-                                                                 const double *fac, const double *thr, double *coef, double *norms) {
+__global__ __launch_bounds__(kOpThreads) void k_cross_correlate(const double *left, const double *right, const double *filt, const double *blocks,
+                                                                 const int *slot, const double *fac, const double *thr, double *coef,
+                                                                 double *norms) {
     constexpr int KD = N1 * N1, K2 = 2 * N1;
     __shared__ double a[4 * KD], b[4 * KD];
     __shared__ double s1d[8 * K2];
     const int k = blockIdx.x;
-    for (int i = threadIdx.x; i < 8 * K2; i += kOpThreads) s1d[i] = in[(size_t)k * 8 * K2 + i];
+    const size_t s0 = (size_t)slot[k];
+    for (int i = threadIdx.x; i < 8 * K2; i += kOpThreads) {
+        const int c = i / (2 * K2), which = (i / K2) & 1;
+        const int off = 1 + ((c >> 1) & 1) - (c & 1) + which;
+        s1d[i] = blocks[(s0 + off) * K2 + i % K2];
+    }
     __syncthreads();
     const double f = fac[k];
     for (int o = threadIdx.x; o < 4 * KD; o += kOpThreads) {
@@ -103,9 +111,9 @@ __global__ __launch_bounds__(kOpThreads) void k_cross_correlate(const double *le
 }
 
 template <int N1> struct CrossL {
-    static mwg_status run(const double *left, const double *right, const double *filt, const double *in, const double *fac, const double *thr,
-                          double *coef, double *norms, int count) {
-        hipLaunchKernelGGL(k_cross_correlate<N1>, dim3(count), dim3(kOpThreads), 0, 0, left, right, filt, in, fac, thr, coef, norms);
+    static mwg_status run(const double *left, const double *right, const double *filt, const double *blocks, const int *slot, const double *fac,
+                          const double *thr, double *coef, double *norms, int count) {
+        hipLaunchKernelGGL(k_cross_correlate<N1>, dim3(count), dim3(kOpThreads), 0, 0, left, right, filt, blocks, slot, fac, thr, coef, norms);
         MWG_LAUNCHED();
         return MWG_OK;
     }
@@ -118,10 +126,10 @@ template <int N1> struct CrossL {
 using namespace mwgdev;
 
 extern "C" mwg_status mwg_operator_nodes(int kp1, const double *left, const double *right, const double *filters, const double *kernel_blocks,
-                                         const double *fac, const double *thr, double *coef, double *norms, int count) {
+                                         const int *slot, const double *fac, const double *thr, double *coef, double *norms, int count) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     if (count < 0) return fail(MWG_ERR_ARG, "negative node count");
     if (count == 0) return MWG_OK;
-    if (!left || !right || !filters || !kernel_blocks || !fac || !thr || !coef || !norms) return fail(MWG_ERR_ARG, "null argument");
-    return dispatch<CrossL>(kp1, left, right, filters, kernel_blocks, fac, thr, coef, norms, count);
+    if (!left || !right || !filters || !kernel_blocks || !slot || !fac || !thr || !coef || !norms) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<CrossL>(kp1, left, right, filters, kernel_blocks, slot, fac, thr, coef, norms, count);
 }

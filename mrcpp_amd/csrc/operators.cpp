@@ -1,6 +1,8 @@
 // Operator construction (host FP64) and band tables for the device kernels.
 #include "operators.h"
 
+#include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -241,50 +243,90 @@ std::array<double, 3> build_trees_device(const std::vector<detail::OperatorTerm 
     };
     std::vector<State> st(M);
     for (int i = 0; i < M; i++) st[i].work = detail::end_nodes(*terms[i]->tree);
-    // per level one upload [kernel blocks (8 x 2K per node) | factors | thresholds]
-    // and one download [coefficients (4 K^2 per node) | norms (5 per node)]
+    // Per level one upload from page-locked memory, [kernel blocks | factors |
+    // thresholds | block slots], and one download [coefficients (4 K^2 per
+    // node) | norms (5 per node)].  A node at (l0, l1) reads the kernel blocks
+    // at translations D - 2 .. D + 1, D = 2 (l1 - l0); a term's nodes with
+    // values of D less than 5 apart share one run of consecutive blocks, so a
+    // level uploads about one block per node instead of eight.
+    static std::mutex pin_mutex;
+    static device::PinnedArray<double> &pin_in = *new device::PinnedArray<double>(), &pin_out = *new device::PinnedArray<double>();
+    std::lock_guard<std::mutex> pin_lock(pin_mutex);
     device::DeviceArray<double> din, dout;
-    std::vector<double> in, outv;
+    std::vector<std::vector<int>> uniqD(M), blockOf(M);
+    std::vector<size_t> kbase(M + 1, 0);
     for (;;) {
         std::vector<size_t> base(M + 1, 0);
         for (int i = 0; i < M; i++) base[i + 1] = base[i] + st[i].work.size();
         const size_t n = base[M];
         if (n == 0) break;
         auto t0 = clk::now();
-        in.resize(n * (8 * K2 + 2));
-        double *fac = in.data() + n * 8 * K2, *thr = fac + n;
+        // runs of kernel blocks per term
 #pragma omp parallel for schedule(dynamic, 1)
+        for (int i = 0; i < M; i++) {
+            const NodeStore<2> &ot = *terms[i]->tree;
+            std::vector<int> &U = uniqD[i], &B = blockOf[i];
+            U.clear();
+            for (int u : st[i].work) U.push_back(2 * (ot.transl(u)[1] - ot.transl(u)[0]));
+            std::sort(U.begin(), U.end());
+            U.erase(std::unique(U.begin(), U.end()), U.end());
+            B.resize(U.size());
+            int total = 0;
+            for (size_t q = 0; q < U.size(); q++) {
+                B[q] = (q > 0 && U[q] - U[q - 1] <= 4) ? B[q - 1] + (U[q] - U[q - 1]) : total;
+                total = B[q] + 4;
+            }
+            kbase[i + 1] = (size_t)total;
+        }
+        for (int i = 0; i < M; i++) kbase[i + 1] += kbase[i];
+        const size_t nblk = kbase[M];
+        if (nblk > (size_t)INT32_MAX) throw Error("operator level too large");
+        const size_t nin = nblk * K2 + 2 * n + (n + 1) / 2;
+        pin_in.reserve(nin);
+        double *blocks = pin_in.data(), *fac = blocks + nblk * K2, *thr = fac + n;
+        int *slot = reinterpret_cast<int *>(thr + n);
+        int mixed = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : mixed)
         for (int i = 0; i < M; i++) {
             NodeStore<2> &ot = *terms[i]->tree;
             NodeStore<1> &kt = *kts[i];
+            const std::vector<int> &U = uniqD[i], &B = blockOf[i];
+            if (st[i].work.empty()) continue;
+            const int scale = ot.scale(st[i].work[0]) + 1;
+            for (size_t q = 0; q < U.size(); q++) {
+                // blocks of this run not yet written by the previous value of D
+                const int first = (q > 0 && U[q] - U[q - 1] <= 4) ? 4 - (U[q] - U[q - 1]) : 0;
+                for (int off = first; off < 4; off++) {
+                    const int a = detail::get_or_generate(kt, scale, {U[q] - 2 + off});
+                    std::memcpy(blocks + (kbase[i] + B[q] + off) * K2, kt.coefs(a), sizeof(double) * K2);
+                }
+            }
             for (size_t j = 0; j < st[i].work.size(); j++) {
                 const int u = st[i].work[j];
                 const size_t k = base[i] + j;
-                const int scale = ot.scale(u) + 1;
-                const auto &l = ot.transl(u);
-                for (int c = 0; c < 4; c++) {
-                    const int l0 = 2 * l[0] + (c & 1), l1 = 2 * l[1] + ((c >> 1) & 1);
-                    const int a = detail::get_or_generate(kt, scale, {l1 - l0 - 1});
-                    const int b = detail::get_or_generate(kt, scale, {l1 - l0});
-                    std::memcpy(in.data() + (k * 8 + 2 * c) * K2, kt.coefs(a), sizeof(double) * K2);
-                    std::memcpy(in.data() + (k * 8 + 2 * c + 1) * K2, kt.coefs(b), sizeof(double) * K2);
-                }
+                mixed |= ot.scale(u) + 1 != scale;
+                const int D = 2 * (ot.transl(u)[1] - ot.transl(u)[0]);
+                const size_t q = std::lower_bound(U.begin(), U.end(), D) - U.begin();
+                slot[k] = (int)(kbase[i] + B[q]);
                 const double two_n = std::pow(2.0, -scale / 2.0);
                 fac[k] = std::sqrt(sf) * two_n;
                 thr[k] = std::max(MachinePrec, ot.normPrec() / (8.0 * (1 << ot.depth(u))));
             }
         }
+        // a term's work nodes are the children of one level, so one scale
+        if (mixed) throw Error("operator tree level mixes scales");
         times[0] += since(t0);
         t0 = clk::now();
-        din.reserve(in.size());
-        din.upload(in.data(), in.size());
+        din.reserve(nin);
+        din.upload(pin_in.data(), nin);
         dout.reserve(n * (4 * kd + 5));
-        device::check(mwg_operator_nodes(K, cc.left.get(), cc.right.get(), filt, din.get(), din.get() + n * 8 * K2, din.get() + n * (8 * K2 + 1),
+        const double *dblocks = din.get(), *dfac = dblocks + nblk * K2, *dthr = dfac + n;
+        device::check(mwg_operator_nodes(K, cc.left.get(), cc.right.get(), filt, dblocks, reinterpret_cast<const int *>(dthr + n), dfac, dthr,
                                          dout.get(), dout.get() + n * 4 * kd, (int)n),
                       "operator nodes");
-        outv.resize(n * (4 * kd + 5));
-        dout.download(outv.data(), outv.size());
-        const double *coef = outv.data(), *nrm = outv.data() + n * 4 * kd;
+        pin_out.reserve(n * (4 * kd + 5));
+        dout.download(pin_out.data(), n * (4 * kd + 5));
+        const double *coef = pin_out.data(), *nrm = pin_out.data() + n * 4 * kd;
         times[1] += since(t0);
         t0 = clk::now();
 #pragma omp parallel for schedule(dynamic, 1)

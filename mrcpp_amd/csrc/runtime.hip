@@ -235,6 +235,21 @@ mwg_status mwg_release_pool(void) {
     return MWG_OK;
 }
 
+mwg_status mwg_host_alloc(void **ptr, size_t bytes) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!ptr) return fail(MWG_ERR_ARG, "null pointer");
+    MWG_HIP(hipHostMalloc(ptr, bytes ? bytes : 1, hipHostMallocDefault));
+    return MWG_OK;
+}
+
+mwg_status mwg_host_free(void *ptr) {
+    if (!ptr || g_exiting) return MWG_OK;
+    MWG_HIP(hipDeviceSynchronize());
+    g_stage_used = 0;
+    MWG_HIP(hipHostFree(ptr));
+    return MWG_OK;
+}
+
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes) {
     if (!bytes) return MWG_OK;
     if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
