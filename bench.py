@@ -131,6 +131,32 @@ def time_operator_build(mr, w, mra):
             "what": "operator construction (kernel expansion, 1D kernel trees, 2D operator trees, transforms, caches)"}
 
 
+def time_tree_io(g):
+    """Writing and reading the apply result (its coefficients in HBM) as a
+    binary tree file and as MADNESS-convention text."""
+    import shutil
+    import tempfile
+    d = tempfile.mkdtemp()
+    try:
+        base, txt = os.path.join(d, "g"), os.path.join(d, "g.txt")
+        out = {}
+        for name, arg, path in (("binary", base, base + ".tree"), ("text", txt, txt)):
+            t0 = time.perf_counter()
+            (g.saveTree if name == "binary" else g.saveTreeTXT)(arg)
+            t1 = time.perf_counter()
+            h = type(g)(g.mra)
+            (h.loadTree if name == "binary" else h.loadTreeTXT)(arg)
+            t2 = time.perf_counter()
+            size = os.path.getsize(path)
+            out[name] = {"save_s": round(t1 - t0, 4), "load_s": round(t2 - t1, 4), "bytes": size,
+                         "save_MB_s": round(size / (t1 - t0) / 1e6, 1), "load_MB_s": round(size / (t2 - t1) / 1e6, 1)}
+        out["nodes"] = g.getNNodes()
+        out["what"] = "saveTree/loadTree and saveTreeTXT/loadTreeTXT of the output tree (host cores; the save includes the copy from HBM)"
+        return out
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def time_batch_operator_build(mr, mra, specs):
     """Construction of the batch's distinct operators: all together with their
     2D trees formed on the GPU (second of two builds) against one by one on
@@ -465,6 +491,7 @@ def main():
             "step_times_s": [round(t, 6) for t in variant["times"]],
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
     res["projection"] = time_projection(mr, w, src, mra)
+    res["tree_io"] = time_tree_io(plain["out"])
     res["operator_build"] = time_operator_build(mr, w, mra)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)

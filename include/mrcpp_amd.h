@@ -42,6 +42,17 @@ double mrcpp_tree_square_norm(void *tree);
 /* real nodes depth first: idx[5 i] = scale, lx, ly, lz, is_end;
  * coefs[8 (k+1)^3 i ...] (may be NULL); norms[9 i] = 8 component norms, square norm */
 int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms);
+/* FunctionTree::saveTree / loadTree (src/trees/FunctionTree.cpp:375-435): binary
+ * "<file>.tree" of every real node in this library's layout (MRCPP's file is its
+ * allocator's memory image); loading needs the MRA the tree was saved with. */
+int mrcpp_tree_save(void *tree, const char *file);
+int mrcpp_tree_load(void *tree, const char *file);
+/* FunctionTree::saveTreeTXT / loadTreeTXT (src/trees/FunctionTree.cpp:127-373):
+ * MADNESS-convention text (values of every end node's children at their
+ * quadrature points, 14 significant digits); loading needs the world cube
+ * [-L, L], L = 2^-rootScale, and the file's order. */
+int mrcpp_tree_save_txt(void *tree, const char *file);
+int mrcpp_tree_load_txt(void *tree, const char *file);
 /* FunctionTree::evalf, integrate (src/trees/FunctionTree.cpp:438-546); dot (src/treebuilders/multiply.cpp:286-318) */
 double mrcpp_tree_evalf(void *tree, double x, double y, double z);
 double mrcpp_tree_integrate(void *tree);

@@ -47,6 +47,10 @@ SIGNATURES = {
     "mrcpp_tree_evalf": (ctypes.c_double, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "mrcpp_tree_integrate": (ctypes.c_double, [_vp]),
     "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
+    "mrcpp_tree_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "mrcpp_tree_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "mrcpp_tree_save_txt": (ctypes.c_int, [_vp, ctypes.c_char_p]),
+    "mrcpp_tree_load_txt": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "mrcpp_operator_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.POINTER(_vp)]),
     "mrcpp_operator_destroy": (None, [_vp]),
     "mrcpp_operators_create": (ctypes.c_int, [_vp, ctypes.c_int, _dp, _dp, ctypes.c_int, ctypes.POINTER(_vp)]),
@@ -215,6 +219,22 @@ class FunctionTree:
 
     def integrate(self):
         return float(lib().mrcpp_tree_integrate(self._h))
+
+    def saveTree(self, file):
+        """Binary file "<file>.tree" (FunctionTree::saveTree)."""
+        _check(lib().mrcpp_tree_save(self._h, os.fsencode(file)))
+
+    def loadTree(self, file):
+        """Read "<file>.tree"; the tree must have the MRA it was saved with."""
+        _check(lib().mrcpp_tree_load(self._h, os.fsencode(file)))
+
+    def saveTreeTXT(self, file):
+        """MADNESS-convention text file (FunctionTree::saveTreeTXT)."""
+        _check(lib().mrcpp_tree_save_txt(self._h, os.fsencode(file)))
+
+    def loadTreeTXT(self, file):
+        """Read a MADNESS-convention text file (FunctionTree::loadTreeTXT)."""
+        _check(lib().mrcpp_tree_load_txt(self._h, os.fsencode(file)))
 
     def export(self, with_coefs=True):
         """(idx[N,5] = scale, lx, ly, lz, is_end; coefs[N, 8 (k+1)^3] or None; norms[N, 9])."""

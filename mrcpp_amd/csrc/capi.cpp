@@ -96,6 +96,22 @@ int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms) {
     return rc < 0 ? -1 : n;
 }
 
+int mrcpp_tree_save(void *tree, const char *file) {
+    return guarded([&] { static_cast<FunctionTree<3> *>(tree)->saveTree(file); });
+}
+
+int mrcpp_tree_load(void *tree, const char *file) {
+    return guarded([&] { static_cast<FunctionTree<3> *>(tree)->loadTree(file); });
+}
+
+int mrcpp_tree_save_txt(void *tree, const char *file) {
+    return guarded([&] { static_cast<FunctionTree<3> *>(tree)->saveTreeTXT(file); });
+}
+
+int mrcpp_tree_load_txt(void *tree, const char *file) {
+    return guarded([&] { static_cast<FunctionTree<3> *>(tree)->loadTreeTXT(file); });
+}
+
 double mrcpp_tree_evalf(void *tree, double x, double y, double z) { return static_cast<FunctionTree<3> *>(tree)->evalf({x, y, z}); }
 
 double mrcpp_tree_integrate(void *tree) { return static_cast<FunctionTree<3> *>(tree)->integrate(); }

@@ -28,6 +28,17 @@ public:
     double integrate();
     double evalf(const Coord<3> &r);
 
+    // FunctionTree::saveTree / loadTree (src/trees/FunctionTree.cpp:375-435):
+    // binary file "<file>.tree" of every real node (this library's layout; the
+    // tree loading it must have the MRA it was saved with).
+    void saveTree(const std::string &file);
+    void loadTree(const std::string &file);
+    // FunctionTree::saveTreeTXT / loadTreeTXT (src/trees/FunctionTree.cpp:127-373):
+    // MADNESS-convention text, the values of every end node's children at
+    // their quadrature points.  Loading needs a world cube [-L, L], L = 2^-rootScale.
+    void saveTreeTXT(const std::string &file);
+    void loadTreeTXT(const std::string &file);
+
     // host node store with coefficients brought up to date
     detail::NodeStore<3> &nodes();
     // host node store without touching coefficients (topology and norms)

@@ -471,12 +471,12 @@ template <int D> void compute_norms(NodeStore<D> &s, int u) {
 // Node-local multiwavelet transform between the 2^D child scaling blocks held in
 // the node's own coefficients and the s/d blocks (MWNode::mwTransform,
 // src/trees/MWNode.cpp:557-594).
-template <int D> void local_transform(NodeStore<D> &s, int u, int dir) {
+template <int D> void local_transform(const NodeStore<D> &s, double *coefs, int dir) {
     constexpr int TD = NodeStore<D>::TD;
     const int kd = s.kp1_d(), m = kd / s.kp1();
     const auto &fb = s.mra().filter();
     std::vector<double> buf((size_t)TD * kd);
-    double *src = s.coefs(u), *dst = buf.data();
+    double *src = coefs, *dst = buf.data();
     for (int dim = 0; dim < D; dim++) {
         const int bit = 1 << dim;
         for (int gt = 0; gt < TD; gt++) {
@@ -490,8 +490,10 @@ template <int D> void local_transform(NodeStore<D> &s, int u, int dir) {
         }
         std::swap(src, dst);
     }
-    if (D % 2 == 1) std::memcpy(s.coefs(u), src, sizeof(double) * TD * kd);
+    if (D % 2 == 1) std::memcpy(coefs, src, sizeof(double) * TD * kd);
 }
+
+template <int D> void local_transform(NodeStore<D> &s, int u, int dir) { local_transform(s, s.coefs(u), dir); }
 
 // Parent s/d (scaling only for a generated parent) -> scaling blocks of the
 // 2^D children, overwriting or accumulating (tree_utils::mw_transform,

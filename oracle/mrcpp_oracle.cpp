@@ -27,6 +27,7 @@
 #include <fstream>
 #include <functional>
 #include <map>
+#include <iomanip>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -1638,6 +1639,220 @@ static double evalf(Tree<3> &t, const std::array<double, 3> &r) {
     return coef * two_n * res;
 }
 
+// ----------------------------------------------------------------------------
+// Text files in MADNESS conventions (FunctionTree::saveTreeTXT / loadTreeTXT,
+// src/trees/FunctionTree.cpp:127-373)
+// ----------------------------------------------------------------------------
+
+// LegendreBasis::calcCVMaps (src/core/LegendreBasis.cpp:98-109): cvMap is the
+// inverse of vcMap, here by Gauss-Jordan elimination with partial pivoting.
+static std::vector<double> cv_map(const Basis &b) {
+    const int n = b.order + 1;
+    std::vector<double> a(b.vcMap), inv(n * n, 0.0);
+    for (int i = 0; i < n; i++) inv[i + n * i] = 1.0;
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::abs(a[r + n * c]) > std::abs(a[piv + n * c])) piv = r;
+        if (piv != c)
+            for (int j = 0; j < n; j++) {
+                std::swap(a[c + n * j], a[piv + n * j]);
+                std::swap(inv[c + n * j], inv[piv + n * j]);
+            }
+        const double d = a[c + n * c];
+        for (int j = 0; j < n; j++) {
+            a[c + n * j] /= d;
+            inv[c + n * j] /= d;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = a[r + n * c];
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; j++) {
+                a[r + n * j] -= f * a[c + n * j];
+                inv[r + n * j] -= f * inv[c + n * j];
+            }
+        }
+    }
+    return inv;
+}
+
+// MWNode::cvTransform (src/trees/MWNode.cpp:448-490) on a node's 8 blocks
+static void cv_transform(const MRA<3> &m, int n, const std::vector<double> &S, bool forward, double *coefs) {
+    const int kp1 = m.kp1, kp1_d = m.kp1_d, kp1_dm1 = kp1 * kp1;
+    std::vector<double> o_vec(8 * kp1_d);
+    double *out_vec = o_vec.data(), *in_vec = coefs;
+    for (int i = 0; i < 3; i++) {
+        for (int t = 0; t < 8; t++) apply_filter(out_vec + t * kp1_d, in_vec + t * kp1_d, S.data(), kp1, kp1_dm1, false);
+        std::swap(in_vec, out_vec);
+    }
+    double sf_prod = m.sf[0] * m.sf[1] * m.sf[2];
+    if (sf_prod <= MachineZero) sf_prod = 1.0;
+    double two_fac = std::pow(2.0, 3 * (n + 1)) / sf_prod;
+    two_fac = forward ? std::sqrt(two_fac) : std::sqrt(1.0 / two_fac);
+    for (int i = 0; i < 8 * kp1_d; i++) coefs[i] = two_fac * in_vec[i];
+}
+
+// MADNESS index order (FunctionTree.cpp:168-176)
+static std::vector<int> madness_map(int kp1) {
+    std::vector<int> map;
+    for (int x = kp1 - 1; x >= 0; x--)
+        for (int y = kp1 - 1; y >= 0; y--)
+            for (int z = kp1 - 1; z >= 0; z--) map.push_back(z * kp1 * kp1 + y * kp1 + x);
+    return map;
+}
+
+// FunctionTree::saveTreeTXT (src/trees/FunctionTree.cpp:310-373)
+static void save_tree_txt(Tree<3> &t, const std::string &fname) {
+    const MRA<3> &m = *t.mra;
+    std::ofstream out(fname);
+    if (!out) throw Error("Unable to open file");
+    out << std::setprecision(14);
+    out << 3 << std::endl;
+    const int rscale = m.n0;
+    double LMRChem = 1.0;
+    for (int i = 0; i > rscale; i--) LMRChem *= 2;
+    for (int d = 0; d < 3; d++) out << -m.sf[d] * LMRChem << " " << m.sf[d] * LMRChem << std::endl;
+    const int kp1 = m.kp1, ncoefs = m.kp1_d;
+    out << kp1 << std::endl;
+    auto ends = end_nodes(t);
+    out << 8 * ends.size() << std::endl;
+    const std::vector<int> mapMRC = madness_map(kp1);
+    const std::vector<double> cv = cv_map(*m.basis);
+    const int L = (int)std::pow(2, -rscale);
+    std::vector<double> values(8 * ncoefs);
+    Node<3> tmp;
+    tmp.tree = &t;
+    for (Node<3> *nd : ends) {
+        tmp.n = nd->n;
+        tmp.coefs = nd->coefs;
+        node_mw_transform(tmp, Filter::Reconstruction);
+        cv_transform(m, nd->n, cv, true, tmp.coefs.data());
+        std::array<int, 3> l;
+        for (int i = 0; i < 3; i++) l[i] = 2 * (nd->l[i] + std::pow(2, nd->n) * L);
+        for (int cix = 0; cix < 8; cix++) {
+            out << nd->n - rscale + 2 << " ";
+            for (int i = 0; i < 3; i++) out << l[i] + ((cix >> i) & 1) << " ";
+            out << std::endl;
+            for (int i = 0; i < ncoefs; i++) out << tmp.coefs[cix * ncoefs + mapMRC[i]] << " ";
+            out << std::endl;
+        }
+    }
+}
+
+// MWTree::getNode(idx, true) (src/trees/MWTree.cpp:340-352): real children created on the way
+static Node<3> &create_node(Tree<3> &t, int n, const std::array<int, 3> &l) {
+    int r = t.mra->boxIndex(n, l);
+    if (r < 0) throw Error("node outside world box");
+    Node<3> *nd = t.roots[r].get();
+    while (nd->n < n) {
+        if (not nd->isBranch()) create_children(*nd);
+        nd = &nd->child(child_index(*nd, n, l));
+    }
+    return *nd;
+}
+
+// FunctionTree::loadTreeTXT (src/trees/FunctionTree.cpp:127-308)
+static void load_tree_txt(Tree<3> &t, const std::string &fname) {
+    const MRA<3> &m = *t.mra;
+    std::ifstream in(fname);
+    if (!in) throw Error("Unable to open file");
+    int NDIM, k;
+    in >> NDIM;
+    if (NDIM != 3) throw Error("dimension");
+    double coord[3][2];
+    for (int d = 0; d < 3; d++) in >> coord[d][0] >> coord[d][1];
+    int p = 1;
+    const int rscale = m.n0;
+    for (int i = rscale; i < 0; i++) p *= 2;
+    const int L = p;
+    for (int d = 0; d < 3; d++)
+        if (std::abs(coord[d][0] + L) > 1.0e-14 or std::abs(coord[d][1] - L) > 1.0e-14) throw Error("world box");
+    in >> k;
+    if (k != m.kp1) throw Error("order");
+    const int kp1 = k, ncoefs = m.kp1_d;
+    const std::vector<int> mapMRC = madness_map(kp1);
+    // fresh roots
+    for (auto &r : t.roots) {
+        Node<3> *kids = r->kids.load();
+        r->kids.store(nullptr);
+        delete[] kids;
+        r->zeroCoefs();
+    }
+    t.nodesAtDepth.assign(1, (int)t.roots.size());
+    std::vector<std::vector<Node<3> *>> NodeTable(50);
+    std::map<Node<3> *, int> mp;
+    int nmax = 0;
+    int nread;
+    in >> nread;
+    while (nread-- > 0) {
+        int n_in;
+        in >> n_in;
+        const int n = n_in + rscale - 1;
+        std::array<int, 3> l_in, l, lp;
+        for (int i = 0; i < 3; i++) in >> l_in[i];
+        for (int i = 0; i < 3; i++) {
+            l[i] = l_in[i] - std::pow(2, n) * L;
+            lp[i] = l_in[i] / 2 - std::pow(2, n - 1) * L;
+        }
+        Node<3> *node = &create_node(t, n - 1, lp);
+        int c_ix = 0, q = 1;
+        for (int i = 0; i < 3; i++) {
+            if (std::abs(l[i]) % 2 == 1) c_ix += q;
+            q *= 2;
+        }
+        if (mp[node] == 0) {
+            node->zeroCoefs();
+            if (node->parent != nullptr)
+                for (int c = 0; c < 8; c++)
+                    if (mp[&node->parent->child(c)] == 0) node->parent->child(c).zeroCoefs();
+        }
+        double *values = node->coefs.data() + c_ix * ncoefs;
+        for (int i = 0; i < ncoefs; i++) in >> values[mapMRC[i]];
+        mp[node]++;
+        nmax = std::max(nmax, n_in);
+        if (mp[node] == 1) NodeTable[n_in].push_back(node);
+    }
+    for (int n = nmax; n > -1; n--)
+        for (Node<3> *node : NodeTable[n]) {
+            cv_transform(m, node->n, m.basis->vcMap, false, node->coefs.data());
+            calc_norms(*node);
+        }
+    for (int n = nmax; n > -1; n--) {
+        for (size_t i = 0; i < NodeTable[n].size(); i++) {
+            Node<3> *node = NodeTable[n][i];
+            if (mp[node] != 8) continue;
+            if (not node->isBranch()) {
+                node_mw_transform(*node, Filter::Compression);
+                node->has_coefs = true;
+                calc_norms(*node);
+            } else {
+                for (int c = 0; c < 8; c++) {
+                    Node<3> &cn = node->child(c);
+                    if (mp[&cn] == 8) continue;
+                    for (int j = 0; j < ncoefs; j++) cn.coefs[j] = node->coefs[j + c * ncoefs];
+                    for (int j = ncoefs; j < 8 * ncoefs; j++) cn.coefs[j] = 0.0;
+                    cn.has_coefs = true;
+                    calc_norms(cn);
+                }
+                node_mw_transform(*node, Filter::Compression);
+                node->has_coefs = true;
+                calc_norms(*node);
+            }
+            if (node->parent != nullptr) {
+                Node<3> *parent = node->parent;
+                if (mp[parent] == 0) NodeTable[n - 1].push_back(parent);
+                int my_ix = -1;
+                for (int c = 0; c < 8; c++)
+                    if (&parent->child(c) == node) my_ix = c;
+                for (int j = 0; j < ncoefs; j++) parent->coefs[j + my_ix * ncoefs] = node->coefs[j];
+                mp[parent]++;
+            }
+        }
+    }
+    calc_square_norm(t);
+}
+
 } // namespace orc
 
 // ============================================================================
@@ -1747,6 +1962,26 @@ int orc_tree_export(void *tp, int *idx, double *coefs, double *norms) {
         if (norms) norms[9 * i + 8] = nd->sq;
     }
     return (int)tab.size();
+}
+
+int orc_tree_save_txt(void *tp, const char *path) {
+    try {
+        save_tree_txt(*static_cast<OrcTree *>(tp)->t, path);
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+int orc_tree_load_txt(void *tp, const char *path) {
+    try {
+        load_tree_txt(*static_cast<OrcTree *>(tp)->t, path);
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
 }
 
 double orc_tree_evalf(void *tp, double x, double y, double z) { return evalf(*static_cast<OrcTree *>(tp)->t, {x, y, z}); }

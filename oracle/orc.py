@@ -52,6 +52,8 @@ class Oracle:
             L.orc_tree_square_norm.argtypes = [ctypes.c_void_p]
             L.orc_tree_export.argtypes = [ctypes.c_void_p, _ip, _dp, _dp]
             L.orc_tree_evalf.restype = ctypes.c_double
+            L.orc_tree_save_txt.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+            L.orc_tree_load_txt.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
             L.orc_tree_evalf.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
             L.orc_tree_dot.restype = ctypes.c_double
             L.orc_tree_dot.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
@@ -143,6 +145,16 @@ class OracleTree:
         centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
         rc = Oracle.lib().orc_tree_build_grid_gaussians(self.h, len(coefs), _ptr(coefs), _ptr(alphas), _ptr(centers))
         if rc != 0:
+            raise RuntimeError(Oracle.error())
+
+    def save_txt(self, path):
+        """FunctionTree::saveTreeTXT restated (MADNESS-convention text)."""
+        if Oracle.lib().orc_tree_save_txt(self.h, os.fsencode(path)) != 0:
+            raise RuntimeError(Oracle.error())
+
+    def load_txt(self, path):
+        """FunctionTree::loadTreeTXT restated."""
+        if Oracle.lib().orc_tree_load_txt(self.h, os.fsencode(path)) != 0:
             raise RuntimeError(Oracle.error())
 
     def num_nodes(self):

@@ -1,0 +1,37 @@
+"""Tree files of an apply result whose coefficients live in HBM
+(FunctionTree::saveTree / saveTreeTXT, src/trees/FunctionTree.cpp:310-435):
+the writers bring the coefficients to the host first, and the files read
+back to the same tree (binary: bit for bit; text: 14 significant digits)."""
+import numpy as np
+import pytest
+
+from mrcpp_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_files_of_a_device_resident_apply_result(mr, tmp_path):
+    w = W.WORKLOADS["gauss1_k5"]
+    mra = mr.MultiResolutionAnalysis((-w["box"], w["box"]), w["order"])
+    gx = mr.GaussExp.from_sources(w["sources"]())
+    f = mr.FunctionTree(mra)
+    mr.build_grid(f, gx)
+    mr.project(w["prec"], f, gx)
+    g = mr.FunctionTree(mra)
+    mr.apply(w["prec"], g, mr.PoissonOperator(mra, w["prec"]), f)
+    base, txt = str(tmp_path / "g"), str(tmp_path / "g.txt")
+    g.saveTree(base)
+    g.saveTreeTXT(txt)
+    ref = g.export()
+    b = mr.FunctionTree(mra)
+    b.loadTree(base)
+    assert all(np.array_equal(x, y) for x, y in zip(ref, b.export()))
+    t = mr.FunctionTree(mra)
+    t.loadTreeTXT(txt)
+    ti, tc, _ = t.export()
+    assert np.array_equal(ti, ref[0])
+    assert np.linalg.norm(tc - ref[1]) / np.linalg.norm(ref[1]) <= 1e-11
+    # a loaded tree is a valid apply input
+    h = mr.FunctionTree(mra)
+    mr.apply(w["prec"], h, mr.PoissonOperator(mra, w["prec"]), b)
+    assert h.getNNodes() > 0
