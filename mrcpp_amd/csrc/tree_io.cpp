@@ -7,8 +7,8 @@
 // objects with their pointers, repaired by NodeAllocator::reassemble on load),
 // so it is readable only by the same MRCPP build.  The binary file here holds
 // the same information in a layout of its own: a header with the MRA, then
-// every real node depth first (scale, translation, child flag, the 2^3 blocks
-// of (k+1)^3 coefficients).
+// every real node depth first (scale, translation, child flag, the 8
+// component norms and the square norm, the 2^3 blocks of (k+1)^3 coefficients).
 #include <charconv>
 #include <cctype>
 #include <cmath>
@@ -364,6 +364,8 @@ void FunctionTree<3>::saveTree(const std::string &file) {
         put<int32_t>(out, s.scale(u));
         for (int d = 0; d < 3; d++) put<int32_t>(out, s.transl(u)[d]);
         put<int32_t>(out, s.hasRealChildren(u) ? 1 : 0);
+        out.write(reinterpret_cast<const char *>(s.norms(u)), sizeof(double) * TD);
+        put<double>(out, s.squareNorm(u));
         out.write(reinterpret_cast<const char *>(s.coefs(u)), sizeof(double) * nc);
     }
     if (!out) throw Error("write failed: " + fname);
@@ -398,10 +400,11 @@ void FunctionTree<3>::loadTree(const std::string &file) {
         get<int32_t>(in); // child flag: the topology follows from the nodes present
         if (n < mra_.getRootScale() || n > mra_.getMaxScale()) throw Error("loadTree: scale out of range");
         const int u = get_or_create(s, n, l);
+        in.read(reinterpret_cast<char *>(s.norms(u)), sizeof(double) * TD);
+        s.setSquareNorm(u, get<double>(in));
         in.read(reinterpret_cast<char *>(s.coefs(u)), sizeof(double) * nc);
         if (!in) throw Error("loadTree: truncated file");
         s.setHasCoefs(u, true);
-        detail::compute_norms(s, u);
     }
     detail::calc_square_norm(s);
 }
