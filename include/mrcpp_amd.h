@@ -42,6 +42,14 @@ double mrcpp_tree_square_norm(void *tree);
 /* real nodes depth first: idx[5 i] = scale, lx, ly, lz, is_end;
  * coefs[8 (k+1)^3 i ...] (may be NULL); norms[9 i] = 8 component norms, square norm */
 int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms);
+/* add(prec, out, FunctionTreeVector, maxIter, absPrec) (src/treebuilders/add.cpp:43-80):
+ * out = sum_i c[i] trees[i], refined from out's end nodes by the wavelet test;
+ * computed on the GPU, 1 <= n <= 8, out distinct from the inputs. */
+int mrcpp_add(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec);
+/* multiply(prec, out, FunctionTreeVector, maxIter, absPrec) (src/treebuilders/multiply.cpp:52-100):
+ * out = prod_i c[i] trees[i] (WaveletAdaptor refinement; the useMaxNorms
+ * variant is not provided), on the GPU, 1 <= n <= 8. */
+int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec);
 /* FunctionTree::saveTree / loadTree (src/trees/FunctionTree.cpp:375-435): binary
  * "<file>.tree" of every real node in this library's layout (MRCPP's file is its
  * allocator's memory image); loading needs the MRA the tree was saved with. */

@@ -70,7 +70,8 @@ enum {
     MWG_KERNEL_NORMS = 6,
     MWG_KERNEL_GROUPS = 7, /* exact group screen and work-unit construction */
     MWG_KERNEL_PROJECT = 8, /* projection of analytic functions onto tree nodes */
-    MWG_KERNEL_KINDS = 9
+    MWG_KERNEL_ARITH = 9,   /* sums and products of trees */
+    MWG_KERNEL_KINDS = 10
 };
 mwg_status mwg_profile(int enable);
 mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds);
@@ -279,6 +280,32 @@ mwg_status mwg_project_nodes(const mwg_projection *pj, const int *scale, const i
  * summation order). */
 mwg_status mwg_operator_nodes(int kp1, const double *left, const double *right, const double *filters, const double *kernel_blocks,
                               const int *slot, const double *fac, const double *thr, double *coef, double *norms, int count);
+
+/* Tree arithmetic over output nodes of any scales (AdditionCalculator /
+ * MultiplicationCalculator::calcNode, src/treebuilders/AdditionCalculator.h:45-70,
+ * src/treebuilders/MultiplicationCalculator.h:45-83).  Mode 0: the node's
+ * coefficients are sum_i c[i] f_i; mode 1: every operand node is reconstructed
+ * to its children and mapped to values at their quadrature points, the values
+ * multiplied (each times c[i]), mapped back and compressed.  in_off[k nin + i]
+ * >= 0: operand i's node k (8 blocks) at that offset in pool[i]; < 0: a
+ * generated node, its scaling block at -in_off - 1 (wavelet blocks zero).
+ * scale[k] is the node's scale, out_off[k] its offset in out; norms[9 k] the 8
+ * component norms and the square norm.  No fused multiply-add: the host's
+ * order of operations. */
+#define MWG_MAX_OPERANDS 8
+typedef struct {
+    int kp1;
+    int mode; /* 0 sum, 1 product */
+    int nin;  /* operands, 1 .. MWG_MAX_OPERANDS */
+    double sf[3];
+    const double *pool[MWG_MAX_OPERANDS];
+    double c[MWG_MAX_OPERANDS];
+    const double *filters; /* mwg_filters_upload */
+    const double *vcmap;   /* [kp1 * kp1] value -> coefficient map, column-major */
+    const double *cvmap;   /* [kp1 * kp1] coefficient -> value map, column-major */
+} mwg_combine;
+mwg_status mwg_combine_nodes(const mwg_combine *cb, const long long *in_off, const int *scale, const long long *out_off, double *out,
+                             double *norms, int count);
 
 /* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;

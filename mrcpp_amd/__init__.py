@@ -47,6 +47,8 @@ SIGNATURES = {
     "mrcpp_tree_evalf": (ctypes.c_double, [_vp, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "mrcpp_tree_integrate": (ctypes.c_double, [_vp]),
     "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
+    "mrcpp_add": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
+    "mrcpp_multiply": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
     "mrcpp_tree_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "mrcpp_tree_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "mrcpp_tree_save_txt": (ctypes.c_int, [_vp, ctypes.c_char_p]),
@@ -103,7 +105,7 @@ SIGNATURES = {
     "mwg_mlp_adam": (ctypes.c_int, [_vp, _vp, _vp, _vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
 }
 
-KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups", "project")
+KERNEL_KINDS = ("screen", "generate", "band", "reduce", "reconstruct", "compress", "norms", "groups", "project", "arith")
 
 
 class MRCPPError(RuntimeError):
@@ -375,6 +377,48 @@ def set_operator_target(target):
 def last_projection_on_device():
     """Whether the last project() ran on the GPU."""
     return bool(lib().mrcpp_last_projection_on_device())
+
+
+def _terms(args, name):
+    """(c_i, tree_i) terms and the trailing (maxIter, absPrec) of add/multiply's
+    two forms: (a, tree_a, b, tree_b, ...) or ([(c, tree), ...], ...)."""
+    if args and isinstance(args[0], (list, tuple)):
+        return [(float(c), t) for c, t in args[0]], args[1:]
+    if len(args) < 4:
+        raise TypeError("%s(prec, out, a, tree_a, b, tree_b[, maxIter, absPrec]) or %s(prec, out, [(c, tree), ...])" % (name, name))
+    return [(float(args[0]), args[1]), (float(args[2]), args[3])], args[4:]
+
+
+def _arith(fn, prec, out, terms, max_iter, abs_prec):
+    c = np.ascontiguousarray([t[0] for t in terms], dtype=np.float64)
+    hs = (_vp * max(len(terms), 1))(*[t[1]._h for t in terms])
+    _check(fn(float(prec), out._h, len(terms), _p(c), hs, int(max_iter), int(bool(abs_prec))))
+
+
+def add(prec, out, *args, maxIter=-1, absPrec=False):
+    """add(prec, out, a, tree_a, b, tree_b[, maxIter, absPrec]) or
+    add(prec, out, [(c, tree), ...][, maxIter, absPrec]) on the GPU."""
+    terms, rest = _terms(args, "add")
+    if rest:
+        maxIter = rest[0]
+    if len(rest) > 1:
+        absPrec = rest[1]
+    _arith(lib().mrcpp_add, prec, out, terms, maxIter, absPrec)
+
+
+def multiply(prec, out, *args, maxIter=-1, absPrec=False):
+    """multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec]) or
+    multiply(prec, out, [(c, tree), ...][, maxIter, absPrec]) on the GPU."""
+    if args and not isinstance(args[0], (list, tuple)):
+        if len(args) < 3:
+            raise TypeError("multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec])")
+        args = ([(args[0], args[1]), (1.0, args[2])],) + tuple(args[3:])
+    terms, rest = _terms(args, "multiply")
+    if rest:
+        maxIter = rest[0]
+    if len(rest) > 1:
+        absPrec = rest[1]
+    _arith(lib().mrcpp_multiply, prec, out, terms, maxIter, absPrec)
 
 
 def dot(bra, ket):

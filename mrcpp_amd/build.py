@@ -18,8 +18,8 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MRCPP_AMD_ARCH", "gfx950")
 
-HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "tree_io.cpp", "apply.cpp", "predictor.cpp", "capi.cpp"]
-HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip", "transform.hip", "project.hip", "opbuild.hip", "mlp.hip"]
+HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "tree_io.cpp", "arith.cpp", "apply.cpp", "predictor.cpp", "capi.cpp"]
+HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip", "transform.hip", "project.hip", "opbuild.hip", "arith.hip", "mlp.hip"]
 
 # IEEE evaluation order on the host (no FMA contraction) keeps host results
 # reproducible across compilers and identical to the CPU checker.
@@ -31,7 +31,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "
 # copying them into AGPRs and back around every group.
 HIP_FILE_FLAGS = {"band.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
                   # operator nodes are formed bit-identically to the host construction
-                  "opbuild.hip": ["-ffp-contract=off"]}
+                  "opbuild.hip": ["-ffp-contract=off"],
+                  "arith.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd):

@@ -241,41 +241,10 @@ void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const 
         pf[p] = prev;
     }
     if (made.empty()) return;
-    // device storage for the new nodes
-    const int total = fm.nReal + fm.nGen;
-    const int firstNew = total - (int)made.size();
-    fm.hostOff.resize(total);
-    for (const auto &m : made) fm.hostOff[m.id] = fm.genBase + (long long)(m.id - fm.nReal) * sh.n3;
-    fm.coefOff.reserve(total, firstNew);
-    fm.coefOff.upload(fm.hostOff.data() + firstNew, made.size(), firstNew);
-    fm.norms.reserve((size_t)total * 9, (size_t)firstNew * 9);
-    fm.isGen.reserve(total, firstNew);
-    std::vector<unsigned char> ones(made.size(), 1);
-    fm.isGen.upload(ones.data(), ones.size(), firstNew);
-    fm.coef.reserve((size_t)fm.genBase + (size_t)fm.nGen * sh.n3, (size_t)fm.genBase + (size_t)(fm.nGen - made.size()) * sh.n3);
-    fm.uploadTable();
-    // one launch per scale, coarse to fine
-    std::stable_sort(made.begin(), made.end(), [](const NewNode &a, const NewNode &b) { return a.scale < b.scale; });
-    // one upload: ids [n] | parents [n] | child indices [n]
-    const size_t n = made.size();
-    std::vector<int> gen(3 * n);
-    for (size_t k = 0; k < n; k++) {
-        gen[k] = made[k].id;
-        gen[n + k] = made[k].parent;
-        gen[2 * n + k] = made[k].child;
-    }
-    Scratch &S = scratch();
-    S.gen.reserve(gen.size());
-    S.gen.upload(gen.data(), gen.size());
-    const int *ids = S.gen.get(), *par = ids + n, *ch = ids + 2 * n;
-    const mwg_ftree fv = fm.view(*job.mra);
-    size_t i = 0;
-    while (i < n) {
-        size_t j = i;
-        while (j < n && made[j].scale == made[i].scale) j++;
-        device::check(mwg_generate_nodes(&fv, job.filters, ids + i, par + i, ch + i, (int)(j - i)), "generate band nodes");
-        i = j;
-    }
+    std::vector<device::GenNode> plan;
+    plan.reserve(made.size());
+    for (const auto &m : made) plan.push_back({m.id, m.parent, m.child, m.scale});
+    device::generate_mirror_nodes(fm, *job.mra, job.filters, plan, scratch().gen);
     job.stats.nGenerated += (long long)made.size();
 }
 
@@ -516,18 +485,6 @@ void transform_and_norms(Job &job, const Shape &sh) {
     }
 }
 
-// Drop the generated band nodes from the input mirror (FunctionTree::deleteGenerated).
-void drop_generated(device::TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, detail::NodeStore<3> &f) {
-    if (fm.nGen == 0) return;
-    fm.table.reset(4 * fm.nReal + 1024);
-    for (int u = 0; u < fm.nReal; u++) fm.table.insert(device::key_of(mra, f.scale(u), f.transl(u)), u);
-    fm.nGen = 0;
-    fm.genParent.clear();
-    fm.genScale.clear();
-    fm.genL.clear();
-    fm.hostOff.resize(fm.nReal);
-    fm.uploadTable();
-}
 
 // TreeBuilder::build bookkeeping after a step: approximate tree norm and
 // WaveletAdaptor splits; returns the next work vector.
@@ -848,7 +805,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
     }
     const auto tc = clk::now();
     for (auto &job : jobs)
-        if (mirrors.insert(job.fm).second) drop_generated(*job.fm, *job.mra, *job.fstore);
+        if (mirrors.insert(job.fm).second) device::drop_generated(*job.fm, *job.mra, *job.fstore);
     const double tCleanup = since(tc);
     const double seconds = since(t_start);
     for (auto &job : jobs) {

@@ -96,6 +96,23 @@ int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms) {
     return rc < 0 ? -1 : n;
 }
 
+namespace {
+FunctionTreeVector tree_vector(int n, const double *c, void **trees) {
+    if (n < 0 || (n > 0 && (!c || !trees))) throw Error("bad input tree list");
+    FunctionTreeVector v;
+    for (int i = 0; i < n; i++) v.emplace_back(c[i], static_cast<FunctionTree<3> *>(trees[i]));
+    return v;
+}
+} // namespace
+
+int mrcpp_add(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec) {
+    return guarded([&] { add(prec, *static_cast<FunctionTree<3> *>(out), tree_vector(n, c, trees), max_iter, abs_prec != 0); });
+}
+
+int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec) {
+    return guarded([&] { multiply(prec, *static_cast<FunctionTree<3> *>(out), tree_vector(n, c, trees), max_iter, abs_prec != 0); });
+}
+
 int mrcpp_tree_save(void *tree, const char *file) {
     return guarded([&] { static_cast<FunctionTree<3> *>(tree)->saveTree(file); });
 }

@@ -125,6 +125,36 @@ LegendreTables::LegendreTables(int order)
     vc_ = SqMat(order + 1);
     for (int j = 0; j <= order; j++)
         for (int i = 0; i <= order; i++) vc_(i, j) = phi(j, nodes_[i]) * weights_[i];
+    // cvMap = vcMap^-1 (LegendreBasis::calcCVMaps, src/core/LegendreBasis.cpp:98-109)
+    // by Gauss-Jordan elimination with partial pivoting
+    const int n = order + 1;
+    SqMat a = vc_;
+    cv_ = SqMat(n);
+    for (int i = 0; i < n; i++) cv_(i, i) = 1.0;
+    for (int c = 0; c < n; c++) {
+        int piv = c;
+        for (int r = c + 1; r < n; r++)
+            if (std::abs(a(r, c)) > std::abs(a(piv, c))) piv = r;
+        if (piv != c)
+            for (int j = 0; j < n; j++) {
+                std::swap(a(c, j), a(piv, j));
+                std::swap(cv_(c, j), cv_(piv, j));
+            }
+        const double d = a(c, c);
+        for (int j = 0; j < n; j++) {
+            a(c, j) /= d;
+            cv_(c, j) /= d;
+        }
+        for (int r = 0; r < n; r++) {
+            if (r == c) continue;
+            const double f = a(r, c);
+            if (f == 0.0) continue;
+            for (int j = 0; j < n; j++) {
+                a(r, j) -= f * a(c, j);
+                cv_(r, j) -= f * cv_(c, j);
+            }
+        }
+    }
 }
 
 double LegendreTables::phi(int j, double x) const {

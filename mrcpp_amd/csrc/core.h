@@ -57,12 +57,16 @@ public:
     const std::vector<double> &nodes() const { return nodes_; }
     const std::vector<double> &weights() const { return weights_; }
     const SqMat &vcMap() const { return vc_; }
+    // coefficient -> value map, the inverse of vcMap formed numerically as
+    // LegendreBasis::calcCVMaps does (src/core/LegendreBasis.cpp:98-109); in
+    // exact arithmetic cv(i, j) = phi_i(x_j)
+    const SqMat &cvMap() const { return cv_; }
 
 private:
     int order_;
     std::vector<std::vector<double>> power_; // per j: coefficients of q^m
     std::vector<double> nodes_, weights_;
-    SqMat vc_;
+    SqMat vc_, cv_;
 };
 
 // Gauss-Legendre nodes/weights on [0,1] with the Newton iteration of

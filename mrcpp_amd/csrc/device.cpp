@@ -2,6 +2,7 @@
 // the key table mirrored on the device.
 #include "device.h"
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 
@@ -168,6 +169,85 @@ mwg_ftree TreeMirror::view(const MultiResolutionAnalysis<3> &mra) const {
     f.norms = norms.get();
     f.is_gen = isGen.get();
     return f;
+}
+
+int plan_mirror_node(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, int n, const std::array<int, 3> &l, std::vector<GenNode> &made) {
+    int id = fm.table.find(key_of(mra, n, l));
+    if (id >= 0) return id;
+    // nearest present ancestor
+    int s = n - 1, anc = -1;
+    for (; s >= mra.getRootScale(); s--) {
+        std::array<int, 3> ls;
+        for (int d = 0; d < 3; d++) ls[d] = l[d] >> (n - s);
+        anc = fm.table.find(key_of(mra, s, ls));
+        if (anc >= 0) break;
+    }
+    if (anc < 0) throw Error("node outside the world box");
+    int prev = anc;
+    for (int t = s + 1; t <= n; t++) {
+        std::array<int, 3> lt;
+        for (int d = 0; d < 3; d++) lt[d] = l[d] >> (n - t);
+        id = fm.nReal + fm.nGen;
+        fm.table.insert(key_of(mra, t, lt), id);
+        fm.nGen++;
+        fm.genParent.push_back(prev);
+        fm.genScale.push_back(t);
+        fm.genL.push_back(lt);
+        int child = 0;
+        for (int d = 0; d < 3; d++) child |= (lt[d] & 1) << d;
+        made.push_back({id, prev, child, t});
+        prev = id;
+    }
+    return prev;
+}
+
+void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,
+                           DeviceArray<int> &staging) {
+    if (made.empty()) return;
+    const int n3 = fm.kp1 * fm.kp1 * fm.kp1;
+    const int total = fm.nReal + fm.nGen;
+    const int firstNew = total - (int)made.size();
+    fm.hostOff.resize(total);
+    for (const auto &m : made) fm.hostOff[m.id] = fm.genBase + (long long)(m.id - fm.nReal) * n3;
+    fm.coefOff.reserve(total, firstNew);
+    fm.coefOff.upload(fm.hostOff.data() + firstNew, made.size(), firstNew);
+    fm.norms.reserve((size_t)total * 9, (size_t)firstNew * 9);
+    fm.isGen.reserve(total, firstNew);
+    std::vector<unsigned char> ones(made.size(), 1);
+    fm.isGen.upload(ones.data(), ones.size(), firstNew);
+    fm.coef.reserve((size_t)fm.genBase + (size_t)fm.nGen * n3, (size_t)fm.genBase + (size_t)(fm.nGen - made.size()) * n3);
+    fm.uploadTable();
+    std::stable_sort(made.begin(), made.end(), [](const GenNode &a, const GenNode &b) { return a.scale < b.scale; });
+    const size_t n = made.size();
+    std::vector<int> gen(3 * n);
+    for (size_t k = 0; k < n; k++) {
+        gen[k] = made[k].id;
+        gen[n + k] = made[k].parent;
+        gen[2 * n + k] = made[k].child;
+    }
+    staging.reserve(gen.size());
+    staging.upload(gen.data(), gen.size());
+    const int *ids = staging.get(), *par = ids + n, *ch = ids + 2 * n;
+    const mwg_ftree fv = fm.view(mra);
+    size_t i = 0;
+    while (i < n) {
+        size_t j = i;
+        while (j < n && made[j].scale == made[i].scale) j++;
+        check(mwg_generate_nodes(&fv, filters, ids + i, par + i, ch + i, (int)(j - i)), "generate nodes");
+        i = j;
+    }
+}
+
+void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f) {
+    if (fm.nGen == 0) return;
+    fm.table.reset(4 * fm.nReal + 1024);
+    for (int u = 0; u < fm.nReal; u++) fm.table.insert(key_of(mra, f.scale(u), f.transl(u)), u);
+    fm.nGen = 0;
+    fm.genParent.clear();
+    fm.genScale.clear();
+    fm.genL.clear();
+    fm.hostOff.resize(fm.nReal);
+    fm.uploadTable();
 }
 
 } // namespace device

@@ -2,6 +2,7 @@
 // arrays, operator band tables and the device mirror of an input tree.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <string>
 #include <unordered_map>
@@ -34,9 +35,11 @@ public:
 
     T *get() const { return ptr_; }
     size_t capacity() const { return cap_; }
-    // grow to at least n elements; keeps the first `keep` elements
+    // grow to at least n elements; keeps the first `keep` elements (at most
+    // the current capacity)
     void reserve(size_t n, size_t keep = 0) {
         if (n <= cap_) return;
+        keep = std::min(keep, cap_);
         size_t nc = cap_ ? cap_ : 1024;
         while (nc < n) nc *= 2;
         void *p = nullptr;
@@ -162,6 +165,27 @@ struct TreeMirror {
 };
 
 uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l);
+
+// A generated node planned into a mirror: its id, parent id, child index in
+// the parent and scale.
+struct GenNode {
+    int id, parent, child, scale;
+};
+
+// Plan node (n, l) of a mirror: its id if present (real or generated), else
+// the chain of generated nodes from its nearest present ancestor, appended
+// to `made` (MWTree::getNode -> MWNode::retrieveNode).  Host only; the new
+// nodes get storage and coefficients in generate_mirror_nodes.
+int plan_mirror_node(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, int n, const std::array<int, 3> &l, std::vector<GenNode> &made);
+
+// Device storage and scaling blocks of planned generated nodes: one upload of
+// ids, parents and child indices (through `staging`), one mwg_generate_nodes
+// launch per scale, coarse to fine.  `made` is sorted by scale.
+void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,
+                           DeviceArray<int> &staging);
+
+// Forget a mirror's generated nodes (FunctionTree::deleteGenerated).
+void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f);
 
 } // namespace device
 } // namespace mrcpp

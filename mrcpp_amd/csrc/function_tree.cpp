@@ -140,22 +140,10 @@ device::TreeMirror &FunctionTree<3>::inputMirror() {
     return *mirror_;
 }
 
-void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp) {
-    build_grid_nodes(out.nodes(), inp);
-    out.invalidate();
-}
+namespace device {
 
-namespace {
-
-thread_local ProjectionTarget g_proj_target = ProjectionTarget::Auto;
-thread_local bool g_proj_device = false;
-
-// Basis quadrature nodes and value -> coefficient map of an order on the
-// device, kept until the process ends (never destroyed: no device call runs
-// from a static destructor).
-struct BasisTables {
-    device::DeviceArray<double> nodes, vcmap;
-};
+// kept until the process ends (never destroyed: no device call runs from a
+// static destructor)
 const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra) {
     static std::map<int, BasisTables *> memo;
     static std::mutex m;
@@ -168,9 +156,24 @@ const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra) {
         slot->nodes.upload(b.nodes().data(), b.nodes().size());
         slot->vcmap.reserve(b.vcMap().a.size());
         slot->vcmap.upload(b.vcMap().a.data(), b.vcMap().a.size());
+        slot->cvmap.reserve(b.cvMap().a.size());
+        slot->cvmap.upload(b.cvMap().a.data(), b.cvMap().a.size());
     }
     return *slot;
 }
+
+} // namespace device
+
+void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp) {
+    build_grid_nodes(out.nodes(), inp);
+    out.invalidate();
+}
+
+namespace {
+
+thread_local ProjectionTarget g_proj_target = ProjectionTarget::Auto;
+thread_local bool g_proj_device = false;
+
 
 // project() with every level's nodes computed on the device (ProjectionCalculator
 // per level, then MWTree::mwTransform(BottomUp) and calcSquareNorm,
@@ -180,7 +183,7 @@ void project_device(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, i
     detail::NodeStore<3> &s = out.topology();
     const auto &mra = out.getMRA();
     const int nc = s.ncoef();
-    const BasisTables &bt = basis_tables(mra);
+    const device::BasisTables &bt = device::basis_tables(mra);
     const double *filt = device::filters(mra.getOrder());
     std::vector<mwg_gauss> gs(inp.size());
     for (int i = 0; i < inp.size(); i++) {

@@ -4,6 +4,7 @@
 
 #include <memory>
 #include <string>
+#include <tuple>
 
 #include "device.h"
 #include "functions.h"
@@ -11,6 +12,14 @@
 #include "trees.h"
 
 namespace mrcpp {
+
+namespace device {
+// Quadrature nodes and the value <-> coefficient maps of an order on the device.
+struct BasisTables {
+    DeviceArray<double> nodes, vcmap, cvmap;
+};
+const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra);
+} // namespace device
 
 template <int D> class FunctionTree;
 
@@ -80,6 +89,25 @@ void set_projection_target(ProjectionTarget t);
 bool last_projection_on_device();
 // dot(bra, ket) (src/treebuilders/multiply.cpp:286-318)
 double dot(FunctionTree<3> &bra, FunctionTree<3> &ket);
+
+// (coefficient, tree) terms (FunctionTreeVector, src/trees/FunctionTreeVector.h)
+using FunctionTreeVector = std::vector<std::tuple<double, FunctionTree<3> *>>;
+// add(prec, out, a, inp_a, b, inp_b, maxIter, absPrec) and add(prec, out,
+// FunctionTreeVector, ...) (src/treebuilders/add.cpp:43-80): TreeBuilder with
+// AdditionCalculator and WaveletAdaptor from out's end nodes, then the
+// bottom-up transform.  Each breadth-first step runs on the MI355X (at most 8
+// terms); the output's coefficients stay in HBM.
+void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter = -1,
+         bool absPrec = false);
+void add(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter = -1, bool absPrec = false);
+// multiply(prec, out, c, inp_a, inp_b, maxIter, absPrec, useMaxNorms) and the
+// FunctionTreeVector form (src/treebuilders/multiply.cpp:52-100) with
+// MultiplicationCalculator and WaveletAdaptor (useMaxNorms, the
+// MultiplicationAdaptor refinement, is not supported).
+void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_a, FunctionTree<3> &inp_b, int maxIter = -1,
+              bool absPrec = false, bool useMaxNorms = false);
+void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter = -1, bool absPrec = false,
+              bool useMaxNorms = false);
 
 // Statistics of the last apply on this thread.
 struct ApplyStats {

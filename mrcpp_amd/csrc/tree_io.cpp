@@ -39,18 +39,6 @@ std::vector<int> madness_order(int kp1) {
     return map;
 }
 
-// coefficient -> value map at the Gauss-Legendre points: cv(i, j) = phi_i(x_j),
-// the inverse of vc(i, j) = phi_j(x_i) w_i because the (k+1)-point rule
-// integrates phi_i phi_j exactly (LegendreBasis::calcCVMaps,
-// src/core/LegendreBasis.cpp:98-109, inverts vc numerically).
-core::SqMat cv_map(const core::LegendreTables &b) {
-    const int n = b.order() + 1;
-    core::SqMat cv(n);
-    for (int i = 0; i < n; i++)
-        for (int j = 0; j < n; j++) cv(i, j) = b.phi(i, b.nodes()[j]);
-    return cv;
-}
-
 // MWNode::cvTransform (src/trees/MWNode.cpp:448-490) on the 8 blocks of c for
 // a node at scale n: Forward maps scaling coefficients of the children to
 // values at their quadrature points, Backward the reverse.
@@ -156,7 +144,7 @@ void FunctionTree<3>::saveTreeTXT(const std::string &fname) {
     head << TD * ends.size() << std::endl;
     out << head.str();
     const std::vector<int> map = madness_order(s.kp1());
-    const core::SqMat cv = cv_map(mra_.basis());
+    const core::SqMat &cv = mra_.basis().cvMap();
     const int L = (int)two_pow(-rscale);
     // end nodes formatted in parallel, a block at a time, written in order
     constexpr int kBlock = 512;

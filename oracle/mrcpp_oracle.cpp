@@ -1640,6 +1640,96 @@ static double evalf(Tree<3> &t, const std::array<double, 3> &r) {
 }
 
 // ----------------------------------------------------------------------------
+// Tree arithmetic: add and multiply (src/treebuilders/add.cpp:43-80,
+// src/treebuilders/multiply.cpp:52-100) with WaveletAdaptor refinement
+// ----------------------------------------------------------------------------
+using TreeVec = std::vector<std::pair<double, Tree<3> *>>;
+
+// AdditionCalculator::calcNode (src/treebuilders/AdditionCalculator.h:45-70):
+// the linear combination of the inputs' nodes, generated from their ancestors
+// when missing (a generated node holds its scaling block only).
+struct AdditionCalculator : Calculator<3> {
+    const TreeVec &inp;
+    explicit AdditionCalculator(const TreeVec &v)
+            : inp(v) {}
+    void calcNode(Node<3> &out) override {
+        out.zeroCoefs();
+        for (const auto &ci : inp) {
+            Node<3> &nd = get_node(*ci.second, out.n, out.l);
+            for (size_t j = 0; j < nd.coefs.size(); j++) out.coefs[j] += ci.first * nd.coefs[j];
+        }
+        out.has_coefs = true;
+        calc_norms(out);
+    }
+};
+
+// MultiplicationCalculator::calcNode (src/treebuilders/MultiplicationCalculator.h:45-83):
+// each input node (a copy) reconstructed to its children and mapped to values
+// at their quadrature points, the values multiplied (times each input's
+// coefficient), mapped back and compressed.
+struct MultiplicationCalculator : Calculator<3> {
+    const TreeVec &inp;
+    std::vector<double> cv;
+    explicit MultiplicationCalculator(const TreeVec &v, const Basis &b)
+            : inp(v)
+            , cv(cv_map_of(b)) {}
+    static std::vector<double> cv_map_of(const Basis &b);
+    void calcNode(Node<3> &out) override {
+        const MRA<3> &m = *out.tree->mra;
+        const int nc = 8 * m.kp1_d;
+        for (int j = 0; j < nc; j++) out.coefs[j] = 1.0;
+        Node<3> tmp;
+        tmp.tree = out.tree;
+        tmp.n = out.n;
+        for (const auto &ci : inp) {
+            Node<3> &nd = get_node(*ci.second, out.n, out.l);
+            tmp.coefs.assign(nc, 0.0);
+            std::copy(nd.coefs.begin(), nd.coefs.end(), tmp.coefs.begin());
+            node_mw_transform(tmp, Filter::Reconstruction);
+            cv_transform_node(m, out.n, cv, true, tmp.coefs.data());
+            for (int j = 0; j < nc; j++) out.coefs[j] *= ci.first * tmp.coefs[j];
+        }
+        cv_transform_node(m, out.n, m.basis->vcMap, false, out.coefs.data());
+        node_mw_transform(out, Filter::Compression);
+        out.has_coefs = true;
+        calc_norms(out);
+    }
+    static void cv_transform_node(const MRA<3> &m, int n, const std::vector<double> &S, bool forward, double *coefs);
+};
+
+template <int D> struct WaveletSplit : Adaptor<D> {
+    double prec;
+    bool absPrec;
+    WaveletSplit(double p, int ms, bool ap)
+            : Adaptor<D>(ms)
+            , prec(p)
+            , absPrec(ap) {}
+    bool split(const Node<D> &nd) const override { return split_check(nd, prec, 1.0, absPrec); }
+};
+
+static void add(double prec, Tree<3> &out, const TreeVec &inp, int maxIter, bool absPrec) {
+    for (const auto &ci : inp)
+        if (ci.second->mra != out.mra) throw Error("Incompatible MRA");
+    AdditionCalculator calc(inp);
+    WaveletSplit<3> ad(prec, out.mra->maxScale(), absPrec);
+    build(out, calc, ad, maxIter);
+    mw_transform_up(out);
+    calc_square_norm(out);
+    for (const auto &ci : inp) delete_generated(*ci.second);
+}
+
+static void multiply(double prec, Tree<3> &out, const TreeVec &inp, int maxIter, bool absPrec) {
+    for (const auto &ci : inp)
+        if (ci.second->mra != out.mra) throw Error("Incompatible MRA");
+    MultiplicationCalculator calc(inp, *out.mra->basis);
+    WaveletSplit<3> ad(prec, out.mra->maxScale(), absPrec);
+    build(out, calc, ad, maxIter);
+    mw_transform_up(out);
+    calc_square_norm(out);
+    for (const auto &ci : inp) delete_generated(*ci.second);
+}
+
+// ----------------------------------------------------------------------------
 // Text files in MADNESS conventions (FunctionTree::saveTreeTXT / loadTreeTXT,
 // src/trees/FunctionTree.cpp:127-373)
 // ----------------------------------------------------------------------------
@@ -1853,6 +1943,11 @@ static void load_tree_txt(Tree<3> &t, const std::string &fname) {
     calc_square_norm(t);
 }
 
+std::vector<double> MultiplicationCalculator::cv_map_of(const Basis &b) { return cv_map(b); }
+void MultiplicationCalculator::cv_transform_node(const MRA<3> &m, int n, const std::vector<double> &S, bool forward, double *coefs) {
+    cv_transform(m, n, S, forward, coefs);
+}
+
 } // namespace orc
 
 // ============================================================================
@@ -1962,6 +2057,23 @@ int orc_tree_export(void *tp, int *idx, double *coefs, double *norms) {
         if (norms) norms[9 * i + 8] = nd->sq;
     }
     return (int)tab.size();
+}
+
+// out = sum_i c_i trees_i (mode 0) or prod_i c_i trees_i (mode 1)
+int orc_arith(int mode, double prec, void *outp, int n, const double *c, void **trees, int maxIter, int absPrec) {
+    try {
+        TreeVec v;
+        for (int i = 0; i < n; i++) v.push_back({c[i], static_cast<OrcTree *>(trees[i])->t.get()});
+        Tree<3> &out = *static_cast<OrcTree *>(outp)->t;
+        if (mode == 0)
+            add(prec, out, v, maxIter, absPrec != 0);
+        else
+            multiply(prec, out, v, maxIter, absPrec != 0);
+        return 0;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return -1;
+    }
 }
 
 int orc_tree_save_txt(void *tp, const char *path) {

@@ -52,6 +52,8 @@ class Oracle:
             L.orc_tree_square_norm.argtypes = [ctypes.c_void_p]
             L.orc_tree_export.argtypes = [ctypes.c_void_p, _ip, _dp, _dp]
             L.orc_tree_evalf.restype = ctypes.c_double
+            L.orc_arith.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_void_p, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_void_p),
+                                    ctypes.c_int, ctypes.c_int]
             L.orc_tree_save_txt.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
             L.orc_tree_load_txt.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
             L.orc_tree_evalf.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double]
@@ -252,3 +254,20 @@ def prec_factor(trees, n, l):
     if v < 0:
         raise RuntimeError(Oracle.error())
     return v
+
+
+def _arith(mode, prec, out, terms, max_iter, abs_prec):
+    c = np.ascontiguousarray([float(a) for a, _ in terms], dtype=np.float64)
+    hs = (ctypes.c_void_p * len(terms))(*[t.h for _, t in terms])
+    if Oracle.lib().orc_arith(mode, prec, out.h, len(terms), _ptr(c), hs, max_iter, int(abs_prec)) != 0:
+        raise RuntimeError(Oracle.error())
+
+
+def add(prec, out, terms, max_iter=-1, abs_prec=False):
+    """add(prec, out, [(c_i, tree_i), ...]) restated (src/treebuilders/add.cpp:43-80)."""
+    _arith(0, prec, out, terms, max_iter, abs_prec)
+
+
+def multiply(prec, out, terms, max_iter=-1, abs_prec=False):
+    """multiply(prec, out, [(c_i, tree_i), ...]) restated (src/treebuilders/multiply.cpp:52-100)."""
+    _arith(1, prec, out, terms, max_iter, abs_prec)
