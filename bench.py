@@ -157,6 +157,35 @@ def time_tree_io(g):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def time_arithmetic(mr, w, f, P):
+    """add, multiply and dot of the input f and a fresh apply result V = P f
+    (held in HBM only), each the median of three, with the output sizes."""
+    V = mr.FunctionTree(f.mra)
+    mr.apply(w["prec"], V, P, f)
+    def med(fn):
+        ts = []
+        for _ in range(3):
+            mr.device_synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            mr.device_synchronize()
+            ts.append(time.perf_counter() - t0)
+        return sorted(ts)[1], r
+    def add_once():
+        h = mr.FunctionTree(f.mra)
+        mr.add(w["prec"], h, 1.0, f, -1.0, V)
+        return h
+    def mul_once():
+        p = mr.FunctionTree(f.mra)
+        mr.multiply(w["prec"], p, 1.0, f, V)
+        return p
+    t_add, h = med(add_once)
+    t_mul, p = med(mul_once)
+    t_dot, d = med(lambda: mr.dot(f, V))
+    return {"add_s": round(t_add, 5), "add_nodes": h.getNNodes(), "multiply_s": round(t_mul, 5), "multiply_nodes": p.getNNodes(),
+            "dot_s": round(t_dot, 6), "dot": d, "what": "add(f - V), multiply(f V), dot(f, V) of the input and the Poisson result, on the GPU"}
+
+
 def time_batch_operator_build(mr, mra, specs):
     """Construction of the batch's distinct operators: all together with their
     2D trees formed on the GPU (second of two builds) against one by one on
@@ -491,6 +520,7 @@ def main():
             "step_times_s": [round(t, 6) for t in variant["times"]],
             "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
     res["projection"] = time_projection(mr, w, src, mra)
+    res["arithmetic"] = time_arithmetic(mr, w, f, P)
     res["tree_io"] = time_tree_io(plain["out"])
     res["operator_build"] = time_operator_build(mr, w, mra)
     if not args.no_cpu_baseline:

@@ -307,6 +307,14 @@ typedef struct {
 mwg_status mwg_combine_nodes(const mwg_combine *cb, const long long *in_off, const int *scale, const long long *out_off, double *out,
                              double *norms, int count);
 
+/* Inner products of count node pairs (dot, src/treebuilders/multiply.cpp:286-318):
+ * partial[2 k] = sum of the scaling blocks' products when root[k] is set
+ * (else 0), partial[2 k + 1] = sum over the 7 wavelet blocks; node k of a at
+ * a + a_off[k], of b at b + b_off[k] (8 blocks each).  Each node's sums run in
+ * a fixed order (deterministic). */
+mwg_status mwg_dot_nodes(int kp1, const double *a, const long long *a_off, const double *b, const long long *b_off, const int *root,
+                         double *partial, int count);
+
 /* Filter bank upload: 8 matrices (compression H0t,H1t,G0t,G1t then
  * reconstruction H0,G0,H1,G1), kp1*kp1 column-major each, from host memory;
  * returns a device pointer (MWFilter::getSubFilter, src/core/MWFilter.cpp:100-133). */

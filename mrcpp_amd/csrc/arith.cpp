@@ -172,6 +172,41 @@ void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVec
 
 } // namespace
 
+double detail::dot_device(FunctionTree<3> &bra, FunctionTree<3> &ket) {
+    device::ensure_init();
+    const detail::NodeStore<3> &a = bra.topology(), &b = ket.topology();
+    device::TreeMirror &ma = bra.inputMirror();
+    device::TreeMirror &mb = ket.inputMirror();
+    std::vector<long long> offs;
+    std::vector<int> root;
+    std::vector<long long> boff;
+    for (int u : detail::real_nodes(a)) {
+        const int v = b.findReal(a.scale(u), a.transl(u));
+        if (v < 0) continue;
+        offs.push_back(ma.hostOff[u]);
+        boff.push_back(mb.hostOff[v]);
+        root.push_back(a.depth(u) == 0 ? 1 : 0);
+    }
+    const size_t n = root.size();
+    if (n == 0) return 0.0;
+    offs.insert(offs.end(), boff.begin(), boff.end());
+    ArithScratch &S = arith_scratch();
+    S.offs.reserve(offs.size());
+    S.offs.upload(offs.data(), offs.size());
+    S.scale.reserve(n);
+    S.scale.upload(root.data(), n);
+    S.norms.reserve(2 * n);
+    device::check(mwg_dot_nodes(a.kp1(), ma.coef.get(), S.offs.get(), mb.coef.get(), S.offs.get() + n, S.scale.get(), S.norms.get(), (int)n), "dot");
+    std::vector<double> part(2 * n);
+    S.norms.download(part.data(), part.size());
+    double result = 0.0;
+    for (size_t k = 0; k < n; k++) {
+        if (root[k]) result += part[2 * k];
+        result += part[2 * k + 1];
+    }
+    return result;
+}
+
 void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter, bool absPrec) {
     combine(Sum, prec, out, {{a, &inp_a}, {b, &inp_b}}, maxIter, absPrec);
 }

@@ -140,11 +140,60 @@ template <int N1> struct CombineL {
     }
 };
 
+// One workgroup per node pair: the scaling and wavelet parts of the pair's
+// inner product, each thread over a fixed stride, then a fixed-order
+// reduction over the workgroup.
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_dot_nodes(const double *a, const long long *a_off, const double *b, const long long *b_off,
+                                                        const int *root, double *partial) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double red[2 * kThreads];
+    const int k = blockIdx.x;
+    const double *x = a + a_off[k], *y = b + b_off[k];
+    double s = 0.0, w = 0.0;
+    if (root[k])
+        for (int o = threadIdx.x; o < N3; o += kThreads) s += x[o] * y[o];
+    for (int o = N3 + threadIdx.x; o < 8 * N3; o += kThreads) w += x[o] * y[o];
+    red[threadIdx.x] = s;
+    red[kThreads + threadIdx.x] = w;
+    __syncthreads();
+    for (int h = kThreads / 2; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            red[threadIdx.x] += red[threadIdx.x + h];
+            red[kThreads + threadIdx.x] += red[kThreads + threadIdx.x + h];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        partial[2 * k] = red[0];
+        partial[2 * k + 1] = red[kThreads];
+    }
+}
+
+template <int N1> struct DotL {
+    static mwg_status run(const double *a, const long long *a_off, const double *b, const long long *b_off, const int *root, double *partial,
+                          int count) {
+        ProfileScope prof_(MWG_KERNEL_ARITH);
+        hipLaunchKernelGGL(k_dot_nodes<N1>, dim3(count), dim3(kThreads), 0, 0, a, a_off, b, b_off, root, partial);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
+
 } // namespace
 
 } // namespace mwgdev
 
 using namespace mwgdev;
+
+extern "C" mwg_status mwg_dot_nodes(int kp1, const double *a, const long long *a_off, const double *b, const long long *b_off, const int *root,
+                                    double *partial, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count < 0) return fail(MWG_ERR_ARG, "negative node count");
+    if (count == 0) return MWG_OK;
+    if (!a || !a_off || !b || !b_off || !root || !partial) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<DotL>(kp1, a, a_off, b, b_off, root, partial, count);
+}
 
 extern "C" mwg_status mwg_combine_nodes(const mwg_combine *cb, const long long *in_off, const int *scale, const long long *out_off, double *out,
                                         double *norms, int count) {

@@ -44,9 +44,14 @@ detail::NodeStore<3> &FunctionTree<3>::nodes() {
 // Legendre integral: root scaling coefficient 0 times 2^{-3n/2}, times the
 // square roots of the scaling factors (src/trees/FunctionTree.cpp:438-454).
 double FunctionTree<3>::integrate() {
-    detail::NodeStore<3> &s = nodes();
+    detail::NodeStore<3> &s = *store_;
     double r = 0.0;
-    for (int u : s.roots()) r += std::pow(2.0, -(3 * s.scale(u)) / 2.0) * s.coefs(u)[0];
+    for (int u : s.roots()) {
+        double c0 = s.coefs(u)[0];
+        // coefficients in HBM: only the roots' first scaling coefficient comes back
+        if (host_stale_) dev_coef_.download(&c0, 1, (size_t)u * s.ncoef());
+        r += std::pow(2.0, -(3 * s.scale(u)) / 2.0) * c0;
+    }
     double jac = 1.0;
     for (int d = 0; d < 3; d++) jac *= std::sqrt(mra_.getWorldBox().getScalingFactor(d));
     return jac * r;
@@ -297,6 +302,8 @@ void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxI
 // trees, summed serially.
 double dot(FunctionTree<3> &bra, FunctionTree<3> &ket) {
     if (bra.getMRA() != ket.getMRA()) throw Error("Trees not compatible");
+    // coefficients in HBM: the node products there, no copy to the host
+    if (bra.hostStale() || ket.hostStale()) return detail::dot_device(bra, ket);
     detail::NodeStore<3> &a = bra.nodes();
     detail::NodeStore<3> &b = ket.nodes();
     const int kd = a.kp1_d();

@@ -90,6 +90,12 @@ bool last_projection_on_device();
 // dot(bra, ket) (src/treebuilders/multiply.cpp:286-318)
 double dot(FunctionTree<3> &bra, FunctionTree<3> &ket);
 
+namespace detail {
+// dot on the device copies of both trees (their input mirrors): node products
+// in one launch, partial sums added on the host in bra's node order.
+double dot_device(FunctionTree<3> &bra, FunctionTree<3> &ket);
+} // namespace detail
+
 // (coefficient, tree) terms (FunctionTreeVector, src/trees/FunctionTreeVector.h)
 using FunctionTreeVector = std::vector<std::tuple<double, FunctionTree<3> *>>;
 // add(prec, out, a, inp_a, b, inp_b, maxIter, absPrec) and add(prec, out,

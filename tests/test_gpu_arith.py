@@ -98,6 +98,47 @@ def test_vector_forms_shared_inputs_and_device_resident_operands(mr, orc):
     assert fv.integrate() == pytest.approx(mr.dot(f, V), rel=1e-6)
 
 
+def _np_dot(a, b, root_scale):
+    """dot from exported trees: every node of a with a match in b, scaling
+    part at the root scale, wavelet part everywhere."""
+    (ai, ac, _), (bi, bc, _) = a.export(), b.export()
+    kd = ac.shape[1] // 8
+    where = {tuple(r[:4]): i for i, r in enumerate(bi)}
+    tot = 0.0
+    for i, r in enumerate(ai):
+        j = where.get(tuple(r[:4]))
+        if j is None:
+            continue
+        if r[0] == root_scale:
+            tot += float(ac[i, :kd] @ bc[j, :kd])
+        tot += float(ac[i, kd:] @ bc[j, kd:])
+    return tot
+
+
+def test_dot_and_integrate_of_trees_in_hbm(mr, orc):
+    """dot and integrate of an apply result without copying it to the host:
+    equal to the same sums formed from the exported coefficients."""
+    w = W.WORKLOADS["gauss16_k7"]
+    mra = mr.MultiResolutionAnalysis((-w["box"], w["box"]), w["order"])
+    f = mr.FunctionTree(mra)
+    gx = mr.GaussExp.from_sources(w["sources"]())
+    mr.build_grid(f, gx)
+    mr.project(w["prec"], f, gx)
+    V = mr.FunctionTree(mra)
+    mr.apply(w["prec"], V, mr.PoissonOperator(mra, w["prec"]), f)
+    d_dev = mr.dot(f, V)
+    i_dev = V.integrate()
+    root = int(f.export(False)[0][0, 0])
+    d_ref = _np_dot(f, V, root)
+    _say("dot", d_dev, d_ref, "integrate", i_dev)
+    assert d_dev == pytest.approx(d_ref, rel=1e-13)
+    idx, coefs, _ = V.export()
+    roots = idx[:, 0] == root
+    i_ref = float(np.sum(coefs[roots, 0])) * 2.0 ** (-1.5 * root)
+    assert i_dev == pytest.approx(i_ref, rel=1e-14, abs=1e-14)
+    assert mr.dot(V, V) == pytest.approx(V.getSquareNorm(), rel=1e-12)
+
+
 def test_arithmetic_arguments_are_checked(mr):
     mra = mr.MultiResolutionAnalysis((-16, 16), 5)
     other = mr.MultiResolutionAnalysis((-32, 32), 5)
