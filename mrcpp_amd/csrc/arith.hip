@@ -19,18 +19,18 @@ namespace mwgdev {
 namespace {
 
 // out(r, j) = sum_i in(i, r) M(i, j) for each of the 8 blocks (the host's
-// math_utils::apply_filter view, in N1 x N1^2 column-major).
+// math_utils::apply_filter view, in N1 x N1^2 column-major), padded LDS layout.
 template <int N1> __device__ inline void map_pass(double *out, const double *in, const double *M) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1;
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
         const int t = o / N3, e = o % N3;
         const int r = e % N2, j = e / N2;
-        const double *x = in + (size_t)t * N3 + N1 * r;
+        const double *x = in + pad<N1>(t * N3 + N1 * r);
         const double *m = M + N1 * j;
         double s = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; i++) s += x[i] * m[i];
-        out[o] = s;
+        out[pad<N1>(o)] = s;
     }
 }
 
@@ -39,16 +39,17 @@ template <int N1> __device__ inline void load_operand(double *dst, const double 
     constexpr int N3 = N1 * N1 * N1;
     const bool gen = off < 0;
     const long long base = gen ? -off - 1 : off;
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = (gen && o >= N3) ? 0.0 : pool[base + o];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[pad<N1>(o)] = (gen && o >= N3) ? 0.0 : pool[base + o];
 }
 
 template <int N1>
 __global__ __launch_bounds__(kThreads) void k_combine(mwg_combine cb, const long long *in_off, const int *scale, const long long *out_off,
                                                       double *out, double *norms) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1, PT = (8 * N3 + kThreads - 1) / kThreads;
-    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
     __shared__ double red[8 * (kThreads / 64)];
     __shared__ double cvm[N2], vcm[N2];
+    __shared__ double sfl[8 * N2];
     const int k = blockIdx.x;
     const long long *offs = in_off + (size_t)k * cb.nin;
     double *dst = out + out_off[k];
@@ -62,17 +63,18 @@ __global__ __launch_bounds__(kThreads) void k_combine(mwg_combine cb, const long
                 const double v = gen ? (o < N3 ? cb.pool[i][-off - 1 + o] : 0.0) : cb.pool[i][off + o];
                 s += cb.c[i] * v;
             }
-            a[o] = s;
+            a[pad<N1>(o)] = s;
             dst[o] = s;
         }
         __syncthreads();
-        block_norms<N1>(a, norms + (size_t)k * 9, red);
+        block_norms<N1, true>(a, norms + (size_t)k * 9, red);
         return;
     }
     for (int i = threadIdx.x; i < N2; i += kThreads) {
         cvm[i] = cb.cvmap[i];
         vcm[i] = cb.vcmap[i];
     }
+    stage_filters<N1>(sfl, cb.filters, 8);
     const int n = scale[k];
     double sfp = cb.sf[0] * cb.sf[1] * cb.sf[2];
     if (sfp <= kMachineZero) sfp = 1.0;
@@ -85,11 +87,11 @@ __global__ __launch_bounds__(kThreads) void k_combine(mwg_combine cb, const long
         __syncthreads(); // the previous operand's values are consumed
         load_operand<N1>(a, cb.pool[i], offs[i]);
         __syncthreads();
-        mw_pass<N1>(b, a, cb.filters, 4, 0);
+        mw_pass<N1>(b, a, sfl, 4, 0);
         __syncthreads();
-        mw_pass<N1>(a, b, cb.filters, 4, 1);
+        mw_pass<N1>(a, b, sfl, 4, 1);
         __syncthreads();
-        mw_pass<N1>(b, a, cb.filters, 4, 2);
+        mw_pass<N1>(b, a, sfl, 4, 2);
         __syncthreads();
         map_pass<N1>(a, b, cvm);
         __syncthreads();
@@ -101,14 +103,14 @@ __global__ __launch_bounds__(kThreads) void k_combine(mwg_combine cb, const long
 #pragma unroll
         for (int p = 0; p < PT; p++) {
             const int o = threadIdx.x + kThreads * p;
-            if (o < 8 * N3) val[p] = val[p] * (ci * (fwd * a[o]));
+            if (o < 8 * N3) val[p] = val[p] * (ci * (fwd * a[pad<N1>(o)]));
         }
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PT; p++) {
         const int o = threadIdx.x + kThreads * p;
-        if (o < 8 * N3) a[o] = val[p];
+        if (o < 8 * N3) a[pad<N1>(o)] = val[p];
     }
     __syncthreads();
     // values -> scaling coefficients of the children, then compression
@@ -118,16 +120,16 @@ __global__ __launch_bounds__(kThreads) void k_combine(mwg_combine cb, const long
     __syncthreads();
     map_pass<N1>(b, a, vcm);
     __syncthreads();
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) b[o] = bwd * b[o];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) b[pad<N1>(o)] = bwd * b[pad<N1>(o)];
     __syncthreads();
-    mw_pass<N1>(a, b, cb.filters, 0, 0);
+    mw_pass<N1>(a, b, sfl, 0, 0);
     __syncthreads();
-    mw_pass<N1>(b, a, cb.filters, 0, 1);
+    mw_pass<N1>(b, a, sfl, 0, 1);
     __syncthreads();
-    mw_pass<N1>(a, b, cb.filters, 0, 2);
+    mw_pass<N1>(a, b, sfl, 0, 2);
     __syncthreads();
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = a[o];
-    block_norms<N1>(a, norms + (size_t)k * 9, red);
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = a[pad<N1>(o)];
+    block_norms<N1, true>(a, norms + (size_t)k * 9, red);
 }
 
 template <int N1> struct CombineL {

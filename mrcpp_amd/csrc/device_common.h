@@ -142,7 +142,23 @@ template <template <int> class Launcher, typename... Args> mwg_status dispatch(i
     }
 }
 
-// One pass of the separable 3D multiwavelet transform over 8 blocks in LDS.
+// LDS layout of a node's 8 blocks in the transform kernels: one padding
+// double after every row of N1 values, so the rows that the lanes of a pass
+// read (N1 apart without padding: lanes on few banks) spread over the banks.
+template <int N1> struct Lds {
+    static constexpr int N3 = N1 * N1 * N1, size = 8 * N3 + 8 * N3 / N1;
+};
+template <int N1> __device__ __forceinline__ int pad(int o) { return o + o / N1; }
+
+// The first `count` filter matrices (N1 x N1 each) of a filter bank into LDS:
+// the passes read every filter element once per output, from LDS rather than
+// through the vector memory path.
+template <int N1> __device__ inline void stage_filters(double *dst, const double *filt, int count) {
+    for (int i = threadIdx.x; i < count * N1 * N1; i += kThreads) dst[i] = filt[i];
+}
+
+// One pass of the separable 3D multiwavelet transform over 8 blocks in LDS
+// (padded layout).
 // dir_base 0: compression sub-filters, 4: reconstruction sub-filters (the
 // device filter array holds H0t,H1t,G0t,G1t then H0,G0,H1,G1).  out[gt] =
 // in[ft0]^T F(2 b + 0) + in[ft1]^T F(2 b + 1), ft0 < ft1 the two blocks that
@@ -157,20 +173,20 @@ template <int N1> __device__ inline void mw_pass(double *out, const double *in, 
         const int ft0 = gt & ~bit, ft1 = gt | bit;
         const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
         const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
-        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
-        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        const double *x0 = in + pad<N1>(ft0 * N3 + N1 * r);
+        const double *x1 = in + pad<N1>(ft1 * N3 + N1 * r);
         double s0 = 0.0, s1 = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; i++) {
             s0 += x0[i] * F0[i];
             s1 += x1[i] * F1[i];
         }
-        out[o] = s0 + s1;
+        out[pad<N1>(o)] = s0 + s1;
     }
 }
 
 // Last reconstruction pass accumulating into the children's scaling blocks,
-// each of the two contributions added separately.
+// each of the two contributions added separately (in: padded LDS layout).
 template <int N1>
 __device__ inline void mw_pass_add_children(double *coef, long long child_off, long long stride, const double *in, const double *filt) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1;
@@ -181,8 +197,8 @@ __device__ inline void mw_pass_add_children(double *coef, long long child_off, l
         const int ft0 = gt & ~bit, ft1 = gt | bit;
         const double *F0 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
         const double *F1 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
-        const double *x0 = in + (size_t)ft0 * N3 + N1 * r;
-        const double *x1 = in + (size_t)ft1 * N3 + N1 * r;
+        const double *x0 = in + pad<N1>(ft0 * N3 + N1 * r);
+        const double *x1 = in + pad<N1>(ft1 * N3 + N1 * r);
         double s0 = 0.0, s1 = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; i++) {
@@ -197,14 +213,15 @@ __device__ inline void mw_pass_add_children(double *coef, long long child_off, l
     }
 }
 
-// Component norms of 8 blocks, written to nrm[0..8] (8 norms + square norm).
-template <int N1> __device__ inline void block_norms(const double *c, double *nrm, double *red) {
+// Component norms of 8 blocks, written to nrm[0..8] (8 norms + square norm);
+// PAD: c in the padded LDS layout.
+template <int N1, bool PAD = false> __device__ inline void block_norms(const double *c, double *nrm, double *red) {
     constexpr int N3 = N1 * N1 * N1;
     double ss[8];
 #pragma unroll
     for (int t = 0; t < 8; t++) ss[t] = 0.0;
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
-        const double v = c[o];
+        const double v = c[PAD ? pad<N1>(o) : o];
         ss[o / N3] += v * v;
     }
 #pragma unroll

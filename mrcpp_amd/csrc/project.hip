@@ -17,21 +17,21 @@ namespace mwgdev {
 
 namespace {
 
-constexpr int kTermsPerRound = 32; // Gaussian terms whose 1D factors share LDS at a time
+constexpr int kTermsPerRound = 24; // Gaussian terms whose 1D factors share LDS at a time
 
 // out(r, j) = sum_i in(i, r) V(i, j) for each of the 8 blocks (in viewed
-// N1 x N1^2, column-major; core::filter_pass).
+// N1 x N1^2, column-major; core::filter_pass), padded LDS layout.
 template <int N1> __device__ inline void vc_pass(double *out, const double *in, const double *V) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1;
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
         const int t = o / N3, e = o % N3;
         const int r = e % N2, j = e / N2;
-        const double *x = in + (size_t)t * N3 + N1 * r;
+        const double *x = in + pad<N1>(t * N3 + N1 * r);
         const double *v = V + N1 * j;
         double s = 0.0;
 #pragma unroll
         for (int i = 0; i < N1; i++) s += x[i] * v[i];
-        out[o] = s;
+        out[pad<N1>(o)] = s;
     }
 }
 
@@ -39,11 +39,13 @@ template <int N1>
 __global__ __launch_bounds__(kThreads) void k_project(mwg_projection pj, const int *scale, const int *lv, const long long *off, double *coef,
                                                       double *norms) {
     constexpr int N2 = N1 * N1, N3 = N2 * N1, PT = (8 * N3 + kThreads - 1) / kThreads;
-    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
     __shared__ double fac1d[kTermsPerRound][3][2 * N1]; // exp(-alpha_d q^2) at the 2 N1 points of each direction
     __shared__ double gcoef[kTermsPerRound];
     __shared__ double red[8 * (kThreads / 64)];
     __shared__ double sx[N1], svc[N2];
+    __shared__ double sfl[4 * N2]; // compression filters
+    stage_filters<N1>(sfl, pj.filters, 4);
     const int k = blockIdx.x;
     const int n = scale[k];
     const int l[3] = {lv[3 * k], lv[3 * k + 1], lv[3 * k + 2]};
@@ -84,7 +86,7 @@ __global__ __launch_bounds__(kThreads) void k_project(mwg_projection pj, const i
 #pragma unroll
     for (int p = 0; p < PT; p++) {
         const int o = threadIdx.x + kThreads * p;
-        if (o < 8 * N3) a[o] = val[p];
+        if (o < 8 * N3) a[pad<N1>(o)] = val[p];
     }
     __syncthreads();
     // values -> coefficients, one direction per pass
@@ -97,18 +99,18 @@ __global__ __launch_bounds__(kThreads) void k_project(mwg_projection pj, const i
     double sfp = pj.sf[0] * pj.sf[1] * pj.sf[2];
     if (sfp <= kMachineZero) sfp = 1.0;
     const double fac = sqrt(1.0 / (ldexp(1.0, 3 * (n + 1)) / sfp));
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) b[o] *= fac;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) b[pad<N1>(o)] *= fac;
     __syncthreads();
     // children's scaling blocks -> the node's scaling and wavelet blocks
-    mw_pass<N1>(a, b, pj.filters, 0, 0);
+    mw_pass<N1>(a, b, sfl, 0, 0);
     __syncthreads();
-    mw_pass<N1>(b, a, pj.filters, 0, 1);
+    mw_pass<N1>(b, a, sfl, 0, 1);
     __syncthreads();
-    mw_pass<N1>(a, b, pj.filters, 0, 2);
+    mw_pass<N1>(a, b, sfl, 0, 2);
     __syncthreads();
     double *dst = coef + off[k];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = a[o];
-    block_norms<N1>(a, norms + (size_t)k * 9, red);
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = a[pad<N1>(o)];
+    block_norms<N1, true>(a, norms + (size_t)k * 9, red);
 }
 
 template <int N1> struct ProjectL {

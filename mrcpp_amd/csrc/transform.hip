@@ -12,24 +12,26 @@ namespace {
 template <int N1>
 __global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child) {
     constexpr int N3 = N1 * N1 * N1;
-    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
     __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double sfl[8 * N1 * N1];
+    stage_filters<N1>(sfl, filt, 8);
     const int k = blockIdx.x;
     const int pid = parent[k], id = ids[k], c = child[k];
     const double *src = f.coef + f.coef_off[pid];
     const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = (o < nblk * N3) ? src[o] : 0.0;
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[pad<N1>(o)] = (o < nblk * N3) ? src[o] : 0.0;
     __syncthreads();
-    mw_pass<N1>(b, a, filt, 4, 0);
+    mw_pass<N1>(b, a, sfl, 4, 0);
     __syncthreads();
-    mw_pass<N1>(a, b, filt, 4, 1);
+    mw_pass<N1>(a, b, sfl, 4, 1);
     __syncthreads();
-    mw_pass<N1>(b, a, filt, 4, 2);
+    mw_pass<N1>(b, a, sfl, 4, 2);
     __syncthreads();
     double *dst = f.coef + f.coef_off[id];
     double ss = 0.0;
     for (int e = threadIdx.x; e < N3; e += kThreads) {
-        const double v = b[c * N3 + e];
+        const double v = b[pad<N1>(c * N3 + e)];
         dst[e] = v;
         ss += v * v;
     }
@@ -50,36 +52,40 @@ __global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double
 template <int N1>
 __global__ __launch_bounds__(kThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
     constexpr int N3 = N1 * N1 * N1;
-    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double sfl[8 * N1 * N1];
+    stage_filters<N1>(sfl, filt, 8);
     const int k = blockIdx.x;
     const double *src = coef + poff[k];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[o] = src[o];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[pad<N1>(o)] = src[o];
     __syncthreads();
-    mw_pass<N1>(b, a, filt, 4, 0);
+    mw_pass<N1>(b, a, sfl, 4, 0);
     __syncthreads();
-    mw_pass<N1>(a, b, filt, 4, 1);
+    mw_pass<N1>(a, b, sfl, 4, 1);
     __syncthreads();
-    mw_pass_add_children<N1>(coef, coff[k], stride, a, filt);
+    mw_pass_add_children<N1>(coef, coff[k], stride, a, sfl);
 }
 
 template <int N1>
 __global__ __launch_bounds__(kThreads) void k_compress(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
     constexpr int N3 = N1 * N1 * N1;
-    __shared__ double a[8 * N3], b[8 * N3];
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double sfl[4 * N1 * N1];
+    stage_filters<N1>(sfl, filt, 4);
     const int k = blockIdx.x;
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
         const int t = o / N3, e = o % N3;
-        a[o] = coef[coff[k] + (long long)t * stride + e];
+        a[pad<N1>(o)] = coef[coff[k] + (long long)t * stride + e];
     }
     __syncthreads();
-    mw_pass<N1>(b, a, filt, 0, 0);
+    mw_pass<N1>(b, a, sfl, 0, 0);
     __syncthreads();
-    mw_pass<N1>(a, b, filt, 0, 1);
+    mw_pass<N1>(a, b, sfl, 0, 1);
     __syncthreads();
-    mw_pass<N1>(b, a, filt, 0, 2);
+    mw_pass<N1>(b, a, sfl, 0, 2);
     __syncthreads();
     double *dst = coef + poff[k];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = b[o];
+    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = b[pad<N1>(o)];
 }
 
 template <int N1> __global__ __launch_bounds__(kThreads) void k_node_norms(const double *coef, const long long *off, double *norms) {
