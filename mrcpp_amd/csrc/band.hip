@@ -30,7 +30,7 @@ namespace mwgdev {
 
 namespace {
 
-constexpr int kGroupMinPerUnit = 16;
+constexpr int kGroupMinPerUnit = 32;
 constexpr int kPrecValuBand = 3; // FP64 sweeps on the vector ALUs (comparison path)
 constexpr int kMaxUnitsPerNode = 512;
 constexpr double kPartialBudget = 1024.0 * 1024.0 * 1024.0; // bytes of partial slots
