@@ -54,6 +54,10 @@ mwg_status mwg_download(void *dst, const void *src, size_t bytes);
 mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes);
 mwg_status mwg_copy(void *dst, const void *src, size_t bytes); /* device to device */
 mwg_status mwg_zero(void *dst, size_t bytes);
+/* Entries of a device key table changed on the host: upd holds n slots, then
+ * n keys, then n values (64-bit each); keys[slot] = key, vals[slot] = value.
+ * Queued on the null stream (no wait). */
+mwg_status mwg_hash_update(unsigned long long *keys, int *vals, const long long *upd, int n);
 mwg_status mwg_synchronize(void);
 mwg_status mwg_release_pool(void);
 

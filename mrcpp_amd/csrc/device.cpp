@@ -97,6 +97,8 @@ void KeyTable::reset(int min_entries) {
     keys_.assign((size_t)1 << log2cap_, 0ull);
     vals_.assign((size_t)1 << log2cap_, -1);
     count_ = 0;
+    full_ = true;
+    dirty_.clear();
 }
 
 int KeyTable::insert(uint64_t key, int id) {
@@ -108,6 +110,8 @@ int KeyTable::insert(uint64_t key, int id) {
         keys_.assign((size_t)1 << log2cap_, 0ull);
         vals_.assign((size_t)1 << log2cap_, -1);
         count_ = 0;
+        full_ = true;
+        dirty_.clear();
         for (size_t i = 0; i < ok.size(); i++)
             if (ok[i]) insert(ok[i], ov[i]);
     }
@@ -118,6 +122,7 @@ int KeyTable::insert(uint64_t key, int id) {
             keys_[s] = key;
             vals_[s] = id;
             count_++;
+            if (!full_) dirty_.push_back((int)s);
             return id;
         }
         if (keys_[s] == key) return vals_[s];
@@ -146,11 +151,29 @@ uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::arr
     return mwgkey::pack(d, o[0], o[1], o[2]);
 }
 
+// The whole table after a reset or rehash; otherwise only the slots filled
+// since the last upload, scattered on the device.
 void TreeMirror::uploadTable() {
-    keys.reserve(table.capacity());
-    vals.reserve(table.capacity());
-    keys.upload(table.keys().data(), table.capacity());
-    vals.upload(table.vals().data(), table.capacity());
+    const size_t cap = table.capacity();
+    if (table.fullUpload() || keys.capacity() < cap || vals.capacity() < cap) {
+        keys.reserve(cap);
+        vals.reserve(cap);
+        keys.upload(table.keys().data(), cap);
+        vals.upload(table.vals().data(), cap);
+    } else if (!table.dirty().empty()) {
+        const std::vector<int> &d = table.dirty();
+        const size_t n = d.size();
+        std::vector<long long> upd(3 * n);
+        for (size_t i = 0; i < n; i++) {
+            upd[i] = d[i];
+            upd[n + i] = (long long)table.keys()[d[i]];
+            upd[2 * n + i] = table.vals()[d[i]];
+        }
+        keyUpdates.reserve(upd.size());
+        keyUpdates.upload(upd.data(), upd.size());
+        check(mwg_hash_update(keys.get(), vals.get(), keyUpdates.get(), (int)n), "key table update");
+    }
+    table.markUploaded();
 }
 
 mwg_ftree TreeMirror::view(const MultiResolutionAnalysis<3> &mra) const {

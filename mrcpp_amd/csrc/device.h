@@ -136,11 +136,20 @@ public:
     int size() const { return count_; }
     const std::vector<unsigned long long> &keys() const { return keys_; }
     const std::vector<int> &vals() const { return vals_; }
+    // slots filled since markUploaded(); fullUpload() after a reset or a rehash
+    bool fullUpload() const { return full_; }
+    const std::vector<int> &dirty() const { return dirty_; }
+    void markUploaded() {
+        full_ = false;
+        dirty_.clear();
+    }
 
 private:
     int log2cap_ = 0, count_ = 0;
     std::vector<unsigned long long> keys_;
     std::vector<int> vals_;
+    bool full_ = true;
+    std::vector<int> dirty_;
 };
 
 // Device mirror of a 3D function tree used as apply input: real nodes keep
@@ -152,6 +161,7 @@ struct TreeMirror {
     KeyTable table;
     DeviceArray<unsigned long long> keys;
     DeviceArray<int> vals;
+    DeviceArray<long long> keyUpdates; // staging of incremental table uploads
     DeviceArray<double> coef, norms;
     DeviceArray<long long> coefOff;
     DeviceArray<unsigned char> isGen;

@@ -122,11 +122,29 @@ int device_cus() {
     return cus;
 }
 
+__global__ __launch_bounds__(256) void k_hash_update(unsigned long long *keys, int *vals, const long long *upd, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long slot = upd[i];
+    keys[slot] = (unsigned long long)upd[n + i];
+    vals[slot] = (int)upd[2 * (size_t)n + i];
+}
+
 } // namespace mwgdev
 
 using namespace mwgdev;
 
 extern "C" {
+
+mwg_status mwg_hash_update(unsigned long long *keys, int *vals, const long long *upd, int n) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (n < 0) return fail(MWG_ERR_ARG, "negative entry count");
+    if (n == 0) return MWG_OK;
+    if (!keys || !vals || !upd) return fail(MWG_ERR_ARG, "null argument");
+    hipLaunchKernelGGL(k_hash_update, dim3((n + 255) / 256), dim3(256), 0, 0, keys, vals, upd, n);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
 
 const char *mwg_last_error(void) { return g_err.c_str(); }
 
