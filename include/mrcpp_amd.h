@@ -50,6 +50,16 @@ int mrcpp_add(double prec, void *out, int n, const double *c, void **trees, int 
  * out = prod_i c[i] trees[i] (WaveletAdaptor refinement; the useMaxNorms
  * variant is not provided), on the GPU, 1 <= n <= 8. */
 int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec);
+/* FunctionTree::rescale (src/trees/FunctionTree.cpp:651-668): end nodes times c,
+ * then the bottom-up transform and norms (on the GPU for trees held in HBM) */
+int mrcpp_tree_rescale(void *tree, double c);
+/* apply(prec, CompFunction out, oper, CompFunction inp, metric, maxIter, absPrec)
+ * (src/treebuilders/apply.h:42, src/treebuilders/apply.cpp:119-139) over real
+ * components: out[4] and inp[ncomp] are tree handles (an out entry may be NULL
+ * when no metric entry selects it); metric: 16 doubles [input][output],
+ * row-major, NULL for the identity.  Pairs writing distinct outputs run as
+ * one batch. */
+int mrcpp_apply_components(double prec, void **out, void *oper, void **inp, int ncomp, const double *metric, int max_iter, int abs_prec);
 /* FunctionTree::saveTree / loadTree (src/trees/FunctionTree.cpp:375-435): binary
  * "<file>.tree" of every real node in this library's layout (MRCPP's file is its
  * allocator's memory image); loading needs the MRA the tree was saved with. */

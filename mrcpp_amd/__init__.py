@@ -49,6 +49,9 @@ SIGNATURES = {
     "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
     "mrcpp_add": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
     "mrcpp_multiply": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
+    "mrcpp_tree_rescale": (ctypes.c_int, [_vp, ctypes.c_double]),
+    "mrcpp_apply_components": (ctypes.c_int, [ctypes.c_double, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp), ctypes.c_int, _dp, ctypes.c_int,
+                                              ctypes.c_int]),
     "mrcpp_tree_save": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "mrcpp_tree_load": (ctypes.c_int, [_vp, ctypes.c_char_p]),
     "mrcpp_tree_save_txt": (ctypes.c_int, [_vp, ctypes.c_char_p]),
@@ -221,6 +224,10 @@ class FunctionTree:
 
     def integrate(self):
         return float(lib().mrcpp_tree_integrate(self._h))
+
+    def rescale(self, c):
+        """FunctionTree::rescale: end nodes times c, bottom-up transform, norms."""
+        _check(lib().mrcpp_tree_rescale(self._h, float(c)))
 
     def saveTree(self, file):
         """Binary file "<file>.tree" (FunctionTree::saveTree)."""
@@ -582,11 +589,45 @@ def apply_batch(tasks):
     return [{k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])} for i in range(n)]
 
 
-def apply(prec, out, oper, inp, maxIter=-1, absPrec=False, precTrees=None):
+class CompFunction:
+    """CompFunction<3> with up to 4 real components (FunctionTree objects)."""
+
+    def __init__(self, mra, ncomp=1):
+        self.mra = mra
+        self.CompD = [FunctionTree(mra) if i < ncomp else None for i in range(4)]
+
+    def Ncomp(self):
+        n = 0
+        for i, t in enumerate(self.CompD):
+            if t is not None:
+                n = i + 1
+        return n
+
+
+def _apply_comp(prec, out, oper, inp, metric, maxIter, absPrec):
+    """apply over CompFunctions (src/treebuilders/apply.cpp:119-139): out gets
+    inp's component count, metric a real 4x4 [input][output] (identity)."""
+    m = np.eye(4) if metric is None else np.ascontiguousarray(metric, dtype=np.float64).reshape(4, 4)
+    nc = inp.Ncomp()
+    out.CompD = [FunctionTree(out.mra) if i < nc else None for i in range(4)]
+    for ic in range(nc):
+        for oc in range(4):
+            if m[ic, oc] ** 2 > 1e-15 and out.CompD[oc] is None:
+                out.CompD[oc] = FunctionTree(out.mra)
+    oh = (_vp * 4)(*[t._h if t is not None else None for t in out.CompD])
+    ih = (_vp * 4)(*[t._h if t is not None else None for t in inp.CompD])
+    _check(lib().mrcpp_apply_components(float(prec), oh, oper._h, ih, nc, _p(np.ascontiguousarray(m.ravel())), int(maxIter), int(bool(absPrec))))
+
+
+def apply(prec, out, oper, inp, maxIter=-1, absPrec=False, precTrees=None, metric=None):
     """apply(prec, out, oper, inp, maxIter, absPrec), or with precision trees
     apply(prec, out, oper, inp, precTrees, maxIter, absPrec) as the reference's
     overload (src/treebuilders/apply.h:43): a list in the fifth place is
-    precTrees.  Returns the run statistics."""
+    precTrees.  With CompFunction arguments the component overload
+    (apply.h:42; metric= a real 4x4 array).  Returns the run statistics
+    (None for CompFunctions)."""
+    if isinstance(out, CompFunction):
+        return _apply_comp(prec, out, oper, inp, metric, maxIter, absPrec)
     if isinstance(maxIter, (list, tuple)):
         # positional form of the overload: (precTrees, maxIter, absPrec) in places 5-7
         trees = list(maxIter)

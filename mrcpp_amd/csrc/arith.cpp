@@ -207,6 +207,51 @@ double detail::dot_device(FunctionTree<3> &bra, FunctionTree<3> &ket) {
     return result;
 }
 
+void FunctionTree<3>::rescale(double c) {
+    detail::NodeStore<3> &s = *store_;
+    const std::vector<int> ends = detail::end_nodes(s);
+    if (host_stale_) {
+        // end nodes times c in place on the device (a one-term sum), then the
+        // bottom-up transform and the norms there
+        device::ensure_init();
+        const int kp1 = s.kp1(), ncoef = s.ncoef();
+        mwg_combine cb{};
+        cb.kp1 = kp1;
+        cb.mode = Sum;
+        cb.nin = 1;
+        cb.c[0] = c;
+        cb.pool[0] = dev_coef_.get();
+        cb.filters = device::filters(mra_.getOrder());
+        for (int d = 0; d < 3; d++) cb.sf[d] = mra_.getWorldBox().getScalingFactor(d);
+        const size_t n = ends.size();
+        std::vector<long long> offs(2 * n);
+        std::vector<int> scales(n);
+        for (size_t k = 0; k < n; k++) {
+            offs[k] = offs[n + k] = (long long)ends[k] * ncoef;
+            scales[k] = s.scale(ends[k]);
+        }
+        ArithScratch &S = arith_scratch();
+        S.offs.reserve(offs.size());
+        S.offs.upload(offs.data(), offs.size());
+        S.scale.reserve(n);
+        S.scale.upload(scales.data(), n);
+        S.norms.reserve(9 * n);
+        device::check(mwg_combine_nodes(&cb, S.offs.get(), S.scale.get(), S.offs.get() + n, dev_coef_.get(), S.norms.get(), (int)n), "rescale");
+        compress_and_norms(*this, cb.filters);
+        markDeviceResident();
+        return;
+    }
+    const int nc = s.ncoef();
+    for (int u : ends) {
+        double *x = s.coefs(u);
+        for (int j = 0; j < nc; j++) x[j] *= c;
+        detail::compute_norms(s, u);
+    }
+    detail::transform_up(s);
+    detail::calc_square_norm(s);
+    invalidate();
+}
+
 void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter, bool absPrec) {
     combine(Sum, prec, out, {{a, &inp_a}, {b, &inp_b}}, maxIter, absPrec);
 }

@@ -113,6 +113,23 @@ int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees,
     return guarded([&] { multiply(prec, *static_cast<FunctionTree<3> *>(out), tree_vector(n, c, trees), max_iter, abs_prec != 0); });
 }
 
+int mrcpp_tree_rescale(void *tree, double c) {
+    return guarded([&] { static_cast<FunctionTree<3> *>(tree)->rescale(c); });
+}
+
+int mrcpp_apply_components(double prec, void **out, void *oper, void **inp, int ncomp, const double *metric, int max_iter, int abs_prec) {
+    return guarded([&] {
+        if (ncomp < 0 || ncomp > 4 || !out || !inp) throw Error("apply: bad component arrays");
+        ComplexDouble m[4][4];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++) m[i][j] = metric ? ComplexDouble(metric[4 * i + j], 0.0) : defaultMetric[i][j];
+        FunctionTree<3> *o[4] = {}, *in[4] = {};
+        for (int i = 0; i < 4; i++) o[i] = static_cast<FunctionTree<3> *>(out[i]);
+        for (int i = 0; i < ncomp; i++) in[i] = static_cast<FunctionTree<3> *>(inp[i]);
+        apply_components(prec, o, *static_cast<ConvolutionOperator<3> *>(oper), in, ncomp, m, max_iter, abs_prec != 0);
+    });
+}
+
 int mrcpp_tree_save(void *tree, const char *file) {
     return guarded([&] { static_cast<FunctionTree<3> *>(tree)->saveTree(file); });
 }

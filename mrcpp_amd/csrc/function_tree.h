@@ -2,6 +2,7 @@
 // optional device-resident copy of its coefficients (src/trees/FunctionTree.h:55-183).
 #pragma once
 
+#include <complex>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -36,6 +37,11 @@ public:
     int getNEndNodes() const;
     double integrate();
     double evalf(const Coord<3> &r);
+
+    // FunctionTree::rescale (src/trees/FunctionTree.cpp:651-668): end-node
+    // coefficients times c, then the bottom-up transform and the norms (on the
+    // device when the coefficients live in HBM).
+    void rescale(double c);
 
     // FunctionTree::saveTree / loadTree (src/trees/FunctionTree.cpp:375-435):
     // binary file "<file>.tree" of every real node (this library's layout; the
@@ -182,5 +188,45 @@ void apply_batch(const std::vector<ApplyTask> &tasks);
 // Per-task statistics of the last apply_batch on this thread (seconds is the
 // whole batch's wall time).
 const std::vector<ApplyStats> &last_batch_stats();
+
+using ComplexDouble = std::complex<double>;
+constexpr ComplexDouble defaultMetric[4][4] = {{1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+
+template <int D> class CompFunction;
+// CompFunction<3> (src/utils/CompFunction.h): a function of up to 4 real
+// components, each a FunctionTree<3> owned by it (the reference's complex
+// components are not provided).
+template <> class CompFunction<3> {
+public:
+    explicit CompFunction(const MultiResolutionAnalysis<3> &mra, int ncomp = 1);
+    CompFunction(const CompFunction &) = delete;
+    CompFunction &operator=(const CompFunction &) = delete;
+    const MultiResolutionAnalysis<3> &getMRA() const { return mra_; }
+    int Ncomp() const { return ncomp_; }
+    bool isreal() const { return true; }
+    // a new empty tree for component i (Ncomp grows to at least i + 1)
+    void alloc_comp(int i);
+    // ncomp new empty components, the others dropped (CompFunction::paramCopy(true))
+    void reset(int ncomp);
+    FunctionTree<3> *CompD[4] = {nullptr, nullptr, nullptr, nullptr};
+
+private:
+    MultiResolutionAnalysis<3> mra_;
+    int ncomp_ = 0;
+    std::unique_ptr<FunctionTree<3>> own_[4];
+};
+
+// apply(prec, CompFunction out, oper, CompFunction inp, metric, maxIter,
+// absPrec) (src/treebuilders/apply.h:42, apply.cpp:119-139): out gets inp's
+// component count; for every (input, output) component pair with a nonzero
+// metric entry the operator is applied and the result rescaled by the entry.
+// When the selected pairs write distinct output components they run as one
+// apply_batch.  The metric must be real.
+void apply(double prec, CompFunction<3> &out, ConvolutionOperator<3> &oper, const CompFunction<3> &inp,
+           const ComplexDouble (*metric)[4] = defaultMetric, int maxIter = -1, bool absPrec = false);
+// The same over caller-owned trees (out[i], inp[i] for i < ncomp); the
+// output components a nonzero metric entry selects must be present.
+void apply_components(double prec, FunctionTree<3> *const *out, ConvolutionOperator<3> &oper, FunctionTree<3> *const *inp, int ncomp,
+                      const ComplexDouble (*metric)[4], int maxIter = -1, bool absPrec = false);
 
 } // namespace mrcpp

@@ -176,3 +176,21 @@ def test_binary_round_trip(mr, orc, tmp_path):
         mr.FunctionTree(mra).loadTree(base)
     with pytest.raises(mr.MRCPPError):
         mr.FunctionTree(mra).loadTree(str(tmp_path / "missing"))
+
+
+def test_rescale_on_the_host(mr, orc):
+    """FunctionTree::rescale: end nodes times c, bottom-up transform, norms:
+    every coefficient scales by c, norms by |c|, the integral by c."""
+    _, mra, f, _, _ = _projected(mr, orc, order=5, box=16, prec=1e-5)
+    i0, c0, n0 = f.export()
+    q0, s0 = f.integrate(), f.getSquareNorm()
+    f.rescale(-2.5)
+    i1, c1, n1 = f.export()
+    assert np.array_equal(i0, i1)
+    assert np.abs(c1 + 2.5 * c0).max() <= 1e-14 * np.abs(c0).max()
+    # component norms to 1e-12 of the node's norm (recompressed small wavelet
+    # norms are differences of nearly equal values)
+    scale = 2.5 * np.abs(n0[:, :8]).max(axis=1, keepdims=True)
+    assert np.all(np.abs(n1[:, :8] - 2.5 * n0[:, :8]) <= 1e-12 * scale + 1e-15 * scale.max())
+    assert f.integrate() == pytest.approx(-2.5 * q0, rel=1e-13)
+    assert f.getSquareNorm() == pytest.approx(6.25 * s0, rel=1e-13)
