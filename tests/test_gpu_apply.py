@@ -114,6 +114,14 @@ def test_apply_deterministic_and_closed_form(mr):
     assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
     # Coulomb self-energy of the unit Gaussian (beta = 100): sqrt(2 beta / pi)
     assert mr.dot(g, f) == pytest.approx(math.sqrt(200.0 / math.pi), rel=10 * w["prec"])
+    # the potential itself, erf(sqrt(beta) r) / r, to the adaptive precision
+    # relative to the output norm (the reference's own check of this case)
+    from scipy.special import erf
+    scale = 10 * w["prec"] * math.sqrt(g.getSquareNorm())
+    for r in (0.02, 0.05, 0.1, 0.3, 1.0, 3.0, 7.0, 15.0):
+        p = np.array([r, 0.013, -0.021])
+        rr = float(np.linalg.norm(p))
+        assert abs(g.evalf(p) - erf(10.0 * rr) / rr) < scale, r
 
 
 def test_apply_rejects_incompatible_mra(mr):
