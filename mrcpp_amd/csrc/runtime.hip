@@ -75,8 +75,17 @@ bool g_exiting = false; // set by an exit handler
 // drain the stream before it starts the copy)
 unsigned char *g_landing = nullptr;
 size_t g_stage_used = 0;
+// Downloads up to kWriteBackMax bytes: a kernel stores them straight into
+// the mapped landing buffer (no copy engine round trip); larger ones use the
+// copy engine.
+constexpr size_t kWriteBackMax = 1ull << 20;
+unsigned char *g_landing_dev = nullptr; // device address of g_landing, or null
 
 } // namespace
+
+__global__ __launch_bounds__(256) void k_write_back(unsigned *dst, const unsigned *src, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
 
 mwg_status fail(mwg_status s, const std::string &msg) {
     g_err = msg;
@@ -296,8 +305,22 @@ mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
     if (!bytes) return MWG_OK;
     if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
     if (bytes <= kStageMaxCopy) {
-        if (!g_landing) MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_landing), kStageMaxCopy, hipHostMallocDefault));
-        MWG_HIP(hipMemcpyAsync(g_landing, src, bytes, hipMemcpyDeviceToHost, 0));
+        if (!g_landing) {
+            MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_landing), kStageMaxCopy, hipHostMallocMapped | hipHostMallocCoherent));
+            void *dev = nullptr;
+            if (hipHostGetDevicePointer(&dev, g_landing, 0) == hipSuccess) g_landing_dev = static_cast<unsigned char *>(dev);
+            else (void)hipGetLastError();
+        }
+        const bool words = bytes % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 4 == 0;
+        if (g_landing_dev && words && bytes <= kWriteBackMax) {
+            const size_t n = bytes / 4;
+            const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+            hipLaunchKernelGGL(k_write_back, dim3(blocks), dim3(256), 0, 0, reinterpret_cast<unsigned *>(g_landing_dev),
+                               static_cast<const unsigned *>(src), n);
+            MWG_LAUNCHED();
+        } else {
+            MWG_HIP(hipMemcpyAsync(g_landing, src, bytes, hipMemcpyDeviceToHost, 0));
+        }
         MWG_HIP(hipStreamSynchronize(0));
         std::memcpy(dst, g_landing, bytes);
     } else {
