@@ -104,6 +104,9 @@ typedef struct {
     const double *term_norm;  /* [matrix/4] max component norm per (depth, term, dl) */
     long long n_mat;          /* number of matrices in mats */
     const void *frag[3];      /* MFMA operand tables per sweep precision (MWG_PREC_*), NULL until built */
+    const long long *screen_off; /* [n_depth + 1] first entry of depth d in screen_bound: (2 band_max[d] + 1)^3 entries, none if band_max[d] < 0 */
+    const double *screen_bound;  /* [entry][8]: per depth, translation difference and f component, the largest
+                                    per-term tuple bound over terms (NaN when no term reaches it) */
 } mwg_band_tables;
 
 /* Sweep precision of the band sum.  The norm screening, the operator
@@ -121,6 +124,15 @@ enum {
  * OperatorTree::getNode, src/trees/OperatorTree.h:53-57, for the device). */
 size_t mwg_fragment_bytes(int kp1, long long n_mat, int precision);
 mwg_status mwg_build_fragments(const mwg_band_tables *op, int precision, void *dst);
+
+/* Screening table of an operator (op->screen_off set, op->screen_bound
+ * ignored): for depth d, translation difference dl (|dl_i| <= band_max[d])
+ * and f component ft, the largest over terms t with |dl_i| <= width_t of
+ * bs_t * prod_t(ft, dl), the per-term norm bound of the tuple screen of
+ * ConvolutionCalculator::applyOperator (src/treebuilders/
+ * ConvolutionCalculator.cpp:297-329) without the band node's norm, written to
+ * dst[8 * entries].  One table lookup then replaces the loop over terms. */
+mwg_status mwg_build_screen_table(const mwg_band_tables *op, long long entries, double *dst);
 
 /* Input function tree resident on the device: an open-addressing hash from a
  * packed node key to a node id, and per-node coefficient offset, 8 component

@@ -66,6 +66,18 @@ detail::DeviceBandTables *create_band_tables(const detail::BandTables &t) {
     v.band_size_max = d->bandSizeMax.get();
     v.term_norm = d->termNorm.get();
     v.n_mat = (long long)t.matNorm.size();
+    // screening table: (2 band_max[d] + 1)^3 translation differences per depth
+    std::vector<long long> off(t.nDepth + 1, 0);
+    for (int dd = 0; dd < t.nDepth; dd++) {
+        const long long nb = t.bandMax[dd] >= 0 ? 2ll * t.bandMax[dd] + 1 : 0;
+        off[dd + 1] = off[dd] + nb * nb * nb;
+    }
+    d->screenOff.reserve(off.size());
+    d->screenOff.upload(off.data(), off.size());
+    d->screenBound.reserve(std::max<long long>(8 * off.back(), 1));
+    v.screen_off = d->screenOff.get();
+    check(mwg_build_screen_table(&v, off.back(), d->screenBound.get()), "screening table");
+    v.screen_bound = d->screenBound.get();
     return d;
 }
 

@@ -109,6 +109,8 @@ struct DeviceBandTables {
     device::DeviceArray<long long> matStart;
     device::DeviceArray<double> mats, matNorm, pairBound, pairBoundW, termNorm;
     device::DeviceArray<int> bandSizeMax;
+    device::DeviceArray<long long> screenOff;
+    device::DeviceArray<double> screenBound; // largest per-term tuple bound per (depth, dl, ft)
     device::DeviceArray<unsigned char> frag[3]; // MFMA operand tables, built on first use
     mwg_band_tables view{};
     // view with the operand table of this sweep precision built

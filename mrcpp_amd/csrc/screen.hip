@@ -81,47 +81,53 @@ __device__ inline bool screen_position(const_as4<mwg_segment> &S, double g_thr, 
         *code = -(anc + 1);
     }
     if (!(a * maxf * kBoundSlack > g_thr)) return false;
-    // Per-term bound, resolved per f component ft: the direction-d factor of
-    // a tuple is the norm of component 2*gbit_d + ftbit_d, so over all gt it is
-    // at most max(n_d(ftbit_d), n_d(2+ftbit_d)); at depth > 0 the (ft,gt) =
-    // (0,0) tuple is skipped, so some direction carries the gt bit.
-    const int M = S.op.M;
-    const int dl3[3] = {x - gl[0], y - gl[1], z - gl[2]};
-    const int maxdl = max(abs(dl3[0]), max(abs(dl3[1]), abs(dl3[2])));
-    const int *wd = S.op.widths + (size_t)d * M * 5;
-    const long long *ms = S.op.mat_start + (size_t)d * M;
-    const int *bsm = S.op.band_size_max + (size_t)d * M;
-    const double *term_norm = S.op.term_norm, *mat_norm = S.op.mat_norm;
-    for (int i = 0; i < M; i++) {
-        const int *wi = wd + i * 5;
-        const int w = wi[4];
-        if (maxdl > w) continue;
-        // cheap per-term bound first: largest component norm per direction
-        const double *tn = term_norm + ms[i] / 4;
-        const double bs = (double)bsm[i];
-        if (!(bs * tn[dl3[0] + w] * tn[dl3[1] + w] * tn[dl3[2] + w] * maxf * kBoundSlack > g_thr)) continue;
-        const double *mn = mat_norm + ms[i];
-        double n[3][4];
-        for (int dim = 0; dim < 3; dim++)
-            for (int c = 0; c < 4; c++) n[dim][c] = (abs(dl3[dim]) <= wi[c]) ? mn[(dl3[dim] + w) * 4 + c] : 0.0;
-        for (int ft = 0; ft < 8; ft++) {
-            if (fnv[ft] < kMachineZero) continue;
-            double g0[3], g1[3], gm[3];
-            for (int dim = 0; dim < 3; dim++) {
-                const int fb = (ft >> dim) & 1;
-                g0[dim] = n[dim][fb];
-                g1[dim] = n[dim][2 + fb];
-                gm[dim] = fmax(g0[dim], g1[dim]);
-            }
-            double prod;
-            if (d > 0 && ft == 0)
-                prod = fmax(g1[0] * gm[1] * gm[2], fmax(gm[0] * g1[1] * gm[2], gm[0] * gm[1] * g1[2]));
-            else
-                prod = gm[0] * gm[1] * gm[2];
-            if (bs * prod * fnv[ft] * kBoundSlack > g_thr) return true;
-        }
+    // Per-term bound per f component, largest over the terms (tuple_bound,
+    // tabulated by k_screen_table): rounded products are monotone, so the
+    // table entry passes exactly when some term's bound passes.
+    const int bm = S.op.band_max[d], nb = 2 * bm + 1;
+    const double *tb = S.op.screen_bound + 8 * (S.op.screen_off[d] + (x - gl[0] + bm) + nb * ((y - gl[1] + bm) + (long long)nb * (z - gl[2] + bm)));
+    for (int ft = 0; ft < 8; ft++) {
+        if (fnv[ft] < kMachineZero) continue;
+        if (tb[ft] * fnv[ft] * kBoundSlack > g_thr) return true;
     }
     return false;
+}
+
+// Per-term bound of the tuples (f component ft, any g component) of term i at
+// translation difference dl3, without the band node's norm: the direction-d
+// factor of a tuple is the norm of component 2*gbit_d + ftbit_d, so over all
+// gt it is at most max(n_d(ftbit_d), n_d(2+ftbit_d)); at depth > 0 the
+// (ft,gt) = (0,0) tuple is skipped, so some direction carries the gt bit.
+// The band-size factor bs is the row maximum over gt.  Negative when term i
+// does not reach dl3.
+__device__ inline void tuple_bound(const mwg_band_tables &op, int d, int i, const int *dl3, double *out) {
+    const int M = op.M;
+    const int *wi = op.widths + ((size_t)d * M + i) * 5;
+    const int w = wi[4];
+    const int maxdl = max(abs(dl3[0]), max(abs(dl3[1]), abs(dl3[2])));
+    if (maxdl > w) {
+        for (int ft = 0; ft < 8; ft++) out[ft] = -1.0;
+        return;
+    }
+    const double bs = (double)op.band_size_max[(size_t)d * M + i];
+    const double *mn = op.mat_norm + op.mat_start[(size_t)d * M + i];
+    double n[3][4];
+    for (int dim = 0; dim < 3; dim++)
+        for (int c = 0; c < 4; c++) n[dim][c] = (abs(dl3[dim]) <= wi[c]) ? mn[(dl3[dim] + w) * 4 + c] : 0.0;
+    for (int ft = 0; ft < 8; ft++) {
+        double g1[3], gm[3];
+        for (int dim = 0; dim < 3; dim++) {
+            const int fb = (ft >> dim) & 1;
+            g1[dim] = n[dim][2 + fb];
+            gm[dim] = fmax(n[dim][fb], g1[dim]);
+        }
+        double prod;
+        if (d > 0 && ft == 0)
+            prod = fmax(g1[0] * gm[1] * gm[2], fmax(gm[0] * g1[1] * gm[2], gm[0] * gm[1] * g1[2]));
+        else
+            prod = gm[0] * gm[1] * gm[2];
+        out[ft] = bs * prod;
+    }
 }
 
 // Screen blocks per output node of a segment (one for an empty band, so its
@@ -252,6 +258,39 @@ Buffer g_cand_pos, g_cand_code, g_blk, g_seg_blk;
 
 using namespace mwgdev;
 
+// One thread per (depth, translation difference): the largest tuple_bound
+// over the terms for each f component, NaN (which passes no threshold) when
+// no term reaches the difference.
+__global__ __launch_bounds__(256) void k_screen_table(mwg_band_tables op, long long entries, double *out) {
+    const long long e = (long long)blockIdx.x * 256 + threadIdx.x;
+    if (e >= entries) return;
+    int d = 0;
+    while (op.screen_off[d + 1] <= e) d++;
+    const int bm = op.band_max[d], nb = 2 * bm + 1;
+    const long long p = e - op.screen_off[d];
+    const int dl3[3] = {(int)(p % nb) - bm, (int)((p / nb) % nb) - bm, (int)(p / ((long long)nb * nb)) - bm};
+    double best[8];
+    bool any = false;
+    for (int ft = 0; ft < 8; ft++) best[ft] = 0.0;
+    for (int i = 0; i < op.M; i++) {
+        double b[8];
+        tuple_bound(op, d, i, dl3, b);
+        if (b[0] < 0.0) continue;
+        any = true;
+        for (int ft = 0; ft < 8; ft++) best[ft] = fmax(best[ft], b[ft]);
+    }
+    for (int ft = 0; ft < 8; ft++) out[8 * e + ft] = any ? best[ft] : __builtin_nan("");
+}
+
+extern "C" mwg_status mwg_build_screen_table(const mwg_band_tables *op, long long entries, double *dst) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!op || !dst || !op->screen_off) return fail(MWG_ERR_ARG, "null argument");
+    if (entries <= 0) return MWG_OK;
+    hipLaunchKernelGGL(k_screen_table, dim3((unsigned)((entries + 255) / 256)), dim3(256), 0, 0, *op, entries, dst);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+
 __global__ __launch_bounds__(256) void k_patch_pairs(int *pair_f, const int *pair, const int *node, int count) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < count) pair_f[pair[i]] = node[i];
@@ -278,6 +317,8 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     std::vector<int> seg_blk(lvl->nseg + 1, 0);
     long long nblk = 0;
     for (int s = 0; s < lvl->nseg; s++) {
+        if (lvl->segs[s].band_w >= 0 && (!lvl->segs[s].op.screen_off || !lvl->segs[s].op.screen_bound))
+            return fail(MWG_ERR_ARG, "operator screening table not built");
         seg_blk[s] = (int)nblk;
         nblk += (long long)screen_blocks(lvl->segs[s].band_w) * lvl->segs[s].count;
         if (nblk >= (1ll << 31) / kScreenThreads) return fail(MWG_ERR_UNSUPPORTED, "level too large for one screen launch");
