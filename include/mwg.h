@@ -94,7 +94,8 @@ typedef struct {
     int wmax;              /* max band width over depths */
     const int *band_max;   /* [n_depth] */
     const int *widths;     /* [n_depth][M][5]: T,C,B,A,max */
-    const int *band_size;  /* [n_depth][M][64] */
+    const int *band_size;  /* [n_depth][M][64]: gt*8+ft -> 64 x product over directions of (2 width + 1) of component 2 gbit + fbit
+                              (a negative width drops its factor); the screen recomputes it from widths */
     const long long *mat_start; /* [n_depth][M] first matrix (dl=-w, comp 0) or -1 */
     const double *mats;    /* matrices (kp1*kp1 each, column-major: O(i_f, j_g)) */
     const double *mat_norm;/* one norm per matrix */

@@ -3,7 +3,7 @@
 // src/treebuilders/ConvolutionCalculator.cpp:224-382).
 //
 // Pipeline, all on one stream:
-//  1. k_group_screen: one wave per candidate pair; lane t tests term c0 + t
+//  1. k_group_screen: two waves per candidate pair; lane t tests term c0 + t
 //     against every (ft, gt) with the exact norm test and the pair's groups
 //     (operator block, ft, gt mask) are written ft-major to a fixed-stride
 //     list; the pair's translation difference and band node go to a pair
@@ -48,12 +48,15 @@ __device__ inline unsigned code_mask(unsigned c) { return (c >> 23) & 0xffu; }
 __device__ inline bool code_fresh(unsigned c) { return (c >> 31) != 0u; }
 
 // ---------------------------------------------------------------------------
-// 1. exact screening, one wave per pair
+// 1. exact screening, two waves per pair
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr, mwg_pairs pr,
-                                                     unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo) {
+constexpr int kScreenWaves = 2; // waves per pair: wave v tests terms 64 v, 64 v + 1, ... (stride 128)
+
+__global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr,
+                                                                   mwg_pairs pr, unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo) {
     __shared__ unsigned char smask[8 * kMaxTerms];
-    const int lane = threadIdx.x;
+    __shared__ int sb40[kMaxTerms]; // band block index of each term (first matrix / 4 + width)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int p = blockIdx.x;
     const int g = pr.pair_g[p];
     const int fid = pr.pair_f[p];
@@ -62,7 +65,7 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
     const int M = S.op.M;
     const double g_thr = node_thr ? node_thr[g] : S.g_thr;
     if (fid < 0) { // unresolved band node: no contribution (the host generates every node first)
-        if (lane == 0) gcount[p] = 0;
+        if (threadIdx.x == 0) gcount[p] = 0;
         return;
     }
     int dl[3], maxdl = 0;
@@ -70,34 +73,69 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
         dl[i] = pr.pair_l[3 * p + i] - l[3 * g + i];
         maxdl = max(maxdl, abs(dl[i]));
     }
-    if (lane == 0) { // the pair's record for the sweeps
+    if (threadIdx.x == 0) { // the pair's record for the sweeps
         pdl[p] = make_int4(dl[0], dl[1], dl[2], fid);
         pfo[p] = S.f.coef_off[fid];
     }
     double fn[8];
     const double *fnorms = S.f.norms;
     for (int t = 0; t < 8; t++) fn[t] = fnorms[(size_t)fid * 9 + t];
-    const auto wd = ro(S.op.widths + (size_t)d * M * 5);
-    const auto bsz = ro(S.op.band_size + (size_t)d * M * 64);
-    const auto ms = ro(S.op.mat_start + (size_t)d * M);
-    const auto mn = ro(S.op.mat_norm);
+    const int *wd = S.op.widths + (size_t)d * M * 5;
+    const int *bsm = S.op.band_size_max + (size_t)d * M;
+    const long long *ms = S.op.mat_start + (size_t)d * M;
+    const double *mn = S.op.mat_norm;
     unsigned napp = 0;
-    for (int c0 = 0; c0 < M; c0 += 64) {
+    for (int c0 = 64 * wv; c0 < M; c0 += 64 * kScreenWaves) {
         const int i = c0 + lane;
         unsigned m8[8];
 #pragma unroll
         for (int ft = 0; ft < 8; ft++) m8[ft] = 0;
-        if (i < M && maxdl <= wd[i * 5 + 4]) {
-            const auto w = wd + i * 5;
-            const int w4 = w[4];
+        int w[5] = {-1, -1, -1, -1, -1};
+        long long m0 = 0;
+        double bmax = 0.0;
+        if (i < M) {
+#pragma unroll
+            for (int c = 0; c < 5; c++) w[c] = wd[i * 5 + c];
+            m0 = ms[i];
+            bmax = (double)bsm[i];
+            sb40[i] = (int)(m0 / 4 + w[4]);
+        }
+        if (maxdl <= w[4]) { // term i reaches the pair (so i < M and its matrices exist)
+            // the 12 component norms at the pair's translation, all loads
+            // issued before any is used (|dl| <= width: in the term's block)
             double nrm[3][4];
 #pragma unroll
             for (int dim = 0; dim < 3; dim++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) nrm[dim][c] = (abs(dl[dim]) <= w[c]) ? mn[ms[i] + (long long)(dl[dim] + w4) * 4 + c] : -1.0;
+                for (int c = 0; c < 4; c++) nrm[dim][c] = mn[m0 + (long long)(dl[dim] + w[4]) * 4 + c];
+#pragma unroll
+            for (int dim = 0; dim < 3; dim++)
+#pragma unroll
+                for (int c = 0; c < 4; c++)
+                    if (abs(dl[dim]) > w[c]) nrm[dim][c] = -1.0;
+            // band-size factor of component c in one direction (a negative
+            // width drops its factor): band_size[gt][ft] = 64 prod_dim bf(c_dim)
+            int bf[4];
+#pragma unroll
+            for (int c = 0; c < 4; c++) bf[c] = w[c] < 0 ? 1 : 2 * w[c] + 1;
 #pragma unroll
             for (int ft = 0; ft < 8; ft++) {
                 if (fn[ft] < kMachineZero) continue;
+                // bound over the g components of (term, ft): the row maximum
+                // of the band size times the largest component norm per
+                // direction (at depth > 0 the (0,0) tuple is skipped, so some
+                // direction carries the g bit); the slack covers the
+                // reassociated roundings of the exact test below
+                double g1[3], gm[3];
+#pragma unroll
+                for (int dim = 0; dim < 3; dim++) {
+                    const int fb = (ft >> dim) & 1;
+                    g1[dim] = fmax(nrm[dim][2 + fb], 0.0);
+                    gm[dim] = fmax(fmax(nrm[dim][fb], 0.0), g1[dim]);
+                }
+                const double prod = (d > 0 && ft == 0) ? fmax(g1[0] * gm[1] * gm[2], fmax(gm[0] * g1[1] * gm[2], gm[0] * gm[1] * g1[2]))
+                                                       : gm[0] * gm[1] * gm[2];
+                if (!(bmax * prod * fn[ft] * kBoundSlack > g_thr)) continue;
                 unsigned m = 0;
 #pragma unroll
                 for (int gt = 0; gt < 8; gt++) {
@@ -110,7 +148,9 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
                     onorm *= nx;
                     onorm *= ny;
                     onorm *= nz;
-                    const double fthr = bsz[i * 64 + gt * 8 + ft] * fn[ft];
+                    const int bs = 64 * bf[2 * (gt & 1) + (ft & 1)] * bf[2 * ((gt >> 1) & 1) + ((ft >> 1) & 1)] *
+                                   bf[2 * ((gt >> 2) & 1) + ((ft >> 2) & 1)];
+                    const double fthr = bs * fn[ft];
                     if (onorm * fthr > g_thr) m |= 1u << gt;
                 }
                 m8[ft] = m;
@@ -121,7 +161,10 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
 #pragma unroll
             for (int ft = 0; ft < 8; ft++) smask[ft * M + i] = (unsigned char)m8[ft];
     }
+    for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
+    if (lane == 0 && napp) atomicAdd(S.counters, (unsigned long long)napp);
     __syncthreads();
+    if (wv != 0) return;
     // emit ft-major, terms ascending
     int base = 0;
     unsigned *dst = codes + (size_t)p * stride;
@@ -134,18 +177,13 @@ __global__ __launch_bounds__(64) void k_group_screen(const mwg_segment *segs, co
             const unsigned long long bal = __ballot(m != 0);
             if (m) {
                 const int rank = __popcll(bal & below);
-                const long long b40 = ms[i] / 4 + wd[i * 5 + 4];
-                dst[base + rank] = group_code(b40, ft, m, first && rank == 0);
+                dst[base + rank] = group_code(sb40[i], ft, m, first && rank == 0);
             }
             if (bal) first = false;
             base += __popcll(bal);
         }
     }
-    for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
-    if (lane == 0) {
-        gcount[p] = base;
-        if (napp) atomicAdd(S.counters, (unsigned long long)napp);
-    }
+    if (lane == 0) gcount[p] = base;
 }
 
 // ---------------------------------------------------------------------------
@@ -915,7 +953,7 @@ template <int N1> struct BandL {
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
-            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
+            hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64 * kScreenWaves), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
                                w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>());
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
