@@ -866,7 +866,16 @@ __global__ __launch_bounds__(256) void k_reduce_units(const mwg_segment *segs, c
     const int Sg = Ug - nonzero_before(v);
     double acc = dst[o];
     const double *src = partial + (size_t)Sg * 8 * N3 + o;
-    for (int j = 0; j + 1 < nu; j++) acc += src[(size_t)j * 8 * N3];
+    // slots added in unit order; their loads issued 8 at a time
+    int j = 0;
+    for (; j + 8 < nu; j += 8) {
+        double t[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) t[q] = src[(size_t)(j + q) * 8 * N3];
+#pragma unroll
+        for (int q = 0; q < 8; q++) acc += t[q];
+    }
+    for (; j + 1 < nu; j++) acc += src[(size_t)j * 8 * N3];
     dst[o] = acc;
 }
 
