@@ -317,9 +317,13 @@ def run_batch_bench(args):
         with_pred = {"ms_per_step": round(float(np.mean(w_times)) * 1e3, 4), "passes": int(max(st["n_levels"] for st in w_sts)),
                      "n_applied": int(sum(st["n_applied"] for st in w_sts)),
                      "same_node_set_trees": int(sum(np.array_equal(a.export(False)[0], b.export(False)[0]) for a, b in zip(p_outs, w_outs)))}
-    mr.profile(True)
+    mr.profile(True, kinds=("band",))  # events around the band kernel only while timing
     c_outs, c_sts, c_times = timed(chosen_step)
     prof = mr.profile_read()
+    mr.profile(False)
+    mr.profile(True)  # one untimed pass with events around every kernel: the breakdown
+    chosen_step()
+    every = mr.profile_read()
     mr.profile(False)
     flop = sum(st["n_applied"] for st in p_sts) * flops_per_tuple(7)
     c_s = float(np.mean(c_times))
@@ -354,7 +358,7 @@ def run_batch_bench(args):
         "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(c_flop / (band_ms_per_step / 1e3) / 1e12, 4),
                      "peak": FP64_PEAK_TFLOPS if args.precision in ("fp64", "fp64_vector") else 157.3, "unit": "TFLOP/s",
                      "launches_per_step": band_launches / (args.steps + max(args.warmup, 1))},
-        "kernel_ms_per_step": {k: round(v[0] / (args.steps + max(args.warmup, 1)), 4) for k, v in prof.items()},
+        "kernel_ms_per_step": {k: round(v[0], 4) for k, v in every.items()},
         "step_times_s": [round(t, 6) for t in c_times],
     }
     res["roofline"]["frac"] = round(res["roofline"]["achieved"] / res["roofline"]["peak"], 5)
@@ -376,13 +380,15 @@ def load_predictor(mr, order):
 
 
 def time_form(mr, run_one, steps, warmup):
-    """warmup untimed runs, then steps timed runs with per-kernel HIP-event
-    timing (Python's cyclic garbage collector paused while timing)."""
+    """warmup untimed runs, then steps timed runs with HIP events around the
+    band kernel only (Python's cyclic garbage collector paused while timing),
+    then up to 5 untimed runs with events around every kernel for the
+    per-kernel breakdown."""
     import gc
     for _ in range(max(warmup, 1)):
         run_one()
     mr.device_synchronize()
-    mr.profile(True)
+    mr.profile(True, kinds=("band",))
     mr.device_synchronize()
     times, stats = [], []
     out = None
@@ -402,7 +408,14 @@ def time_form(mr, run_one, steps, warmup):
         gc.enable()
     prof = mr.profile_read()
     mr.profile(False)
-    return dict(s=elapsed / steps, times=times, stats=stats, prof=prof, out=out)
+    nb = min(steps, 5)
+    mr.profile(True)
+    for _ in range(nb):
+        run_one()
+    every = mr.profile_read()
+    mr.profile(False)
+    breakdown = {k: v[0] / nb for k, v in every.items()}  # ms per apply
+    return dict(s=elapsed / steps, times=times, stats=stats, prof=prof, out=out, breakdown=breakdown)
 
 
 def roofline(prof, flop, steps, precision, workload):
@@ -496,10 +509,10 @@ def main():
                    "passes": int(cst["n_levels"]), "n_predicted": int(cst.get("n_predicted", 0)),
                    "n_cropped": int(cst.get("n_cropped", 0)), "n_fallback": int(cst.get("n_fallback", 0)),
                    "n_applied": int(cst["n_applied"]), "step_times_s": [round(t, 6) for t in chosen["times"]],
-                   "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in chosen["prof"].items()}},
+                   "kernel_ms_per_step": {k: round(v, 4) for k, v in chosen["breakdown"].items()}},
         "plain_fp64": {"ms_per_step": round(plain["s"] * 1e3, 4), "s_per_tree": plain["s"], "gflops": round(flop / plain["s"] / 1e9, 3),
                        "passes": int(pst["n_levels"]), "step_times_s": [round(t, 6) for t in plain["times"]],
-                       "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in plain["prof"].items()},
+                       "kernel_ms_per_step": {k: round(v, 4) for k, v in plain["breakdown"].items()},
                        "host_breakdown_s": {k: pst[k] for k in ("seconds", "t_setup", "t_screen", "t_generate", "t_band", "t_transform",
                                                                 "t_host", "t_cleanup")}},
         "speedup_vs_plain": round(plain["s"] / chosen["s"], 3),
@@ -518,7 +531,7 @@ def main():
             "n_fallback": int(vst.get("n_fallback", 0)), "same_node_set": vsame,
             "rel_l2": float(np.linalg.norm(c2 - c0) / np.linalg.norm(c0)) if vsame else None,
             "step_times_s": [round(t, 6) for t in variant["times"]],
-            "kernel_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in variant["prof"].items()}}
+            "kernel_ms_per_step": {k: round(v, 4) for k, v in variant["breakdown"].items()}}
     res["projection"] = time_projection(mr, w, src, mra)
     res["arithmetic"] = time_arithmetic(mr, w, f, P)
     res["tree_io"] = time_tree_io(plain["out"])

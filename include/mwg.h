@@ -78,6 +78,10 @@ enum {
     MWG_KERNEL_KINDS = 10
 };
 mwg_status mwg_profile(int enable);
+/* As mwg_profile, bracketing only the launches of the kinds whose bit
+ * (1 << MWG_KERNEL_*) is set in mask (0: off), so that timing one kernel
+ * adds no events to the others. */
+mwg_status mwg_profile_kinds(unsigned mask);
 mwg_status mwg_profile_read(double *ms, long long *launches, int nkinds);
 
 /* Band tables of a convolution operator at one apply precision: widths and

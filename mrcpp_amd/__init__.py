@@ -96,6 +96,7 @@ SIGNATURES = {
     "mwg_device_count": (ctypes.c_int, []),
     "mwg_synchronize": (ctypes.c_int, []),
     "mwg_profile": (ctypes.c_int, [ctypes.c_int]),
+    "mwg_profile_kinds": (ctypes.c_int, [ctypes.c_uint]),
     "mwg_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "mwg_release_pool": (ctypes.c_int, []),
     "mwg_init": (ctypes.c_int, [ctypes.c_int]),
@@ -441,9 +442,17 @@ def device_synchronize():
         raise MRCPPError(lib().mwg_last_error().decode())
 
 
-def profile(enable):
-    """Start (True) or stop recording per-kernel HIP-event timings."""
-    if lib().mwg_profile(int(bool(enable))) != 0:
+def profile(enable, kinds=None):
+    """Start (True) or stop recording per-kernel HIP-event timings; kinds
+    (names from KERNEL_KINDS) limits the events to those kernels."""
+    if kinds is None:
+        rc = lib().mwg_profile(int(bool(enable)))
+    else:
+        mask = 0
+        for k in kinds:
+            mask |= 1 << KERNEL_KINDS.index(k)
+        rc = lib().mwg_profile_kinds(mask if enable else 0)
+    if rc != 0:
         raise MRCPPError(lib().mwg_last_error().decode())
 
 

@@ -14,7 +14,7 @@ namespace {
 
 thread_local std::string g_err;
 bool g_ready = false;
-bool g_profile = false;
+unsigned g_profile = 0; // kinds (bit 1 << kind) whose launches are bracketed by events
 
 struct TimedLaunch {
     int kind;
@@ -96,14 +96,14 @@ mwg_status need_init() { return g_ready ? MWG_OK : fail(MWG_ERR_ARG, "mwg_init h
 
 ProfileScope::ProfileScope(int k)
         : kind(k) {
-    if (!g_profile) return;
+    if (!(g_profile >> k & 1u)) return;
     start = take_event();
     stop = take_event();
     (void)hipEventRecord(start, 0);
 }
 
 ProfileScope::~ProfileScope() {
-    if (!g_profile) return;
+    if (!(g_profile >> kind & 1u)) return;
     (void)hipEventRecord(stop, 0);
     timed_launches().push_back({kind, start, stop});
 }
@@ -219,11 +219,13 @@ mwg_status mwg_free(void *ptr) {
     return MWG_OK;
 }
 
-mwg_status mwg_profile(int enable) {
+mwg_status mwg_profile(int enable) { return mwg_profile_kinds(enable ? ~0u : 0u); }
+
+mwg_status mwg_profile_kinds(unsigned mask) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     MWG_HIP(hipDeviceSynchronize());
     recycle_events();
-    g_profile = enable != 0;
+    g_profile = mask;
     if (g_profile) { // enough events for a few thousand launches, each recorded once so first use costs nothing
         while (event_pool().size() < 8192) {
             hipEvent_t e{};
