@@ -164,24 +164,70 @@ template <int N1> __device__ inline void stage_filters(double *dst, const double
 // in[ft0]^T F(2 b + 0) + in[ft1]^T F(2 b + 1), ft0 < ft1 the two blocks that
 // agree with gt outside direction dim, b = bit dim of gt, in viewed
 // (N1 x N1^2) column-major (math_utils::apply_filter).
-template <int N1> __device__ inline void mw_pass(double *out, const double *in, const double *filt, int dir_base, int dim) {
-    constexpr int N2 = N1 * N1, N3 = N2 * N1;
+// Register tiles of a pass: a thread forms J = N1/2 output columns j of R = 4
+// rows r (2 for N1 = 10, within the register budget), rows N2/R apart so that consecutive lanes read consecutive (padded)
+// rows; every input row element and filter element is read once per tile
+// instead of once per output.
+template <int N1> struct PassTile {
+    static constexpr int N2 = N1 * N1, J = N1 / 2, R = N1 >= 10 ? 2 : 4, RB = N2 / R, TILES = 8 * 2 * RB;
+    static_assert(N1 % 2 == 0 && N2 % R == 0, "pass tile shape");
+};
+
+// s0[p][q] = sum_i in[ft0][r_q][i] F0[j_p][i], s1 likewise with ft1 / F1, the
+// sum over i in increasing order (the order of MRCPP's apply_filter loops).
+template <int N1>
+__device__ inline void pass_tile(const double *in, const double *filt, int dir_base, int dim, int t, int &gt, int &j0, int &rb,
+                                 double (&s0)[N1 / 2][PassTile<N1>::R], double (&s1)[N1 / 2][PassTile<N1>::R]) {
+    using P = PassTile<N1>;
+    constexpr int N2 = P::N2, N3 = N2 * N1, J = P::J, R = P::R, RB = P::RB;
+    gt = t / (2 * RB);
+    j0 = ((t / RB) % 2) * J;
+    rb = t % RB;
     const int bit = 1 << dim;
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
-        const int gt = o / N3, e = o % N3;
-        const int r = e % N2, j = e / N2;
-        const int ft0 = gt & ~bit, ft1 = gt | bit;
-        const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
-        const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
-        const double *x0 = in + pad<N1>(ft0 * N3 + N1 * r);
-        const double *x1 = in + pad<N1>(ft1 * N3 + N1 * r);
-        double s0 = 0.0, s1 = 0.0;
+    const int ft0 = gt & ~bit, ft1 = gt | bit;
+    const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j0;
+    const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j0;
 #pragma unroll
-        for (int i = 0; i < N1; i++) {
-            s0 += x0[i] * F0[i];
-            s1 += x1[i] * F1[i];
+    for (int p = 0; p < J; p++)
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            s0[p][q] = 0.0;
+            s1[p][q] = 0.0;
         }
-        out[pad<N1>(o)] = s0 + s1;
+#pragma unroll 2
+    for (int i = 0; i < N1; i++) {
+        double x0[R], x1[R], f0[J], f1[J];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            x0[q] = in[pad<N1>(ft0 * N3 + N1 * (rb + q * RB) + i)];
+            x1[q] = in[pad<N1>(ft1 * N3 + N1 * (rb + q * RB) + i)];
+        }
+#pragma unroll
+        for (int p = 0; p < J; p++) {
+            f0[p] = F0[N1 * p + i];
+            f1[p] = F1[N1 * p + i];
+        }
+#pragma unroll
+        for (int p = 0; p < J; p++)
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                s0[p][q] += x0[q] * f0[p];
+                s1[p][q] += x1[q] * f1[p];
+            }
+    }
+}
+
+template <int N1> __device__ inline void mw_pass(double *out, const double *in, const double *filt, int dir_base, int dim) {
+    using P = PassTile<N1>;
+    constexpr int N2 = P::N2, N3 = N2 * N1;
+    for (int t = threadIdx.x; t < P::TILES; t += kThreads) {
+        double s0[P::J][P::R], s1[P::J][P::R];
+        int gt, j0, rb;
+        pass_tile<N1>(in, filt, dir_base, dim, t, gt, j0, rb, s0, s1);
+#pragma unroll
+        for (int p = 0; p < P::J; p++)
+#pragma unroll
+            for (int q = 0; q < P::R; q++) out[pad<N1>(gt * N3 + (j0 + p) * N2 + rb + q * P::RB)] = s0[p][q] + s1[p][q];
     }
 }
 
@@ -189,27 +235,22 @@ template <int N1> __device__ inline void mw_pass(double *out, const double *in, 
 // each of the two contributions added separately (in: padded LDS layout).
 template <int N1>
 __device__ inline void mw_pass_add_children(double *coef, long long child_off, long long stride, const double *in, const double *filt) {
-    constexpr int N2 = N1 * N1, N3 = N2 * N1;
-    const int dim = 2, bit = 4;
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
-        const int gt = o / N3, e = o % N3;
-        const int r = e % N2, j = e / N2;
-        const int ft0 = gt & ~bit, ft1 = gt | bit;
-        const double *F0 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j;
-        const double *F1 = filt + (size_t)(4 + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j;
-        const double *x0 = in + pad<N1>(ft0 * N3 + N1 * r);
-        const double *x1 = in + pad<N1>(ft1 * N3 + N1 * r);
-        double s0 = 0.0, s1 = 0.0;
+    using P = PassTile<N1>;
+    constexpr int N2 = P::N2;
+    for (int t = threadIdx.x; t < P::TILES; t += kThreads) {
+        double s0[P::J][P::R], s1[P::J][P::R];
+        int gt, j0, rb;
+        pass_tile<N1>(in, filt, 4, 2, t, gt, j0, rb, s0, s1);
 #pragma unroll
-        for (int i = 0; i < N1; i++) {
-            s0 += x0[i] * F0[i];
-            s1 += x1[i] * F1[i];
-        }
-        double *dst = coef + child_off + (long long)gt * stride + e;
-        double v = *dst;
-        v += s0;
-        v += s1;
-        *dst = v;
+        for (int p = 0; p < P::J; p++)
+#pragma unroll
+            for (int q = 0; q < P::R; q++) {
+                double *dst = coef + child_off + (long long)gt * stride + (j0 + p) * N2 + rb + q * P::RB;
+                double v = *dst;
+                v += s0[p][q];
+                v += s1[p][q];
+                *dst = v;
+            }
     }
 }
 
