@@ -53,8 +53,10 @@ __device__ inline bool code_fresh(unsigned c) { return (c >> 31) != 0u; }
 constexpr int kScreenWaves = 2; // waves per pair: wave v tests terms 64 v, 64 v + 1, ... (stride 128)
 
 __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr,
-                                                                   mwg_pairs pr, unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo) {
+                                                                   mwg_pairs pr, unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo,
+                                                                   int *papp) {
     __shared__ unsigned char smask[8 * kMaxTerms];
+    __shared__ unsigned s_napp[kScreenWaves];
     __shared__ int sb40[kMaxTerms]; // band block index of each term (first matrix / 4 + width)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int p = blockIdx.x;
@@ -65,7 +67,10 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
     const int M = S.op.M;
     const double g_thr = node_thr ? node_thr[g] : S.g_thr;
     if (fid < 0) { // unresolved band node: no contribution (the host generates every node first)
-        if (threadIdx.x == 0) gcount[p] = 0;
+        if (threadIdx.x == 0) {
+            gcount[p] = 0;
+            papp[p] = 0;
+        }
         return;
     }
     int dl[3], maxdl = 0;
@@ -161,10 +166,17 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
 #pragma unroll
             for (int ft = 0; ft < 8; ft++) smask[ft * M + i] = (unsigned char)m8[ft];
     }
+    // applied tuples of the pair (added to the segment's counter per output
+    // node by k_unit_count: one atomic per node instead of one per wave)
     for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
-    if (lane == 0 && napp) atomicAdd(S.counters, (unsigned long long)napp);
+    if (lane == 0) s_napp[wv] = napp;
     __syncthreads();
     if (wv != 0) return;
+    if (lane == 0) {
+        unsigned tot = 0;
+        for (int v = 0; v < kScreenWaves; v++) tot += s_napp[v];
+        papp[p] = (int)tot;
+    }
     // emit ft-major, terms ascending
     int base = 0;
     unsigned *dst = codes + (size_t)p * stride;
@@ -199,11 +211,15 @@ __device__ inline int units_group_size(int total, int slots, int T) {
 
 // Units per output node, packed with a nonzero flag so one scan yields both
 // the first unit and the first partial slot of every node.
-__global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npairs, const int *gstart, int slots, long long *packed, int *counter) {
+__global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npairs, const int *gstart, int slots, long long *packed, int *counter,
+                                                    const mwg_segment *segs, const int *node_seg, const int *papp) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g == 0) *counter = 0;
     if (g >= L) return;
     const int gf = pr.g_first[g], np = pr.g_npairs[g];
+    unsigned long long app = 0; // applied tuples of the node's pairs
+    for (int q = gf; q < gf + np; q++) app += (unsigned)papp[q];
+    if (app) atomicAdd(segs[node_seg[g]].counters, app);
     const int T = gstart[gf + np] - gstart[gf];
     const int G = units_group_size(gstart[npairs], slots, T);
     const int nu = (T + G - 1) / G;
@@ -897,7 +913,7 @@ __global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs
 }
 
 struct BandWork {
-    Buffer codes, gstart, uscan, units, partial, counter, grec, pdl, pfo;
+    Buffer codes, gstart, uscan, units, partial, counter, grec, pdl, pfo, papp;
 };
 BandWork &work() {
     static BandWork w;
@@ -958,16 +974,18 @@ template <int N1> struct BandL {
         if (mfma && w.grec.ensure(sizeof(int2) * (size_t)npairs * stride) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.pdl.ensure(sizeof(int4) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.pfo.ensure(sizeof(long long) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
+        if (w.papp.ensure(sizeof(int) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         int *gstart = w.gstart.as<int>();
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
             hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64 * kScreenWaves), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
-                               w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>());
+                               w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>(), w.papp.as<int>());
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
-            hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>());
+            hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>(),
+                               dev.segs, dev.node_seg, w.papp.as<int>());
             MWG_LAUNCHED();
             s = scan_exclusive(uscan, uscan, L);
             if (s != MWG_OK) return s;
