@@ -17,7 +17,7 @@ namespace mwgdev {
 
 namespace {
 
-constexpr int kTermsPerRound = 24; // Gaussian terms whose 1D factors share LDS at a time
+constexpr int kTermsPerRound = 16; // Gaussian terms whose 1D factors share LDS at a time (LDS <= 80 KB at k = 7: two workgroups per CU)
 
 // out(r, j) = sum_i in(i, r) V(i, j) for each of the 8 blocks (in viewed
 // N1 x N1^2, column-major; core::filter_pass), padded LDS layout.
@@ -44,8 +44,6 @@ __global__ __launch_bounds__(kThreads) void k_project(mwg_projection pj, const i
     __shared__ double gcoef[kTermsPerRound];
     __shared__ double red[8 * (kThreads / 64)];
     __shared__ double sx[N1], svc[N2];
-    __shared__ double sfl[4 * N2]; // compression filters
-    stage_filters<N1>(sfl, pj.filters, 4);
     const int k = blockIdx.x;
     const int n = scale[k];
     const int l[3] = {lv[3 * k], lv[3 * k + 1], lv[3 * k + 2]};
@@ -102,11 +100,11 @@ __global__ __launch_bounds__(kThreads) void k_project(mwg_projection pj, const i
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) b[pad<N1>(o)] *= fac;
     __syncthreads();
     // children's scaling blocks -> the node's scaling and wavelet blocks
-    mw_pass<N1>(a, b, sfl, 0, 0);
+    mw_pass<N1>(a, b, pj.filters, 0, 0); // filters read through the cache
     __syncthreads();
-    mw_pass<N1>(b, a, sfl, 0, 1);
+    mw_pass<N1>(b, a, pj.filters, 0, 1);
     __syncthreads();
-    mw_pass<N1>(a, b, sfl, 0, 2);
+    mw_pass<N1>(a, b, pj.filters, 0, 2);
     __syncthreads();
     double *dst = coef + off[k];
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = a[pad<N1>(o)];
