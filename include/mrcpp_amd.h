@@ -6,6 +6,12 @@
  * call it stands for (paths relative to the MRCPP source tree).  Handles are
  * opaque; every function returning int returns 0 on success and -1 on error,
  * with mrcpp_last_error() describing the failure.
+ *
+ * Threads: every call that may touch the GPU holds one process-wide lock, so
+ * calls from several host threads are served one after the other (results
+ * equal those of the same calls made in sequence).  Settings documented as
+ * "on this thread" (sweep precision, projection and operator targets) and
+ * mrcpp_last_error() are per calling thread.
  */
 #ifndef MRCPP_AMD_H
 #define MRCPP_AMD_H
@@ -22,6 +28,10 @@ void mrcpp_set_filter_dir(const char *dir);
 /* MultiResolutionAnalysis<3>(BoundingBox<3>({lo, hi}), LegendreBasis(order), depth)
  * (src/trees/MultiResolutionAnalysis.h:51-53) */
 int mrcpp_mra_create(int box_lo, int box_hi, int order, int depth, void **mra);
+/* MultiResolutionAnalysis<3>(BoundingBox<3>(scale, corner, nboxes, sf), LegendreBasis(order), depth)
+ * (src/trees/BoundingBox.cpp:100-106, src/trees/MultiResolutionAnalysis.h:53): a world of
+ * nboxes[d] root boxes from translation corner[d] at root scale `scale`; sf NULL means 1.0. */
+int mrcpp_mra_create_box(int scale, const int *corner, const int *nboxes, const double *sf, int order, int depth, void **mra);
 void mrcpp_mra_destroy(void *mra);
 /* root scale, scaling factor and max scale of the world box */
 int mrcpp_mra_info(void *mra, int *root_scale, double *scaling_factor, int *max_scale);

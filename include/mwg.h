@@ -10,6 +10,10 @@
  * counts.  Every call returns MWG_OK or an error code; mwg_last_error()
  * describes the last failure of the calling thread.  Work is enqueued on the
  * library's stream; calls that return host data synchronise it.
+ *
+ * Threads: the device layer's scratch buffers, staging ring and stream are
+ * per process and unlocked; callers serialise (the mrcpp_* ABI of
+ * mrcpp_amd.h and the C++ apply() hold a process-wide lock for that).
  */
 #ifndef MWG_H
 #define MWG_H
@@ -187,6 +191,15 @@ typedef struct {
     const double *node_thr;  /* [count] screening threshold of each node, or NULL for its segment's g_thr
                                 (prec * precFunc(idx) * sqrt(|g|^2 / M) of apply with precTrees,
                                 src/treebuilders/ConvolutionCalculator.cpp:241-246) */
+    /* Optional screening margins (NULL: not computed; no reference
+     * counterpart).  For node i and j < 4: g_margin[4 i + j] = sum over the
+     * node's applied tuples whose test value |Ox||Oy||Oz| * bs * |f_ft| is at
+     * most margin_ratio[j] times the node's threshold, of |Ox||Oy||Oz| |f_ft|
+     * (Frobenius norms).  It bounds the 2-norm of what those tuples add to the
+     * node, i.e. how far the node can move when it is recomputed with a
+     * threshold up to margin_ratio[j] times higher. */
+    double *g_margin;        /* [count][4] */
+    double margin_ratio[4];
 } mwg_level;
 
 /* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the

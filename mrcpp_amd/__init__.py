@@ -34,6 +34,7 @@ SIGNATURES = {
     "mrcpp_last_error": (ctypes.c_char_p, []),
     "mrcpp_set_filter_dir": (None, [ctypes.c_char_p]),
     "mrcpp_mra_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "mrcpp_mra_create_box": (ctypes.c_int, [ctypes.c_int, _ip, _ip, _dp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
     "mrcpp_mra_destroy": (None, [_vp]),
     "mrcpp_mra_info": (ctypes.c_int, [_vp, _ip, _dp, _ip]),
     "mrcpp_tree_create": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
@@ -175,13 +176,21 @@ class GaussExp:
 
 
 class MultiResolutionAnalysis:
-    """MultiResolutionAnalysis<3>(BoundingBox<3>({lo, hi}), LegendreBasis(order), depth)."""
+    """MultiResolutionAnalysis<3>(BoundingBox<3>({lo, hi}), LegendreBasis(order), depth), or with
+    box = (scale, corner, nboxes[, sf]) the world BoundingBox<3>(scale, corner, nboxes, sf):
+    nboxes[d] root boxes from translation corner[d] at root scale `scale`."""
 
     def __init__(self, box, order, depth=30):
-        self.box = (int(box[0]), int(box[1]))
         self.order = int(order)
         h = _vp()
-        _check(lib().mrcpp_mra_create(self.box[0], self.box[1], self.order, int(depth), ctypes.byref(h)))
+        if len(box) == 2:
+            self.box = (int(box[0]), int(box[1]))
+            _check(lib().mrcpp_mra_create(self.box[0], self.box[1], self.order, int(depth), ctypes.byref(h)))
+        else:
+            scale, corner, nb = int(box[0]), np.asarray(box[1], dtype=np.int32), np.asarray(box[2], dtype=np.int32)
+            sf = np.asarray(box[3] if len(box) > 3 else (1.0, 1.0, 1.0), dtype=np.float64)
+            self.box = (scale, tuple(corner.tolist()), tuple(nb.tolist()), tuple(sf.tolist()))
+            _check(lib().mrcpp_mra_create_box(scale, _p(corner, _ip), _p(nb, _ip), _p(sf), self.order, int(depth), ctypes.byref(h)))
         self._h = h
         n0, sf, ms = ctypes.c_int(), ctypes.c_double(), ctypes.c_int()
         _check(lib().mrcpp_mra_info(h, ctypes.byref(n0), ctypes.byref(sf), ctypes.byref(ms)))

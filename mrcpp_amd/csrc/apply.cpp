@@ -34,7 +34,7 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 struct Scratch {
     device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, gen, missing, patch;
     device::DeviceArray<long long> pOff, cOff;
-    device::DeviceArray<double> gNorms;
+    device::DeviceArray<double> gNorms, gMargin;
     device::DeviceArray<long long> lvl; // a level's inputs in one upload: g_off | node thresholds | translations
     device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
@@ -170,7 +170,11 @@ struct Job {
     PredictorContext ctx;
     std::vector<int> replayWork; // nodes of the breadth-first step being judged (computed)
     std::vector<char> speculative; // computed ahead of its step (with an earlier step's threshold)
-    bool edgeChecked = false;      // knife-edge nodes of the current replay step recomputed
+    bool edgeChecked = false;      // uncertain nodes of the current replay step recomputed
+    bool marginPass = false;       // the next pass computes speculative nodes: record their screening margins
+    std::vector<std::array<double, 4>> margin; // per node: screening margins of the pass that computed it
+    std::vector<double> thrUsed;               // per node: the screening threshold it was computed with
+    double normErr = 0.0; // bound on |approximate tree norm - the plain method's| from kept speculative nodes
     // precision trees: precFunc of every output node (by id, -1 until needed)
     std::unique_ptr<PrecFactors> precFunc;
     std::vector<double> precFac;
@@ -196,10 +200,11 @@ void eval_prec_factors(Job &job) {
         if (job.precFac[u] < 0.0) job.precFac[u] = (*job.precFunc)(g.scale(u), g.transl(u));
 }
 
-// Relative distance from the split threshold within which a node computed
-// ahead of its step is recomputed with its own step's screening threshold
-// before the split is decided.
-constexpr double kKnifeEdge = 1e-7;
+// Threshold ratios of the screening margins recorded for speculative nodes
+// (mwg_level::margin_ratio): a node computed with threshold t and judged at a
+// step whose own threshold t' satisfies t' <= kMarginRatio[j] t differs from
+// its recomputation at t' by at most its margin j in 2-norm.
+constexpr double kMarginRatio[4] = {1.02, 1.1, 1.5, 4.0};
 
 struct Shape {
     int kp1, n3, ncoef;
@@ -291,8 +296,11 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
             at++;
         }
     }
-    bool perNode = false;
-    for (const auto &sp : specs) perNode = perNode || jobs[sp.job].precFunc != nullptr;
+    bool perNode = false, margins = false;
+    for (const auto &sp : specs) {
+        perNode = perNode || jobs[sp.job].precFunc != nullptr;
+        margins = margins || jobs[sp.job].marginPass;
+    }
     // one upload: g_off [L] | node thresholds [L] (precision trees only) | translations [3L ints]
     const size_t nThr = perNode ? (size_t)L : 0, nL8 = (3 * (size_t)L + 1) / 2;
     std::vector<long long> packed((size_t)L + nThr + nL8, 0);
@@ -319,6 +327,11 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.node_thr = perNode ? reinterpret_cast<const double *>(S.lvl.get() + L) : nullptr;
     lvl.l = reinterpret_cast<const int *>(S.lvl.get() + L + nThr);
     lvl.g_norms = S.gNorms.get();
+    if (margins) {
+        S.gMargin.reserve(4 * (size_t)L);
+        lvl.g_margin = S.gMargin.get();
+        for (int j = 0; j < 4; j++) lvl.margin_ratio[j] = kMarginRatio[j];
+    }
 
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0;
@@ -399,8 +412,9 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     t0 = clk::now();
     // band sums, output coefficients and norms (zero when no pair survives)
     device::check(mwg_band_apply(&lvl, &pr, npairs, (int)g_sweep), "band apply");
-    std::vector<double> nrm(9 * (size_t)L);
+    std::vector<double> nrm(9 * (size_t)L), mgn(margins ? 4 * (size_t)L : 0);
     S.gNorms.download(nrm.data(), nrm.size());
+    if (margins) S.gMargin.download(mgn.data(), mgn.size());
     const double tBand = since(t0);
     std::set<int> touched;
     for (const auto &sp : specs) touched.insert(sp.job);
@@ -411,12 +425,21 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     }
     at = 0;
     for (const auto &sp : specs) {
-        detail::NodeStore<3> &g = *jobs[sp.job].g;
+        Job &job = jobs[sp.job];
+        detail::NodeStore<3> &g = *job.g;
+        if (job.marginPass) {
+            job.margin.resize(g.size());
+            job.thrUsed.resize(g.size(), -1.0);
+        }
         for (int u : sp.nodes) {
             double *cn = g.norms(u);
             for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)at + t];
             g.setSquareNorm(u, nrm[9 * (size_t)at + 8]);
             g.setHasCoefs(u, true);
+            if (job.marginPass) {
+                for (int j = 0; j < 4; j++) job.margin[u][j] = mgn[4 * (size_t)at + j];
+                job.thrUsed[u] = job.precFunc ? node_threshold(job, u) : job.gthr;
+            }
             at++;
         }
     }
@@ -486,6 +509,15 @@ void transform_and_norms(Job &job, const Shape &sh) {
 }
 
 
+// TreeBuilder::calcWaveletNorm (src/treebuilders/TreeBuilder.cpp:156-163):
+// the wavelet square norms of one step's nodes at depth >= 0, in work order.
+double step_wavelet_norm(const detail::NodeStore<3> &g, const std::vector<int> &work) {
+    double w = 0.0;
+    for (int u : work)
+        if (g.depth(u) >= 0) w += g.waveletNorm(u);
+    return w;
+}
+
 // TreeBuilder::build bookkeeping after a step: approximate tree norm and
 // WaveletAdaptor splits; returns the next work vector.
 void finish_step(Job &job) {
@@ -496,8 +528,9 @@ void finish_step(Job &job) {
         for (int u : job.work)
             if (g.depth(u) >= 0) job.sNorm += g.scalingNorm(u);
     }
-    for (int u : job.work)
-        if (g.depth(u) >= 0) job.wNorm += g.waveletNorm(u);
+    // the step's wavelet norms summed first, then added once
+    // (wNorm += calcWaveletNorm(*workVec), src/treebuilders/TreeBuilder.cpp:58)
+    job.wNorm += step_wavelet_norm(g, job.work);
     g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
     if (job.iter >= job.task.maxIter && job.task.maxIter >= 0) job.work.clear();
     std::vector<int> next;
@@ -571,6 +604,7 @@ void pregrow(Job &job) {
     }
     job.work.swap(all);
     job.mode = Mode::Replay;
+    job.marginPass = true;
 }
 
 long long subtree_size(const detail::NodeStore<3> &g, int u) {
@@ -588,6 +622,24 @@ long long subtree_size(const detail::NodeStore<3> &g, int u) {
     return n;
 }
 
+// Bound on how far speculative node u (computed with threshold thrUsed[u])
+// can be from its computation with its own step's screening threshold: the
+// tuples the higher threshold drops each add at most |Ox||Oy||Oz| |f_ft| in
+// 2-norm (operator norms are Frobenius, which bound the spectral norm), and the
+// margin sums exactly those over every tuple below ratio * thrUsed.  Infinite
+// when no recorded ratio covers the step's threshold.
+double speculative_bound(const Job &job, int u) {
+    const double used = job.thrUsed[u], own = node_threshold(job, u);
+    if (!(used > 0.0) || !(own >= used)) return HUGE_VAL;
+    const double ratio = own / used;
+    if (ratio == 1.0) return 0.0; // the same threshold: the same tuples
+    for (int j = 0; j < 4; j++)
+        if (kMarginRatio[j] >= ratio * (1.0 + 1e-12)) return job.margin[u][j];
+    return HUGE_VAL;
+}
+
+bool is_speculative(const Job &job, int u) { return u < (int)job.speculative.size() && job.speculative[u]; }
+
 // The breadth-first steps of TreeBuilder::build replayed on computed nodes:
 // the wavelet test decides every split with the running approximate norm;
 // predicted children it confirms are kept, the others removed; children it
@@ -595,20 +647,33 @@ long long subtree_size(const detail::NodeStore<3> &g, int u) {
 void replay(Job &job) {
     detail::NodeStore<3> &g = *job.g;
     const int maxScale = job.mra->getMaxScale();
+    job.marginPass = false;
     job.work.clear();
     while (!job.replayWork.empty()) {
+        // speculative nodes of this step and their bounds (own threshold:
+        // the approximate norm after the previous step, still in treeSquareNorm)
+        std::vector<std::pair<int, double>> spec;
+        for (int u : job.replayWork)
+            if (is_speculative(job, u)) spec.push_back({u, speculative_bound(job, u)});
+        auto err_of = [&](int u, double B) { return g.depth(u) >= 0 ? (2.0 * std::sqrt(g.waveletNorm(u)) + B) * B : 0.0; };
         if (!job.edgeChecked) {
-            // nodes computed ahead whose split decision sits on the threshold:
-            // recompute them with this step's screening threshold first
-            double w = job.wNorm;
-            for (int u : job.replayWork)
-                if (g.depth(u) >= 0) w += g.waveletNorm(u);
+            // A speculative node whose split could differ from the one its
+            // recomputation would get is recomputed with this step's screening
+            // threshold first: its wavelet norm is within B of the recomputed
+            // one, and the split threshold within dthr of the plain method's
+            // (the approximate norm holds every kept speculative node's error).
+            const double w = job.wNorm + step_wavelet_norm(g, job.replayWork);
             const double norm = (job.sNorm < 0.0 || w < 0.0) ? -1.0 : job.sNorm + w;
+            double err = job.normErr;
+            for (const auto &sb : spec) err += err_of(sb.first, sb.second);
             std::vector<int> edge;
-            for (int u : job.replayWork) {
-                if (u >= (int)job.speculative.size() || !job.speculative[u]) continue;
+            for (const auto &sb : spec) {
+                const int u = sb.first;
+                const double B = sb.second;
                 const double thr = detail::wavelet_threshold(node_prec(job, u), norm, job.task.absPrec, g.scale(u));
-                if (std::fabs(std::sqrt(g.waveletNorm(u)) / thr - 1.0) < kKnifeEdge) edge.push_back(u);
+                const double dthr = (!job.task.absPrec && norm > 0.0) ? thr * (std::sqrt(1.0 + err / norm) - 1.0) : 0.0;
+                if (!std::isfinite(B) || !std::isfinite(dthr) || std::fabs(std::sqrt(g.waveletNorm(u)) - thr) <= B * (1.0 + 1e-9) + dthr + 1e-12 * thr)
+                    edge.push_back(u);
             }
             job.edgeChecked = true;
             if (!edge.empty()) {
@@ -619,8 +684,9 @@ void replay(Job &job) {
             }
         }
         job.edgeChecked = false;
-        for (int u : job.replayWork)
-            if (g.depth(u) >= 0) job.wNorm += g.waveletNorm(u);
+        for (const auto &sb : spec)
+            if (is_speculative(job, sb.first)) job.normErr += err_of(sb.first, sb.second);
+        job.wNorm += step_wavelet_norm(g, job.replayWork);
         g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
         const bool stop = job.iter >= job.task.maxIter && job.task.maxIter >= 0;
         std::vector<int> next, pending;
@@ -687,6 +753,7 @@ void set_sweep_precision(SweepPrecision p) { g_sweep = p; }
 SweepPrecision sweep_precision() { return g_sweep; }
 
 void apply_batch(const std::vector<ApplyTask> &tasks) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
     const auto t_start = clk::now();
     g_batch_stats.clear();
     if (tasks.empty()) return;

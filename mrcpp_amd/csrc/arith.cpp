@@ -147,8 +147,12 @@ void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVec
             for (int u : work)
                 if (g.depth(u) >= 0) sNorm += g.scalingNorm(u);
         }
+        // the step's wavelet norms summed first, then added once
+        // (wNorm += calcWaveletNorm(*workVec), src/treebuilders/TreeBuilder.cpp:58)
+        double stepW = 0.0;
         for (int u : work)
-            if (g.depth(u) >= 0) wNorm += g.waveletNorm(u);
+            if (g.depth(u) >= 0) stepW += g.waveletNorm(u);
+        wNorm += stepW;
         g.treeSquareNorm = (sNorm < 0.0 || wNorm < 0.0) ? -1.0 : sNorm + wNorm;
         if (iter >= maxIter && maxIter >= 0) work.clear();
         std::vector<int> next;

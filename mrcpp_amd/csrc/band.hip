@@ -52,11 +52,19 @@ __device__ inline bool code_fresh(unsigned c) { return (c >> 31) != 0u; }
 // ---------------------------------------------------------------------------
 constexpr int kScreenWaves = 2; // waves per pair: wave v tests terms 64 v, 64 v + 1, ... (stride 128)
 
+// Screening margins (optional): ratios r_j of the node threshold below which
+// an applied tuple's |O| |f_ft| is summed into margin bin j.
+struct MarginSpec {
+    double ratio[4];
+    double *pair; // [pair][4], nullptr: margins not computed
+};
+
 __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr,
                                                                    mwg_pairs pr, unsigned *codes, int stride, int *gcount, int4 *pdl, long long *pfo,
-                                                                   int *papp) {
+                                                                   int *papp, MarginSpec mg) {
     __shared__ unsigned char smask[8 * kMaxTerms];
     __shared__ unsigned s_napp[kScreenWaves];
+    __shared__ double s_mg[kScreenWaves][4];
     __shared__ int sb40[kMaxTerms]; // band block index of each term (first matrix / 4 + width)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int p = blockIdx.x;
@@ -71,8 +79,12 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
             gcount[p] = 0;
             papp[p] = 0;
         }
+        if (mg.pair && threadIdx.x < 4) mg.pair[4 * (size_t)p + threadIdx.x] = 0.0;
         return;
     }
+    double mg_thr[4], mg_sum[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int j = 0; j < 4; j++) mg_thr[j] = mg.ratio[j] * g_thr;
     int dl[3], maxdl = 0;
     for (int i = 0; i < 3; i++) {
         dl[i] = pr.pair_l[3 * p + i] - l[3 * g + i];
@@ -156,7 +168,16 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
                     const int bs = 64 * bf[2 * (gt & 1) + (ft & 1)] * bf[2 * ((gt >> 1) & 1) + ((ft >> 1) & 1)] *
                                    bf[2 * ((gt >> 2) & 1) + ((ft >> 2) & 1)];
                     const double fthr = bs * fn[ft];
-                    if (onorm * fthr > g_thr) m |= 1u << gt;
+                    const double val = onorm * fthr;
+                    if (val > g_thr) {
+                        m |= 1u << gt;
+                        if (mg.pair) {
+                            const double c = onorm * fn[ft];
+#pragma unroll
+                            for (int j = 0; j < 4; j++)
+                                if (val <= mg_thr[j]) mg_sum[j] += c;
+                        }
+                    }
                 }
                 m8[ft] = m;
                 napp += __popc(m);
@@ -170,12 +191,25 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
     // node by k_unit_count: one atomic per node instead of one per wave)
     for (int off = 32; off > 0; off >>= 1) napp += __shfl_xor(napp, off);
     if (lane == 0) s_napp[wv] = napp;
+    if (mg.pair) { // fixed-order tree over lanes, then waves in order
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            double x = mg_sum[j];
+            for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+            if (lane == 0) s_mg[wv][j] = x;
+        }
+    }
     __syncthreads();
     if (wv != 0) return;
     if (lane == 0) {
         unsigned tot = 0;
         for (int v = 0; v < kScreenWaves; v++) tot += s_napp[v];
         papp[p] = (int)tot;
+    }
+    if (mg.pair && lane < 4) {
+        double x = 0.0;
+        for (int v = 0; v < kScreenWaves; v++) x += s_mg[v][lane];
+        mg.pair[4 * (size_t)p + lane] = x;
     }
     // emit ft-major, terms ascending
     int base = 0;
@@ -212,7 +246,7 @@ __device__ inline int units_group_size(int total, int slots, int T) {
 // Units per output node, packed with a nonzero flag so one scan yields both
 // the first unit and the first partial slot of every node.
 __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npairs, const int *gstart, int slots, long long *packed, int *counter,
-                                                    const mwg_segment *segs, const int *node_seg, const int *papp) {
+                                                    const mwg_segment *segs, const int *node_seg, const int *papp, const double *pmg, double *g_margin) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g == 0) *counter = 0;
     if (g >= L) return;
@@ -220,6 +254,14 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
     unsigned long long app = 0; // applied tuples of the node's pairs
     for (int q = gf; q < gf + np; q++) app += (unsigned)papp[q];
     if (app) atomicAdd(segs[node_seg[g]].counters, app);
+    if (g_margin) { // the node's screening margins, its pairs in order
+        double m[4] = {0.0, 0.0, 0.0, 0.0};
+        for (int q = gf; q < gf + np; q++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) m[j] += pmg[4 * (size_t)q + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) g_margin[4 * (size_t)g + j] = m[j];
+    }
     const int T = gstart[gf + np] - gstart[gf];
     const int G = units_group_size(gstart[npairs], slots, T);
     const int nu = (T + G - 1) / G;
@@ -902,18 +944,20 @@ __global__ __launch_bounds__(kThreads) void k_level_norms(const mwg_segment *seg
     block_norms<N1>(node_out(segs, node_seg, g_off, g), g_norms + (size_t)g * 9, red);
 }
 
-// A level without candidate pairs: zero coefficients and norms.
+// A level without candidate pairs: zero coefficients, norms and margins.
 template <int N1>
-__global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs, const int *node_seg, const long long *g_off, double *g_norms) {
+__global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs, const int *node_seg, const long long *g_off, double *g_norms,
+                                                         double *g_margin) {
     constexpr int N3 = N1 * N1 * N1;
     const int g = blockIdx.x;
     double *dst = node_out(segs, node_seg, g_off, g);
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = 0.0;
     if (threadIdx.x < 9) g_norms[(size_t)g * 9 + threadIdx.x] = 0.0;
+    if (g_margin && threadIdx.x < 4) g_margin[(size_t)g * 4 + threadIdx.x] = 0.0;
 }
 
 struct BandWork {
-    Buffer codes, gstart, uscan, units, partial, counter, grec, pdl, pfo, papp;
+    Buffer codes, gstart, uscan, units, partial, counter, grec, pdl, pfo, papp, pmg;
 };
 BandWork &work() {
     static BandWork w;
@@ -953,7 +997,7 @@ template <int N1> struct BandL {
         if (st != MWG_OK) return st;
         if (npairs == 0) {
             ProfileScope prof_(MWG_KERNEL_REDUCE);
-            hipLaunchKernelGGL(k_zero_level<N1>, dim3(L), dim3(kThreads), 0, 0, dev.segs, dev.node_seg, lvl->g_off, lvl->g_norms);
+            hipLaunchKernelGGL(k_zero_level<N1>, dim3(L), dim3(kThreads), 0, 0, dev.segs, dev.node_seg, lvl->g_off, lvl->g_norms, lvl->g_margin);
             MWG_LAUNCHED();
             return MWG_OK;
         }
@@ -975,17 +1019,23 @@ template <int N1> struct BandL {
         if (w.pdl.ensure(sizeof(int4) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.pfo.ensure(sizeof(long long) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         if (w.papp.ensure(sizeof(int) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
+        MarginSpec mg{};
+        if (lvl->g_margin) {
+            if (w.pmg.ensure(sizeof(double) * 4 * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
+            for (int j = 0; j < 4; j++) mg.ratio[j] = lvl->margin_ratio[j];
+            mg.pair = w.pmg.as<double>();
+        }
         int *gstart = w.gstart.as<int>();
         long long *uscan = w.uscan.as<long long>();
         {
             ProfileScope prof_(MWG_KERNEL_GROUPS);
             hipLaunchKernelGGL(k_group_screen, dim3(npairs), dim3(64 * kScreenWaves), 0, 0, dev.segs, dev.node_seg, lvl->l, lvl->node_thr, *pr,
-                               w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>(), w.papp.as<int>());
+                               w.codes.as<unsigned>(), stride, gstart, w.pdl.as<int4>(), w.pfo.as<long long>(), w.papp.as<int>(), mg);
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
             hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>(),
-                               dev.segs, dev.node_seg, w.papp.as<int>());
+                               dev.segs, dev.node_seg, w.papp.as<int>(), mg.pair, lvl->g_margin);
             MWG_LAUNCHED();
             s = scan_exclusive(uscan, uscan, L);
             if (s != MWG_OK) return s;

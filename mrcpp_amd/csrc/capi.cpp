@@ -12,7 +12,10 @@ using namespace mrcpp;
 namespace {
 thread_local std::string g_error;
 
+// Every entry point runs under the process-wide device lock: two host threads
+// calling in at once (ctypes releases the GIL) are served one after the other.
 template <typename F> int guarded(F &&f) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
     try {
         f();
         return 0;
@@ -33,6 +36,22 @@ int mrcpp_mra_create(int box_lo, int box_hi, int order, int depth, void **mra) {
     return guarded([&] { *mra = new MultiResolutionAnalysis<3>({box_lo, box_hi}, order, depth); });
 }
 
+int mrcpp_mra_create_box(int scale, const int *corner, const int *nboxes, const double *sf, int order, int depth, void **mra) {
+    return guarded([&] {
+        if (!corner || !nboxes) throw Error("null box argument");
+        std::array<int, 3> c, nb;
+        std::array<double, 3> s;
+        for (int d = 0; d < 3; d++) {
+            if (nboxes[d] < 1) throw Error("Invalid number of root boxes");
+            c[d] = corner[d];
+            nb[d] = nboxes[d];
+            s[d] = sf ? sf[d] : 1.0;
+            if (!(s[d] > 0.0)) throw Error("Invalid scaling factor");
+        }
+        *mra = new MultiResolutionAnalysis<3>(BoundingBox<3>(scale, c, nb, s), LegendreBasis(order), depth);
+    });
+}
+
 void mrcpp_mra_destroy(void *mra) { delete static_cast<MultiResolutionAnalysis<3> *>(mra); }
 
 int mrcpp_mra_info(void *mra, int *root_scale, double *scaling_factor, int *max_scale) {
@@ -48,7 +67,10 @@ int mrcpp_tree_create(void *mra, void **tree) {
     return guarded([&] { *tree = new FunctionTree<3>(*static_cast<MultiResolutionAnalysis<3> *>(mra)); });
 }
 
-void mrcpp_tree_destroy(void *tree) { delete static_cast<FunctionTree<3> *>(tree); }
+void mrcpp_tree_destroy(void *tree) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
+    delete static_cast<FunctionTree<3> *>(tree);
+}
 
 static GaussExp<3> gauss_sum(int n, const double *coefs, const double *alphas, const double *centers) {
     GaussExp<3> ge;
@@ -146,11 +168,20 @@ int mrcpp_tree_load_txt(void *tree, const char *file) {
     return guarded([&] { static_cast<FunctionTree<3> *>(tree)->loadTreeTXT(file); });
 }
 
-double mrcpp_tree_evalf(void *tree, double x, double y, double z) { return static_cast<FunctionTree<3> *>(tree)->evalf({x, y, z}); }
+double mrcpp_tree_evalf(void *tree, double x, double y, double z) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
+    return static_cast<FunctionTree<3> *>(tree)->evalf({x, y, z});
+}
 
-double mrcpp_tree_integrate(void *tree) { return static_cast<FunctionTree<3> *>(tree)->integrate(); }
+double mrcpp_tree_integrate(void *tree) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
+    return static_cast<FunctionTree<3> *>(tree)->integrate();
+}
 
-double mrcpp_tree_dot(void *bra, void *ket) { return dot(*static_cast<FunctionTree<3> *>(bra), *static_cast<FunctionTree<3> *>(ket)); }
+double mrcpp_tree_dot(void *bra, void *ket) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
+    return dot(*static_cast<FunctionTree<3> *>(bra), *static_cast<FunctionTree<3> *>(ket));
+}
 
 int mrcpp_operator_create(void *mra, int kind, double mu, double prec, void **oper) {
     return guarded([&] {
@@ -164,7 +195,10 @@ int mrcpp_operator_create(void *mra, int kind, double mu, double prec, void **op
     });
 }
 
-void mrcpp_operator_destroy(void *oper) { delete static_cast<ConvolutionOperator<3> *>(oper); }
+void mrcpp_operator_destroy(void *oper) {
+    std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
+    delete static_cast<ConvolutionOperator<3> *>(oper);
+}
 
 int mrcpp_operator_size(void *oper) { return static_cast<ConvolutionOperator<3> *>(oper)->size(); }
 

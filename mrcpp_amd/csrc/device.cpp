@@ -24,6 +24,11 @@ mwg_status init_status() {
 }
 } // namespace
 
+std::recursive_mutex &api_mutex() {
+    static std::recursive_mutex *m = new std::recursive_mutex(); // never destroyed
+    return *m;
+}
+
 void ensure_init() { check(init_status(), "device initialisation"); }
 
 bool available() { return init_status() == MWG_OK; }

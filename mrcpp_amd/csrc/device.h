@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -16,6 +17,11 @@ namespace mrcpp {
 namespace device {
 
 void ensure_init();
+// Process-wide lock of the device path.  The device layer keeps its scratch
+// buffers, staging ring and stream per process, so calls that use it run one
+// at a time: every mrcpp_* C ABI entry point and apply_batch() hold this
+// (recursive) lock for their whole call.
+std::recursive_mutex &api_mutex();
 // whether a GPU is present and the device layer initialised
 bool available();
 void check(mwg_status s, const char *what);

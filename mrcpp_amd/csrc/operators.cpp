@@ -349,8 +349,10 @@ std::array<double, 3> build_trees_device(const std::vector<detail::OperatorTerm 
                 for (int u : S.work)
                     if (ot.depth(u) >= 0) S.sNorm += ot.scalingNorm(u);
             }
+            double stepW = 0.0; // the step's sum first, then added once (TreeBuilder.cpp:58)
             for (int u : S.work)
-                if (ot.depth(u) >= 0) S.wNorm += ot.waveletNorm(u);
+                if (ot.depth(u) >= 0) stepW += ot.waveletNorm(u);
+            S.wNorm += stepW;
             ot.treeSquareNorm = (S.sNorm < 0.0 || S.wNorm < 0.0) ? -1.0 : S.sNorm + S.wNorm;
             std::vector<int> next;
             const int maxScale = ot.mra().getMaxScale();

@@ -685,8 +685,12 @@ template <int D> std::vector<int> build_tree(NodeStore<D> &s, const BuildHooks<D
             for (int u : work)
                 if (s.depth(u) >= 0) sNorm += s.scalingNorm(u);
         }
+        // the step's wavelet norms summed first, then added once
+        // (wNorm += calcWaveletNorm(*workVec), src/treebuilders/TreeBuilder.cpp:58)
+        double stepW = 0.0;
         for (int u : work)
-            if (s.depth(u) >= 0) wNorm += s.waveletNorm(u);
+            if (s.depth(u) >= 0) stepW += s.waveletNorm(u);
+        wNorm += stepW;
         s.treeSquareNorm = (sNorm < 0.0 || wNorm < 0.0) ? -1.0 : sNorm + wNorm;
         if (iter >= maxIter && maxIter >= 0) work.clear();
         // TreeAdaptor::splitNodeVector (src/treebuilders/TreeAdaptor.h:41-52)

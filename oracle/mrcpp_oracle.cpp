@@ -917,8 +917,12 @@ template <int D> static void build(Tree<D> &t, Calculator<D> &calc, Adaptor<D> &
             for (Node<D> *nd : work)
                 if (nd->depth() >= 0) sNorm += nd->scalingNorm();
         }
+        // the step's wavelet norms summed first, then added once
+        // (wNorm += calcWaveletNorm(*workVec), src/treebuilders/TreeBuilder.cpp:58,156-163)
+        double stepW = 0.0;
         for (Node<D> *nd : work)
-            if (nd->depth() >= 0) wNorm += nd->waveletNorm();
+            if (nd->depth() >= 0) stepW += nd->waveletNorm();
+        wNorm += stepW;
         if (sNorm < 0.0 or wNorm < 0.0)
             t.squareNorm = -1.0;
         else

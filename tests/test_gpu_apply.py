@@ -134,3 +134,131 @@ def test_apply_rejects_incompatible_mra(mr):
     P = mr.PoissonOperator(a, 1e-3)
     with pytest.raises(mr.MRCPPError):
         mr.apply(1e-3, mr.FunctionTree(b), P, f)
+
+
+def test_concurrent_threads_serialised(mr):
+    """Two host threads calling apply() at once (ctypes releases the GIL) get
+    the trees the same calls give one after the other: the C ABI serialises
+    every call that uses the device."""
+    import threading
+    mra = mr.MultiResolutionAnalysis((-20, 20), 5)
+    jobs = []
+    for seed, mu in ((5, None), (6, 1.5)):
+        src = W.random_sources(seed, 4, center_range=3.0, beta_range=(2.0, 200.0))
+        f = mr.FunctionTree(mra)
+        gx = mr.GaussExp.from_sources(src)
+        mr.build_grid(f, gx)
+        mr.project(1e-5, f, gx)
+        P = mr.PoissonOperator(mra, 1e-5) if mu is None else mr.HelmholtzOperator(mra, mu, 1e-5)
+        jobs.append((f, P))
+    seq = []
+    for f, P in jobs:
+        g = mr.FunctionTree(mra)
+        mr.apply(1e-5, g, P, f)
+        seq.append(g.export())
+    results = [[None] * 3 for _ in jobs]
+    errors = []
+
+    def worker(j):
+        try:
+            f, P = jobs[j]
+            for r in range(3):
+                g = mr.FunctionTree(mra)
+                mr.apply(1e-5, g, P, f)
+                results[j][r] = g.export()
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    ts = [threading.Thread(target=worker, args=(j,)) for j in range(len(jobs))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(120)
+    assert not errors, errors
+    for j, (si, sc, _) in enumerate(seq):
+        for r in range(3):
+            ri, rc, _ = results[j][r]
+            assert np.array_equal(ri, si) and np.array_equal(rc, sc), (j, r)
+
+
+STEP0_CASES = [
+    ("gauss1_k5_poisson", W.single_gaussian(100.0), 5, 20, 1e-5, None),
+    ("gauss1_k5_helmholtz", W.single_gaussian(100.0), 5, 20, 1e-5, 1.0),
+    ("gauss4_k7_poisson", W.random_sources(11, 4, 4.0), 7, 32, 1e-6, None),
+    ("gauss16_k7_poisson", W.WORKLOADS["gauss16_k7"]["sources"](), 7, 32, 1e-6, None),
+]
+
+
+@pytest.mark.parametrize("name,src,order,box,prec,mu", STEP0_CASES, ids=[c[0] for c in STEP0_CASES])
+def test_first_step_matches_brute_force(mr, name, src, order, box, prec, mu):
+    """maxIter = 0 on the GPU against the brute-force restatement of the
+    first breadth-first step (tests/step0_brute.py): the applied-tuple count
+    exactly, the root coefficients to 1e-13."""
+    from tests.step0_brute import step0
+    mra = mr.MultiResolutionAnalysis((-box, box), order)
+    f = mr.FunctionTree(mra)
+    gx = mr.GaussExp.from_sources(src)
+    mr.build_grid(f, gx)
+    mr.project(prec, f, gx)
+    P = mr.PoissonOperator(mra, prec) if mu is None else mr.HelmholtzOperator(mra, mu, prec)
+    g = mr.FunctionTree(mra)
+    st = mr.apply(prec, g, P, f, maxIter=0)
+    fi, fc, fn = f.export()
+    n0 = int(fi[:, 0].min())
+
+    def roots(idx, c, n):
+        sel = sorted(np.where(idx[:, 0] == n0)[0], key=lambda r: (idx[r, 3], idx[r, 2], idx[r, 1]))
+        return idx[sel, 1:4], c[sel], n[sel]
+
+    fl, fcoef, fnorm = roots(fi, fc, fn)
+    gi, gc, gn = g.export()
+    assert (gi[:, 0] == n0).all() and len(gi) == len(fl)
+    gl, gcoef, _ = roots(gi, gc, gn)
+    assert np.array_equal(gl, fl)
+    count, ref = step0(fl, fcoef, fnorm, order + 1, P.bandwidths(prec), lambda i, l, c: P.node(i, 0, l, c)[0])
+    rel = np.linalg.norm(gcoef - ref) / np.linalg.norm(ref)
+    _say(name, "applied", st["n_applied"], "brute force", count, "rel L2", rel)
+    assert st["n_applied"] == count
+    assert rel <= 1e-13
+
+
+def test_applied_count_grows_with_precision(mr):
+    src = W.single_gaussian(100.0)
+    mra = mr.MultiResolutionAnalysis((-20, 20), 5)
+    gx = mr.GaussExp.from_sources(src)
+    counts, nodes = [], []
+    for prec in (1e-3, 1e-4, 1e-5, 1e-6):
+        f = mr.FunctionTree(mra)
+        mr.build_grid(f, gx)
+        mr.project(prec, f, gx)
+        g = mr.FunctionTree(mra)
+        st = mr.apply(prec, g, mr.PoissonOperator(mra, prec), f)
+        counts.append(st["n_applied"])
+        nodes.append(st["n_output_nodes"])
+    _say("applied tuples", counts, "output nodes", nodes)
+    assert all(a < b for a, b in zip(counts, counts[1:])), counts
+    assert all(a <= b for a, b in zip(nodes, nodes[1:])), nodes
+
+
+def test_reference_poisson_energy_kat(mr):
+    """MRCPP's own Poisson test (tests/operators/poisson_operator.cpp:129-154)
+    at its parameters, in a Legendre basis of order 5: the world of
+    tests/factory_functions.h (root scale 1, corner (-1, 0, 1), 1 x 2 x 3 root
+    boxes), a normalised Gaussian of exponent 1e4 at (-0.2, 0.5, 1.0)
+    projected at 1e-4, the Poisson operator built at 1e-4 and applied at
+    1e-3; dot(P f, f) equals the analytic Coulomb self-energy
+    sqrt(2 beta / pi) (GaussFunc::calcCoulombEnergy,
+    src/functions/GaussFunc.cpp:210-237) to relative 1e-3."""
+    beta = 1.0e4
+    src = W.GaussianSources([1.0], [beta], [(-0.2, 0.5, 1.0)])
+    mra = mr.MultiResolutionAnalysis((1, (-1, 0, 1), (1, 2, 3)), 5)
+    f = mr.FunctionTree(mra)
+    mr.project(1e-4, f, mr.GaussExp.from_sources(src))
+    P = mr.PoissonOperator(mra, 1e-4)
+    g = mr.FunctionTree(mra)
+    st = mr.apply(1e-3, g, P, f)
+    e_num = mr.dot(g, f)
+    e_ana = math.sqrt(2.0 * beta / math.pi)
+    _say("KAT: E_num", e_num, "E_ana", e_ana, "rel", abs(e_num - e_ana) / e_ana, "nodes", st["n_output_nodes"], "terms", P.size())
+    assert abs(f.integrate() - 1.0) < 1e-3
+    assert abs(e_num - e_ana) <= 1e-3 * abs(e_ana)

@@ -128,3 +128,47 @@ def test_trained_predictor_saves_passes(mr):
         pred_passes += st1["n_levels"]
     _say("device passes: plain", plain_passes, "with predictor", pred_passes)
     assert pred_passes < plain_passes
+
+
+def test_cfg5_held_out(mr):
+    """The committed order-7 predictor on the 50 held-out trees of the
+    predictor workload (4..32 Gaussians, prec 1e-6, Poisson and Helmholtz
+    mu in [0.5,3]), batched: every tree has the plain apply's node set,
+    coefficients within 1e-6 relative L2 of it, and the batch needs fewer
+    device passes than the plain batch."""
+    import os
+    specs = W.predictor_sources()[200:250]
+    mra = mr.MultiResolutionAnalysis((-32, 32), 7)
+    ops, trees = {}, []
+    for sp in specs:
+        f = mr.FunctionTree(mra)
+        gx = mr.GaussExp.from_sources(sp["sources"])
+        mr.build_grid(f, gx)
+        mr.project(sp["prec"], f, gx)
+        key = (sp["mu"], sp["prec"])
+        if key not in ops:
+            ops[key] = mr.PoissonOperator(mra, sp["prec"]) if sp["mu"] is None else mr.HelmholtzOperator(mra, sp["mu"], sp["prec"])
+        trees.append((sp["prec"], ops[key], f, np.asarray(sp["sources"].centers)))
+    params = np.load(os.path.join(os.path.dirname(mr.__file__), "share", "predictor_k7.npy"))
+    pred = mr.RefinePredictor(params=params, threshold=0.5)
+    plain = [mr.FunctionTree(mra) for _ in trees]
+    pst = mr.apply_predicted([(t[0], o, t[1], t[2]) for t, o in zip(trees, plain)])
+    guess = [mr.FunctionTree(mra) for _ in trees]
+    gst = mr.apply_predicted([(t[0], o, t[1], t[2]) for t, o in zip(trees, guess)], predictor=pred, centers=[t[3] for t in trees])
+    same, worst = 0, 0.0
+    for i, (a, b) in enumerate(zip(plain, guess)):
+        ai, ac, _ = a.export()
+        bi, bc, _ = b.export()
+        if np.array_equal(ai, bi):
+            same += 1
+            worst = max(worst, np.linalg.norm(bc - ac) / np.linalg.norm(ac))
+        else:
+            _say("tree", i, "node sets differ:", len(ai), len(bi))
+    p_passes = max(s["n_levels"] for s in pst)
+    g_passes = max(s["n_levels"] for s in gst)
+    _say("held-out: %d/50 identical node sets, worst rel L2 %.2e, passes plain %d predicted %d, predicted %d cropped %d fallback %d"
+         % (same, worst, p_passes, g_passes, sum(s["n_predicted"] for s in gst), sum(s["n_cropped"] for s in gst),
+            sum(s["n_fallback"] for s in gst)))
+    assert same == len(trees)
+    assert worst <= 1e-6
+    assert g_passes < p_passes
