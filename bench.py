@@ -204,25 +204,54 @@ def time_batch_operator_build(mr, mra, specs):
             "what": "construction of the batch's distinct Poisson/Helmholtz operators (one device pass vs one by one on the host cores)"}
 
 
-def cpu_baseline(workload, w, src):
-    """The CPU oracle's apply on the same workload (one full tree), timed on
-    this host's cores."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(workload, w, src, repeats=5):
+    """The CPU oracle's apply on the same workload (one full tree per run),
+    timed on this host's cores: one warm-up, then the median of `repeats`
+    runs with every OpenMP thread, then one run on a single thread."""
     import oracle as O
     omra = O.OracleMRA((-w["box"], w["box"]), w["order"])
     of = O.OracleTree(omra)
     of.project_gaussians(w["prec"], src.coefs, src.exponents, src.centers)
     op = O.OracleOperator(omra, w["prec"], mu=w["mu"])
-    og = O.OracleTree(omra)
-    st = O.apply(w["prec"], og, op, of)
+    cores = O.Oracle.max_threads()
+
+    def one():
+        og = O.OracleTree(omra)
+        return O.apply(w["prec"], og, op, of)
+
+    one()  # warm-up
+    runs = [one() for _ in range(repeats)]
+    secs = sorted(r["seconds"] for r in runs)
+    med = secs[len(secs) // 2]
+    st = runs[0]
+    O.Oracle.set_threads(1)
+    try:
+        s1 = one()["seconds"]
+    finally:
+        O.Oracle.set_threads(cores)
     gflop = st["n_applied"] * flops_per_tuple(w["order"]) / 1e9
     return {
-        "value": gflop / st["seconds"],
+        "value": gflop / med,
         "unit": "GFLOP/s",
-        "cores": O.Oracle.max_threads(),
+        "cores": cores,
         "kind": "port",
-        "sample": "one full apply of %s on the host (OpenMP over output nodes): %.3f s/tree, %d applied tuples"
-                  % (workload, st["seconds"], st["n_applied"]),
-        "s_per_tree": st["seconds"],
+        "sample": "full applies of %s on the host (OpenMP over output nodes): 1 warm-up, median of %d at %d threads %.3f s/tree, "
+                  "1 thread %.3f s/tree, %d applied tuples" % (workload, repeats, cores, med, s1, st["n_applied"]),
+        "s_per_tree": med,
+        "s_per_tree_runs": [round(s, 4) for s in secs],
+        "s_per_tree_1_thread": s1,
+        "value_1_thread": gflop / s1,
+        "cpu_model": cpu_model(),
         "n_applied": st["n_applied"],
         "n_output_nodes": st["n_g"],
     }
@@ -488,37 +517,41 @@ def main():
     rel = float(np.linalg.norm(c1 - c0) / np.linalg.norm(c0)) if same else None
     res = {
         "metric": METRIC,
-        "value": round(flop / chosen["s"] / 1e9, 3),
+        # the headline is the FP64 drop-in (the reference's arithmetic); the
+        # chosen reduced-precision form is reported beside it with its error
+        "value": round(flop / plain["s"] / 1e9, 3),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(chosen["s"] * 1e3, 4),
-        "s_per_tree": chosen["s"],
+        "ms_per_step": round(plain["s"] * 1e3, 4),
+        "s_per_tree": plain["s"],
         "higher_is_better": True,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": DTYPES[args.precision],
+        "dtype": "f64",
         "data": "synthetic seeded Gaussian sources",
         "config": {"workload": args.workload, "order": order, "prec": prec, "box": [-w["box"], w["box"]],
                    "n_gaussians": len(src), "operator": "Poisson" if w["mu"] is None else "Helmholtz mu=%g" % w["mu"],
                    "separation_rank": P.size(), "n_applied": int(n_applied),
                    "n_candidate_pairs": int(pst["n_pairs"]), "n_generated_nodes": int(pst["n_generated"]),
                    "n_output_nodes": int(pst["n_output_nodes"]), "parallelism": "single GPU"},
-        "chosen": {"form": "%s sweeps%s" % (args.precision, " + refinement predictor" if pred is not None else ""),
-                   "passes": int(cst["n_levels"]), "n_predicted": int(cst.get("n_predicted", 0)),
-                   "n_cropped": int(cst.get("n_cropped", 0)), "n_fallback": int(cst.get("n_fallback", 0)),
-                   "n_applied": int(cst["n_applied"]), "step_times_s": [round(t, 6) for t in chosen["times"]],
-                   "kernel_ms_per_step": {k: round(v, 4) for k, v in chosen["breakdown"].items()}},
-        "plain_fp64": {"ms_per_step": round(plain["s"] * 1e3, 4), "s_per_tree": plain["s"], "gflops": round(flop / plain["s"] / 1e9, 3),
+        "plain_fp64": {"form": "FP64 sweeps (drop-in)", "ms_per_step": round(plain["s"] * 1e3, 4), "s_per_tree": plain["s"],
+                       "gflops": round(flop / plain["s"] / 1e9, 3),
                        "passes": int(pst["n_levels"]), "step_times_s": [round(t, 6) for t in plain["times"]],
                        "kernel_ms_per_step": {k: round(v, 4) for k, v in plain["breakdown"].items()},
                        "host_breakdown_s": {k: pst[k] for k in ("seconds", "t_setup", "t_screen", "t_generate", "t_band", "t_transform",
                                                                 "t_host", "t_cleanup")}},
-        "speedup_vs_plain": round(plain["s"] / chosen["s"], 3),
-        "error_vs_plain_fp64": {"same_node_set": same, "rel_l2": rel, "nodes_plain": int(len(i0)), "nodes_chosen": int(len(i1))},
-        "roofline": roofline(chosen["prof"], flop, args.steps, args.precision, args.workload),
-        "roofline_plain_fp64": roofline(plain["prof"], flop, args.steps, "fp64", args.workload),
+        "roofline": roofline(plain["prof"], flop, args.steps, "fp64", args.workload),
+        "chosen": {"form": "%s sweeps%s" % (args.precision, " + refinement predictor" if pred is not None else ""),
+                   "dtype": DTYPES[args.precision], "ms_per_step": round(chosen["s"] * 1e3, 4), "s_per_tree": chosen["s"],
+                   "gflops": round(flop / chosen["s"] / 1e9, 3), "speedup_vs_plain": round(plain["s"] / chosen["s"], 3),
+                   "error_vs_plain_fp64": {"same_node_set": same, "rel_l2": rel, "nodes_plain": int(len(i0)), "nodes_chosen": int(len(i1))},
+                   "roofline": roofline(chosen["prof"], flop, args.steps, args.precision, args.workload),
+                   "passes": int(cst["n_levels"]), "n_predicted": int(cst.get("n_predicted", 0)),
+                   "n_cropped": int(cst.get("n_cropped", 0)), "n_fallback": int(cst.get("n_fallback", 0)),
+                   "n_applied": int(cst["n_applied"]), "step_times_s": [round(t, 6) for t in chosen["times"]],
+                   "kernel_ms_per_step": {k: round(v, 4) for k, v in chosen["breakdown"].items()}},
     }
     if variant is not None:
         vst = variant["stats"][-1]
@@ -538,8 +571,8 @@ def main():
     res["operator_build"] = time_operator_build(mr, w, mra)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
-        res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]
-        res["plain_gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]
+        res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]
+        res["chosen_gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / chosen["s"]
     print(json.dumps(res))
 
 

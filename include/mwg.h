@@ -123,7 +123,7 @@ typedef struct {
 enum {
     MWG_PREC_FP64 = 0,   /* FP64 MFMA */
     MWG_PREC_FP32 = 1,   /* FP32 MFMA, band sum accumulated in FP64 */
-    MWG_PREC_BF16X3 = 2  /* split BF16 (hi*hi + hi*lo + lo*hi) MFMA with FP32 products, FP64 band sum */
+    MWG_PREC_BF16X3 = 2  /* three-part split BF16 (hh + hm + mh + hl + lh + mm) MFMA, FP32 products and sums, FP64 band sum */
 };
 
 /* Size and construction of the MFMA operand table of one precision: every
@@ -176,6 +176,9 @@ typedef struct {
     unsigned long long *counters; /* device [2] the segment adds to: applied (f, ft, gt, term) tuples, candidate pairs */
 } mwg_segment;
 
+/* Screening-margin bins of a level (mwg_level::g_margin). */
+#define MWG_MARGINS 8
+
 /* One breadth-first level: count output nodes, ordered by segment.  segs is
  * HOST memory (the library copies it per call); the arrays are device
  * memory.  Output coefficients go to segs[s].g_coef + g_off[i] for the
@@ -192,14 +195,14 @@ typedef struct {
                                 (prec * precFunc(idx) * sqrt(|g|^2 / M) of apply with precTrees,
                                 src/treebuilders/ConvolutionCalculator.cpp:241-246) */
     /* Optional screening margins (NULL: not computed; no reference
-     * counterpart).  For node i and j < 4: g_margin[4 i + j] = sum over the
+     * counterpart).  For node i and j < MWG_MARGINS: g_margin[MWG_MARGINS i + j] = sum over the
      * node's applied tuples whose test value |Ox||Oy||Oz| * bs * |f_ft| is at
      * most margin_ratio[j] times the node's threshold, of |Ox||Oy||Oz| |f_ft|
      * (Frobenius norms).  It bounds the 2-norm of what those tuples add to the
      * node, i.e. how far the node can move when it is recomputed with a
      * threshold up to margin_ratio[j] times higher. */
-    double *g_margin;        /* [count][4] */
-    double margin_ratio[4];
+    double *g_margin;        /* [count][MWG_MARGINS] */
+    double margin_ratio[MWG_MARGINS];
 } mwg_level;
 
 /* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the

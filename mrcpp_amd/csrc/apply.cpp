@@ -172,7 +172,7 @@ struct Job {
     std::vector<char> speculative; // computed ahead of its step (with an earlier step's threshold)
     bool edgeChecked = false;      // uncertain nodes of the current replay step recomputed
     bool marginPass = false;       // the next pass computes speculative nodes: record their screening margins
-    std::vector<std::array<double, 4>> margin; // per node: screening margins of the pass that computed it
+    std::vector<std::array<double, MWG_MARGINS>> margin; // per node: screening margins of the pass that computed it
     std::vector<double> thrUsed;               // per node: the screening threshold it was computed with
     double normErr = 0.0; // bound on |approximate tree norm - the plain method's| from kept speculative nodes
     // precision trees: precFunc of every output node (by id, -1 until needed)
@@ -187,7 +187,9 @@ double node_prec(const Job &job, int u) { return job.precFunc ? job.task.prec * 
 // src/treebuilders/ConvolutionCalculator.cpp:241-246).
 double node_threshold(const Job &job, int u) {
     double t = job.g->treeSquareNorm;
-    if (t > 0.0) t = job.task.prec * job.precFac[u] * std::sqrt(t / (double)job.task.oper->size());
+    // precFunc is 1 without precision trees (precFac is then empty)
+    const double fac = job.precFunc ? job.precFac[u] : 1.0;
+    if (t > 0.0) t = job.task.prec * fac * std::sqrt(t / (double)job.task.oper->size());
     return t;
 }
 
@@ -203,8 +205,11 @@ void eval_prec_factors(Job &job) {
 // Threshold ratios of the screening margins recorded for speculative nodes
 // (mwg_level::margin_ratio): a node computed with threshold t and judged at a
 // step whose own threshold t' satisfies t' <= kMarginRatio[j] t differs from
-// its recomputation at t' by at most its margin j in 2-norm.
-constexpr double kMarginRatio[4] = {1.02, 1.1, 1.5, 4.0};
+// its recomputation at t' by at most its margin j in 2-norm.  The screening
+// threshold grows with the square root of the approximate tree norm; after
+// the first two steps it typically grows by 0.1-0.3 % more for Poisson trees
+// and by up to 30 % for Helmholtz trees, hence the fine bins near 1.
+constexpr double kMarginRatio[MWG_MARGINS] = {1.0003, 1.001, 1.003, 1.01, 1.03, 1.1, 1.3, 2.0};
 
 struct Shape {
     int kp1, n3, ncoef;
@@ -328,9 +333,9 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.l = reinterpret_cast<const int *>(S.lvl.get() + L + nThr);
     lvl.g_norms = S.gNorms.get();
     if (margins) {
-        S.gMargin.reserve(4 * (size_t)L);
+        S.gMargin.reserve(MWG_MARGINS * (size_t)L);
         lvl.g_margin = S.gMargin.get();
-        for (int j = 0; j < 4; j++) lvl.margin_ratio[j] = kMarginRatio[j];
+        for (int j = 0; j < MWG_MARGINS; j++) lvl.margin_ratio[j] = kMarginRatio[j];
     }
 
     mwg_pairs pr{};
@@ -412,7 +417,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     t0 = clk::now();
     // band sums, output coefficients and norms (zero when no pair survives)
     device::check(mwg_band_apply(&lvl, &pr, npairs, (int)g_sweep), "band apply");
-    std::vector<double> nrm(9 * (size_t)L), mgn(margins ? 4 * (size_t)L : 0);
+    std::vector<double> nrm(9 * (size_t)L), mgn(margins ? MWG_MARGINS * (size_t)L : 0);
     S.gNorms.download(nrm.data(), nrm.size());
     if (margins) S.gMargin.download(mgn.data(), mgn.size());
     const double tBand = since(t0);
@@ -437,7 +442,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
             g.setSquareNorm(u, nrm[9 * (size_t)at + 8]);
             g.setHasCoefs(u, true);
             if (job.marginPass) {
-                for (int j = 0; j < 4; j++) job.margin[u][j] = mgn[4 * (size_t)at + j];
+                for (int j = 0; j < MWG_MARGINS; j++) job.margin[u][j] = mgn[MWG_MARGINS * (size_t)at + j];
                 job.thrUsed[u] = job.precFunc ? node_threshold(job, u) : job.gthr;
             }
             at++;
@@ -633,7 +638,7 @@ double speculative_bound(const Job &job, int u) {
     if (!(used > 0.0) || !(own >= used)) return HUGE_VAL;
     const double ratio = own / used;
     if (ratio == 1.0) return 0.0; // the same threshold: the same tuples
-    for (int j = 0; j < 4; j++)
+    for (int j = 0; j < MWG_MARGINS; j++)
         if (kMarginRatio[j] >= ratio * (1.0 + 1e-12)) return job.margin[u][j];
     return HUGE_VAL;
 }
@@ -707,6 +712,15 @@ void replay(Job &job) {
         job.iter++;
         job.replayWork.swap(next);
         if (!pending.empty()) {
+            // the pass that computes the missed children also recomputes the
+            // step's speculative nodes with the step's own threshold (the
+            // pass's threshold is the one after this step), so the step is
+            // then judged without an uncertainty check and its own pass
+            for (int u : job.replayWork)
+                if (is_speculative(job, u)) {
+                    job.speculative[u] = 0;
+                    pending.push_back(u);
+                }
             job.stats.nFallback += (long long)pending.size();
             job.work.swap(pending);
             return;
@@ -896,6 +910,13 @@ void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, Func
     t.precTrees = &precTrees;
     apply_batch({t});
     g_stats = g_batch_stats[0];
+}
+
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, FunctionTreeVector<3> &precTrees, int maxIter,
+           bool absPrec) {
+    std::vector<FunctionTree<3> *> trees;
+    for (auto &t : precTrees) trees.push_back(std::get<1>(t));
+    apply(prec, out, oper, inp, trees, maxIter, absPrec);
 }
 
 } // namespace mrcpp

@@ -69,7 +69,7 @@ void compress_and_norms(FunctionTree<3> &out, const double *filters) {
     detail::calc_square_norm(g);
 }
 
-void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter, bool absPrec) {
+void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec) {
     const char *what = mode == Sum ? "add" : "multiply";
     if (inp.empty()) throw Error(std::string(what) + ": no input trees");
     if (inp.size() > MWG_MAX_OPERANDS) throw Error(std::string(what) + ": at most 8 input trees");
@@ -256,20 +256,37 @@ void FunctionTree<3>::rescale(double c) {
     invalidate();
 }
 
-void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter, bool absPrec) {
+void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter, bool absPrec,
+         bool /*conjugate*/) {
     combine(Sum, prec, out, {{a, &inp_a}, {b, &inp_b}}, maxIter, absPrec);
 }
 
-void add(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter, bool absPrec) { combine(Sum, prec, out, inp, maxIter, absPrec); }
-
-void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_a, FunctionTree<3> &inp_b, int maxIter, bool absPrec,
-              bool useMaxNorms) {
-    multiply(prec, out, {{c, &inp_a}, {1.0, &inp_b}}, maxIter, absPrec, useMaxNorms);
+void add(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec, bool /*conjugate*/) {
+    combine(Sum, prec, out, inp, maxIter, absPrec);
 }
 
-void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter, bool absPrec, bool useMaxNorms) {
+void add(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter, bool absPrec, bool conjugate) {
+    FunctionTreeVector<3> v;
+    for (auto *t : inp) v.push_back({1.0, t});
+    add(prec, out, v, maxIter, absPrec, conjugate);
+}
+
+void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_a, FunctionTree<3> &inp_b, int maxIter, bool absPrec,
+              bool useMaxNorms, bool conjugate) {
+    multiply(prec, out, {{c, &inp_a}, {1.0, &inp_b}}, maxIter, absPrec, useMaxNorms, conjugate);
+}
+
+void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec, bool useMaxNorms,
+              bool /*conjugate*/) {
     if (useMaxNorms) throw Error("multiply: the max-norm refinement (useMaxNorms) is not supported");
     combine(Product, prec, out, inp, maxIter, absPrec);
+}
+
+void multiply(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter, bool absPrec, bool useMaxNorms,
+              bool conjugate) {
+    FunctionTreeVector<3> v;
+    for (auto *t : inp) v.push_back({1.0, t});
+    multiply(prec, out, v, maxIter, absPrec, useMaxNorms, conjugate);
 }
 
 } // namespace mrcpp

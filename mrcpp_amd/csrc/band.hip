@@ -55,8 +55,8 @@ constexpr int kScreenWaves = 2; // waves per pair: wave v tests terms 64 v, 64 v
 // Screening margins (optional): ratios r_j of the node threshold below which
 // an applied tuple's |O| |f_ft| is summed into margin bin j.
 struct MarginSpec {
-    double ratio[4];
-    double *pair; // [pair][4], nullptr: margins not computed
+    double ratio[MWG_MARGINS];
+    double *pair; // [pair][MWG_MARGINS], nullptr: margins not computed
 };
 
 __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_segment *segs, const int *node_seg, const int *l, const double *node_thr,
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
                                                                    int *papp, MarginSpec mg) {
     __shared__ unsigned char smask[8 * kMaxTerms];
     __shared__ unsigned s_napp[kScreenWaves];
-    __shared__ double s_mg[kScreenWaves][4];
+    __shared__ double s_mg[kScreenWaves][MWG_MARGINS];
     __shared__ int sb40[kMaxTerms]; // band block index of each term (first matrix / 4 + width)
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int p = blockIdx.x;
@@ -79,12 +79,15 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
             gcount[p] = 0;
             papp[p] = 0;
         }
-        if (mg.pair && threadIdx.x < 4) mg.pair[4 * (size_t)p + threadIdx.x] = 0.0;
+        if (mg.pair && threadIdx.x < MWG_MARGINS) mg.pair[MWG_MARGINS * (size_t)p + threadIdx.x] = 0.0;
         return;
     }
-    double mg_thr[4], mg_sum[4] = {0.0, 0.0, 0.0, 0.0};
+    double mg_thr[MWG_MARGINS], mg_sum[MWG_MARGINS];
 #pragma unroll
-    for (int j = 0; j < 4; j++) mg_thr[j] = mg.ratio[j] * g_thr;
+    for (int j = 0; j < MWG_MARGINS; j++) {
+        mg_thr[j] = mg.ratio[j] * g_thr;
+        mg_sum[j] = 0.0;
+    }
     int dl[3], maxdl = 0;
     for (int i = 0; i < 3; i++) {
         dl[i] = pr.pair_l[3 * p + i] - l[3 * g + i];
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
                         if (mg.pair) {
                             const double c = onorm * fn[ft];
 #pragma unroll
-                            for (int j = 0; j < 4; j++)
+                            for (int j = 0; j < MWG_MARGINS; j++)
                                 if (val <= mg_thr[j]) mg_sum[j] += c;
                         }
                     }
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
     if (lane == 0) s_napp[wv] = napp;
     if (mg.pair) { // fixed-order tree over lanes, then waves in order
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
+        for (int j = 0; j < MWG_MARGINS; j++) {
             double x = mg_sum[j];
             for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
             if (lane == 0) s_mg[wv][j] = x;
@@ -206,10 +209,10 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
         for (int v = 0; v < kScreenWaves; v++) tot += s_napp[v];
         papp[p] = (int)tot;
     }
-    if (mg.pair && lane < 4) {
+    if (mg.pair && lane < MWG_MARGINS) {
         double x = 0.0;
         for (int v = 0; v < kScreenWaves; v++) x += s_mg[v][lane];
-        mg.pair[4 * (size_t)p + lane] = x;
+        mg.pair[MWG_MARGINS * (size_t)p + lane] = x;
     }
     // emit ft-major, terms ascending
     int base = 0;
@@ -255,12 +258,14 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
     for (int q = gf; q < gf + np; q++) app += (unsigned)papp[q];
     if (app) atomicAdd(segs[node_seg[g]].counters, app);
     if (g_margin) { // the node's screening margins, its pairs in order
-        double m[4] = {0.0, 0.0, 0.0, 0.0};
+        double m[MWG_MARGINS];
+#pragma unroll
+        for (int j = 0; j < MWG_MARGINS; j++) m[j] = 0.0;
         for (int q = gf; q < gf + np; q++)
 #pragma unroll
-            for (int j = 0; j < 4; j++) m[j] += pmg[4 * (size_t)q + j];
+            for (int j = 0; j < MWG_MARGINS; j++) m[j] += pmg[MWG_MARGINS * (size_t)q + j];
 #pragma unroll
-        for (int j = 0; j < 4; j++) g_margin[4 * (size_t)g + j] = m[j];
+        for (int j = 0; j < MWG_MARGINS; j++) g_margin[MWG_MARGINS * (size_t)g + j] = m[j];
     }
     const int T = gstart[gf + np] - gstart[gf];
     const int G = units_group_size(gstart[npairs], slots, T);
@@ -545,11 +550,15 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
 // PREC selects the operand types:
 //   FP64: 16x16x4 f64, LDS [contracted index][row], accumulate in place;
 //   FP32: 16x16x4 f32, same layout, each group's product added to FP64;
-//   BF16X3: 16x16x32 bf16 with the contraction concatenated as
-//     [A_hi | A_hi | A_lo | 0] x [B_hi ; B_lo ; B_hi ; 0] (8 wide each), so one
-//     instruction forms hi*hi + hi*lo + lo*hi with FP32 products; LDS in FP32
-//     [row][contracted index] with 4 floats of padding after every N1 rows;
-//     each group's product added to FP64.
+//   BF16X3: 16x16x32 bf16 with each operand split into three bf16 parts
+//     x = hi + mid + lo (about 27 significant bits together) and the
+//     contraction concatenated in two instructions per 8-wide block,
+//     [A_hi | A_hi | A_mid | A_hi] x [B_hi ; B_mid ; B_hi ; B_lo] and
+//     [A_lo | A_mid | 0 | 0] x [B_hi ; B_mid ; 0 ; 0], so the six products
+//     hi*hi + hi*mid + mid*hi + hi*lo + lo*hi + mid*mid (everything down to
+//     2^-16 of the leading one) accumulate exactly formed in FP32; LDS in FP32
+//     [row][contracted index] with 4 floats of padding after every N1 rows,
+//     split into parts as it is read; each group's product added to FP64.
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
@@ -557,7 +566,7 @@ template <int N1, int PREC> struct BandGeom {
     using S = SweepShape<N1>;
     static constexpr bool kBF = PREC == MWG_PREC_BF16X3;
     static constexpr int KB = (N1 + 7) / 8;                  // 8-wide contraction blocks (BF16X3)
-    static constexpr int KSTEPS = kBF ? KB : S::KS;         // instructions per tile
+    static constexpr int KSTEPS = kBF ? 2 * KB : S::KS;     // instructions (operand fragments) per tile
     static constexpr int PK = 8 * KB;                        // row length (BF16X3 layout)
     static constexpr int ROWS = 16 * S::MT;
     // elements of one [contracted][row] or [row][contracted] block in LDS
@@ -600,17 +609,22 @@ __device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename Band
             const float4 v0 = *reinterpret_cast<const float4 *>(row);
             const float4 v1 = *reinterpret_cast<const float4 *>(row + 4);
             const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
-            bf16x8 a;
+            bf16x8 a1, a2;
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const __bf16 hi = (__bf16)x[e];
-                const __bf16 lo = (__bf16)(x[e] - (float)hi);
-                a[e] = lq == 3 ? (__bf16)0.0f : (lq == 2 ? lo : hi);
+                const float r1 = x[e] - (float)hi; // exact
+                const __bf16 mid = (__bf16)r1;
+                const __bf16 lo = (__bf16)(r1 - (float)mid);
+                a1[e] = lq == 2 ? mid : hi;
+                a2[e] = lq == 0 ? lo : (lq == 1 ? mid : (__bf16)0.0f);
             }
-            const uint4 bb = b[kb];
-            bf16x8 bv;
-            __builtin_memcpy(&bv, &bb, sizeof(bv));
-            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bv, c, 0, 0, 0);
+            bf16x8 b1, b2;
+            const uint4 bb1 = b[2 * kb], bb2 = b[2 * kb + 1];
+            __builtin_memcpy(&b1, &bb1, sizeof(b1));
+            __builtin_memcpy(&b2, &bb2, sizeof(b2));
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b2, c, 0, 0, 0);
         }
     } else {
 #pragma unroll
@@ -859,7 +873,8 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
 // (matrix block b4, f bit, column tile nt, step ks, lane) of P[k][n] =
 // O_(2 (n / N1) + f)[k][n % N1], zero in the padding; FP64/FP32 hold
 // k = 4 ks + lane/16, n = 16 nt + lane%16; BF16X3 holds 8 bf16 for
-// k = 8 ks + 0..7 in the slot lane/16 of [hi ; lo ; hi ; 0].
+// k = 8 (ks / 2) + 0..7 in the slot lane/16 of [hi ; mid ; hi ; lo] (even ks)
+// or [hi ; mid ; 0 ; 0] (odd ks).
 template <int N1, int PREC> __global__ __launch_bounds__(256) void k_build_frag(const double *mats, long long nb4, typename BandTypes<PREC>::B *out) {
     using S = SweepShape<N1>;
     using G = BandGeom<N1, PREC>;
@@ -881,13 +896,19 @@ template <int N1, int PREC> __global__ __launch_bounds__(256) void k_build_frag(
         return mats[(size_t)(b4 * 4 + 2 * (nn / N1) + fb) * S::N2 + kk + N1 * (nn % N1)];
     };
     if constexpr (PREC == MWG_PREC_BF16X3) {
-        const int slot = lane / 16;
+        // three-part split of the FP64 entry (residuals exact in FP64); fragment
+        // 2 kb + 0 holds [hi ; mid ; hi ; lo], 2 kb + 1 holds [hi ; mid ; 0 ; 0]
+        // for the 8-wide block kb, slot lane / 16
+        const int slot = lane / 16, kb = ks / 2, part = ks % 2;
         bf16x8 val;
         for (int e = 0; e < 8; e++) {
-            const float x = (float)P(8 * ks + e);
-            const __bf16 hi = (__bf16)x;
-            const __bf16 lo = (__bf16)(x - (float)hi);
-            val[e] = slot == 3 ? (__bf16)0.0f : (slot == 1 ? lo : hi);
+            const double x = P(8 * kb + e);
+            const __bf16 hi = (__bf16)(float)x;
+            const double r1 = x - (double)(float)hi;
+            const __bf16 mid = (__bf16)(float)r1;
+            const __bf16 lo = (__bf16)(float)(r1 - (double)(float)mid);
+            const __bf16 parts[2][4] = {{hi, mid, hi, lo}, {hi, mid, (__bf16)0.0f, (__bf16)0.0f}};
+            val[e] = parts[part][slot];
         }
         uint4 o;
         __builtin_memcpy(&o, &val, sizeof(o));
@@ -953,7 +974,7 @@ __global__ __launch_bounds__(kThreads) void k_zero_level(const mwg_segment *segs
     double *dst = node_out(segs, node_seg, g_off, g);
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = 0.0;
     if (threadIdx.x < 9) g_norms[(size_t)g * 9 + threadIdx.x] = 0.0;
-    if (g_margin && threadIdx.x < 4) g_margin[(size_t)g * 4 + threadIdx.x] = 0.0;
+    if (g_margin && threadIdx.x < MWG_MARGINS) g_margin[(size_t)g * MWG_MARGINS + threadIdx.x] = 0.0;
 }
 
 struct BandWork {
@@ -1021,8 +1042,8 @@ template <int N1> struct BandL {
         if (w.papp.ensure(sizeof(int) * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
         MarginSpec mg{};
         if (lvl->g_margin) {
-            if (w.pmg.ensure(sizeof(double) * 4 * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
-            for (int j = 0; j < 4; j++) mg.ratio[j] = lvl->margin_ratio[j];
+            if (w.pmg.ensure(sizeof(double) * MWG_MARGINS * (size_t)npairs) != MWG_OK) return MWG_ERR_NOMEM;
+            for (int j = 0; j < MWG_MARGINS; j++) mg.ratio[j] = lvl->margin_ratio[j];
             mg.pair = w.pmg.as<double>();
         }
         int *gstart = w.gstart.as<int>();

@@ -119,9 +119,9 @@ int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms) {
 }
 
 namespace {
-FunctionTreeVector tree_vector(int n, const double *c, void **trees) {
+FunctionTreeVector<3> tree_vector(int n, const double *c, void **trees) {
     if (n < 0 || (n > 0 && (!c || !trees))) throw Error("bad input tree list");
-    FunctionTreeVector v;
+    FunctionTreeVector<3> v;
     for (int i = 0; i < n; i++) v.emplace_back(c[i], static_cast<FunctionTree<3> *>(trees[i]));
     return v;
 }

@@ -78,4 +78,24 @@ void apply(double prec, CompFunction<3> &out, ConvolutionOperator<3> &oper, cons
     apply_components(prec, out.CompD, oper, inp.CompD, inp.Ncomp(), metric, maxIter, absPrec);
 }
 
+void apply(double prec, CompFunction<3> &out, ConvolutionOperator<3> &oper, CompFunction<3> &inp, FunctionTreeVector<3> *precTrees,
+           const ComplexDouble (*metric)[4], int maxIter, bool absPrec) {
+    if (&out == &inp) throw Error("apply: output and input must differ");
+    if (out.getMRA() != inp.getMRA()) throw Error("Incompatible MRA");
+    if (!precTrees) throw Error("apply: null precision tree vectors");
+    if (!metric) metric = defaultMetric;
+    out.reset(inp.Ncomp());
+    // (input, output) pairs in the reference's order, one apply each, the
+    // output components being the ones paramCopy allocates
+    for (int ic = 0; ic < inp.Ncomp(); ic++)
+        for (int oc = 0; oc < 4; oc++) {
+            if (!(std::norm(metric[ic][oc]) > MachinePrec)) continue;
+            if (std::abs(metric[ic][oc].imag()) > MachinePrec) throw Error("apply: complex metric entries need complex components");
+            if (!out.CompD[oc]) throw Error("apply: the metric selects an output component beyond the input's count");
+            apply(prec, *out.CompD[oc], oper, *inp.CompD[ic], precTrees[ic], maxIter, absPrec);
+            const double m = metric[ic][oc].real();
+            if (std::abs(m - 1.0) > MachinePrec) out.CompD[oc]->rescale(m);
+        }
+}
+
 } // namespace mrcpp

@@ -2,10 +2,14 @@
 #include "core.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
+
+#include <dlfcn.h>
+#include <sys/stat.h>
 
 namespace mrcpp {
 
@@ -35,7 +39,28 @@ void set_filter_dir(const std::string &dir) {
     filter_dir_storage() = dir;
 }
 
-const std::string &filter_dir() { return filter_dir_storage(); }
+// Filter folder when none was set: $MWFILTERS_DIR if it names a folder (as
+// details::find_filters, src/utils/details.cpp:53-68, looks first), else the
+// share/mwfilters folder beside this library's lib/ folder.
+const std::string &filter_dir() {
+    std::lock_guard<std::mutex> lk(core_mutex());
+    std::string &d = filter_dir_storage();
+    if (d.empty()) {
+        struct stat st {};
+        const char *env = std::getenv("MWFILTERS_DIR");
+        if (env && stat(env, &st) == 0 && S_ISDIR(st.st_mode)) {
+            d = env;
+        } else {
+            Dl_info info{};
+            if (dladdr(reinterpret_cast<const void *>(&set_filter_dir), &info) && info.dli_fname) {
+                std::string so = info.dli_fname;
+                const size_t cut = so.find_last_of('/');
+                d = (cut == std::string::npos ? std::string(".") : so.substr(0, cut)) + "/../share/mwfilters";
+            }
+        }
+    }
+    return d;
+}
 
 namespace core {
 

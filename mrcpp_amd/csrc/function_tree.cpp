@@ -169,9 +169,15 @@ const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra) {
 
 } // namespace device
 
-void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp) {
-    build_grid_nodes(out.nodes(), inp);
+void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter) {
+    build_grid_nodes(out.nodes(), inp, maxIter);
     out.invalidate();
+}
+
+void build_grid(FunctionTree<3> &out, const GaussFunc<3> &inp, int maxIter) {
+    GaussExp<3> g;
+    g.append(inp);
+    build_grid(out, g, maxIter);
 }
 
 namespace {
@@ -295,6 +301,28 @@ void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxI
         return;
     }
     project_nodes(out.nodes(), prec, inp, maxIter, absPrec);
+    out.invalidate();
+}
+
+void project(double prec, FunctionTree<3> &out, const GaussFunc<3> &inp, int maxIter, bool absPrec) {
+    GaussExp<3> g;
+    g.append(inp);
+    project(prec, out, g, maxIter, absPrec);
+}
+
+namespace {
+struct PointFunction {
+    const std::function<double(const Coord<3> &)> &f;
+    double evalf(const Coord<3> &r) const { return f(r); }
+};
+} // namespace
+
+void project(double prec, FunctionTree<3> &out, std::function<double(const Coord<3> &r)> func, int maxIter, bool absPrec) {
+    if (!func) throw Error("project: empty function");
+    g_proj_device = false;
+    // node values come from arbitrary host code: one node per call, in order
+    // within a node, nodes of a level in parallel as MRCPP does
+    project_nodes(out.nodes(), prec, PointFunction{func}, maxIter, absPrec);
     out.invalidate();
 }
 

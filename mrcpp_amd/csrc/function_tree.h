@@ -3,6 +3,7 @@
 #pragma once
 
 #include <complex>
+#include <functional>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -22,9 +23,11 @@ struct BasisTables {
 const BasisTables &basis_tables(const MultiResolutionAnalysis<3> &mra);
 } // namespace device
 
-template <int D> class FunctionTree;
+// FunctionTree<D, T> (src/trees/FunctionTree.h:55): D = 3 and real (double)
+// coefficients are provided; FunctionTree<3> names FunctionTree<3, double>.
+template <int D, typename T = double> class FunctionTree;
 
-template <> class FunctionTree<3> {
+template <> class FunctionTree<3, double> {
 public:
     explicit FunctionTree(const MultiResolutionAnalysis<3> &mra, const std::string &name = "nn");
     FunctionTree(const FunctionTree &) = delete;
@@ -81,13 +84,21 @@ private:
     std::unique_ptr<device::TreeMirror> mirror_;
 };
 
-// build_grid(tree, GaussExp) (src/treebuilders/grid.cpp:106-123)
-void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp);
+// build_grid(tree, GaussExp, maxIter) (src/treebuilders/grid.cpp:106-123)
+// and build_grid(tree, Gaussian, maxIter) (grid.cpp:78-84, the same refinement
+// for one term)
+void build_grid(FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter = -1);
+void build_grid(FunctionTree<3> &out, const GaussFunc<3> &inp, int maxIter = -1);
 // project(prec, tree, GaussExp) (src/treebuilders/project.cpp:85-102).  Each
 // breadth-first level is computed on the MI355X when one is present (the
 // function values, value -> coefficient maps, compression and norms; the
 // host keeps the topology and the split test), else on the host cores.
 void project(double prec, FunctionTree<3> &out, const GaussExp<3> &inp, int maxIter = -1, bool absPrec = false);
+// project(prec, tree, GaussFunc) (src/treebuilders/project.h:33): a one-term expansion
+void project(double prec, FunctionTree<3> &out, const GaussFunc<3> &inp, int maxIter = -1, bool absPrec = false);
+// project(prec, tree, std::function) (src/treebuilders/project.h:34,
+// project.cpp:39-43): any function of a point, evaluated on the host cores
+void project(double prec, FunctionTree<3> &out, std::function<double(const Coord<3> &r)> func, int maxIter = -1, bool absPrec = false);
 enum class ProjectionTarget { Auto = 0, Host = 1, Device = 2 };
 // Where later project() calls on this thread run (Auto: the GPU if present).
 void set_projection_target(ProjectionTarget t);
@@ -102,24 +113,47 @@ namespace detail {
 double dot_device(FunctionTree<3> &bra, FunctionTree<3> &ket);
 } // namespace detail
 
-// (coefficient, tree) terms (FunctionTreeVector, src/trees/FunctionTreeVector.h)
-using FunctionTreeVector = std::vector<std::tuple<double, FunctionTree<3> *>>;
+// (coefficient, tree) terms (src/trees/FunctionTreeVector.h:35-36) and the
+// accessors of src/trees/FunctionTreeVector.h:42-104.
+template <int D, typename T = double> using CoefsFunctionTree = std::tuple<T, FunctionTree<D, T> *>;
+template <int D, typename T = double> using FunctionTreeVector = std::vector<CoefsFunctionTree<D, T>>;
+template <int D, typename T> void clear(FunctionTreeVector<D, T> &fs, bool dealloc = false) {
+    if (dealloc)
+        for (auto &t : fs) delete std::get<1>(t);
+    fs.clear();
+}
+template <int D, typename T> T get_coef(const FunctionTreeVector<D, T> &fs, int i) { return std::get<0>(fs[i]); }
+template <int D, typename T> FunctionTree<D, T> &get_func(FunctionTreeVector<D, T> &fs, int i) { return *std::get<1>(fs[i]); }
+template <int D, typename T> const FunctionTree<D, T> &get_func(const FunctionTreeVector<D, T> &fs, int i) { return *std::get<1>(fs[i]); }
+template <int D, typename T> int get_n_nodes(const FunctionTreeVector<D, T> &fs) {
+    int n = 0;
+    for (const auto &t : fs)
+        if (std::get<1>(t)) n += std::get<1>(t)->getNNodes();
+    return n;
+}
 // add(prec, out, a, inp_a, b, inp_b, maxIter, absPrec) and add(prec, out,
 // FunctionTreeVector, ...) (src/treebuilders/add.cpp:43-80): TreeBuilder with
 // AdditionCalculator and WaveletAdaptor from out's end nodes, then the
 // bottom-up transform.  Each breadth-first step runs on the MI355X (at most 8
 // terms); the output's coefficients stay in HBM.
+// conjugate (src/treebuilders/add.h:31-33, multiply.h:41-55) conjugates the
+// operands of complex trees; real trees are their own conjugates, so it is
+// accepted and has no effect.  The std::vector forms take every coefficient as 1.
 void add(double prec, FunctionTree<3> &out, double a, FunctionTree<3> &inp_a, double b, FunctionTree<3> &inp_b, int maxIter = -1,
-         bool absPrec = false);
-void add(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter = -1, bool absPrec = false);
+         bool absPrec = false, bool conjugate = false);
+void add(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter = -1, bool absPrec = false, bool conjugate = false);
+void add(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter = -1, bool absPrec = false,
+         bool conjugate = false);
 // multiply(prec, out, c, inp_a, inp_b, maxIter, absPrec, useMaxNorms) and the
 // FunctionTreeVector form (src/treebuilders/multiply.cpp:52-100) with
 // MultiplicationCalculator and WaveletAdaptor (useMaxNorms, the
 // MultiplicationAdaptor refinement, is not supported).
 void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_a, FunctionTree<3> &inp_b, int maxIter = -1,
-              bool absPrec = false, bool useMaxNorms = false);
-void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector &inp, int maxIter = -1, bool absPrec = false,
-              bool useMaxNorms = false);
+              bool absPrec = false, bool useMaxNorms = false, bool conjugate = false);
+void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter = -1, bool absPrec = false,
+              bool useMaxNorms = false, bool conjugate = false);
+void multiply(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter = -1, bool absPrec = false,
+              bool useMaxNorms = false, bool conjugate = false);
 
 // Statistics of the last apply on this thread.
 struct ApplyStats {
@@ -155,6 +189,10 @@ void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, Func
 // (src/treebuilders/apply.h:43, src/treebuilders/apply.cpp:214-251): the
 // precision of output node idx is prec / max over precTrees of the largest
 // norm at or below idx (scaled to unit volume).
+void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, FunctionTreeVector<3> &precTrees,
+           int maxIter = -1, bool absPrec = false);
+// The same with the precision trees as plain pointers (their coefficients in
+// a FunctionTreeVector are not used by the reference either).
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp,
            const std::vector<FunctionTree<3> *> &precTrees, int maxIter = -1, bool absPrec = false);
 
@@ -224,6 +262,12 @@ private:
 // apply_batch.  The metric must be real.
 void apply(double prec, CompFunction<3> &out, ConvolutionOperator<3> &oper, const CompFunction<3> &inp,
            const ComplexDouble (*metric)[4] = defaultMetric, int maxIter = -1, bool absPrec = false);
+// apply(prec, CompFunction out, oper, CompFunction inp, precTrees, metric,
+// maxIter, absPrec) (src/treebuilders/apply.h:44, apply.cpp:252-270): as
+// above with precTrees[i] the precision trees of input component i; a null
+// metric is the identity.
+void apply(double prec, CompFunction<3> &out, ConvolutionOperator<3> &oper, CompFunction<3> &inp, FunctionTreeVector<3> *precTrees,
+           const ComplexDouble (*metric)[4] = nullptr, int maxIter = -1, bool absPrec = false);
 // The same over caller-owned trees (out[i], inp[i] for i < ncomp); the
 // output components a nonzero metric entry selects must be present.
 void apply_components(double prec, FunctionTree<3> *const *out, ConvolutionOperator<3> &oper, FunctionTree<3> *const *inp, int ncomp,

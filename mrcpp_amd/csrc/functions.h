@@ -14,10 +14,28 @@ namespace mrcpp {
 // (src/functions/GaussFunc.cpp:47-66, src/functions/Gaussian.cpp:116-135).
 template <int D> class GaussFunc {
 public:
-    GaussFunc(double alpha, double coef, const Coord<D> &pos = {})
+    // GaussFunc(beta, alpha, pos, pow) (src/functions/GaussFunc.h:56): exponent,
+    // coefficient, centre; Cartesian powers other than zero are not provided
+    GaussFunc(double alpha, double coef, const Coord<D> &pos = {}, const std::array<int, D> &pow = {})
             : coef_(coef)
             , pos_(pos) {
         alpha_.fill(alpha);
+        for (int p : pow)
+            if (p != 0) throw Error("GaussFunc: only zero Cartesian powers are supported");
+    }
+    // Coulomb interaction of two unit-charge Gaussians
+    // (GaussFunc::calcCoulombEnergy, src/functions/GaussFunc.cpp:210-237):
+    // sqrt(4 a / pi) F0(a R^2) with a = p q / (p + q) and the Boys function
+    // F0(x) = sqrt(pi / x) erf(sqrt(x)) / 2, F0(0) = 1.
+    double calcCoulombEnergy(const GaussFunc<D> &g) const {
+        if (D != 3) throw Error("calcCoulombEnergy: three dimensions only");
+        const double p = alpha_[0], q = g.alpha_[0];
+        const double a = p * q / (p + q);
+        double r2 = 0.0;
+        for (int d = 0; d < D; d++) r2 += (pos_[d] - g.pos_[d]) * (pos_[d] - g.pos_[d]);
+        const double x = a * r2;
+        const double f0 = x < 1e-14 ? 1.0 - x / 3.0 : 0.5 * std::sqrt(pi / x) * std::erf(std::sqrt(x));
+        return std::sqrt(4.0 * a / pi) * f0;
     }
     double evalf(const Coord<D> &r) const {
         double q2 = 0.0;
@@ -143,7 +161,7 @@ template <int D> bool analytic_split(const NodeStore<D> &s, int u, const GaussFu
 
 // build_grid(tree, GaussExp): refine the empty grid once per Gaussian
 // (src/treebuilders/grid.cpp:106-123 with AnalyticAdaptor.h:42-50).
-template <int D> void build_grid_nodes(detail::NodeStore<D> &s, const GaussExp<D> &g) {
+template <int D> void build_grid_nodes(detail::NodeStore<D> &s, const GaussExp<D> &g, int maxIter = -1) {
     for (int i = 0; i < g.size(); i++) {
         const GaussFunc<D> &gi = g.getFunc(i);
         detail::BuildHooks<D> h;
@@ -155,7 +173,7 @@ template <int D> void build_grid_nodes(detail::NodeStore<D> &s, const GaussExp<D
             }
         };
         h.split = [&gi](const detail::NodeStore<D> &st, int u) { return detail::analytic_split(st, u, gi); };
-        detail::build_tree(s, h, -1);
+        detail::build_tree(s, h, maxIter);
     }
 }
 

@@ -45,11 +45,19 @@ public:
         nbox_.fill(box[0] == 0 ? 1 : 2);
         sf_.fill(size);
     }
-    BoundingBox(int n, const std::array<int, D> &corner, const std::array<int, D> &nb, const std::array<double, D> &sf)
+    // root scale n, corner translation, root boxes per direction (one where
+    // not positive) and scaling factors (all-zero: 1.0 each), as
+    // src/trees/BoundingBox.cpp:100-106,211-217,247-253 default them
+    explicit BoundingBox(int n, const std::array<int, D> &corner = {}, const std::array<int, D> &nb = {}, const std::array<double, D> &sf = {})
             : scale_(n)
             , corner_(corner)
             , nbox_(nb)
-            , sf_(sf) {}
+            , sf_(sf) {
+        for (int &x : nbox_) x = x > 0 ? x : 1;
+        if (sf_ == std::array<double, D>{}) sf_.fill(1.0);
+        for (double x : sf_)
+            if (!(x > 0.0)) throw Error("Non-positive scaling factor");
+    }
 
     int getScale() const { return scale_; }
     int size(int d) const { return nbox_[d]; }

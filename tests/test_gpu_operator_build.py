@@ -1,10 +1,12 @@
 """Operator construction with the 2D operator trees formed on the MI355X
 (CrossCorrelationCalculator per refinement level for all terms at once,
-src/treebuilders/CrossCorrelationCalculator.cpp:36-96) against the host
-construction, which the oracle matches bit for bit (tests/test_host_parity.py).
+src/treebuilders/CrossCorrelationCalculator.cpp:36-96) against the CPU
+oracle's construction (oracle/mrcpp_oracle.cpp build_operator) and against
+the product's host construction.
 
-Bar: bit-identical.  Every cached operator node (matrix and norm), the band
-widths at the apply precision and the result of an apply with either operator.
+Bar: bit-identical.  The kernel expansion, every cached operator node (matrix
+and norm), the band widths at the apply precision, and the result of an apply
+with either operator.
 """
 import sys
 
@@ -56,18 +58,21 @@ def _same_operator(H, D, prec):
 
 
 @pytest.mark.parametrize("name,order,box,prec,mu", CASES, ids=[c[0] for c in CASES])
-def test_device_operator_trees_are_bit_identical(mr, name, order, box, prec, mu):
+def test_device_operator_trees_are_bit_identical(mr, orc, name, order, box, prec, mu):
     mra = mr.MultiResolutionAnalysis((-box, box), order)
     H = _build(mr, mra, prec, mu, "host")
     D = _build(mr, mra, prec, mu, "device")
     assert not H.built_on_device() and D.built_on_device()
     _say(name, "terms", D.size(), "build s host", H.build_seconds(), "device", D.build_seconds())
+    # the device construction against the oracle's, node by node
+    O = orc.OracleOperator(orc.OracleMRA((-box, box), order), prec, mu=mu)
+    checked_oracle = _same_operator(O, D, prec)
     checked = _same_operator(H, D, prec)
-    _say(name, "nodes compared", checked)
-    assert checked > 100
+    _say(name, "nodes compared: oracle", checked_oracle, "host", checked)
+    assert checked_oracle > 100 and checked == checked_oracle
 
 
-def test_batched_device_construction(mr):
+def test_batched_device_construction(mr, orc):
     """Poisson and Helmholtz operators of different precisions built in one
     device pass equal their own host constructions."""
     mra = mr.MultiResolutionAnalysis((-20, 20), 5)
@@ -76,10 +81,12 @@ def test_batched_device_construction(mr):
     assert len(D) == len(specs) and all(op.built_on_device() for op in D)
     ph = D[0].build_phases()
     _say("batched build s", D[0].build_seconds(), ph)
+    omra = orc.OracleMRA((-20, 20), 5)
     for (mu, prec), d in zip(specs, D):
         h = _build(mr, mra, prec, mu, "host")
         assert isinstance(d, mr.PoissonOperator if mu is None else mr.HelmholtzOperator)
         assert _same_operator(h, d, prec) > 50
+        assert _same_operator(orc.OracleOperator(omra, prec, mu=mu), d, prec) > 50
 
 
 def test_apply_with_device_built_operator(mr):
