@@ -371,6 +371,17 @@ mwg_status mwg_mlp_forward(const float *params, const float *features, int n, fl
  * pos_weight * y * softplus(-z) + (1 - y) * softplus(z); grad[PARAMS] = the
  * mean loss.  grad holds MWG_MLP_PARAMS + 1 floats.  Deterministic. */
 mwg_status mwg_mlp_gradient(const float *params, const float *features, const float *labels, int n, float pos_weight, float *grad);
+/* mwg_mlp_gradient without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate (a training step captured in a
+ * HIP graph, loops that forbid hipMalloc).  Same arguments and the same
+ * result bit for bit.  Its scratch (512 per-wave partial vectors, 10,358,784
+ * bytes) is a file-scope device array that each GPU holds from mwg_init until
+ * the process ends, whatever n is (no input size exceeds it).  Every call on a
+ * GPU shares that GPU's array and writes all of it, so nothing carries over
+ * between calls; calls that overlap on one GPU (several host threads or
+ * streams) race on it and must be serialised by the caller.  Needs mwg_init
+ * for the current device. */
+mwg_status mwg_mlp_gradient_noalloc(const float *params, const float *features, const float *labels, int n, float pos_weight, float *grad);
 /* One Adam step (step counts from 1) on params with first/second moments m, v. */
 mwg_status mwg_mlp_adam(float *params, const float *grad, float *m, float *v, int step, float lr, float beta1, float beta2, float eps);
 

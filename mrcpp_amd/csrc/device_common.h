@@ -23,6 +23,9 @@ constexpr double kBoundSlack = 1.0 + 1.0e-9;
 
 mwg_status fail(mwg_status s, const std::string &msg);
 mwg_status need_init();
+// Device address of the allocation-free gradient's scratch on `device`
+// (current), looked up once per device.
+mwg_status mlp_scratch_setup(int device);
 
 // Per-kernel timing with HIP events on the launch stream (mwg_profile).
 struct ProfileScope {

@@ -13,6 +13,8 @@
 // wave order, so gradients do not depend on scheduling.
 #include "device_common.h"
 
+#include <mutex>
+
 namespace mwgdev {
 
 namespace {
@@ -143,6 +145,35 @@ Buffer g_partial;
 
 } // namespace mwgdev
 
+// Scratch of mwg_mlp_gradient_noalloc: the kGradWaves per-wave partial
+// vectors, 512 x 5058 floats = 10,358,784 bytes per device, held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never the
+// device memory: kernels get the address hipGetSymbolAddress returns for the
+// current device (scratch_dev, filled by mwgdev::mlp_scratch_setup).
+static __device__ __align__(256) float scratch[mwgdev::kGradWaves * (MWG_MLP_PARAMS + 1)];
+
+namespace mwgdev {
+
+namespace {
+constexpr int kScratchDevices = 64;
+float *scratch_dev[kScratchDevices] = {};
+std::mutex scratch_mutex;
+} // namespace
+
+// Device address of `scratch` on `device`, which must be current: looked up
+// once per device (called by mwg_init).
+mwg_status mlp_scratch_setup(int device) {
+    if (device < 0 || device >= kScratchDevices) return fail(MWG_ERR_UNSUPPORTED, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(scratch_mutex);
+    if (scratch_dev[device]) return MWG_OK;
+    void *p = nullptr;
+    MWG_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    scratch_dev[device] = static_cast<float *>(p);
+    return MWG_OK;
+}
+
+} // namespace mwgdev
+
 using namespace mwgdev;
 
 extern "C" {
@@ -166,6 +197,26 @@ mwg_status mwg_mlp_gradient(const float *params, const float *features, const fl
     hipLaunchKernelGGL(k_mlp_grad_partial, dim3(kGradWaves / 4), dim3(256), 0, 0, params, features, labels, n, pos_weight, g_partial.as<float>());
     MWG_LAUNCHED();
     hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((MWG_MLP_PARAMS + 1 + 255) / 256), dim3(256), 0, 0, g_partial.as<float>(), n, grad);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+
+mwg_status mwg_mlp_gradient_noalloc(const float *params, const float *features, const float *labels, int n, float pos_weight, float *grad) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (n < 0) return fail(MWG_ERR_ARG, "negative sample count");
+    if (!params || !grad || (n > 0 && (!features || !labels))) return fail(MWG_ERR_ARG, "null argument");
+    int dev = -1;
+    MWG_HIP(hipGetDevice(&dev));
+    float *part = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(scratch_mutex);
+        if (dev >= 0 && dev < kScratchDevices) part = scratch_dev[dev];
+    }
+    if (!part) return fail(MWG_ERR_ARG, "mwg_init has not been called for the current device");
+    // every partial vector is written whole by each call: nothing to clear
+    hipLaunchKernelGGL(k_mlp_grad_partial, dim3(kGradWaves / 4), dim3(256), 0, 0, params, features, labels, n, pos_weight, part);
+    MWG_LAUNCHED();
+    hipLaunchKernelGGL(k_mlp_grad_reduce, dim3((MWG_MLP_PARAMS + 1 + 255) / 256), dim3(256), 0, 0, part, n, grad);
     MWG_LAUNCHED();
     return MWG_OK;
 }

@@ -168,6 +168,8 @@ mwg_status mwg_init(int device) {
     MWG_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(MWG_ERR_ARG, "no such device");
     MWG_HIP(hipSetDevice(device));
+    const mwg_status st = mlp_scratch_setup(device);
+    if (st != MWG_OK) return st;
     // registered after the runtime's own exit handlers, so it runs before
     // them: device memory held by static objects is not freed at exit
     static const bool registered = std::atexit([] { g_exiting = true; }) == 0;
