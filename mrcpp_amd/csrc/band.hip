@@ -545,7 +545,8 @@ __global__ __launch_bounds__(UnitShape<N1>::NT) void k_band_units(const mwg_segm
 // F^T [Ox_0 | Ox_1] (tiles shared out over the waves), the y sweep for each
 // needed gx, then each wave runs the z sweep of its own (gx, gy) (rotated
 // from unit to unit) into its accumulators, the columns of a gz outside the
-// mask zeroed in the operand.  LDS holds F, T1 = [gx][i1][i2 + N1 j0], T2 = [gx][gy][i2][j0 + N1 j1].
+// mask zeroed in the operand.  LDS holds F = [i0][i1 + N1 i2], T1 = [gx][i1][j0 + N1 i2],
+// T2 = [gx][gy][i2][j1 + N1 j0] (layouts and swizzles in BandGeom).
 //
 // PREC selects the operand types:
 //   FP64: 16x16x4 f64, LDS [contracted index][row], accumulate in place;
@@ -565,14 +566,43 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 template <int N1, int PREC> struct BandGeom {
     using S = SweepShape<N1>;
     static constexpr bool kBF = PREC == MWG_PREC_BF16X3;
+    static constexpr bool k64 = PREC == MWG_PREC_FP64;
     static constexpr int KB = (N1 + 7) / 8;                  // 8-wide contraction blocks (BF16X3)
     static constexpr int KSTEPS = kBF ? 2 * KB : S::KS;     // instructions (operand fragments) per tile
     static constexpr int PK = 8 * KB;                        // row length (BF16X3 layout)
     static constexpr int ROWS = 16 * S::MT;
-    // elements of one [contracted][row] or [row][contracted] block in LDS
-    static constexpr int BLK = kBF ? ROWS * PK + (ROWS / N1 + 1) * 4 : S::KP * S::PM;
-    // LDS index of (contracted index kk, row m)
-    __device__ static inline int at(int kk, int m) { return kBF ? m * PK + (m / N1) * 4 + kk : kk * S::PM + m; }
+    // Matrix-core layouts (FP64, FP32): block [contracted index kk][row m],
+    // row pitch PM = 16 mod 32 elements, the rows XOR-swizzled inside their
+    // aligned 32-row group so that every 16-lane (and 32-lane) group of an
+    // operand read or a result write touches distinct LDS banks: reads take
+    // 16 consecutive rows of one kk; result writes put the 16 columns of one
+    // output row side by side (the column index is the fastest row index of
+    // the next block).  FP32 swizzles with 16 and needs rows up to the next 32.
+    static constexpr int ROWS_SW = k64 ? ROWS : 32 * ((S::MT + 1) / 2);
+    static constexpr int PM = ROWS_SW + ((16 - ROWS_SW % 32) + 32) % 32;
+    static constexpr int CORE = ((S::KP * PM + 63) / 64) * 64;
+    static constexpr int BF_BLK = ROWS * PK + (ROWS / N1 + 1) * 4;
+    static constexpr int BLKF = kBF ? BF_BLK : S::KP * PM;     // F
+    static constexpr int B1 = kBF ? BF_BLK : CORE + (k64 ? 0 : 32);  // stride of the two T1 blocks (gx)
+    static constexpr int B2 = kBF ? BF_BLK : CORE + (k64 ? 16 : 0);  // stride of the four T2 blocks (2 gx + gy)
+    // BF16X3: [row][contracted index] in FP32 with 4 floats of padding after every N1 rows
+    __device__ static inline int at_bf(int kk, int m) { return m * PK + (m / N1) * 4 + kk; }
+    // F: kk = i0, m = i1 + N1 i2
+    __device__ static inline int atF(int kk, int m) { return kBF ? at_bf(kk, m) : kk * PM + (m ^ ((2 * kk) & 15)); }
+    // T1 block b = gx: kk = i1, m = j0 + N1 i2
+    __device__ static inline int atT1(int b, int kk, int m) {
+        if constexpr (kBF) return b * B1 + at_bf(kk, m);
+        else if constexpr (k64) return b * B1 + kk * PM + (m ^ (8 * (b & 1)));
+        else return b * B1 + kk * PM + (m ^ (16 * ((kk >> 2) & 1)) ^ (8 * (b & 1)));
+    }
+    // T2 block b = 2 gx + gy: kk = i2, m = j1 + N1 j0
+    __device__ static inline int atT2(int b, int kk, int m) {
+        if constexpr (kBF) return b * B2 + at_bf(kk, m);
+        else if constexpr (k64) return b * B2 + kk * PM + (m ^ (8 * (b & 1)));
+        else return b * B2 + kk * PM + (m ^ (16 * ((m >> 5) & 1)) ^ (8 * (b & 1)));
+    }
+    // operand element (kk, m) of block `blk` of array `arr` (0: F, 1: T1, 2: T2)
+    __device__ static inline int at(int arr, int blk, int kk, int m) { return arr == 0 ? atF(kk, m) : arr == 1 ? atT1(blk, kk, m) : atT2(blk, kk, m); }
 };
 
 template <int PREC> struct BandTypes;
@@ -592,11 +622,12 @@ template <> struct BandTypes<MWG_PREC_BF16X3> {
     using MM = Mfma<float>;
 };
 
-// One 16x16 tile of a sweep: rows 16 mt.. of the LDS block src times the
-// operand fragments b[0..KSTEPS) of column tile nt.
-template <int N1, int PREC>
-__device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename BandTypes<PREC>::L *src, const typename BandTypes<PREC>::B *b, int mt,
-                                                             int lane) {
+// One 16x16 tile of a sweep: rows 16 mt.. of block blk of LDS array ARR
+// (0: F, 1: T1, 2: T2, each starting at lds) times the operand fragments
+// b[0..KSTEPS) of column tile nt.
+template <int N1, int PREC, int ARR>
+__device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename BandTypes<PREC>::L *lds, int blk, const typename BandTypes<PREC>::B *b,
+                                                             int mt, int lane) {
     using G = BandGeom<N1, PREC>;
     using MM = typename BandTypes<PREC>::MM;
     typename MM::C c = {};
@@ -605,7 +636,7 @@ __device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename Band
         const int m = 16 * mt + li;
 #pragma unroll
         for (int kb = 0; kb < G::KB; kb++) {
-            const float *row = src + G::at(8 * kb, m);
+            const float *row = lds + G::at(ARR, blk, 8 * kb, m);
             const float4 v0 = *reinterpret_cast<const float4 *>(row);
             const float4 v1 = *reinterpret_cast<const float4 *>(row + 4);
             const float x[8] = {v0.x, v0.y, v0.z, v0.w, v1.x, v1.y, v1.z, v1.w};
@@ -628,7 +659,7 @@ __device__ inline typename BandTypes<PREC>::MM::C sweep_tile(const typename Band
         }
     } else {
 #pragma unroll
-        for (int ks = 0; ks < G::KSTEPS; ks++) c = MM::mma(src[G::at(4 * ks + lq, 16 * mt + li)], b[ks], c);
+        for (int ks = 0; ks < G::KSTEPS; ks++) c = MM::mma(lds[G::at(ARR, blk, 4 * ks + lq, 16 * mt + li)], b[ks], c);
     }
     return c;
 }
@@ -650,12 +681,12 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
     using LT = typename BandTypes<PREC>::L;
     using BT = typename BandTypes<PREC>::B;
     using MM = typename BandTypes<PREC>::MM;
-    constexpr int N2 = S::N2, N3 = S::N3, MT = S::MT, NT = S::NT, KSTEPS = G::KSTEPS, BLK = G::BLK;
+    constexpr int N2 = S::N2, N3 = S::N3, MT = S::MT, NT = S::NT, KSTEPS = G::KSTEPS;
     constexpr int FRAG = KSTEPS * NT * 64; // operand elements per (matrix block, f bit)
     constexpr int FPT = (N3 + 255) / 256;  // band block elements per thread
-    __shared__ __align__(16) LT sF[BLK];
-    __shared__ __align__(16) LT sT1[2 * BLK];
-    __shared__ __align__(16) LT sT2[4 * BLK];
+    __shared__ __align__(16) LT sF[G::BLKF];
+    __shared__ __align__(16) LT sT1[2 * G::B1];
+    __shared__ __align__(16) LT sT2[4 * G::B2];
     __shared__ int s_unit;
 
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -665,9 +696,9 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
     const auto ud4 = ro(reinterpret_cast<const int *>(units));
     const int total = unit_first(ro(uscan)[L]);
 
-    for (int i = tid; i < BLK; i += 256) sF[i] = LT(0);
-    for (int i = tid; i < 2 * BLK; i += 256) sT1[i] = LT(0);
-    for (int i = tid; i < 4 * BLK; i += 256) sT2[i] = LT(0);
+    for (int i = tid; i < G::BLKF; i += 256) sF[i] = LT(0);
+    for (int i = tid; i < 2 * G::B1; i += 256) sT1[i] = LT(0);
+    for (int i = tid; i < 4 * G::B2; i += 256) sT2[i] = LT(0);
 
     for (int u; (u = take_unit(counter, &s_unit, total)) >= 0;) {
         // unit: groups [gi0, gi0 + ng) of one output node, contiguous in level order
@@ -748,7 +779,7 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
         take(ca, 0, ga);
         fetch(ga, ax, ay, az);
         // F[i0][i1 + N1 i2] for element e = i0 + N1 (i1 + N1 i2)
-        for (int e = tid; e < N3; e += 256) sF[G::at(e % N1, e / N1)] = (LT)ga.fsrc[e];
+        for (int e = tid; e < N3; e += 256) sF[G::atF(e % N1, e / N1)] = (LT)ga.fsrc[e];
         __syncthreads();
         // group i with its fragments in (fx, fy, fz); group i + 1 is read into nq
         // and its fragments requested into (nfx, nfy, nfz).  Two buffer sets
@@ -779,16 +810,16 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
             const unsigned gm = cur.mask;
             unsigned xneed, yneed;
             sweep_needs(gm, xneed, yneed);
-            // x sweep: T1[gx][i1][i2 + N1 j0] = sum_i0 F[i0][i1 + N1 i2] Ox_gx(i0, j0)
+            // x sweep: T1[gx][i1][j0 + N1 i2] = sum_i0 F[i0][i1 + N1 i2] Ox_gx(i0, j0)
             for (int t = w; t < MT * NT; t += 4) {
                 const int mt = t / NT, nt = t % NT;
-                const auto c = sweep_tile<N1, PREC>(sF, fx[nt], mt, lane);
+                const auto c = sweep_tile<N1, PREC, 0>(sF, 0, fx[nt], mt, lane);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int line = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
                     if (line < N2 && nn < 2 * N1) {
                         const int gx = nn / N1, j0 = nn % N1, i1 = line % N1, i2 = line / N1;
-                        sT1[gx * BLK + G::at(i1, i2 + N1 * j0)] = c[r];
+                        sT1[G::atT1(gx, i1, j0 + N1 * i2)] = c[r];
                     }
                 }
             }
@@ -797,28 +828,28 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
 #pragma unroll
                 for (int k = 0; k < FPT; k++) {
                     const int e = tid + 256 * k;
-                    if (e < N3) sF[G::at(e % N1, e / N1)] = (LT)fpre[k];
+                    if (e < N3) sF[G::atF(e % N1, e / N1)] = (LT)fpre[k];
                 }
             }
-            // y sweep: T2[gx][gy][i2][j0 + N1 j1] = sum_i1 T1[gx][i1][i2 + N1 j0] Oy_gy(i1, j1)
+            // y sweep: T2[gx][gy][i2][j1 + N1 j0] = sum_i1 T1[gx][i1][j0 + N1 i2] Oy_gy(i1, j1)
             for (int t = w; t < 2 * MT * NT; t += 4) {
                 const int gx = t / (MT * NT), tt = t % (MT * NT);
                 if (!(xneed >> gx & 1)) continue;
                 const int mt = tt / NT, nt = tt % NT;
-                const auto c = sweep_tile<N1, PREC>(sT1 + gx * BLK, fy[nt], mt, lane);
+                const auto c = sweep_tile<N1, PREC, 1>(sT1, gx, fy[nt], mt, lane);
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
                     const int m = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
                     if (m < N2 && nn < 2 * N1) {
-                        const int i2 = m % N1, j0 = m / N1, gy = nn / N1, j1 = nn % N1;
-                        sT2[(gx * 2 + gy) * BLK + G::at(i2, j0 + N1 * j1)] = c[r];
+                        const int j0 = m % N1, i2 = m / N1, gy = nn / N1, j1 = nn % N1;
+                        sT2[G::atT2(gx * 2 + gy, i2, j1 + N1 * j0)] = c[r];
                     }
                 }
             }
             __syncthreads();
-            // z sweep of this wave's (gx, gy): G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[gx][gy][i2][j0 + N1 j1] Oz_gz(i2, j2)
+            // z sweep of this wave's (gx, gy): G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[gx][gy][i2][j1 + N1 j0] Oz_gz(i2, j2)
             if (yneed >> (gxw + 2 * gyw) & 1) {
-                const LT *src = sT2 + (gxw * 2 + gyw) * BLK;
+                const int zb2 = gxw * 2 + gyw;
                 // the columns of a gz outside the mask zeroed
                 BT zb[NT][KSTEPS];
 #pragma unroll
@@ -835,9 +866,9 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
                         if constexpr (PREC == MWG_PREC_FP64) {
 #pragma unroll
                             for (int ks = 0; ks < KSTEPS; ks++)
-                                acc[mt][nt] = MM::mma(src[G::at(4 * ks + (lane >> 4), 16 * mt + li)], zb[nt][ks], acc[mt][nt]);
+                                acc[mt][nt] = MM::mma(sT2[G::atT2(zb2, 4 * ks + (lane >> 4), 16 * mt + li)], zb[nt][ks], acc[mt][nt]);
                         } else {
-                            const auto c = sweep_tile<N1, PREC>(src, zb[nt], mt, lane);
+                            const auto c = sweep_tile<N1, PREC, 2>(sT2, zb2, zb[nt], mt, lane);
 #pragma unroll
                             for (int r = 0; r < 4; r++) acc[mt][nt][r] += (double)c[r];
                         }
@@ -862,8 +893,9 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
                 for (int r = 0; r < 4; r++) {
                     const int m = 16 * mt + MM::row(lane, r), nn = 16 * nt + li;
                     if (m < N2 && nn < 2 * N1) {
-                        const int gz = nn / N1, j2 = nn % N1;
-                        dst[(gxw + 2 * gyw + 4 * gz) * N3 + m + N2 * j2] = acc[mt][nt][r];
+                        // row m = j1 + N1 j0 of T2: output element j0 + N1 j1 + N2 j2
+                        const int gz = nn / N1, j2 = nn % N1, j1 = m % N1, j0 = m / N1;
+                        dst[(gxw + 2 * gyw + 4 * gz) * N3 + j0 + N1 * j1 + N2 * j2] = acc[mt][nt][r];
                     }
                 }
     }
