@@ -301,6 +301,10 @@ def setup_batch(mr, W):
 
 
 def run_batch_bench(args):
+    print(json.dumps(batch_bench(args, args.steps, args.warmup)))
+
+
+def batch_bench(args, steps, warmup):
     """Chosen form (one apply_batch of all trees at --precision) against the
     plain form (one FP64 apply per tree), same trees; one step = all trees."""
     import numpy as np
@@ -330,10 +334,10 @@ def run_batch_bench(args):
         return outs, sts
 
     def timed(step):
-        for _ in range(max(args.warmup, 1)):
+        for _ in range(max(warmup, 1)):
             step()
         times = []
-        for _ in range(args.steps):
+        for _ in range(steps):
             t0 = time.perf_counter()
             outs, sts = step()
             times.append(time.perf_counter() - t0)
@@ -343,9 +347,21 @@ def run_batch_bench(args):
     with_pred = None
     if pred is not None and not args.no_predictor:
         w_outs, w_sts, w_times = timed(lambda: chosen_step(pred))
-        with_pred = {"ms_per_step": round(float(np.mean(w_times)) * 1e3, 4), "passes": int(max(st["n_levels"] for st in w_sts)),
+        wnum = wden = 0.0
+        wsame = 0
+        for a, b in zip(p_outs, w_outs):
+            ai, ac, _ = a.export()
+            bi, bc, _ = b.export()
+            if np.array_equal(ai, bi):
+                wsame += 1
+                wnum += float(np.sum((bc - ac) ** 2))
+                wden += float(np.sum(ac ** 2))
+        with_pred = {"form": "one apply_batch of all trees, %s sweeps + refinement predictor" % args.precision,
+                     "ms_per_step": round(float(np.mean(w_times)) * 1e3, 4),
+                     "speedup_vs_plain": round(float(np.mean(p_times)) / float(np.mean(w_times)), 3),
+                     "passes": int(max(st["n_levels"] for st in w_sts)),
                      "n_applied": int(sum(st["n_applied"] for st in w_sts)),
-                     "same_node_set_trees": int(sum(np.array_equal(a.export(False)[0], b.export(False)[0]) for a, b in zip(p_outs, w_outs)))}
+                     "same_node_set_trees": wsame, "rel_l2_same_trees": (wnum / wden) ** 0.5 if wden > 0 else None}
     mr.profile(True, kinds=("band",))  # events around the band kernel only while timing
     c_outs, c_sts, c_times = timed(chosen_step)
     prof = mr.profile_read()
@@ -368,10 +384,10 @@ def run_batch_bench(args):
             num += float(np.sum((bc - ac) ** 2))
             den += float(np.sum(ac ** 2))
     band_ms, band_launches = prof["band"]
-    band_ms_per_step = band_ms / (args.steps + max(args.warmup, 1))
+    band_ms_per_step = band_ms / (steps + max(warmup, 1))
     c_flop = sum(st["n_applied"] for st in c_sts) * flops_per_tuple(7)
     res = {
-        "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64" if args.precision in ("fp64", "fp64_vector") else args.precision + " sweeps, f64 band sum",
         "precision": args.precision, "form": "one apply_batch of all trees, %s sweeps" % args.precision,
@@ -386,13 +402,14 @@ def run_batch_bench(args):
                                 "applied_chosen": int(sum(st["n_applied"] for st in c_sts))},
         "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(c_flop / (band_ms_per_step / 1e3) / 1e12, 4),
                      "peak": FP64_PEAK_TFLOPS if args.precision in ("fp64", "fp64_vector") else 157.3, "unit": "TFLOP/s",
-                     "launches_per_step": band_launches / (args.steps + max(args.warmup, 1))},
+                     "launches_per_step": band_launches / (steps + max(warmup, 1))},
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in every.items()},
         "step_times_s": [round(t, 6) for t in c_times],
     }
     res["roofline"]["frac"] = round(res["roofline"]["achieved"] / res["roofline"]["peak"], 5)
     res["operator_build"] = time_batch_operator_build(mr, mra, specs)
-    print(json.dumps(res))
+    mr.set_sweep_precision("fp64")
+    return res
 
 
 PEAKS = {"fp64": FP64_PEAK_TFLOPS, "fp64_vector": FP64_PEAK_TFLOPS, "fp32": 157.3, "bf16x3": 2500.0}
@@ -569,6 +586,13 @@ def main():
     res["arithmetic"] = time_arithmetic(mr, w, f, P)
     res["tree_io"] = time_tree_io(plain["out"])
     res["operator_build"] = time_operator_build(mr, w, mra)
+    # the north star's comparison on the 64-tree batch: one apply_batch of all
+    # trees in the chosen form (with and without the refinement predictor)
+    # against the plain FP64 apply of each tree in turn
+    b = batch_bench(args, 3, 1)
+    res["batch64_k7"] = {k: b[k] for k in ("form", "ms_per_step", "s_per_tree", "plain_fp64_sequential", "speedup_vs_plain",
+                                           "error_vs_plain_fp64", "with_refinement_predictor", "passes_chosen", "passes_plain", "roofline",
+                                           "config")}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]

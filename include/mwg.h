@@ -59,7 +59,8 @@ mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes);
 mwg_status mwg_copy(void *dst, const void *src, size_t bytes); /* device to device */
 mwg_status mwg_zero(void *dst, size_t bytes);
 /* Entries of a device key table changed on the host: upd holds n slots, then
- * n keys, then n values (64-bit each); keys[slot] = key, vals[slot] = value.
+ * n keys of two words each (hi, lo), then n values (64-bit each);
+ * keys[2 slot], keys[2 slot + 1] = the key's words, vals[slot] = value.
  * Queued on the null stream (no wait). */
 mwg_status mwg_hash_update(unsigned long long *keys, int *vals, const long long *upd, int n);
 mwg_status mwg_synchronize(void);
@@ -152,7 +153,7 @@ typedef struct {
     int root_scale;
     int corner[3];
     int nbox[3];
-    const unsigned long long *hash_keys; /* [hash_cap], 0 = empty */
+    const unsigned long long *hash_keys; /* [hash_cap][2]: node key words (hi, lo) of keys.h, hi = 0 for an empty slot */
     const int *hash_vals;                /* [hash_cap] */
     int hash_cap;                        /* power of two */
     double *coef;                        /* coefficient pool */

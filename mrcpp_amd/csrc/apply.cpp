@@ -233,7 +233,7 @@ void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const 
         for (int s = prevScale + 1; s <= scale; s++) {
             std::array<int, 3> ls;
             for (int d = 0; d < 3; d++) ls[d] = l[d] >> (scale - s);
-            const uint64_t key = device::key_of(*job.mra, s, ls);
+            const mwgkey::Key key = device::key_of(*job.mra, s, ls);
             int id = fm.table.find(key);
             if (id < 0) {
                 id = fm.nReal + fm.nGen;

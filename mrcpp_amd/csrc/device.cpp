@@ -111,59 +111,61 @@ const double *filters(int order) {
 void KeyTable::reset(int min_entries) {
     log2cap_ = 10;
     while ((1 << log2cap_) < 2 * std::max(min_entries, 1)) log2cap_++;
-    keys_.assign((size_t)1 << log2cap_, 0ull);
+    keys_.assign((size_t)2 << log2cap_, 0ull);
     vals_.assign((size_t)1 << log2cap_, -1);
     count_ = 0;
     full_ = true;
     dirty_.clear();
 }
 
-int KeyTable::insert(uint64_t key, int id) {
+int KeyTable::insert(const mwgkey::Key &key, int id) {
     if (2 * (count_ + 1) > capacity()) {
         // grow and rehash
         std::vector<unsigned long long> ok = keys_;
         std::vector<int> ov = vals_;
         log2cap_++;
-        keys_.assign((size_t)1 << log2cap_, 0ull);
+        keys_.assign((size_t)2 << log2cap_, 0ull);
         vals_.assign((size_t)1 << log2cap_, -1);
         count_ = 0;
         full_ = true;
         dirty_.clear();
-        for (size_t i = 0; i < ok.size(); i++)
-            if (ok[i]) insert(ok[i], ov[i]);
+        for (size_t i = 0; i < ov.size(); i++)
+            if (ok[2 * i]) insert(mwgkey::Key{ok[2 * i], ok[2 * i + 1]}, ov[i]);
     }
     const uint32_t mask = (uint32_t)capacity() - 1;
     uint32_t s = mwgkey::slot(key, log2cap_);
     while (true) {
-        if (keys_[s] == 0) {
-            keys_[s] = key;
+        if (keys_[2 * (size_t)s] == 0) {
+            keys_[2 * (size_t)s] = key.hi;
+            keys_[2 * (size_t)s + 1] = key.lo;
             vals_[s] = id;
             count_++;
             if (!full_) dirty_.push_back((int)s);
             return id;
         }
-        if (keys_[s] == key) return vals_[s];
+        if (keys_[2 * (size_t)s] == key.hi && keys_[2 * (size_t)s + 1] == key.lo) return vals_[s];
         s = (s + 1) & mask;
     }
 }
 
-int KeyTable::find(uint64_t key) const {
+int KeyTable::find(const mwgkey::Key &key) const {
     const uint32_t mask = (uint32_t)capacity() - 1;
     uint32_t s = mwgkey::slot(key, log2cap_);
     while (true) {
-        if (keys_[s] == 0) return -1;
-        if (keys_[s] == key) return vals_[s];
+        if (keys_[2 * (size_t)s] == 0) return -1;
+        if (keys_[2 * (size_t)s] == key.hi && keys_[2 * (size_t)s + 1] == key.lo) return vals_[s];
         s = (s + 1) & mask;
     }
 }
 
-uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l) {
+mwgkey::Key key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l) {
     const int d = scale - mra.getRootScale();
-    if (d < 0 || d > mwgkey::kMaxDepth) throw Error("node depth outside the device key range (max depth 18)");
-    int o[3];
+    if (d < 0 || d > mwgkey::kMaxDepth) throw Error("node depth outside the device key range (max depth 30)");
+    uint32_t o[3];
     for (int i = 0; i < 3; i++) {
-        o[i] = l[i] - mra.getWorldBox().getCornerIndex()[i] * (1 << d);
-        if (o[i] < 0 || o[i] >= (1 << mwgkey::kOffsetBits)) throw Error("node translation outside the device key range");
+        const long long v = (long long)l[i] - (long long)mra.getWorldBox().getCornerIndex()[i] * (1ll << d);
+        if (v < 0 || v >= (1ll << 32)) throw Error("node translation outside the device key range");
+        o[i] = (uint32_t)v;
     }
     return mwgkey::pack(d, o[0], o[1], o[2]);
 }
@@ -172,19 +174,20 @@ uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::arr
 // since the last upload, scattered on the device.
 void TreeMirror::uploadTable() {
     const size_t cap = table.capacity();
-    if (table.fullUpload() || keys.capacity() < cap || vals.capacity() < cap) {
-        keys.reserve(cap);
+    if (table.fullUpload() || keys.capacity() < 2 * cap || vals.capacity() < cap) {
+        keys.reserve(2 * cap);
         vals.reserve(cap);
-        keys.upload(table.keys().data(), cap);
+        keys.upload(table.keys().data(), 2 * cap);
         vals.upload(table.vals().data(), cap);
     } else if (!table.dirty().empty()) {
         const std::vector<int> &d = table.dirty();
         const size_t n = d.size();
-        std::vector<long long> upd(3 * n);
+        std::vector<long long> upd(4 * n);
         for (size_t i = 0; i < n; i++) {
             upd[i] = d[i];
-            upd[n + i] = (long long)table.keys()[d[i]];
-            upd[2 * n + i] = table.vals()[d[i]];
+            upd[n + 2 * i] = (long long)table.keys()[2 * (size_t)d[i]];
+            upd[n + 2 * i + 1] = (long long)table.keys()[2 * (size_t)d[i] + 1];
+            upd[3 * n + i] = table.vals()[d[i]];
         }
         keyUpdates.reserve(upd.size());
         keyUpdates.upload(upd.data(), upd.size());

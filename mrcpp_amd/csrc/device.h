@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/mwg.h"
+#include "keys.h"
 #include "operators.h"
 #include "trees.h"
 
@@ -139,9 +140,10 @@ public:
     int log2cap() const { return log2cap_; }
     int capacity() const { return 1 << log2cap_; }
     // insert, or return the existing id
-    int insert(uint64_t key, int id);
-    int find(uint64_t key) const;
+    int insert(const mwgkey::Key &key, int id);
+    int find(const mwgkey::Key &key) const;
     int size() const { return count_; }
+    // two words (hi, lo) per slot
     const std::vector<unsigned long long> &keys() const { return keys_; }
     const std::vector<int> &vals() const { return vals_; }
     // slots filled since markUploaded(); fullUpload() after a reset or a rehash
@@ -182,7 +184,7 @@ struct TreeMirror {
     void uploadTable();
 };
 
-uint64_t key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l);
+mwgkey::Key key_of(const MultiResolutionAnalysis<3> &mra, int scale, const std::array<int, 3> &l);
 
 // A generated node planned into a mirror: its id, parent id, child index in
 // the parent and scale.

@@ -16,7 +16,7 @@
 namespace mwgdev {
 
 constexpr int kThreads = 256;
-constexpr int kMaxTerms = 512;
+constexpr int kMaxTerms = 1000; // MRCPP's MaxSepRank (api/constants.h:36)
 constexpr double kMachineZero = 1.0e-14;
 // relative slack of the screening bounds against rounding
 constexpr double kBoundSlack = 1.0 + 1.0e-9;
@@ -89,17 +89,18 @@ template <typename T> __device__ inline T ld(const_as4<T> &x) {
 }
 
 __device__ inline int f_lookup(const mwg_ftree &f, int depth, int x, int y, int z) {
-    const int ox = x - f.corner[0] * (1 << depth);
-    const int oy = y - f.corner[1] * (1 << depth);
-    const int oz = z - f.corner[2] * (1 << depth);
-    const uint64_t key = mwgkey::pack(depth, ox, oy, oz);
+    const long long span = 1ll << depth;
+    const uint32_t ox = (uint32_t)((long long)x - f.corner[0] * span);
+    const uint32_t oy = (uint32_t)((long long)y - f.corner[1] * span);
+    const uint32_t oz = (uint32_t)((long long)z - f.corner[2] * span);
+    const mwgkey::Key key = mwgkey::pack(depth, ox, oy, oz);
     const int log2cap = 31 - __clz(f.hash_cap);
     const uint32_t mask = (uint32_t)f.hash_cap - 1;
     uint32_t s = mwgkey::slot(key, log2cap);
     for (;;) {
-        const unsigned long long k = f.hash_keys[s];
-        if (k == key) return f.hash_vals[s];
-        if (k == 0ull) return -1;
+        const unsigned long long hi = f.hash_keys[2 * (size_t)s];
+        if (hi == 0ull) return -1;
+        if (hi == key.hi && f.hash_keys[2 * (size_t)s + 1] == key.lo) return f.hash_vals[s];
         s = (s + 1) & mask;
     }
 }

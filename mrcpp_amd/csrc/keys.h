@@ -1,10 +1,10 @@
-// Packed node keys and the open-addressing hash shared by host and device.
+// Node keys and the open-addressing hash shared by host and device.
 //
-// key = (depth+1) << 57 | ox << 38 | oy << 19 | oz, where depth = scale - root
-// scale and o = translation - corner * 2^depth.  Translations must satisfy
-// 0 <= o < 2^19, i.e. depth <= 18 for a box of two root nodes per direction;
-// deeper trees are rejected by the host before anything reaches the device.
-// Key 0 marks an empty hash slot.
+// A key is two 64-bit words: hi = (depth+1) << 32 | ox, lo = oy << 32 | oz,
+// where depth = scale - root scale (0..30, MRCPP's MaxDepth, api/constants.h:33)
+// and o = translation - corner * 2^depth (0 <= o < 2^32, which holds for up to
+// three root boxes per direction at depth 30).  A hash slot stores its key's
+// two words side by side; hi = 0 marks an empty slot.
 #pragma once
 
 #include <stdint.h>
@@ -17,13 +17,25 @@
 
 namespace mwgkey {
 
-constexpr int kOffsetBits = 19;
-constexpr int kMaxDepth = 18;
+constexpr int kMaxDepth = 30;
 
-MWG_HD uint64_t pack(int depth, int ox, int oy, int oz) {
-    return ((uint64_t)(depth + 1) << 57) | ((uint64_t)ox << 38) | ((uint64_t)oy << 19) | (uint64_t)oz;
+struct Key {
+    uint64_t hi, lo;
+};
+
+MWG_HD bool operator==(const Key &a, const Key &b) { return a.hi == b.hi && a.lo == b.lo; }
+
+MWG_HD Key pack(int depth, uint32_t ox, uint32_t oy, uint32_t oz) {
+    return Key{((uint64_t)(depth + 1) << 32) | (uint64_t)ox, ((uint64_t)oy << 32) | (uint64_t)oz};
 }
 
-MWG_HD uint32_t slot(uint64_t key, int log2cap) { return (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> (64 - log2cap)); }
+// multiplicative hash of both words (top log2cap bits)
+MWG_HD uint32_t slot(const Key &k, int log2cap) {
+    uint64_t h = k.hi * 0x9E3779B97F4A7C15ull;
+    h ^= (k.lo + (h >> 31)) * 0xC2B2AE3D27D4EB4Full;
+    h ^= h >> 29;
+    h *= 0x9E3779B97F4A7C15ull;
+    return (uint32_t)(h >> (64 - log2cap));
+}
 
 } // namespace mwgkey

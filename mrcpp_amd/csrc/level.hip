@@ -39,7 +39,7 @@ mwg_status check_level(const mwg_level *lvl) {
         if (S.first != next || S.count < 0) return fail(MWG_ERR_ARG, "segments do not partition the level in order");
         next += S.count;
         if (S.f.kp1 != lvl->kp1 || S.op.kp1 != lvl->kp1) return fail(MWG_ERR_ARG, "operator, level and tree orders differ");
-        if (S.op.M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 512");
+        if (S.op.M > kMaxTerms) return fail(MWG_ERR_UNSUPPORTED, "separation rank above 1000");
         if (S.op.n_mat / 4 >= (1ll << 20)) return fail(MWG_ERR_UNSUPPORTED, "operator tables above 2^20 matrix blocks");
         if (!S.g_coef || !S.counters) return fail(MWG_ERR_ARG, "segment without output pool or counters");
         if (S.band_w >= 0) {

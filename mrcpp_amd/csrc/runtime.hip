@@ -135,8 +135,9 @@ __global__ __launch_bounds__(256) void k_hash_update(unsigned long long *keys, i
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const long long slot = upd[i];
-    keys[slot] = (unsigned long long)upd[n + i];
-    vals[slot] = (int)upd[2 * (size_t)n + i];
+    keys[2 * slot] = (unsigned long long)upd[n + 2 * (size_t)i];
+    keys[2 * slot + 1] = (unsigned long long)upd[n + 2 * (size_t)i + 1];
+    vals[slot] = (int)upd[3 * (size_t)n + i];
 }
 
 } // namespace mwgdev

@@ -81,6 +81,27 @@ def test_apply_small_order3_many_sources(mr, orc):
     assert r["st"]["n_generated"] > 0
 
 
+def test_apply_deep_tree_beyond_depth_18(mr, orc):
+    """A Gaussian of exponent 1e12 (width 7e-7) in the box [-2, 2]^3: the input
+    tree reaches 23 levels below the root and the output 20, past the 64-bit
+    key range of depth 18; MRCPP allows 30 (api/constants.h:33).  Same bar as
+    the other applies against the oracle, and the Coulomb self-energy
+    sqrt(2 beta / pi) to 1e-3."""
+    beta = 1.0e12
+    src = W.GaussianSources([1.0], [beta], [(0.1, -0.2, 0.05)])
+    r = run_pair(mr, orc, src, 5, 2, 1e-4, None)
+    rel = assert_parity(r, 1e-4)
+    fi, _, _ = r["f"].export(False)
+    gi, _, _ = r["g"].export(False)
+    fdepth = int(fi[:, 0].max() - fi[:, 0].min())
+    gdepth = int(gi[:, 0].max() - gi[:, 0].min())
+    e = mr.dot(r["g"], r["f"])
+    e_ana = math.sqrt(2.0 * beta / math.pi)
+    _say("deep tree: input depth", fdepth, "output depth", gdepth, "rel L2", rel, "energy rel", abs(e - e_ana) / e_ana)
+    assert fdepth > 18 and gdepth > 18
+    assert abs(e - e_ana) <= 1e-3 * e_ana
+
+
 def test_apply_on_existing_grid(mr, orc):
     """An output grid present before apply is computed in the first pass (all
     scales at once), as ConvolutionCalculator::getInitialWorkVector does."""
