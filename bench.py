@@ -486,6 +486,7 @@ def main():
                     help="band sweep precision of the chosen form (the plain form is always fp64)")
     ap.add_argument("--predictor", action="store_true", help="chosen form with the refinement predictor")
     ap.add_argument("--no-predictor", action="store_true", help="do not time the refinement-predictor variant beside the chosen form")
+    ap.add_argument("--no-batch", action="store_true", help="leave out the 64-tree batch comparison (profiling runs of the single tree)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
@@ -589,10 +590,11 @@ def main():
     # the north star's comparison on the 64-tree batch: one apply_batch of all
     # trees in the chosen form (with and without the refinement predictor)
     # against the plain FP64 apply of each tree in turn
-    b = batch_bench(args, 3, 1)
-    res["batch64_k7"] = {k: b[k] for k in ("form", "ms_per_step", "s_per_tree", "plain_fp64_sequential", "speedup_vs_plain",
-                                           "error_vs_plain_fp64", "with_refinement_predictor", "passes_chosen", "passes_plain", "roofline",
-                                           "config")}
+    if not args.no_batch:
+        b = batch_bench(args, 3, 1)
+        res["batch64_k7"] = {k: b[k] for k in ("form", "ms_per_step", "s_per_tree", "plain_fp64_sequential", "speedup_vs_plain",
+                                               "error_vs_plain_fp64", "with_refinement_predictor", "passes_chosen", "passes_plain",
+                                               "roofline", "kernel_ms_per_step", "config")}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]

@@ -459,57 +459,86 @@ double node_budget(const Job &job, int scale) {
     return 8.0 * (box + 256.0) + 128.0 * (96.0 * ht.M + 48.0); // screen candidates; per pair: group codes, level-order records, pair record
 }
 
-void transform_and_norms(Job &job, const Shape &sh) {
+// The final transforms of every tree of a batch together: TopDown
+// (MWTree::mwTransformDown, overwrite=false) from the root depth by depth,
+// then BottomUp (MWTree::mwTransformUp), one launch per depth for the parents
+// of all trees, then the component norms of every node in one launch.  Node
+// offsets are taken from the lowest output pool of the batch (each tree keeps
+// its own pool); trees are independent, so merging them changes no result.
+void transform_and_norms(std::vector<Job> &jobs, const Shape &sh) {
     Scratch &S = scratch();
-    detail::NodeStore<3> &g = *job.g;
-    FunctionTree<3> &out = *job.task.out;
-    const auto tab = detail::real_nodes_by_depth(g);
-    // parents with children per depth: TopDown (MWTree::mwTransformDown,
-    // overwrite=false) from the root, then BottomUp (MWTree::mwTransformUp)
-    std::vector<std::vector<int>> br(tab.size());
-    for (size_t d = 0; d < tab.size(); d++)
-        for (int u : tab[d])
-            if (g.hasRealChildren(u)) br[d].push_back(u);
+    double *base = nullptr;
+    for (auto &job : jobs) {
+        double *c = job.task.out->deviceCoefs().get();
+        if (c && (!base || c < base)) base = c;
+    }
+    if (!base) return;
+    std::vector<std::vector<std::vector<int>>> br(jobs.size()); // [job][depth] parents with children
+    std::vector<long long> poolOff(jobs.size());
+    size_t depths = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        detail::NodeStore<3> &g = *jobs[j].g;
+        poolOff[j] = (long long)(((uintptr_t)jobs[j].task.out->deviceCoefs().get() - (uintptr_t)base) / sizeof(double));
+        const auto tab = detail::real_nodes_by_depth(g);
+        br[j].resize(tab.size());
+        for (size_t d = 0; d < tab.size(); d++)
+            for (int u : tab[d])
+                if (g.hasRealChildren(u)) br[j][d].push_back(u);
+        depths = std::max(depths, tab.size());
+    }
     struct Pass {
         size_t at, n;
         bool up;
     };
     std::vector<Pass> passes;
     std::vector<long long> po, co;
-    auto add = [&](const std::vector<int> &parents, bool up) {
-        if (parents.empty()) return;
-        passes.push_back({po.size(), parents.size(), up});
-        for (int u : parents) {
-            po.push_back((long long)u * sh.ncoef);
-            co.push_back((long long)g.child0(u) * sh.ncoef);
+    auto add = [&](size_t d, bool up) {
+        const size_t at = po.size();
+        for (size_t j = 0; j < jobs.size(); j++) {
+            if (d >= br[j].size()) continue;
+            const detail::NodeStore<3> &g = *jobs[j].g;
+            for (int u : br[j][d]) {
+                po.push_back(poolOff[j] + (long long)u * sh.ncoef);
+                co.push_back(poolOff[j] + (long long)g.child0(u) * sh.ncoef);
+            }
         }
+        if (po.size() > at) passes.push_back({at, po.size() - at, up});
     };
-    for (size_t d = 0; d < br.size(); d++) add(br[d], false);
-    for (int d = (int)br.size() - 2; d >= 0; d--) add(br[d], true);
+    for (size_t d = 0; d < depths; d++) add(d, false);
+    for (int d = (int)depths - 2; d >= 0; d--) add((size_t)d, true);
     // then the component norms of every node: offsets appended to po
-    const std::vector<int> all = detail::real_nodes(g);
+    std::vector<std::vector<int>> all(jobs.size());
     const size_t normAt = po.size();
-    for (int u : all) po.push_back((long long)u * sh.ncoef);
+    for (size_t j = 0; j < jobs.size(); j++) {
+        all[j] = detail::real_nodes(*jobs[j].g);
+        for (int u : all[j]) po.push_back(poolOff[j] + (long long)u * sh.ncoef);
+    }
+    const size_t nAll = po.size() - normAt;
     S.pOff.reserve(po.size());
     S.cOff.reserve(std::max<size_t>(co.size(), 1));
     S.pOff.upload(po.data(), po.size());
     S.cOff.upload(co.data(), co.size());
-    double *coef = out.deviceCoefs().get();
+    const double *filters = jobs[0].filters;
     for (const Pass &p : passes) {
         if (p.up)
-            device::check(mwg_compress(sh.kp1, job.filters, coef, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef), "compress");
+            device::check(mwg_compress(sh.kp1, filters, base, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef), "compress");
         else
-            device::check(mwg_reconstruct_add(sh.kp1, job.filters, coef, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef),
+            device::check(mwg_reconstruct_add(sh.kp1, filters, base, S.pOff.get() + p.at, S.cOff.get() + p.at, (int)p.n, sh.ncoef),
                           "reconstruct");
     }
-    S.gNorms.reserve(9 * all.size());
-    device::check(mwg_node_norms(sh.kp1, coef, S.pOff.get() + normAt, S.gNorms.get(), (int)all.size()), "norms");
-    std::vector<double> nrm(9 * all.size());
+    S.gNorms.reserve(9 * nAll);
+    device::check(mwg_node_norms(sh.kp1, base, S.pOff.get() + normAt, S.gNorms.get(), (int)nAll), "norms");
+    std::vector<double> nrm(9 * nAll);
     S.gNorms.download(nrm.data(), nrm.size());
-    for (size_t i = 0; i < all.size(); i++) {
-        double *cn = g.norms(all[i]);
-        for (int t = 0; t < 8; t++) cn[t] = nrm[9 * i + t];
-        g.setSquareNorm(all[i], nrm[9 * i + 8]);
+    size_t k = 0;
+    for (size_t j = 0; j < jobs.size(); j++) {
+        detail::NodeStore<3> &g = *jobs[j].g;
+        for (int u : all[j]) {
+            double *cn = g.norms(u);
+            for (int t = 0; t < 8; t++) cn[t] = nrm[9 * k + t];
+            g.setSquareNorm(u, nrm[9 * k + 8]);
+            k++;
+        }
     }
 }
 
@@ -872,16 +901,17 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
     std::vector<unsigned long long> na(2 * jobs.size());
     S.counters.download(na.data(), na.size());
     std::set<device::TreeMirror *> mirrors;
+    for (auto &job : jobs) collect_samples(job);
+    auto tt = clk::now();
+    transform_and_norms(jobs, sh);
+    const double tTransform = since(tt);
     for (size_t i = 0; i < jobs.size(); i++) {
         Job &job = jobs[i];
-        collect_samples(job);
-        auto tt = clk::now();
-        transform_and_norms(job, sh);
         detail::calc_square_norm(*job.g);
         job.task.out->markDeviceResident();
         job.stats.nApplied = (long long)na[2 * i];
         job.stats.nPairs = (long long)na[2 * i + 1];
-        job.stats.tTransform = since(tt);
+        job.stats.tTransform = tTransform;
         job.stats.nOutputNodes = (long long)detail::real_nodes(*job.g).size();
     }
     const auto tc = clk::now();
