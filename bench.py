@@ -404,6 +404,8 @@ def batch_bench(args, steps, warmup):
                      "peak": FP64_PEAK_TFLOPS if args.precision in ("fp64", "fp64_vector") else 157.3, "unit": "TFLOP/s",
                      "launches_per_step": band_launches / (steps + max(warmup, 1))},
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in every.items()},
+        "host_breakdown_s": {k: c_sts[0][k] for k in ("seconds", "t_setup", "t_screen", "t_generate", "t_band", "t_transform", "t_host",
+                                                      "t_cleanup")},
         "step_times_s": [round(t, 6) for t in c_times],
     }
     res["roofline"]["frac"] = round(res["roofline"]["achieved"] / res["roofline"]["peak"], 5)

@@ -41,8 +41,10 @@ int mwg_device_count(void);
 
 /* Device memory helpers (bytes).  Freed blocks are kept in a per-size pool
  * and reused by later allocations; mwg_release_pool returns them.  mwg_free
- * waits for the device first, so work enqueued before it has finished with
- * the block. */
+ * does not wait: the library's work runs in order on the null stream, so
+ * work enqueued after a free (the only work that can get the block again)
+ * runs after the work enqueued before it.  Callers that touch device memory
+ * from other streams must synchronise before mwg_free. */
 mwg_status mwg_malloc(void **ptr, size_t bytes);
 mwg_status mwg_free(void *ptr);
 /* Page-locked host memory for large transfers: mwg_upload / mwg_download with

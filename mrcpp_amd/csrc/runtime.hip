@@ -210,10 +210,10 @@ mwg_status mwg_malloc(void **ptr, size_t bytes) {
 mwg_status mwg_free(void *ptr) {
     if (!ptr) return MWG_OK;
     if (g_exiting) return MWG_OK; // process exit: the runtime may be gone, the memory goes with the process
-    // the block is handed out again only after the work enqueued before this
-    // free (kernels, asynchronous copies) has finished with it
-    MWG_HIP(hipDeviceSynchronize());
-    g_stage_used = 0;
+    // No wait: every kernel and copy of the library runs on the null stream
+    // in order, and a block handed out again is only used by work enqueued
+    // after this free, which the stream runs after the work that still uses
+    // the block (synchronous host copies are ordered on the same stream).
     std::lock_guard<std::mutex> lk(pool_mutex());
     auto it = block_size().find(ptr);
     if (it == block_size().end()) return fail(MWG_ERR_ARG, "mwg_free of an unknown pointer");
