@@ -57,9 +57,13 @@ int mrcpp_tree_export(void *tree, int *idx, double *coefs, double *norms);
  * computed on the GPU, 1 <= n <= 8, out distinct from the inputs. */
 int mrcpp_add(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec);
 /* multiply(prec, out, FunctionTreeVector, maxIter, absPrec) (src/treebuilders/multiply.cpp:52-100):
- * out = prod_i c[i] trees[i] (WaveletAdaptor refinement; the useMaxNorms
- * variant is not provided), on the GPU, 1 <= n <= 8. */
+ * out = prod_i c[i] trees[i] with WaveletAdaptor refinement, on the GPU, 1 <= n <= 8. */
 int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec);
+/* multiply(prec, out, FunctionTreeVector, maxIter, absPrec, useMaxNorms)
+ * (src/treebuilders/multiply.cpp:104-119): use_max_norms != 0 refines with
+ * MultiplicationAdaptor (src/treebuilders/MultiplicationAdaptor.h:46-66) from
+ * the two operands' largest norms (n must be 2); 0 is mrcpp_multiply. */
+int mrcpp_multiply_ex(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec, int use_max_norms);
 /* FunctionTree::rescale (src/trees/FunctionTree.cpp:651-668): end nodes times c,
  * then the bottom-up transform and norms (on the GPU for trees held in HBM) */
 int mrcpp_tree_rescale(void *tree, double c);

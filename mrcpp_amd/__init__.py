@@ -50,6 +50,7 @@ SIGNATURES = {
     "mrcpp_tree_dot": (ctypes.c_double, [_vp, _vp]),
     "mrcpp_add": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
     "mrcpp_multiply": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]),
+    "mrcpp_multiply_ex": (ctypes.c_int, [ctypes.c_double, _vp, ctypes.c_int, _dp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "mrcpp_tree_rescale": (ctypes.c_int, [_vp, ctypes.c_double]),
     "mrcpp_apply_components": (ctypes.c_int, [ctypes.c_double, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp), ctypes.c_int, _dp, ctypes.c_int,
                                               ctypes.c_int]),
@@ -423,19 +424,24 @@ def add(prec, out, *args, maxIter=-1, absPrec=False):
     _arith(lib().mrcpp_add, prec, out, terms, maxIter, absPrec)
 
 
-def multiply(prec, out, *args, maxIter=-1, absPrec=False):
-    """multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec]) or
-    multiply(prec, out, [(c, tree), ...][, maxIter, absPrec]) on the GPU."""
+def multiply(prec, out, *args, maxIter=-1, absPrec=False, useMaxNorms=False):
+    """multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec, useMaxNorms]) or
+    multiply(prec, out, [(c, tree), ...][, maxIter, absPrec, useMaxNorms]) on the GPU;
+    useMaxNorms refines with MRCPP's MultiplicationAdaptor (two trees)."""
     if args and not isinstance(args[0], (list, tuple)):
         if len(args) < 3:
-            raise TypeError("multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec])")
+            raise TypeError("multiply(prec, out, c, tree_a, tree_b[, maxIter, absPrec, useMaxNorms])")
         args = ([(args[0], args[1]), (1.0, args[2])],) + tuple(args[3:])
     terms, rest = _terms(args, "multiply")
     if rest:
         maxIter = rest[0]
     if len(rest) > 1:
         absPrec = rest[1]
-    _arith(lib().mrcpp_multiply, prec, out, terms, maxIter, absPrec)
+    if len(rest) > 2:
+        useMaxNorms = rest[2]
+    c = np.ascontiguousarray([t[0] for t in terms], dtype=np.float64)
+    hs = (_vp * max(len(terms), 1))(*[t[1]._h for t in terms])
+    _check(lib().mrcpp_multiply_ex(float(prec), out._h, len(terms), _p(c), hs, int(maxIter), int(bool(absPrec)), int(bool(useMaxNorms))))
 
 
 def dot(bra, ket):

@@ -17,6 +17,7 @@
 
 #include "function_tree.h"
 #include "keys.h"
+#include "max_norms.h"
 #include "predictor.h"
 
 namespace mrcpp {
@@ -63,90 +64,17 @@ constexpr double kBatchBudget = 16.0e9;
 class PrecFactors {
 public:
     explicit PrecFactors(const std::vector<FunctionTree<3> *> &trees) {
-        for (FunctionTree<3> *t : trees) {
-            Src src;
-            src.s = &t->nodes();
-            const detail::NodeStore<3> &s = *src.s;
-            src.maxSq.assign(s.size(), -1.0);
-            const std::vector<int> order = detail::real_nodes(s); // parents before children
-            for (auto it = order.rbegin(); it != order.rend(); ++it) {
-                const int u = *it;
-                double m = scaled(s.scale(u), s.squareNorm(u));
-                if (s.hasRealChildren(u))
-                    for (int c = 0; c < 8; c++) m = std::max(m, src.maxSq[s.child0(u) + c]);
-                src.maxSq[u] = m;
-            }
-            src_.push_back(std::move(src));
-        }
+        for (FunctionTree<3> *t : trees) src_.emplace_back(t->nodes());
     }
 
     double operator()(int n, const std::array<int, 3> &l) {
         double maxNorm = src_.empty() ? 1.0 : 0.0;
-        for (Src &src : src_) maxNorm = std::max(maxNorm, std::sqrt(maxSquareNorm(src, n, l)));
+        for (detail::MaxNorms &m : src_) maxNorm = std::max(maxNorm, std::sqrt(m.at(n, l).maxSq));
         return 1.0 / maxNorm;
     }
 
 private:
-    using Key = std::pair<int, std::array<int, 3>>;
-    struct Gen {
-        double sq;
-        std::vector<double> block; // scaling block
-    };
-    struct Src {
-        const detail::NodeStore<3> *s = nullptr;
-        std::vector<double> maxSq;
-        std::map<Key, Gen> gen;
-    };
-    std::vector<Src> src_;
-
-    static double scaled(int n, double sq) { return std::pow(2.0, 3 * n) * sq; }
-
-    static double maxSquareNorm(Src &src, int n, const std::array<int, 3> &l) {
-        const detail::NodeStore<3> &s = *src.s;
-        const int u = s.findReal(n, l);
-        if (u >= 0) return src.maxSq[u] > 0.0 ? src.maxSq[u] : scaled(n, s.squareNorm(u));
-        auto hit = src.gen.find({n, l});
-        if (hit != src.gen.end()) return scaled(n, hit->second.sq);
-        // nearest real or generated ancestor, then generate down to (n, l)
-        int an = n, real = -1;
-        std::array<int, 3> al = l;
-        const Gen *gen = nullptr;
-        for (;;) {
-            if (an <= s.rootScale()) throw Error("precision tree: node outside the world box");
-            an--;
-            for (int d = 0; d < 3; d++) al[d] >>= 1;
-            if ((real = s.findReal(an, al)) >= 0) break;
-            auto g = src.gen.find({an, al});
-            if (g != src.gen.end()) {
-                gen = &g->second;
-                break;
-            }
-        }
-        const int kd = s.kp1_d();
-        std::vector<double> kids((size_t)8 * kd), parent;
-        if (gen) parent = gen->block;
-        for (int sc = an + 1; sc <= n; sc++) {
-            const bool scalingOnly = real < 0 || sc > an + 1;
-            detail::children_scaling_blocks(s, (sc == an + 1 && real >= 0) ? s.coefs(real) : parent.data(), scalingOnly, kids.data());
-            std::array<int, 3> pl;
-            for (int d = 0; d < 3; d++) pl[d] = l[d] >> (n - sc + 1);
-            for (int c = 0; c < 8; c++) {
-                std::array<int, 3> cl;
-                for (int d = 0; d < 3; d++) cl[d] = 2 * pl[d] + ((c >> d) & 1);
-                Gen g;
-                g.block.assign(kids.begin() + (size_t)c * kd, kids.begin() + (size_t)(c + 1) * kd);
-                double ss = 0.0;
-                for (double x : g.block) ss += x * x;
-                const double cn = std::sqrt(ss); // MWNode::calcNorms of a generated node: the scaling component only
-                g.sq = cn * cn;
-                src.gen.emplace(Key{sc, cl}, std::move(g));
-            }
-            std::array<int, 3> ol;
-            for (int d = 0; d < 3; d++) ol[d] = l[d] >> (n - sc);
-            parent = src.gen.at({sc, ol}).block;
-        }
-        return scaled(n, src.gen.at({n, l}).sq);
-    }
+    std::vector<detail::MaxNorms> src_;
 };
 
 // One tree of a batch and its state across breadth-first steps.

@@ -10,6 +10,7 @@
 #include <set>
 
 #include "function_tree.h"
+#include "max_norms.h"
 
 namespace mrcpp {
 
@@ -69,7 +70,22 @@ void compress_and_norms(FunctionTree<3> &out, const double *filters) {
     detail::calc_square_norm(g);
 }
 
-void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec) {
+// MultiplicationAdaptor::splitNode (src/treebuilders/MultiplicationAdaptor.h:46-66):
+// split where maxW0 maxS1 + maxW1 maxS0 + maxW0 maxW1 > prec (square roots of
+// the operands' largest scaled norms at the node's index) unless both
+// operands are leaves there.
+struct MaxNormSplit {
+    detail::MaxNorms a, b;
+    bool operator()(double prec, int n, const std::array<int, 3> &l) {
+        const detail::MaxNorms::Info p0 = a.at(n, l), p1 = b.at(n, l);
+        const double maxW0 = std::sqrt(p0.maxWSq), maxW1 = std::sqrt(p1.maxWSq);
+        const double maxS0 = std::sqrt(p0.maxSq), maxS1 = std::sqrt(p1.maxSq);
+        const double multNorm = maxW0 * maxS1 + maxW1 * maxS0 + maxW0 * maxW1;
+        return multNorm > prec && !(p0.leaf && p1.leaf);
+    }
+};
+
+void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec, bool useMaxNorms = false) {
     const char *what = mode == Sum ? "add" : "multiply";
     if (inp.empty()) throw Error(std::string(what) + ": no input trees");
     if (inp.size() > MWG_MAX_OPERANDS) throw Error(std::string(what) + ": at most 8 input trees");
@@ -99,6 +115,12 @@ void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVec
     cb.vcmap = bt.vcmap.get();
     cb.cvmap = bt.cvmap.get();
 
+    // the operands' largest norms, before any node is generated below their leaves
+    std::unique_ptr<MaxNormSplit> maxNorms;
+    if (useMaxNorms) {
+        if (inp.size() != 2) throw Error("multiply: the max-norm refinement needs exactly two input trees");
+        maxNorms.reset(new MaxNormSplit{detail::MaxNorms(std::get<1>(inp[0])->nodes()), detail::MaxNorms(std::get<1>(inp[1])->nodes())});
+    }
     ArithScratch &S = arith_scratch();
     detail::NodeStore<3> &g = out.topology();
     std::vector<int> work = detail::end_nodes(g); // TreeCalculator::getInitialWorkVector
@@ -158,7 +180,9 @@ void combine(Mode mode, double prec, FunctionTree<3> &out, const FunctionTreeVec
         std::vector<int> next;
         for (int u : work) {
             if (g.hasChildren(u) || g.scale(u) + 2 > maxScale) continue;
-            if (detail::wavelet_split(prec, g.treeSquareNorm, absPrec, g.scale(u), g.waveletNorm(u))) {
+            const bool split = maxNorms ? (*maxNorms)(prec, g.scale(u), g.transl(u))
+                                        : detail::wavelet_split(prec, g.treeSquareNorm, absPrec, g.scale(u), g.waveletNorm(u));
+            if (split) {
                 const int c0 = g.createChildren(u, false);
                 for (int c = 0; c < 8; c++) next.push_back(c0 + c);
             }
@@ -278,8 +302,7 @@ void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_
 
 void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter, bool absPrec, bool useMaxNorms,
               bool /*conjugate*/) {
-    if (useMaxNorms) throw Error("multiply: the max-norm refinement (useMaxNorms) is not supported");
-    combine(Product, prec, out, inp, maxIter, absPrec);
+    combine(Product, prec, out, inp, maxIter, absPrec, useMaxNorms);
 }
 
 void multiply(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter, bool absPrec, bool useMaxNorms,

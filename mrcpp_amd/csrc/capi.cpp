@@ -135,6 +135,12 @@ int mrcpp_multiply(double prec, void *out, int n, const double *c, void **trees,
     return guarded([&] { multiply(prec, *static_cast<FunctionTree<3> *>(out), tree_vector(n, c, trees), max_iter, abs_prec != 0); });
 }
 
+int mrcpp_multiply_ex(double prec, void *out, int n, const double *c, void **trees, int max_iter, int abs_prec, int use_max_norms) {
+    return guarded([&] {
+        multiply(prec, *static_cast<FunctionTree<3> *>(out), tree_vector(n, c, trees), max_iter, abs_prec != 0, use_max_norms != 0);
+    });
+}
+
 int mrcpp_tree_rescale(void *tree, double c) {
     return guarded([&] { static_cast<FunctionTree<3> *>(tree)->rescale(c); });
 }

@@ -145,9 +145,10 @@ void add(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, in
 void add(double prec, FunctionTree<3> &out, const std::vector<FunctionTree<3> *> &inp, int maxIter = -1, bool absPrec = false,
          bool conjugate = false);
 // multiply(prec, out, c, inp_a, inp_b, maxIter, absPrec, useMaxNorms) and the
-// FunctionTreeVector form (src/treebuilders/multiply.cpp:52-100) with
-// MultiplicationCalculator and WaveletAdaptor (useMaxNorms, the
-// MultiplicationAdaptor refinement, is not supported).
+// FunctionTreeVector form (src/treebuilders/multiply.cpp:52-119) with
+// MultiplicationCalculator and WaveletAdaptor, or with useMaxNorms the
+// MultiplicationAdaptor refinement from the two operands' largest norms
+// (src/treebuilders/MultiplicationAdaptor.h:46-66).
 void multiply(double prec, FunctionTree<3> &out, double c, FunctionTree<3> &inp_a, FunctionTree<3> &inp_b, int maxIter = -1,
               bool absPrec = false, bool useMaxNorms = false, bool conjugate = false);
 void multiply(double prec, FunctionTree<3> &out, const FunctionTreeVector<3> &inp, int maxIter = -1, bool absPrec = false,

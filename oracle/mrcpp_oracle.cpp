@@ -449,7 +449,8 @@ template <int D> struct Node {
     std::vector<double> coefs;
     double cn[TD];
     double sq = -1.0;
-    double maxSq = -1.0; // MWNode::maxSquareNorm (src/trees/MWNode.h:180)
+    double maxSq = -1.0;  // MWNode::maxSquareNorm (src/trees/MWNode.h:180)
+    double maxWSq = -1.0; // MWNode::maxWSquareNorm
     omp_lock_t lock;
 
     Node() {
@@ -991,16 +992,21 @@ template <int D> static bool split_check(const Node<D> &nd, double prec, double 
 // positive maximum was set.
 template <int D> static double scaled_square_norm(const Node<D> &nd) { return std::pow(2.0, D * nd.n) * nd.sq; }
 template <int D> static double max_square_norm(const Node<D> &nd) { return nd.maxSq > 0.0 ? nd.maxSq : scaled_square_norm(nd); }
+// MWNode::calcScaledWSquareNorm / getMaxWSquareNorm (src/trees/MWNode.h:85,213)
+template <int D> static double scaled_wsquare_norm(const Node<D> &nd) { return std::pow(2.0, D * nd.n) * nd.waveletNorm(); }
+template <int D> static double max_wsquare_norm(const Node<D> &nd) { return nd.maxWSq > 0.0 ? nd.maxWSq : scaled_wsquare_norm(nd); }
 
 // MWNode::setMaxSquareNorm (src/trees/MWNode.cpp:1257-1268) from every root
 // (MWTree::makeMaxSquareNorms, src/trees/MWTree.cpp:536-543).
 template <int D> static void set_max_square_norm(Node<D> &nd) {
     nd.maxSq = scaled_square_norm(nd);
+    nd.maxWSq = scaled_wsquare_norm(nd);
     if (nd.isEnd()) return;
     for (int i = 0; i < Node<D>::TD; i++) {
         Node<D> &c = nd.child(i);
         set_max_square_norm(c);
         nd.maxSq = std::max(nd.maxSq, c.maxSq);
+        nd.maxWSq = std::max(nd.maxWSq, c.maxWSq);
     }
 }
 template <int D> static void make_max_square_norms(Tree<D> &t) {
@@ -1722,12 +1728,39 @@ static void add(double prec, Tree<3> &out, const TreeVec &inp, int maxIter, bool
     for (const auto &ci : inp) delete_generated(*ci.second);
 }
 
-static void multiply(double prec, Tree<3> &out, const TreeVec &inp, int maxIter, bool absPrec) {
+// MultiplicationAdaptor::splitNode (src/treebuilders/MultiplicationAdaptor.h:46-66)
+struct MultiplicationAdaptor : Adaptor<3> {
+    double prec;
+    Tree<3> *t0, *t1;
+    MultiplicationAdaptor(double p, int ms, Tree<3> *a, Tree<3> *b)
+            : Adaptor<3>(ms)
+            , prec(p)
+            , t0(a)
+            , t1(b) {}
+    bool split(const Node<3> &nd) const override {
+        Node<3> &p0 = get_node(*t0, nd.n, nd.l);
+        Node<3> &p1 = get_node(*t1, nd.n, nd.l);
+        const double maxW0 = std::sqrt(max_wsquare_norm(p0)), maxW1 = std::sqrt(max_wsquare_norm(p1));
+        const double maxS0 = std::sqrt(max_square_norm(p0)), maxS1 = std::sqrt(max_square_norm(p1));
+        const double multNorm = maxW0 * maxS1 + maxW1 * maxS0 + maxW0 * maxW1;
+        return multNorm > prec && !(!p0.isBranch() && !p1.isBranch());
+    }
+};
+
+static void multiply(double prec, Tree<3> &out, const TreeVec &inp, int maxIter, bool absPrec, bool useMaxNorms = false) {
     for (const auto &ci : inp)
         if (ci.second->mra != out.mra) throw Error("Incompatible MRA");
     MultiplicationCalculator calc(inp, *out.mra->basis);
-    WaveletSplit<3> ad(prec, out.mra->maxScale(), absPrec);
-    build(out, calc, ad, maxIter);
+    if (useMaxNorms) {
+        // multiply.cpp:110-114
+        if (inp.size() != 2) throw Error("Invalid tree vec size");
+        for (const auto &ci : inp) make_max_square_norms(*ci.second);
+        MultiplicationAdaptor ad(prec, out.mra->maxScale(), inp[0].second, inp[1].second);
+        build(out, calc, ad, maxIter);
+    } else {
+        WaveletSplit<3> ad(prec, out.mra->maxScale(), absPrec);
+        build(out, calc, ad, maxIter);
+    }
     mw_transform_up(out);
     calc_square_norm(out);
     for (const auto &ci : inp) delete_generated(*ci.second);
@@ -2064,6 +2097,7 @@ int orc_tree_export(void *tp, int *idx, double *coefs, double *norms) {
 }
 
 // out = sum_i c_i trees_i (mode 0) or prod_i c_i trees_i (mode 1)
+// (mode 2: the product with useMaxNorms)
 int orc_arith(int mode, double prec, void *outp, int n, const double *c, void **trees, int maxIter, int absPrec) {
     try {
         TreeVec v;
@@ -2072,7 +2106,7 @@ int orc_arith(int mode, double prec, void *outp, int n, const double *c, void **
         if (mode == 0)
             add(prec, out, v, maxIter, absPrec != 0);
         else
-            multiply(prec, out, v, maxIter, absPrec != 0);
+            multiply(prec, out, v, maxIter, absPrec != 0, mode == 2);
         return 0;
     } catch (const std::exception &e) {
         g_last_error = e.what();

@@ -268,6 +268,7 @@ def add(prec, out, terms, max_iter=-1, abs_prec=False):
     _arith(0, prec, out, terms, max_iter, abs_prec)
 
 
-def multiply(prec, out, terms, max_iter=-1, abs_prec=False):
-    """multiply(prec, out, [(c_i, tree_i), ...]) restated (src/treebuilders/multiply.cpp:52-100)."""
-    _arith(1, prec, out, terms, max_iter, abs_prec)
+def multiply(prec, out, terms, max_iter=-1, abs_prec=False, use_max_norms=False):
+    """multiply(prec, out, [(c_i, tree_i), ...]) restated (src/treebuilders/multiply.cpp:52-119);
+    use_max_norms refines with MultiplicationAdaptor (two trees)."""
+    _arith(2 if use_max_norms else 1, prec, out, terms, max_iter, abs_prec)

@@ -114,3 +114,68 @@ def test_multiply_by_a_constant(orc):
     assert L.orc_tree_integrate(out.h) == pytest.approx(3.0 * L.orc_tree_integrate(f.h), rel=1e-5)
     for r in ((0.1, 0.2, -0.3), (0.2, 0.0, -0.2)):
         assert L.orc_tree_evalf(out.h, *r) == pytest.approx(3.0 * L.orc_tree_evalf(f.h, *r), rel=1e-5)
+
+
+def _scaled_maxima(idx, norms):
+    """Per exported node (scale, lx, ly, lz): the largest 2^(3n) |c|^2 and
+    2^(3n) |w|^2 over the node and its descendants, and whether it is a leaf."""
+    key = {tuple(int(x) for x in r[:4]): i for i, r in enumerate(idx)}
+    sq = norms[:, 8] * 2.0 ** (3 * idx[:, 0])
+    wsq = np.sum(norms[:, 1:8] ** 2, axis=1) * 2.0 ** (3 * idx[:, 0])
+    ms, mw, leaf = sq.copy(), wsq.copy(), np.ones(len(idx), bool)
+    for i in np.argsort(-idx[:, 0]):  # deepest first: children before parents
+        n, l = int(idx[i, 0]), idx[i, 1:4]
+        if n > idx[:, 0].min():
+            p = key[(n - 1, *(int(x) >> 1 for x in l))]
+            ms[p] = max(ms[p], ms[i])
+            mw[p] = max(mw[p], mw[i])
+            leaf[p] = False
+    return key, ms, mw, leaf
+
+
+def test_multiply_with_max_norms(orc):
+    """multiply(..., useMaxNorms=true) (MultiplicationAdaptor,
+    src/treebuilders/MultiplicationAdaptor.h:46-66): the product of two
+    Gaussians integrates to the closed-form overlap and matches the Gaussian
+    product pointwise; the output never goes deeper than the deeper input; a
+    node both inputs hold as real nodes is split exactly when
+    maxW0 maxS1 + maxW1 maxS0 + maxW0 maxW1 > prec (recomputed here from the
+    exported inputs) and neither is a leaf of both; a coarse prec refines
+    nothing."""
+    m = orc.OracleMRA((-16, 16), 5)
+    a, b = 3.0, 40.0
+    A, B = np.array([0.3, -0.2, 0.1]), np.array([-0.4, 0.5, 0.0])
+    f = _gauss_tree(orc, m, a, A, prec=1e-6)
+    g = _gauss_tree(orc, m, b, B, 2.0, prec=1e-6)
+    prec = 1e-6
+    out = orc.OracleTree(m)
+    orc.multiply(prec, out, [(1.0, f), (1.0, g)], use_max_norms=True)
+    L = _lib(orc)
+    exact = 2.0 * (a * b / math.pi ** 2) ** 1.5 * (math.pi / (a + b)) ** 1.5 * math.exp(-a * b / (a + b) * np.sum((A - B) ** 2))
+    assert L.orc_tree_integrate(out.h) == pytest.approx(exact, rel=1e-5)
+    for r in ((0.0, 0.1, 0.0), (-0.2, 0.3, 0.05), (-0.4, 0.45, 0.02)):
+        r = np.array(r)
+        want = 2.0 * (a * b / math.pi ** 2) ** 1.5 * math.exp(-a * np.sum((r - A) ** 2) - b * np.sum((r - B) ** 2))
+        assert L.orc_tree_evalf(out.h, *r) == pytest.approx(want, rel=1e-3, abs=1e-5)
+    fi, _, fn = f.export()
+    gi, _, gn = g.export()
+    oi, _, _ = out.export()
+    assert oi[:, 0].max() <= max(fi[:, 0].max(), gi[:, 0].max())
+    fk, fms, fmw, fleaf = _scaled_maxima(fi, fn)
+    gk, gms, gmw, gleaf = _scaled_maxima(gi, gn)
+    ok, _, _, oleaf = _scaled_maxima(oi, np.zeros((len(oi), 9)))
+    checked = 0
+    for key, i in ok.items():
+        if key not in fk or key not in gk:
+            continue
+        p, q = fk[key], gk[key]
+        w0, w1, s0, s1 = math.sqrt(fmw[p]), math.sqrt(gmw[q]), math.sqrt(fms[p]), math.sqrt(gms[q])
+        split = w0 * s1 + w1 * s0 + w0 * w1 > prec and not (fleaf[p] and gleaf[q])
+        if key[0] + 2 > m.max_scale:
+            continue
+        assert split == (not oleaf[i]), key
+        checked += 1
+    assert checked > 20
+    coarse = orc.OracleTree(m)
+    orc.multiply(1e3, coarse, [(1.0, f), (1.0, g)], use_max_norms=True)
+    assert len(coarse.export()[0]) == 8  # the 2 x 2 x 2 roots only

@@ -1,5 +1,6 @@
-"""add and multiply on the MI355X (src/treebuilders/add.cpp:43-80,
-src/treebuilders/multiply.cpp:52-100) against the oracle's restatement, which
+"""add and multiply (WaveletAdaptor and, with useMaxNorms, MultiplicationAdaptor
+refinement) on the MI355X (src/treebuilders/add.cpp:43-80,
+src/treebuilders/multiply.cpp:52-119) against the oracle's restatement, which
 tests/test_arith_oracle.py pins to closed forms.
 
 Inputs are projected on the host so both sides start from bit-identical trees.
@@ -67,6 +68,12 @@ def test_add_and_multiply_match_the_oracle(mr, orc, name, order, box, prec):
     op = orc.OracleTree(omra)
     orc.multiply(prec, op, [(2.0, of), (1.0, og)])
     _compare(name + " multiply", p, op, 1e-14)
+    # the max-norm refinement (MultiplicationAdaptor, useMaxNorms = true)
+    q = mr.FunctionTree(mra)
+    mr.multiply(prec, q, 2.0, f, g, useMaxNorms=True)
+    oq = orc.OracleTree(omra)
+    orc.multiply(prec, oq, [(2.0, of), (1.0, og)], use_max_norms=True)
+    _compare(name + " multiply useMaxNorms", q, oq, 1e-14)
 
 
 def test_vector_forms_shared_inputs_and_device_resident_operands(mr, orc):
@@ -154,3 +161,5 @@ def test_arithmetic_arguments_are_checked(mr):
         mr.add(1e-4, mr.FunctionTree(mra), [(1.0, f)] * 9)  # more than 8 terms
     with pytest.raises(mr.MRCPPError):
         mr.multiply(1e-4, mr.FunctionTree(mra), [])
+    with pytest.raises(mr.MRCPPError):
+        mr.multiply(1e-4, mr.FunctionTree(mra), [(1.0, f)] * 3, useMaxNorms=True)  # the max-norm refinement takes two trees
