@@ -669,15 +669,31 @@ void replay(Job &job) {
         job.iter++;
         job.replayWork.swap(next);
         if (!pending.empty()) {
-            // the pass that computes the missed children also recomputes the
-            // step's speculative nodes with the step's own threshold (the
-            // pass's threshold is the one after this step), so the step is
-            // then judged without an uncertainty check and its own pass
+            // The pass that computes the missed children also recomputes,
+            // with the step's own screening threshold (the pass's), every
+            // speculative node of the step that might split: the step is then
+            // judged without a pass of its own for them.  A node whose wavelet
+            // norm stays below the split threshold of the smallest possible
+            // approximate norm (the missed children adding nothing, every
+            // speculative norm at the low end of its bound) keeps its value;
+            // the uncertainty check of its step still judges it.
+            std::vector<std::pair<int, double>> spec;
+            double low = job.wNorm - job.normErr;
             for (int u : job.replayWork)
                 if (is_speculative(job, u)) {
-                    job.speculative[u] = 0;
-                    pending.push_back(u);
+                    const double B = speculative_bound(job, u);
+                    spec.push_back({u, B});
+                    if (g.depth(u) >= 0 && std::isfinite(B)) low += std::pow(std::max(0.0, std::sqrt(g.waveletNorm(u)) - B), 2);
                 }
+            const double normLow = (job.sNorm < 0.0 || low < 0.0) ? -1.0 : job.sNorm + low;
+            for (const auto &sb : spec) {
+                const int u = sb.first;
+                const double thrLow = detail::wavelet_threshold(node_prec(job, u), normLow, job.task.absPrec, g.scale(u));
+                const bool surelyNoSplit = std::isfinite(sb.second) && normLow > 0.0 && std::sqrt(g.waveletNorm(u)) + sb.second < thrLow;
+                if (surelyNoSplit) continue;
+                job.speculative[u] = 0;
+                pending.push_back(u);
+            }
             job.stats.nFallback += (long long)pending.size();
             job.work.swap(pending);
             return;
