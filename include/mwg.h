@@ -260,6 +260,19 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
  * input tree of its output node's segment. */
 mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_pairs *pairs, int npairs, int precision);
 
+/* Work distribution of the matrix-core band kernel at order 7 (kp1 = 8) for
+ * MWG_PREC_FP64 and MWG_PREC_FP32.  MWG_BAND_ROWS: each wave of a workgroup
+ * owns 16 rows of every sweep, so only the x -> y step crosses waves (one
+ * workgroup barrier per group) and the z sweep is shared evenly whatever the
+ * group's gt mask.  MWG_BAND_TILES: the tiles of each sweep are shared out
+ * over the waves (two barriers per group), the layout of every other order
+ * and precision.  MWG_BAND_AUTO (the default) is MWG_BAND_ROWS where it
+ * applies.  The layouts give bit-identical results.  Process-wide; no
+ * reference counterpart. */
+enum { MWG_BAND_AUTO = 0, MWG_BAND_TILES = 1, MWG_BAND_ROWS = 2 };
+mwg_status mwg_set_band_layout(int layout);
+int mwg_band_layout(void);
+
 /* Add each parent's reconstructed s/d into the scaling block of its 8 children
  * (MWTree::mwTransform(TopDown, false) -> tree_utils::mw_transform,
  * src/trees/MWTree.cpp:194-216, src/utils/tree_utils.cpp:113-216).  Children

@@ -99,6 +99,8 @@ SIGNATURES = {
     "mwg_synchronize": (ctypes.c_int, []),
     "mwg_profile": (ctypes.c_int, [ctypes.c_int]),
     "mwg_profile_kinds": (ctypes.c_int, [ctypes.c_uint]),
+    "mwg_set_band_layout": (ctypes.c_int, [ctypes.c_int]),
+    "mwg_band_layout": (ctypes.c_int, []),
     "mwg_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "mwg_release_pool": (ctypes.c_int, []),
     "mwg_init": (ctypes.c_int, [ctypes.c_int]),
@@ -496,6 +498,24 @@ def set_sweep_precision(name):
 def sweep_precision():
     code = lib().mrcpp_get_sweep_precision()
     return {v: k for k, v in SWEEP_PRECISIONS.items()}[code]
+
+
+BAND_LAYOUTS = {"auto": 0, "tiles": 1, "rows": 2}
+
+
+def set_band_layout(name):
+    """Work distribution of the order-7 FP64/FP32 band kernel (process-wide):
+    'rows' (each wave owns 16 rows of every sweep, one workgroup barrier per
+    group), 'tiles' (sweep tiles shared out over the waves, two barriers per
+    group) or 'auto' (the default: 'rows').  Results are bit-identical."""
+    if name not in BAND_LAYOUTS:
+        raise ValueError("band layout must be one of %s" % sorted(BAND_LAYOUTS))
+    if lib().mwg_set_band_layout(BAND_LAYOUTS[name]) != 0:
+        raise MRCPPError(lib().mwg_last_error().decode())
+
+
+def band_layout():
+    return {v: k for k, v in BAND_LAYOUTS.items()}[lib().mwg_band_layout()]
 
 
 _fp = ctypes.POINTER(ctypes.c_float)

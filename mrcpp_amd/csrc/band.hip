@@ -901,6 +901,237 @@ __global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// 3c. order 7 (N1 = 8) on the matrix cores with wave-owned rows (FP64, FP32).
+//
+// The same sweeps and instructions as k_band_mfma<8, PREC>, distributed so
+// that only the x -> y step crosses waves.  Wave w computes the x sweep rows
+// i1 + N1 i2 with i2 in {2w, 2w+1} (it writes and reads its own F rows), and
+// the y and z sweeps for the output rows with j0 in {2w, 2w+1}: the y sweep
+// reads T1 rows written by every wave (one barrier per group; T1 is kept
+// twice, by group parity, so the next group's x sweep does not wait for the
+// slowest wave's y sweep) and writes T2 rows only this wave reads; the z
+// sweep runs every needed (gx, gy) block, so the four waves do equal work
+// whatever the gt mask.  Every output element goes through the same products
+// in the same order as in k_band_mfma: the results are bit-identical.
+//
+// LDS layouts (element offsets; conflict-free for these access patterns:
+// each 32-lane half of an access touches distinct banks, and for FP64 each
+// 16-lane quarter too):
+//   F[kk = i0][m = i1 + N1 i2], T1[gx][kk = i1][j0 + N1 i2],
+//   T2 (per wave) [b = 2 gx + gy][kk = i2][r = j1 + N1 (j0 - 2w)].
+// ---------------------------------------------------------------------------
+template <int PREC> struct RowGeom;
+template <> struct RowGeom<MWG_PREC_FP64> {
+    static constexpr int F = 512, T1B = 520, T1 = 2 * T1B, T2W = 512;
+    __device__ static inline int atF(int kk, int m) { return kk * 64 + (m ^ (2 * kk) ^ (16 * (kk & 1))); }
+    __device__ static inline int atT1(int gx, int kk, int j0, int i2) { return gx * T1B + kk * 64 + (((j0 + 8 * i2) ^ (2 * (i2 >> 1))) ^ (16 * (kk & 1))); }
+    __device__ static inline int atT2(int b, int kk, int r) {
+        const int gy = b & 1;
+        return b * 128 + ((kk * 16 + (r ^ (8 * gy))) ^ (16 * gy));
+    }
+};
+template <> struct RowGeom<MWG_PREC_FP32> {
+    static constexpr int F = 512, T1B = 552, T1 = 2 * T1B, T2W = 576;
+    __device__ static inline int atF(int kk, int m) { return kk * 64 + (m ^ (16 * (kk & 1)) ^ (4 * (kk >> 1))); }
+    __device__ static inline int atT1(int gx, int kk, int j0, int i2) { return gx * T1B + kk * 68 + ((j0 + 8 * i2) ^ (2 * (i2 >> 2))); }
+    __device__ static inline int atT2(int b, int kk, int r) { return b * 144 + kk * 16 + (r ^ (8 * ((kk >> 1) & 1))); }
+};
+
+template <int PREC>
+__global__ __launch_bounds__(256) void k_band_rows(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
+                                                   const int4 *pdl, const long long *pfo, const int *gstart, const int4 *units,
+                                                   const long long *uscan, int L, double *partial, int *counter) {
+    constexpr int N1 = 8, N3 = 512, KSTEPS = 2;
+    constexpr int FRAG = KSTEPS * 64; // operand elements per (matrix block, f bit)
+    using R = RowGeom<PREC>;
+    using LT = typename BandTypes<PREC>::L;
+    using BT = typename BandTypes<PREC>::B;
+    using MM = typename BandTypes<PREC>::MM;
+    __shared__ __align__(16) LT sF[R::F];
+    __shared__ __align__(16) LT sT1[2 * R::T1];
+    __shared__ __align__(16) LT sT2[4 * R::T2W];
+    __shared__ int s_unit;
+
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lq = lane >> 4;
+    const auto useg = ro(segs);
+    const auto unode = ro(node_seg);
+    const auto ud4 = ro(reinterpret_cast<const int *>(units));
+    const int total = unit_first(ro(uscan)[L]);
+    LT *const myT2 = sT2 + w * R::T2W;
+
+    for (int i = tid; i < R::F; i += 256) sF[i] = LT(0);
+    for (int i = tid; i < 2 * R::T1; i += 256) sT1[i] = LT(0);
+    for (int i = tid; i < 4 * R::T2W; i += 256) sT2[i] = LT(0);
+
+    // this wave's F rows: band block elements 128 w + lane + 64 k
+    auto write_f = [&](const double (&v)[2]) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) sF[R::atF(lane & 7, 16 * w + 8 * k + (lane >> 3))] = (LT)v[k];
+    };
+
+    for (int u; (u = take_unit(counter, &s_unit, total)) >= 0;) {
+        // unit: groups [gi0, gi0 + ng) of one output node, contiguous in level order
+        const int p0 = ud4[4 * (size_t)u], ng = ud4[4 * (size_t)u + 2], slot = ud4[4 * (size_t)u + 3];
+        const int gi0 = ro(gstart)[p0] + ud4[4 * (size_t)u + 1];
+        const int g = ro(pr.pair_g)[p0];
+        const_as4<mwg_segment> &SG = useg[unode[g]];
+        const BT *frag = static_cast<const BT *>(SG.op.frag[PREC]);
+        const double *coef = SG.f.coef;
+        double *dst = slot < 0 ? SG.g_coef + ro(g_off)[g] : partial + (size_t)slot * 8 * N3;
+        dvec4 acc[4]; // z sweep result of block b = 2 gx + gy, rows j1 + N1 (j0 - 2w), columns gz, j2
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = dvec4{0.0, 0.0, 0.0, 0.0};
+
+        // group records, 64 to a chunk, lane j holding group 64 c + j (as in k_band_mfma)
+        struct Chunk {
+            int code, d0, d1, d2, flo, fhi;
+        };
+        auto load_chunk = [&](int c, Chunk &ch) {
+            const int i = 64 * c + lane;
+            if (i < ng) {
+                const int2 r = grec[gi0 + i];
+                const int4 d = pdl[r.y];
+                const long long fo = pfo[r.y];
+                ch = Chunk{r.x, d.x, d.y, d.z, (int)(unsigned)(fo & 0xffffffffll), (int)(fo >> 32)};
+            }
+        };
+        auto take = [&](const Chunk &ch, int i, Group &q) {
+            const int j = i & 63;
+            const unsigned code = (unsigned)__builtin_amdgcn_readlane(ch.code, j);
+            q.b40 = code_block(code);
+            q.ft = code_ft(code);
+            q.mask = code_mask(code);
+            q.fresh_f = code_fresh(code);
+            q.dl[0] = __builtin_amdgcn_readlane(ch.d0, j);
+            q.dl[1] = __builtin_amdgcn_readlane(ch.d1, j);
+            q.dl[2] = __builtin_amdgcn_readlane(ch.d2, j);
+            const long long fo = ((long long)__builtin_amdgcn_readlane(ch.fhi, j) << 32) | (unsigned)__builtin_amdgcn_readlane(ch.flo, j);
+            q.fsrc = coef + fo + (long long)q.ft * N3;
+        };
+        using Frags = BT[KSTEPS];
+        auto fetch = [&](const Group &q, Frags &fx, Frags &fy, Frags &fz) {
+#pragma unroll
+            for (int ks = 0; ks < KSTEPS; ks++) {
+                const int o = ks * 64 + lane;
+                fx[ks] = frag[(size_t)((q.b40 + q.dl[0]) * 2 + (q.ft & 1)) * FRAG + o];
+                fy[ks] = frag[(size_t)((q.b40 + q.dl[1]) * 2 + ((q.ft >> 1) & 1)) * FRAG + o];
+                fz[ks] = frag[(size_t)((q.b40 + q.dl[2]) * 2 + ((q.ft >> 2) & 1)) * FRAG + o];
+            }
+        };
+
+        Chunk ca{}, cb{};
+        load_chunk(0, ca);
+        if (ng > 64) load_chunk(1, cb);
+        Group ga, gb;
+        Frags ax, ay, az, bx, by, bz;
+        double fpre[2];
+        take(ca, 0, ga);
+        fetch(ga, ax, ay, az);
+        {
+            double v[2];
+#pragma unroll
+            for (int k = 0; k < 2; k++) v[k] = ga.fsrc[128 * w + lane + 64 * k];
+            write_f(v);
+        }
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        auto step = [&](int i, Group &cur, Frags &fx, Frags &fy, Frags &fz, Group &nq, Frags &nfx, Frags &nfy, Frags &nfz) -> bool {
+#pragma unroll
+            for (int ks = 0; ks < KSTEPS; ks++) {
+                settled(fx[ks]);
+                settled(fy[ks]);
+                settled(fz[ks]);
+            }
+            const bool have_next = i + 1 < ng;
+            if (have_next && ((i + 1) & 63) == 0) take(cb, i + 1, nq);
+            else take(ca, have_next ? i + 1 : i, nq);
+            const bool next_f = have_next && nq.fresh_f;
+#pragma unroll
+            for (int k = 0; k < 2; k++) fpre[k] = nq.fsrc[128 * w + lane + 64 * k];
+            fetch(nq, nfx, nfy, nfz);
+            const unsigned gm = cur.mask;
+            unsigned xneed, yneed;
+            sweep_needs(gm, xneed, yneed);
+            LT *const t1 = sT1 + (i & 1) * R::T1;
+            // x sweep, rows m = i1 + N1 i2 of this wave: T1[gx][i1][j0 + N1 i2] = sum_i0 F[i0][m] Ox_gx(i0, j0)
+            {
+                typename MM::C c = {};
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) c = MM::mma(sF[R::atF(4 * ks + lq, 16 * w + li)], fx[ks], c);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rho = MM::row(lane, r);
+                    t1[R::atT1(li >> 3, rho & 7, li & 7, 2 * w + (rho >> 3))] = c[r];
+                }
+            }
+            __syncthreads();
+            // y sweep, rows (j0 = 2w + (row & 1), i2 = row >> 1):
+            // T2[gx][gy][i2][j1 + N1 (j0 - 2w)] = sum_i1 T1[gx][i1][j0 + N1 i2] Oy_gy(i1, j1)
+#pragma unroll
+            for (int gx = 0; gx < 2; gx++) {
+                if (!(xneed >> gx & 1)) continue;
+                typename MM::C c = {};
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) c = MM::mma(t1[R::atT1(gx, 4 * ks + lq, 2 * w + (li & 1), li >> 1)], fy[ks], c);
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int rho = MM::row(lane, r);
+                    myT2[R::atT2(2 * gx + (li >> 3), rho >> 1, (li & 7) + 8 * (rho & 1))] = c[r];
+                }
+            }
+            // T2 and F rows pass between lanes of this wave only: the wave's LDS
+            // accesses complete in issue order, so a compiler fence suffices
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            // z sweep of every needed (gx, gy), rows j1 + N1 (j0 - 2w):
+            // G[gt][j0 + N1 j1 + N2 j2] += sum_i2 T2[gx][gy][i2][.] Oz_gz(i2, j2), columns of a gz outside the mask zeroed
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int gx = b >> 1, gy = b & 1;
+                if (!(yneed >> (gx + 2 * gy) & 1)) continue;
+                const bool keep = (gm >> (gx + 2 * gy + 4 * (li >> 3))) & 1;
+                BT zb[KSTEPS];
+#pragma unroll
+                for (int ks = 0; ks < KSTEPS; ks++) zb[ks] = keep ? fz[ks] : BT{};
+                if constexpr (PREC == MWG_PREC_FP64) {
+#pragma unroll
+                    for (int ks = 0; ks < KSTEPS; ks++) acc[b] = MM::mma(myT2[R::atT2(b, 4 * ks + lq, li)], zb[ks], acc[b]);
+                } else {
+                    typename MM::C c = {};
+#pragma unroll
+                    for (int ks = 0; ks < KSTEPS; ks++) c = MM::mma(myT2[R::atT2(b, 4 * ks + lq, li)], zb[ks], c);
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[b][r] += (double)c[r];
+                }
+            }
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            // the next band block into this wave's F rows (its x sweep above has read them)
+            if (next_f) write_f(fpre);
+            __atomic_signal_fence(__ATOMIC_SEQ_CST);
+            if (have_next && ((i + 1) & 63) == 0) {
+                ca = cb;
+                if (i + 1 + 64 < ng) load_chunk((i + 1) / 64 + 1, cb);
+            }
+            return have_next;
+        };
+        for (int i = 0;; i += 2) {
+            if (!step(i, ga, ax, ay, az, gb, bx, by, bz)) break;
+            if (!step(i + 1, gb, bx, by, bz, ga, ax, ay, az)) break;
+        }
+        // blocks gt = gx + 2 gy + 4 gz, output rows j0 in {2w, 2w + 1}
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const int gx = b >> 1, gy = b & 1;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int rho = MM::row(lane, r);
+                const int j1 = rho & 7, j0 = 2 * w + (rho >> 3), gz = li >> 3, j2 = li & 7;
+                dst[(gx + 2 * gy + 4 * gz) * N3 + j0 + N1 * j1 + 64 * j2] = acc[b][r];
+            }
+        }
+    }
+}
+
 // Operand table of one precision from the FP64 operator matrices.  Element
 // (matrix block b4, f bit, column tile nt, step ks, lane) of P[k][n] =
 // O_(2 (n / N1) + f)[k][n % N1], zero in the padding; FP64/FP32 hold
@@ -1034,8 +1265,21 @@ template <int N1, int PREC> int mfma_grid() {
     static const int grid = resident_grid(k_band_mfma<N1, PREC>, 256);
     return grid;
 }
+template <int PREC> int rows_grid() {
+    static const int grid = resident_grid(k_band_rows<PREC>, 256);
+    return grid;
+}
+int g_band_layout = MWG_BAND_AUTO;
 template <int N1, int PREC>
 void launch_mfma(const mwg_level *lvl, const LevelDevice &dev, const mwg_pairs *pr, BandWork &w, int stride) {
+    if constexpr (N1 == 8 && (PREC == MWG_PREC_FP64 || PREC == MWG_PREC_FP32)) {
+        if (g_band_layout != MWG_BAND_TILES) {
+            hipLaunchKernelGGL((k_band_rows<PREC>), dim3(rows_grid<PREC>()), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->g_off, *pr,
+                               w.grec.as<int2>(), w.pdl.as<int4>(), w.pfo.as<long long>(), w.gstart.as<int>(), w.units.as<int4>(),
+                               w.uscan.as<long long>(), lvl->count, w.partial.as<double>(), w.counter.as<int>());
+            return;
+        }
+    }
     hipLaunchKernelGGL((k_band_mfma<N1, PREC>), dim3(mfma_grid<N1, PREC>()), dim3(256), 0, 0, dev.segs, dev.node_seg, lvl->g_off, *pr,
                        w.grec.as<int2>(), w.pdl.as<int4>(), w.pfo.as<long long>(), w.gstart.as<int>(), w.units.as<int4>(),
                        w.uscan.as<long long>(), lvl->count, w.partial.as<double>(), w.counter.as<int>());
@@ -1180,6 +1424,14 @@ extern "C" mwg_status mwg_build_fragments(const mwg_band_tables *op, int precisi
         return fail(MWG_ERR_UNSUPPORTED, "no operand table for this precision");
     return dispatch<FragL>(op->kp1, op, precision, dst);
 }
+
+extern "C" mwg_status mwg_set_band_layout(int layout) {
+    if (layout != MWG_BAND_AUTO && layout != MWG_BAND_TILES && layout != MWG_BAND_ROWS) return fail(MWG_ERR_ARG, "unknown band layout");
+    g_band_layout = layout;
+    return MWG_OK;
+}
+
+extern "C" int mwg_band_layout(void) { return g_band_layout; }
 
 extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_pairs *pairs, int npairs, int precision) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
