@@ -237,6 +237,16 @@ typedef struct {
  * not written: grow the arrays and screen again). */
 mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
 
+/* One upload for a level: `bytes` of the caller's own level data (returned at
+ * *data_dev, 256-byte aligned) together with the level's segment table, the
+ * segment of every output node and the pair screen's block table, so that
+ * mwg_pair_screen and mwg_band_apply on this level (the same segments, same
+ * count) upload nothing more.  Only lvl->segs, nseg and count are read; the
+ * caller then points lvl->g_off, lvl->l and lvl->node_thr into *data_dev.
+ * Queued on the null stream; *data_dev stays valid until the next call.  No
+ * reference counterpart. */
+mwg_status mwg_level_upload(const mwg_level *lvl, const void *data, size_t bytes, void **data_dev);
+
 /* pair_f[pair[i]] = node[i] for count patches (band nodes generated on the host's plan). */
 mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, int count);
 
@@ -247,6 +257,14 @@ mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, i
  * tree_utils::mw_transform, src/utils/tree_utils.cpp:113-216). */
 mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
                               int count);
+/* As mwg_generate_nodes, and the new nodes are recorded in the tree's own
+ * arrays first: coef_off[ids[i]] = offs[i] (the block is written there) and
+ * is_gen[ids[i]] = 1, so a caller extending a tree needs no separate offset
+ * and flag uploads.  coef_off_w and is_gen_w must be f->coef_off and
+ * f->is_gen.  Nodes of one call must not be each other's parents (one call
+ * per scale, coarse to fine). */
+mwg_status mwg_generate_nodes_recorded(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
+                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w, int count);
 
 /* Exact screening and the 3-sweep band sum of every candidate pair, then the
  * output coefficients of the level and their component norms

@@ -33,10 +33,10 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, gen, missing, patch;
+    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, missing;
+    device::DeviceArray<long long> gen; // one upload per level: generated nodes and pair patches
     device::DeviceArray<long long> pOff, cOff;
     device::DeviceArray<double> gNorms, gMargin;
-    device::DeviceArray<long long> lvl; // a level's inputs in one upload: g_off | node thresholds | translations
     device::DeviceArray<unsigned long long> counters; // [job][2]: applied tuples, candidate pairs
 };
 
@@ -143,9 +143,11 @@ struct Shape {
     int kp1, n3, ncoef;
 };
 
-// Extend a tree's input mirror with the generated nodes some pairs need and
-// patch those pairs (pf[i] < 0, band translation pl, target scale sc[i]).
-void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const std::vector<int> &pl, const std::vector<int> &sc) {
+// Plan the generated nodes some pairs of a tree need into its input mirror
+// (queued in `batch`) and set those pairs' band nodes (pf[i] < 0: ancestor
+// code; band translation pl, target scale sc[i]).
+void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const std::vector<int> &pl, const std::vector<int> &sc,
+                         device::MirrorBatch &batch) {
     device::TreeMirror &fm = *job.fm;
     struct NewNode {
         int id, parent, child, scale;
@@ -182,7 +184,7 @@ void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const 
     std::vector<device::GenNode> plan;
     plan.reserve(made.size());
     for (const auto &m : made) plan.push_back({m.id, m.parent, m.child, m.scale});
-    device::generate_mirror_nodes(fm, *job.mra, job.filters, plan, scratch().gen);
+    batch.add(fm, *job.mra, job.filters, plan);
     job.stats.nGenerated += (long long)made.size();
 }
 
@@ -245,8 +247,6 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
             for (int u : sp.nodes) thr[i++] = jobs[sp.job].precFunc ? node_threshold(jobs[sp.job], u) : jobs[sp.job].gthr;
     }
     std::memcpy(packed.data() + L + nThr, lv.data(), sizeof(int) * lv.size());
-    S.lvl.reserve(packed.size());
-    S.lvl.upload(packed.data(), packed.size());
     S.gNorms.reserve(9 * (size_t)L);
     S.gFirst.reserve(L);
     S.gCount.reserve(L);
@@ -256,9 +256,13 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.count = L;
     lvl.nseg = (int)segs.size();
     lvl.segs = segs.data();
-    lvl.g_off = S.lvl.get();
-    lvl.node_thr = perNode ? reinterpret_cast<const double *>(S.lvl.get() + L) : nullptr;
-    lvl.l = reinterpret_cast<const int *>(S.lvl.get() + L + nThr);
+    // the level data goes with the segment tables in one upload
+    void *lvlDev = nullptr;
+    device::check(mwg_level_upload(&lvl, packed.data(), sizeof(long long) * packed.size(), &lvlDev), "level upload");
+    const long long *lvlData = static_cast<const long long *>(lvlDev);
+    lvl.g_off = lvlData;
+    lvl.node_thr = perNode ? reinterpret_cast<const double *>(lvlData + L) : nullptr;
+    lvl.l = reinterpret_cast<const int *>(lvlData + L + nThr);
     lvl.g_norms = S.gNorms.get();
     if (margins) {
         S.gMargin.reserve(MWG_MARGINS * (size_t)L);
@@ -321,6 +325,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         std::map<int, std::vector<int>> byJob;
         for (int i : order) byJob[nodeJob[rec[6 * i + 2]]].push_back(i);
         std::vector<int> pp, pn; // patched pairs and their band node ids
+        device::MirrorBatch batch;
         for (auto &kv : byJob) {
             std::vector<int> f, l3, sc;
             for (int i : kv.second) {
@@ -328,16 +333,17 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
                 for (int d = 0; d < 3; d++) l3.push_back(rec[6 * i + 3 + d]);
                 sc.push_back(segs[segOf[rec[6 * i + 2]]].scale);
             }
-            generate_band_nodes(jobs[kv.first], sh, f, l3, sc);
+            generate_band_nodes(jobs[kv.first], sh, f, l3, sc, batch);
             for (size_t k = 0; k < kv.second.size(); k++) {
                 pp.push_back(rec[6 * kv.second[k]]);
                 pn.push_back(f[k]);
             }
         }
-        pp.insert(pp.end(), pn.begin(), pn.end()); // one upload: [pairs | nodes]
-        S.patch.reserve(pp.size());
-        S.patch.upload(pp.data(), pp.size());
-        device::check(mwg_patch_pairs(&pr, S.patch.get(), S.patch.get() + nmissing, nmissing), "patch pairs");
+        // one upload for every tree's new nodes and the patches [pairs | nodes]
+        const size_t at = batch.addInts(pp.data(), pp.size());
+        batch.addInts(pn.data(), pn.size());
+        const int *patch = batch.run(S.gen) + at;
+        device::check(mwg_patch_pairs(&pr, patch, patch + nmissing, nmissing), "patch pairs");
         // mirrors may have moved: refresh every segment's input view
         for (size_t s = 0; s < segs.size(); s++) segs[s].f = jobs[nodeJob[segs[s].first]].fm->view(*jobs[nodeJob[segs[s].first]].mra);
         tGenerate = since(t0);

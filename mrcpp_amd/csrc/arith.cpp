@@ -22,7 +22,8 @@ enum Mode { Sum = 0, Product = 1 };
 // device call from a static destructor).
 struct ArithScratch {
     device::DeviceArray<long long> offs;
-    device::DeviceArray<int> scale, staging;
+    device::DeviceArray<int> scale;
+    device::DeviceArray<long long> staging;
     device::DeviceArray<double> norms;
 };
 ArithScratch &arith_scratch() {

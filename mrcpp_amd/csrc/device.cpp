@@ -3,6 +3,7 @@
 #include "device.h"
 
 #include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -116,6 +117,21 @@ void KeyTable::reset(int min_entries) {
     count_ = 0;
     full_ = true;
     dirty_.clear();
+    sinceMark_.clear();
+    markValid_ = false;
+}
+
+bool KeyTable::rollback() {
+    if (!markValid_) return false;
+    for (int slot : sinceMark_) {
+        keys_[2 * (size_t)slot] = 0;
+        keys_[2 * (size_t)slot + 1] = 0;
+        vals_[slot] = -1;
+        count_--;
+        if (!full_) dirty_.push_back(slot);
+    }
+    sinceMark_.clear();
+    return true;
 }
 
 int KeyTable::insert(const mwgkey::Key &key, int id) {
@@ -129,6 +145,8 @@ int KeyTable::insert(const mwgkey::Key &key, int id) {
         count_ = 0;
         full_ = true;
         dirty_.clear();
+        markValid_ = false; // entries moved: a rollback would not find them
+        sinceMark_.clear();
         for (size_t i = 0; i < ov.size(); i++)
             if (ok[2 * i]) insert(mwgkey::Key{ok[2 * i], ok[2 * i + 1]}, ov[i]);
     }
@@ -141,6 +159,7 @@ int KeyTable::insert(const mwgkey::Key &key, int id) {
             vals_[s] = id;
             count_++;
             if (!full_) dirty_.push_back((int)s);
+            if (markValid_) sinceMark_.push_back((int)s);
             return id;
         }
         if (keys_[2 * (size_t)s] == key.hi && keys_[2 * (size_t)s + 1] == key.lo) return vals_[s];
@@ -244,8 +263,7 @@ int plan_mirror_node(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, int 
     return prev;
 }
 
-void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,
-                           DeviceArray<int> &staging) {
+void MirrorBatch::add(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made) {
     if (made.empty()) return;
     const int n3 = fm.kp1 * fm.kp1 * fm.kp1;
     const int total = fm.nReal + fm.nGen;
@@ -253,38 +271,100 @@ void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra
     fm.hostOff.resize(total);
     for (const auto &m : made) fm.hostOff[m.id] = fm.genBase + (long long)(m.id - fm.nReal) * n3;
     fm.coefOff.reserve(total, firstNew);
-    fm.coefOff.upload(fm.hostOff.data() + firstNew, made.size(), firstNew);
     fm.norms.reserve((size_t)total * 9, (size_t)firstNew * 9);
     fm.isGen.reserve(total, firstNew);
-    std::vector<unsigned char> ones(made.size(), 1);
-    fm.isGen.upload(ones.data(), ones.size(), firstNew);
     fm.coef.reserve((size_t)fm.genBase + (size_t)fm.nGen * n3, (size_t)fm.genBase + (size_t)(fm.nGen - made.size()) * n3);
-    fm.uploadTable();
-    std::stable_sort(made.begin(), made.end(), [](const GenNode &a, const GenNode &b) { return a.scale < b.scale; });
-    const size_t n = made.size();
-    std::vector<int> gen(3 * n);
-    for (size_t k = 0; k < n; k++) {
-        gen[k] = made[k].id;
-        gen[n + k] = made[k].parent;
-        gen[2 * n + k] = made[k].child;
+    Entry e{&fm, &mra, filters, 0, 0, 0, 0, made.size(), {}};
+    // key table: a full upload now (grown or first), else its changed slots
+    // go with the batch (re-applying them after a later full upload of the
+    // same table writes the same values: slots are only ever filled)
+    const size_t cap = fm.table.capacity();
+    if (fm.table.fullUpload() || fm.keys.capacity() < 2 * cap || fm.vals.capacity() < cap) {
+        fm.uploadTable();
+    } else {
+        const std::vector<int> &d = fm.table.dirty();
+        e.upd = words_.size();
+        e.nupd = d.size();
+        words_.resize(e.upd + 4 * d.size());
+        long long *u = words_.data() + e.upd;
+        const size_t nd = d.size();
+        for (size_t k = 0; k < nd; k++) {
+            u[k] = d[k];
+            u[nd + 2 * k] = (long long)fm.table.keys()[2 * (size_t)d[k]];
+            u[nd + 2 * k + 1] = (long long)fm.table.keys()[2 * (size_t)d[k] + 1];
+            u[3 * nd + k] = fm.table.vals()[d[k]];
+        }
+        fm.table.markUploaded();
     }
-    staging.reserve(gen.size());
-    staging.upload(gen.data(), gen.size());
-    const int *ids = staging.get(), *par = ids + n, *ch = ids + 2 * n;
-    const mwg_ftree fv = fm.view(mra);
-    size_t i = 0;
-    while (i < n) {
+    std::stable_sort(made.begin(), made.end(), [](const GenNode &a, const GenNode &b) { return a.scale < b.scale; });
+    e.offs = words_.size();
+    for (const auto &m : made) words_.push_back(fm.hostOff[m.id]);
+    e.ids = ints_.size();
+    ints_.resize(e.ids + 3 * made.size());
+    int *g = ints_.data() + e.ids;
+    const size_t n = made.size();
+    for (size_t k = 0; k < n; k++) {
+        g[k] = made[k].id;
+        g[n + k] = made[k].parent;
+        g[2 * n + k] = made[k].child;
+    }
+    for (size_t i = 0; i < n;) {
         size_t j = i;
         while (j < n && made[j].scale == made[i].scale) j++;
-        check(mwg_generate_nodes(&fv, filters, ids + i, par + i, ch + i, (int)(j - i)), "generate nodes");
+        e.runs.push_back({i, j - i});
         i = j;
     }
+    entries_.push_back(std::move(e));
+}
+
+size_t MirrorBatch::addInts(const int *p, size_t n) {
+    const size_t at = ints_.size();
+    ints_.insert(ints_.end(), p, p + n);
+    return at;
+}
+
+const int *MirrorBatch::run(DeviceArray<long long> &staging) {
+    const size_t nw = words_.size(), total = nw + (ints_.size() + 1) / 2;
+    if (total == 0) return nullptr;
+    std::vector<long long> buf(total, 0);
+    std::copy(words_.begin(), words_.end(), buf.begin());
+    if (!ints_.empty()) std::memcpy(buf.data() + nw, ints_.data(), sizeof(int) * ints_.size());
+    staging.reserve(total);
+    staging.upload(buf.data(), total);
+    const long long *W = staging.get();
+    const int *I = reinterpret_cast<const int *>(W + nw);
+    for (const Entry &e : entries_) {
+        TreeMirror &fm = *e.fm;
+        if (e.nupd) check(mwg_hash_update(fm.keys.get(), fm.vals.get(), W + e.upd, (int)e.nupd), "key table update");
+        const mwg_ftree fv = fm.view(*e.mra); // after every reserve of the batch
+        const int *ids = I + e.ids, *par = ids + e.n, *ch = ids + 2 * e.n;
+        for (const auto &r : e.runs)
+            check(mwg_generate_nodes_recorded(&fv, e.filters, ids + r.first, par + r.first, ch + r.first, W + e.offs + r.first, fm.coefOff.get(),
+                                              fm.isGen.get(), (int)r.second),
+                  "generate nodes");
+    }
+    entries_.clear();
+    words_.clear();
+    ints_.clear();
+    return I;
+}
+
+void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,
+                           DeviceArray<long long> &staging) {
+    MirrorBatch b;
+    b.add(fm, mra, filters, made);
+    b.run(staging);
 }
 
 void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f) {
     if (fm.nGen == 0) return;
-    fm.table.reset(4 * fm.nReal + 1024);
-    for (int u = 0; u < fm.nReal; u++) fm.table.insert(key_of(mra, f.scale(u), f.transl(u)), u);
+    // the generated nodes' slots emptied (only those go to the device), or
+    // after a rehash the real nodes' table rebuilt
+    if (!fm.table.rollback()) {
+        fm.table.reset(4 * fm.nReal + 1024);
+        for (int u = 0; u < fm.nReal; u++) fm.table.insert(key_of(mra, f.scale(u), f.transl(u)), u);
+        fm.table.mark();
+    }
     fm.nGen = 0;
     fm.genParent.clear();
     fm.genScale.clear();

@@ -148,6 +148,16 @@ public:
     const std::vector<int> &vals() const { return vals_; }
     // slots filled since markUploaded(); fullUpload() after a reset or a rehash
     bool fullUpload() const { return full_; }
+    // Remember the current entries; rollback() empties every slot filled
+    // since (queued as changed slots for the next upload) and returns true,
+    // or returns false when a rehash moved the entries in between.  Entries
+    // present at mark() were all inserted before the removed ones, so their
+    // probe chains never pass through a removed slot.
+    void mark() {
+        sinceMark_.clear();
+        markValid_ = true;
+    }
+    bool rollback();
     const std::vector<int> &dirty() const { return dirty_; }
     void markUploaded() {
         full_ = false;
@@ -160,6 +170,8 @@ private:
     std::vector<int> vals_;
     bool full_ = true;
     std::vector<int> dirty_;
+    std::vector<int> sinceMark_;
+    bool markValid_ = false;
 };
 
 // Device mirror of a 3D function tree used as apply input: real nodes keep
@@ -198,11 +210,42 @@ struct GenNode {
 // nodes get storage and coefficients in generate_mirror_nodes.
 int plan_mirror_node(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, int n, const std::array<int, 3> &l, std::vector<GenNode> &made);
 
-// Device storage and scaling blocks of planned generated nodes: one upload of
-// ids, parents and child indices (through `staging`), one mwg_generate_nodes
-// launch per scale, coarse to fine.  `made` is sorted by scale.
+// Planned generated nodes of several mirrors (a mirror may appear more than
+// once) and integers of the caller, sent to the device in one upload: per
+// entry its key-table updates and, per new node, its coefficient offset, id,
+// parent and child index.  run() then launches, entry by entry, the table
+// update and one recording generation per scale, coarse to fine
+// (mwg_hash_update, mwg_generate_nodes_recorded).
+class MirrorBatch {
+public:
+    // Reserves the mirror's device storage for the nodes of `made` (new ids
+    // following the mirror's previous nodes) and queues them; sorts `made`
+    // by scale.
+    void add(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made);
+    // Integers sent with the same upload; returns the index of the first.
+    size_t addInts(const int *p, size_t n);
+    // One upload, then the launches; returns the device address of the
+    // integers added with addInts (valid until `staging` is reused).
+    const int *run(DeviceArray<long long> &staging);
+    bool empty() const { return entries_.empty() && ints_.empty(); }
+
+private:
+    struct Entry {
+        TreeMirror *fm;
+        const MultiResolutionAnalysis<3> *mra;
+        const double *filters;
+        size_t upd, nupd;              // table updates: words_[upd ..), 4 nupd words
+        size_t offs, ids, n;           // offsets words_[offs ..), ids / parents / children ints_[ids ..)
+        std::vector<std::pair<size_t, size_t>> runs; // (first, count) of each scale
+    };
+    std::vector<Entry> entries_;
+    std::vector<long long> words_;
+    std::vector<int> ints_;
+};
+
+// One mirror's planned nodes through a MirrorBatch of their own.
 void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,
-                           DeviceArray<int> &staging);
+                           DeviceArray<long long> &staging);
 
 // Forget a mirror's generated nodes (FunctionTree::deleteGenerated).
 void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f);

@@ -55,6 +55,13 @@ struct LevelDevice {
 };
 mwg_status check_level(const mwg_level *lvl);
 mwg_status upload_level(const mwg_level *lvl, LevelDevice &out);
+// First screen block of every segment of a level (nseg + 1 entries, the
+// last the total); fails when a segment's screening table is not built or the
+// level is too large for one screen launch.
+mwg_status screen_block_table(const mwg_level *lvl, std::vector<int> &seg_blk);
+// The level's screen block table staged by mwg_level_upload when it equals
+// seg_blk (same segments), else null.
+const int *staged_seg_blk(const mwg_level *lvl, const std::vector<int> &seg_blk);
 
 // Exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in and
 // out may alias.  Enqueued on the null stream.

@@ -116,6 +116,7 @@ device::TreeMirror &FunctionTree<3>::inputMirror() {
     m->genBase = (long long)N * m->ncoef;
     m->table.reset(4 * N + 1024);
     for (int u = 0; u < N; u++) m->table.insert(device::key_of(mra_, s.scale(u), s.transl(u)), u);
+    m->table.mark(); // generated nodes inserted later are removed by rollback
     const size_t nc = m->ncoef;
     std::vector<double> nrm((size_t)N * 9);
     m->hostOff.resize(N);

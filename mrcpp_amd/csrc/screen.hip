@@ -305,16 +305,9 @@ extern "C" mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const i
     return MWG_OK;
 }
 
-extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
-    if (need_init() != MWG_OK) return MWG_ERR_ARG;
-    if (!out || !out->count) return fail(MWG_ERR_ARG, "null argument");
-    mwg_status st = check_level(lvl);
-    if (st != MWG_OK) return st;
-    if (lvl->count == 0) {
-        MWG_HIP(hipMemset(out->count, 0, 3 * sizeof(int)));
-        return MWG_OK;
-    }
-    std::vector<int> seg_blk(lvl->nseg + 1, 0);
+namespace mwgdev {
+mwg_status screen_block_table(const mwg_level *lvl, std::vector<int> &seg_blk) {
+    seg_blk.assign(lvl->nseg + 1, 0);
     long long nblk = 0;
     for (int s = 0; s < lvl->nseg; s++) {
         if (lvl->segs[s].band_w >= 0 && (!lvl->segs[s].op.screen_off || !lvl->segs[s].op.screen_bound))
@@ -324,19 +317,39 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
         if (nblk >= (1ll << 31) / kScreenThreads) return fail(MWG_ERR_UNSUPPORTED, "level too large for one screen launch");
     }
     seg_blk[lvl->nseg] = (int)nblk;
+    return MWG_OK;
+}
+} // namespace mwgdev
+
+extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!out || !out->count) return fail(MWG_ERR_ARG, "null argument");
+    mwg_status st = check_level(lvl);
+    if (st != MWG_OK) return st;
+    if (lvl->count == 0) {
+        MWG_HIP(hipMemset(out->count, 0, 3 * sizeof(int)));
+        return MWG_OK;
+    }
+    std::vector<int> seg_blk;
+    if ((st = screen_block_table(lvl, seg_blk)) != MWG_OK) return st;
+    const long long nblk = seg_blk[lvl->nseg];
     LevelDevice dev;
     if ((st = upload_level(lvl, dev)) != MWG_OK) return st;
     if (g_cand_pos.ensure(sizeof(int) * (size_t)nblk * kScreenThreads) != MWG_OK) return MWG_ERR_NOMEM;
     if (g_cand_code.ensure(sizeof(int) * (size_t)nblk * kScreenThreads) != MWG_OK) return MWG_ERR_NOMEM;
     if (g_blk.ensure(sizeof(int) * ((size_t)nblk + 1)) != MWG_OK) return MWG_ERR_NOMEM;
-    if (g_seg_blk.ensure(sizeof(int) * seg_blk.size()) != MWG_OK) return MWG_ERR_NOMEM;
-    if ((st = mwg_upload_async(g_seg_blk.p, seg_blk.data(), sizeof(int) * seg_blk.size())) != MWG_OK) return st;
+    const int *seg_blk_dev = staged_seg_blk(lvl, seg_blk);
+    if (!seg_blk_dev) {
+        if (g_seg_blk.ensure(sizeof(int) * seg_blk.size()) != MWG_OK) return MWG_ERR_NOMEM;
+        if ((st = mwg_upload_async(g_seg_blk.p, seg_blk.data(), sizeof(int) * seg_blk.size())) != MWG_OK) return st;
+        seg_blk_dev = g_seg_blk.as<int>();
+    }
     ProfileScope prof_(MWG_KERNEL_SCREEN);
-    hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
+    hipLaunchKernelGGL(k_screen_positions, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, lvl->l,
                        lvl->node_thr, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
     MWG_LAUNCHED();
     if ((st = scan_exclusive(g_blk.as<int>(), g_blk.as<int>(), (int)nblk)) != MWG_OK) return st;
-    hipLaunchKernelGGL(k_screen_scatter, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, g_seg_blk.as<int>(), lvl->l,
+    hipLaunchKernelGGL(k_screen_scatter, dim3((unsigned)nblk), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, lvl->l,
                        g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), *out);
     MWG_LAUNCHED();
     return MWG_OK;

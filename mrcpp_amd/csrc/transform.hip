@@ -9,8 +9,11 @@ namespace {
 
 // Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
 // + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
+// With offs given, the node is also recorded in the mirror: coef_off_w[id] =
+// offs[k] (where its block goes) and is_gen_w[id] = 1.
 template <int N1>
-__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child) {
+__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child,
+                                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w) {
     constexpr int N3 = N1 * N1 * N1;
     __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
     __shared__ double red[8 * (kThreads / 64)];
@@ -28,7 +31,11 @@ __global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double
     __syncthreads();
     mw_pass<N1>(b, a, sfl, 4, 2);
     __syncthreads();
-    double *dst = f.coef + f.coef_off[id];
+    double *dst = f.coef + (offs ? offs[k] : f.coef_off[id]);
+    if (offs && threadIdx.x == 0) {
+        coef_off_w[id] = offs[k];
+        is_gen_w[id] = 1;
+    }
     double ss = 0.0;
     for (int e = threadIdx.x; e < N3; e += kThreads) {
         const double v = b[pad<N1>(c * N3 + e)];
@@ -95,9 +102,10 @@ template <int N1> __global__ __launch_bounds__(kThreads) void k_node_norms(const
 }
 
 template <int N1> struct GenerateL {
-    static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count) {
+    static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count, const long long *offs,
+                          long long *coef_off_w, unsigned char *is_gen_w) {
         ProfileScope prof_(MWG_KERNEL_GENERATE);
-        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch);
+        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch, offs, coef_off_w, is_gen_w);
         MWG_LAUNCHED();
         return MWG_OK;
     }
@@ -139,7 +147,17 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     if (count <= 0) return MWG_OK;
     if (!f || !filters || !ids || !parent || !child_idx) return fail(MWG_ERR_ARG, "null argument");
-    return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count);
+    return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count, (const long long *)nullptr, (long long *)nullptr,
+                               (unsigned char *)nullptr);
+}
+
+mwg_status mwg_generate_nodes_recorded(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
+                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!f || !filters || !ids || !parent || !child_idx || !offs || !coef_off_w || !is_gen_w) return fail(MWG_ERR_ARG, "null argument");
+    if (coef_off_w != f->coef_off || is_gen_w != f->is_gen) return fail(MWG_ERR_ARG, "recorded offsets and flags must be the tree's own arrays");
+    return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count, offs, coef_off_w, is_gen_w);
 }
 
 mwg_status mwg_reconstruct_add(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,

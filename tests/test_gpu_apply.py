@@ -145,6 +145,36 @@ def test_apply_deterministic_and_closed_form(mr):
         assert abs(g.evalf(p) - erf(10.0 * rr) / rr) < scale, r
 
 
+def test_reused_input_equals_fresh_input(mr):
+    """Generated band nodes are removed from the input tree's device table
+    after each apply (their slots emptied): applies of different operators
+    and precisions on one input equal the same applies on a freshly
+    projected copy, bit for bit."""
+    src = W.WORKLOADS["gauss16_k7"]["sources"]()
+    mra = mr.MultiResolutionAnalysis((-32, 32), 7)
+    gx = mr.GaussExp.from_sources(src)
+
+    def projected():
+        f = mr.FunctionTree(mra)
+        mr.build_grid(f, gx)
+        mr.project(1e-6, f, gx)
+        return f
+
+    shared = projected()
+    for prec, mu in ((1e-6, None), (1e-5, 2.0), (1e-7, None), (1e-6, None)):
+        P = mr.PoissonOperator(mra, prec) if mu is None else mr.HelmholtzOperator(mra, mu, prec)
+        outs = []
+        for f in (shared, projected()):
+            g = mr.FunctionTree(mra)
+            st = mr.apply(prec, g, P, f)
+            outs.append((g.export(), st["n_applied"], st["n_generated"]))
+        (a, na, ga), (b, nb, gb) = outs
+        _say("prec", prec, "mu", mu, "applied", na, nb, "generated", ga, gb)
+        assert na == nb and ga == gb
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 def test_apply_rejects_incompatible_mra(mr):
     a = mr.MultiResolutionAnalysis((-20, 20), 5)
     b = mr.MultiResolutionAnalysis((-32, 32), 5)
