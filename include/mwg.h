@@ -214,16 +214,25 @@ typedef struct {
  * (MWTree::getNode -> MWNode::retrieveNode, src/trees/MWNode.cpp:1097-1120). */
 typedef struct {
     int capacity;
-    int *count;          /* device int[3]: pairs written/needed, pairs whose band node is missing,
-                            nonzero if a segment's band_w is not its tables' band width */
+    int *count;          /* device int[4]: pairs written/needed, missing-node records, nonzero if a
+                            segment's band_w is not its tables' band width, pairs whose band node is
+                            missing */
     int *pair_g;         /* [capacity] output-node index in the level */
     int *pair_f;         /* [capacity] */
     int *pair_l;         /* [capacity][3] band node translation */
     int *g_first;        /* [level count] first pair of each output node */
     int *g_npairs;       /* [level count] */
-    int *missing;        /* [missing_cap][6]: pair, code, node, translation of each pair whose band
-                            node is missing, in no particular order (count[1] of them) */
+    int *missing;        /* [missing_cap][6]: pair, code, node, translation: one record per distinct
+                            missing band node of a segment (per missing pair where the node's offsets
+                            exceed 17 bits or the segment index 12 bits), in no particular order
+                            (count[1] of them) */
     int missing_cap;
+    int *missing_pairs;  /* [capacity][2]: every pair whose band node is missing (count[3]): the pair
+                            and its record, as a dedup slot (>= 0) or -(record + 1) */
+    unsigned long long *dedup_keys; /* [dedup_cap], all zero before the screen; mwg_patch_pairs
+                            returns the slots it used to zero */
+    int *dedup_rec;      /* [dedup_cap] record of each claimed slot */
+    int dedup_cap;       /* power of two, at least 2 capacity; 0: one record per missing pair */
 } mwg_pairs;
 
 /* Screen the band of every output node of a level and emit candidate pairs
@@ -247,8 +256,10 @@ mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
  * reference counterpart. */
 mwg_status mwg_level_upload(const mwg_level *lvl, const void *data, size_t bytes, void **data_dev);
 
-/* pair_f[pair[i]] = node[i] for count patches (band nodes generated on the host's plan). */
-mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, int count);
+/* For each of the count (= count[3] of the screen) pairs whose band node was
+ * missing: pair_f = node_of_record[its record] (band nodes generated on the
+ * host's plan of the records); the dedup slots used return to zero. */
+mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *node_of_record, int count);
 
 /* Generate scaling-only band nodes below existing parents: node ids[i] gets
  * the scaling block of child child_idx[i] of node parent[i], written at

@@ -8,6 +8,7 @@
 // (tree, scale) segments, so one screen and one band launch serve all trees.
 // A tree's result does not depend on the other trees of the batch.
 #include <algorithm>
+#include <climits>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -33,7 +34,9 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 
 // Device scratch reused across levels and applies.
 struct Scratch {
-    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, missing;
+    device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, missing, missingPairs, dedupRec;
+    device::DeviceArray<unsigned long long> dedupKeys;
+    bool dedupClean = false; // every slot of dedupKeys is zero
     device::DeviceArray<long long> gen; // one upload per level: generated nodes and pair patches
     device::DeviceArray<long long> pOff, cOff;
     device::DeviceArray<double> gNorms, gMargin;
@@ -271,11 +274,11 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     }
 
     mwg_pairs pr{};
-    int npairs = 0, nmissing = 0;
+    int npairs = 0, nmissing = 0, nMissingPairs = 0;
     size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
     // S.missing: the screen's counters (4 ints) then the missing-node records
     // (6 ints each), so one download returns the counts and the first records
-    constexpr size_t kHead = 4, kEagerRecords = 2048;
+    constexpr size_t kHead = 4, kEagerRecords = 512;
     size_t mcap = std::max<size_t>(S.missing.capacity() > kHead ? (S.missing.capacity() - kHead) / 6 : 0, 1024);
     auto bind_pairs = [&]() {
         S.pairG.reserve(cap);
@@ -291,18 +294,41 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         pr.g_npairs = S.gCount.get();
         pr.missing = S.missing.get() + kHead;
         pr.missing_cap = (int)((S.missing.capacity() - kHead) / 6);
+        S.missingPairs.reserve(2 * (size_t)pr.capacity);
+        pr.missing_pairs = S.missingPairs.get();
+        // deduplication table of missing band nodes: at least twice the pair
+        // capacity (so every probe ends), zero before each screen
+        size_t dcap = 1024;
+        while (dcap < 2 * (size_t)pr.capacity) dcap *= 2;
+        if (S.dedupKeys.capacity() < dcap) {
+            S.dedupKeys.reserve(dcap);
+            S.dedupRec.reserve(dcap);
+            S.dedupClean = false;
+        }
+        pr.dedup_keys = S.dedupKeys.get();
+        pr.dedup_rec = S.dedupRec.get();
+        pr.dedup_cap = (int)dcap;
+        if (!S.dedupClean) S.dedupKeys.zero(S.dedupKeys.capacity());
+        S.dedupClean = true;
     };
     bind_pairs();
     auto t0 = clk::now();
     std::vector<int> head;
     for (;;) {
+        // a screen claims table slots for missing band nodes; the patch of
+        // their pairs returns them to zero
+        S.dedupClean = false;
         device::check(mwg_pair_screen(&lvl, &pr), "pair screen");
         head.resize(kHead + 6 * std::min<size_t>(kEagerRecords, (size_t)pr.missing_cap));
         S.missing.download(head.data(), head.size());
         if (head[2]) throw Error("pair screen: band width does not match the operator tables");
         npairs = head[0];
         nmissing = head[1];
-        if (npairs <= pr.capacity && nmissing <= pr.missing_cap) break;
+        nMissingPairs = head[3];
+        if (npairs <= pr.capacity && nmissing <= pr.missing_cap) {
+            if (nMissingPairs == 0) S.dedupClean = true;
+            break;
+        }
         cap = std::max(cap, (size_t)npairs + npairs / 4);
         mcap = std::max(mcap, (size_t)nmissing + nmissing / 4);
         bind_pairs();
@@ -315,35 +341,60 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         std::vector<int> rec(6 * (size_t)nmissing);
         if ((size_t)nmissing <= kEagerRecords) std::copy(head.begin() + kHead, head.begin() + kHead + rec.size(), rec.begin());
         else S.missing.download(rec.data(), rec.size(), kHead);
-        std::vector<int> order(nmissing);
-        for (int i = 0; i < nmissing; i++) order[i] = i;
-        std::sort(order.begin(), order.end(), [&](int a, int b) { return rec[6 * a] < rec[6 * b]; }); // pair order: deterministic plan
         std::vector<int> segOf(L);
         for (size_t s = 0; s < segs.size(); s++)
             for (int i = 0; i < segs[s].count; i++) segOf[segs[s].first + i] = (int)s;
-        // per tree: its missing pairs, planned and generated on its mirror
-        std::map<int, std::vector<int>> byJob;
-        for (int i : order) byJob[nodeJob[rec[6 * i + 2]]].push_back(i);
-        std::vector<int> pp, pn; // patched pairs and their band node ids
-        device::MirrorBatch batch;
-        for (auto &kv : byJob) {
-            std::vector<int> f, l3, sc;
-            for (int i : kv.second) {
-                f.push_back(rec[6 * i + 1]);
-                for (int d = 0; d < 3; d++) l3.push_back(rec[6 * i + 3 + d]);
-                sc.push_back(segs[segOf[rec[6 * i + 2]]].scale);
-            }
-            generate_band_nodes(jobs[kv.first], sh, f, l3, sc, batch);
-            for (size_t k = 0; k < kv.second.size(); k++) {
-                pp.push_back(rec[6 * kv.second[k]]);
-                pn.push_back(f[k]);
-            }
+        // The screen gives one record per distinct missing band node of a
+        // segment (all pairs of that node share it).  Records are planned in
+        // this deterministic order; records naming the same node (segments
+        // of one tree and scale, or nodes beyond the screen's key range) are
+        // planned once.
+        struct Miss {
+            int job, anc, scale, l[3], rec;
+        };
+        std::vector<Miss> ms(nmissing);
+        for (int i = 0; i < nmissing; i++) {
+            const int *r = rec.data() + 6 * (size_t)i;
+            ms[i] = Miss{nodeJob[r[2]], r[1], segs[segOf[r[2]]].scale, {r[3], r[4], r[5]}, i};
         }
-        // one upload for every tree's new nodes and the patches [pairs | nodes]
-        const size_t at = batch.addInts(pp.data(), pp.size());
-        batch.addInts(pn.data(), pn.size());
-        const int *patch = batch.run(S.gen) + at;
-        device::check(mwg_patch_pairs(&pr, patch, patch + nmissing, nmissing), "patch pairs");
+        auto same = [](const Miss &a, const Miss &b) {
+            return a.job == b.job && a.anc == b.anc && a.scale == b.scale && a.l[0] == b.l[0] && a.l[1] == b.l[1] && a.l[2] == b.l[2];
+        };
+        std::sort(ms.begin(), ms.end(), [](const Miss &a, const Miss &b) {
+            if (a.job != b.job) return a.job < b.job;
+            if (a.scale != b.scale) return a.scale < b.scale;
+            if (a.anc != b.anc) return a.anc < b.anc;
+            for (int d = 0; d < 3; d++)
+                if (a.l[d] != b.l[d]) return a.l[d] < b.l[d];
+            return a.rec < b.rec;
+        });
+        std::vector<int> nodeOfRecord(nmissing);
+        device::MirrorBatch batch;
+        for (size_t a = 0; a < ms.size();) {
+            // the distinct band nodes of one tree
+            const int job = ms[a].job;
+            std::vector<int> f, l3, sc;
+            std::vector<size_t> first;
+            size_t b = a;
+            for (; b < ms.size() && ms[b].job == job; b++)
+                if (b == a || !same(ms[b], ms[b - 1])) {
+                    first.push_back(b);
+                    f.push_back(ms[b].anc);
+                    l3.insert(l3.end(), ms[b].l, ms[b].l + 3);
+                    sc.push_back(ms[b].scale);
+                }
+            generate_band_nodes(jobs[job], sh, f, l3, sc, batch);
+            for (size_t k = 0; k < first.size(); k++) {
+                const size_t e = k + 1 < first.size() ? first[k + 1] : b;
+                for (size_t i = first[k]; i < e; i++) nodeOfRecord[ms[i].rec] = f[k];
+            }
+            a = b;
+        }
+        // one upload for every tree's new nodes and the node of every record
+        const size_t at = batch.addInts(nodeOfRecord.data(), nodeOfRecord.size());
+        const int *nodes = batch.run(S.gen) + at;
+        device::check(mwg_patch_pairs(&pr, nodes, nMissingPairs), "patch pairs");
+        S.dedupClean = true;
         // mirrors may have moved: refresh every segment's input view
         for (size_t s = 0; s < segs.size(); s++) segs[s].f = jobs[nodeJob[segs[s].first]].fm->view(*jobs[nodeJob[segs[s].first]].mra);
         tGenerate = since(t0);
@@ -502,10 +553,25 @@ void finish_step(Job &job) {
     g.treeSquareNorm = (job.sNorm < 0.0 || job.wNorm < 0.0) ? -1.0 : job.sNorm + job.wNorm;
     if (job.iter >= job.task.maxIter && job.task.maxIter >= 0) job.work.clear();
     std::vector<int> next;
+    // without precision trees the split threshold depends on the scale only
+    // (the same value wavelet_split forms, computed once per scale)
+    int thrScale = INT_MIN;
+    double thr = 0.0;
     for (int u : job.work) {
         if (g.hasChildren(u)) continue;
-        if (g.scale(u) + 2 > mra.getMaxScale()) continue;
-        if (detail::wavelet_split(node_prec(job, u), g.treeSquareNorm, job.task.absPrec, g.scale(u), g.waveletNorm(u))) {
+        const int sc = g.scale(u);
+        if (sc + 2 > mra.getMaxScale()) continue;
+        bool split;
+        if (job.precFunc) {
+            split = detail::wavelet_split(node_prec(job, u), g.treeSquareNorm, job.task.absPrec, sc, g.waveletNorm(u));
+        } else {
+            if (sc != thrScale) {
+                thrScale = sc;
+                thr = detail::wavelet_threshold(job.task.prec, g.treeSquareNorm, job.task.absPrec, sc);
+            }
+            split = job.task.prec > 0.0 && std::sqrt(g.waveletNorm(u)) > thr;
+        }
+        if (split) {
             const int c0 = g.createChildren(u, false);
             for (int k = 0; k < 8; k++) next.push_back(c0 + k);
         }

@@ -183,6 +183,7 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
     if (blockIdx.x == 0 && threadIdx.x == 0) { // counters the scatter adds to
         count[1] = 0;
         count[2] = 0;
+        count[3] = 0;
     }
     int code = 0;
     bool ok = false;
@@ -199,6 +200,27 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
         cand_code[(size_t)blockIdx.x * kScreenThreads + excl] = code;
     }
     if (threadIdx.x == 0) blk_count[blockIdx.x] = total;
+}
+
+// Key of missing band node (x, y, z) of segment s for the deduplication
+// table: the segment and the node's offsets from the world corner, or 0 when
+// they do not fit (17 bits per offset, 12 for the segment).
+__device__ inline unsigned long long dedup_key(const_as4<mwg_segment> &S, int s, int x, int y, int z) {
+    const int d = S.o_depth;
+    const long long ox = (long long)x - (long long)S.f.corner[0] * (1ll << d);
+    const long long oy = (long long)y - (long long)S.f.corner[1] * (1ll << d);
+    const long long oz = (long long)z - (long long)S.f.corner[2] * (1ll << d);
+    const long long lim = 1ll << 17;
+    if (s >= 4096 || ox < 0 || oy < 0 || oz < 0 || ox >= lim || oy >= lim || oz >= lim) return 0ull;
+    return (1ull << 63) | ((unsigned long long)s << 51) | ((unsigned long long)ox << 34) | ((unsigned long long)oy << 17) | (unsigned long long)oz;
+}
+__device__ inline unsigned dedup_slot(unsigned long long k) {
+    k ^= k >> 31;
+    k *= 0x7fb5d329728ea185ull;
+    k ^= k >> 27;
+    k *= 0x81dadef4bc2dd44dull;
+    k ^= k >> 33;
+    return (unsigned)k;
 }
 
 // Place the survivors of every block at their scanned offset.
@@ -218,15 +240,42 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(const mwg_seg
         const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
         const int x = bx.s[0] + p % bx.n[0], y = bx.s[1] + (p / bx.n[0]) % bx.n[1], z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
         if (code < 0) {
-            const int m = atomicAdd(out.count + 1, 1);
-            if (m < out.missing_cap) {
-                int *rec = out.missing + 6 * (size_t)m;
-                rec[0] = q;
-                rec[1] = code;
-                rec[2] = sb.g;
-                rec[3] = x;
-                rec[4] = y;
-                rec[5] = z;
+            // one record per distinct missing band node of the segment: the
+            // first pair to claim its key in the table writes it; every
+            // missing pair lists its slot (or, without a table entry, its
+            // own record as -(record + 1))
+            const int k = atomicAdd(out.count + 3, 1);
+            int e = 0;
+            bool own = true;
+            const unsigned long long key = dedup_key(S, sb.s, x, y, z);
+            if (key && out.dedup_cap > 0) {
+                const unsigned mask = (unsigned)out.dedup_cap - 1;
+                for (unsigned h = dedup_slot(key) & mask;; h = (h + 1) & mask) {
+                    const unsigned long long prev = atomicCAS(out.dedup_keys + h, 0ull, key);
+                    if (prev == 0ull || prev == key) {
+                        own = prev == 0ull;
+                        e = (int)h;
+                        break;
+                    }
+                }
+            }
+            if (own) {
+                const int m = atomicAdd(out.count + 1, 1);
+                if (m < out.missing_cap) {
+                    int *rec = out.missing + 6 * (size_t)m;
+                    rec[0] = q;
+                    rec[1] = code;
+                    rec[2] = sb.g;
+                    rec[3] = x;
+                    rec[4] = y;
+                    rec[5] = z;
+                }
+                if (key && out.dedup_cap > 0) out.dedup_rec[e] = m;
+                else e = -(m + 1);
+            }
+            if (k < out.capacity) {
+                out.missing_pairs[2 * (size_t)k] = q;
+                out.missing_pairs[2 * (size_t)k + 1] = e;
             }
         }
         if (q < out.capacity) {
@@ -291,16 +340,23 @@ extern "C" mwg_status mwg_build_screen_table(const mwg_band_tables *op, long lon
     return MWG_OK;
 }
 
-__global__ __launch_bounds__(256) void k_patch_pairs(int *pair_f, const int *pair, const int *node, int count) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < count) pair_f[pair[i]] = node[i];
+// Band node of every pair whose node was missing, from its record's node;
+// the deduplication slots used go back to zero.
+__global__ __launch_bounds__(256) void k_patch_pairs(mwg_pairs pr, const int *node_of_record, int count) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= count) return;
+    const int q = pr.missing_pairs[2 * (size_t)k], e = pr.missing_pairs[2 * (size_t)k + 1];
+    const int m = e >= 0 ? pr.dedup_rec[e] : -e - 1;
+    pr.pair_f[q] = node_of_record[m];
+    if (e >= 0) pr.dedup_keys[e] = 0ull;
 }
 
-extern "C" mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *pair, const int *node, int count) {
+extern "C" mwg_status mwg_patch_pairs(mwg_pairs *pairs, const int *node_of_record, int count) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;
     if (count <= 0) return MWG_OK;
-    if (!pairs || !pair || !node) return fail(MWG_ERR_ARG, "null argument");
-    hipLaunchKernelGGL(k_patch_pairs, dim3((count + 255) / 256), dim3(256), 0, 0, pairs->pair_f, pair, node, count);
+    if (!pairs || !node_of_record || !pairs->missing_pairs) return fail(MWG_ERR_ARG, "null argument");
+    if (count > pairs->capacity) return fail(MWG_ERR_ARG, "more missing pairs than the pair arrays hold");
+    hipLaunchKernelGGL(k_patch_pairs, dim3((count + 255) / 256), dim3(256), 0, 0, *pairs, node_of_record, count);
     MWG_LAUNCHED();
     return MWG_OK;
 }
@@ -326,8 +382,12 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     if (!out || !out->count) return fail(MWG_ERR_ARG, "null argument");
     mwg_status st = check_level(lvl);
     if (st != MWG_OK) return st;
+    if (!out->missing_pairs) return fail(MWG_ERR_ARG, "no missing-pair list");
+    if (out->dedup_cap < 0 || (out->dedup_cap > 0 && ((out->dedup_cap & (out->dedup_cap - 1)) != 0 || (long long)out->dedup_cap < 2ll * out->capacity ||
+                                                       !out->dedup_keys || !out->dedup_rec)))
+        return fail(MWG_ERR_ARG, "deduplication table must be a power of two of at least twice the pair capacity");
     if (lvl->count == 0) {
-        MWG_HIP(hipMemset(out->count, 0, 3 * sizeof(int)));
+        MWG_HIP(hipMemset(out->count, 0, 4 * sizeof(int)));
         return MWG_OK;
     }
     std::vector<int> seg_blk;

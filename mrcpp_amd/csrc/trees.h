@@ -350,7 +350,7 @@ private:
         child0_.push_back(-1);
         gen_.push_back(gen ? 1 : 0);
         has_coefs_.push_back(0);
-        for (int t = 0; t < TD; t++) norm_.push_back(-1.0);
+        norm_.insert(norm_.end(), TD, -1.0);
         sq_.push_back(-1.0);
         index(u);
         if (!gen) {
