@@ -25,7 +25,16 @@ sys.path.insert(0, HERE)
 FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (vector and matrix)
 HBM_PEAK_GBS = 8000.0
 METRIC = "Poisson apply s/tree + GFLOP/s (order 7, \u03b5=1e-6), chosen vs plain FP64 GPU"
-BAND_KERNEL = "k_band_mfma"
+
+
+def band_kernel(precision, order=7):
+    """Name of the band kernel that runs: at order 7 the FP64 and FP32 sweeps
+    use the wave-owned-row kernel (unless the tile layout is selected), every
+    other case the tile-shared one."""
+    import mrcpp_amd as mr
+    if order == 7 and precision in ("fp64", "fp32") and mr.band_layout() != "tiles":
+        return "k_band_rows"
+    return "k_band_mfma"
 COPY_PROBE_BYTES = 256 << 20
 
 
@@ -51,7 +60,7 @@ def copy_probe():
 def measured_traffic(workload, precision="fp64"):
     """HBM bytes per band-kernel launch from the committed counter summary
     (lines 'workload: <name>' and 'traffic_bytes_per_launch_<precision>: <int>')."""
-    path = os.path.join(HERE, "profiles", "%s_%s_traffic.md" % (BAND_KERNEL, workload))
+    path = os.path.join(HERE, "profiles", "%s_%s_traffic.md" % (band_kernel(precision), workload))
     if not os.path.exists(path):
         return None, None
     kv = {}
@@ -400,7 +409,7 @@ def batch_bench(args, steps, warmup):
         "speedup_vs_plain": round(p_s / c_s, 3),
         "error_vs_plain_fp64": {"rel_l2_same_trees": (num / den) ** 0.5 if den > 0 else None, "same_node_set_trees": same, "trees": n,
                                 "applied_chosen": int(sum(st["n_applied"] for st in c_sts))},
-        "roofline": {"bound": "mfma", "kernel": BAND_KERNEL, "achieved": round(c_flop / (band_ms_per_step / 1e3) / 1e12, 4),
+        "roofline": {"bound": "mfma", "kernel": band_kernel(args.precision), "achieved": round(c_flop / (band_ms_per_step / 1e3) / 1e12, 4),
                      "peak": FP64_PEAK_TFLOPS if args.precision in ("fp64", "fp64_vector") else 157.3, "unit": "TFLOP/s",
                      "launches_per_step": band_launches / (steps + max(warmup, 1))},
         "kernel_ms_per_step": {k: round(v[0], 4) for k, v in every.items()},
@@ -471,7 +480,7 @@ def roofline(prof, flop, steps, precision, workload):
     achieved = flop / (band_ms / steps / 1e3) / 1e12  # algorithmic TFLOP/s while the band kernel runs
     peak = PEAKS[precision]
     traffic, src = measured_traffic(workload, precision)
-    return {"bound": "mfma", "kernel": BAND_KERNEL, "precision": precision, "achieved": round(achieved, 4), "peak": peak,
+    return {"bound": "mfma", "kernel": band_kernel(precision), "precision": precision, "achieved": round(achieved, 4), "peak": peak,
             "unit": "TFLOP/s", "frac": round(achieved / peak, 5), "traffic": traffic, "traffic_source": src,
             "launches_per_step": launches / steps, "avg_launch_ms": band_ms / max(launches, 1), "flop_per_launch": flop / max(launches / steps, 1)}
 
@@ -596,7 +605,7 @@ def main():
         b = batch_bench(args, 3, 1)
         res["batch64_k7"] = {k: b[k] for k in ("form", "ms_per_step", "s_per_tree", "plain_fp64_sequential", "speedup_vs_plain",
                                                "error_vs_plain_fp64", "with_refinement_predictor", "passes_chosen", "passes_plain",
-                                               "roofline", "kernel_ms_per_step", "config")}
+                                               "roofline", "kernel_ms_per_step", "host_breakdown_s", "config")}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]
