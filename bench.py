@@ -398,7 +398,7 @@ def batch_bench(args, steps, warmup):
     res = {
         "metric": METRIC, "value": round(flop / c_s / 1e9, 3), "unit": "GFLOP/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
-        "dtype": "f64" if args.precision in ("fp64", "fp64_vector") else args.precision + " sweeps, f64 band sum",
+        "dtype": DTYPES[args.precision],
         "precision": args.precision, "form": "one apply_batch of all trees, %s sweeps" % args.precision,
         "with_refinement_predictor": with_pred, "data": "synthetic seeded Gaussian sources",
         "passes_chosen": int(max(st["n_levels"] for st in c_sts)), "passes_plain": int(max(st["n_levels"] for st in p_sts)),

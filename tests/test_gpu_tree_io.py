@@ -45,3 +45,46 @@ def test_files_of_a_device_resident_apply_result(mr, tmp_path):
     h = mr.FunctionTree(mra)
     mr.apply(w["prec"], h, mr.PoissonOperator(mra, w["prec"]), b)
     assert h.getNNodes() > 0
+
+
+def _text_tree(path):
+    """Header lines, box lines (scale and translation) and value rows of a
+    MADNESS-convention text tree: after the six header lines, one box line
+    and one line of values per end node."""
+    with open(path) as fh:
+        lines = fh.read().splitlines()
+    head, body = lines[:6], lines[6:]
+    boxes = [ln.split() for ln in body[0::2]]
+    vals = np.array([np.array(ln.split(), dtype=float) for ln in body[1::2]])
+    return head, boxes, vals
+
+
+def test_text_file_of_apply_matches_the_oracle(mr, orc, tmp_path):
+    """saveTreeTXT of the GPU apply result (coefficients in HBM) against the
+    oracle writer's file of the oracle's apply of the same input: the same
+    header, boxes and order (every integer token), values equal to 1e-12 of
+    the largest (the two results agree to about 1e-15)."""
+    w = dict(W.WORKLOADS["gauss1_k5"], box=16)
+    src = w["sources"]()
+    b = (-w["box"], w["box"])
+    omra = orc.OracleMRA(b, w["order"])
+    of = orc.OracleTree(omra)
+    of.project_gaussians(w["prec"], src.coefs, src.exponents, src.centers)
+    og = orc.OracleTree(omra)
+    orc.apply(w["prec"], og, orc.OracleOperator(omra, w["prec"]), of)
+    mra = mr.MultiResolutionAnalysis(b, w["order"])
+    gx = mr.GaussExp.from_sources(src)
+    f = mr.FunctionTree(mra)
+    mr.build_grid(f, gx)
+    mr.project(w["prec"], f, gx)
+    g = mr.FunctionTree(mra)
+    mr.apply(w["prec"], g, mr.PoissonOperator(mra, w["prec"]), f)
+    p, o = str(tmp_path / "gpu.txt"), str(tmp_path / "oracle.txt")
+    g.saveTreeTXT(p)
+    og.save_txt(o)
+    ph, pb, pv = _text_tree(p)
+    oh, ob, ov = _text_tree(o)
+    assert ph == oh
+    assert pb == ob
+    assert pv.shape == ov.shape
+    assert np.abs(pv - ov).max() <= 1e-12 * np.abs(ov).max()
