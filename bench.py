@@ -335,8 +335,8 @@ def batch_bench(args, steps, warmup):
     pred = load_predictor(mr, 7)
     centers = [np.asarray(sp["sources"].centers) for sp in specs]
 
-    def chosen_step(predictor=None):
-        mr.set_sweep_precision(args.precision)
+    def chosen_step(predictor=None, precision=None):
+        mr.set_sweep_precision(precision or args.precision)
         outs = [mr.FunctionTree(mra) for _ in trees]
         sts = mr.apply_predicted([(prec, g, P, f) for (prec, P, f), g in zip(trees, outs)], predictor=predictor, centers=centers)
         mr.device_synchronize()
@@ -353,6 +353,12 @@ def batch_bench(args, steps, warmup):
         return outs, sts, times
 
     p_outs, p_sts, p_times = timed(plain_step)
+    # the batching gain alone: one apply_batch in plain FP64 (bit-identical
+    # trees to the applies one by one)
+    b_outs, b_sts, b_times = timed(lambda: chosen_step(precision="fp64"))
+    b_ident = sum(1 for a, b in zip(p_outs, b_outs) if all(np.array_equal(x, y) for x, y in zip(a.export(), b.export())))
+    fp64_batch = {"form": "one apply_batch of all trees, fp64 sweeps", "ms_per_step": round(float(np.mean(b_times)) * 1e3, 4),
+                  "speedup_vs_plain": round(float(np.mean(p_times)) / float(np.mean(b_times)), 3), "bit_identical_trees": b_ident}
     with_pred = None
     if pred is not None and not args.no_predictor:
         w_outs, w_sts, w_times = timed(lambda: chosen_step(pred))
@@ -400,7 +406,7 @@ def batch_bench(args, steps, warmup):
         "ms_per_step": round(c_s * 1e3, 4), "s_per_tree": c_s / n, "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
         "dtype": DTYPES[args.precision],
         "precision": args.precision, "form": "one apply_batch of all trees, %s sweeps" % args.precision,
-        "with_refinement_predictor": with_pred, "data": "synthetic seeded Gaussian sources",
+        "with_refinement_predictor": with_pred, "fp64_batch": fp64_batch, "data": "synthetic seeded Gaussian sources",
         "passes_chosen": int(max(st["n_levels"] for st in c_sts)), "passes_plain": int(max(st["n_levels"] for st in p_sts)),
         "config": {"workload": "batch64_k7", "trees": n, "order": 7, "box": [-32, 32], "precs": sorted({sp["prec"] for sp in specs}),
                    "n_gaussians": [len(sp["sources"]) for sp in specs], "operators": "Poisson (even) / Helmholtz mu~U[0.5,3] (odd)",
@@ -498,6 +504,7 @@ def main():
     ap.add_argument("--predictor", action="store_true", help="chosen form with the refinement predictor")
     ap.add_argument("--no-predictor", action="store_true", help="do not time the refinement-predictor variant beside the chosen form")
     ap.add_argument("--no-batch", action="store_true", help="leave out the 64-tree batch comparison (profiling runs of the single tree)")
+    ap.add_argument("--no-autotune", action="store_true", help="leave out the launch-shape autotuner run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
@@ -604,8 +611,18 @@ def main():
     if not args.no_batch:
         b = batch_bench(args, 3, 1)
         res["batch64_k7"] = {k: b[k] for k in ("form", "ms_per_step", "s_per_tree", "plain_fp64_sequential", "speedup_vs_plain",
-                                               "error_vs_plain_fp64", "with_refinement_predictor", "passes_chosen", "passes_plain",
+                                               "error_vs_plain_fp64", "with_refinement_predictor", "fp64_batch", "passes_chosen", "passes_plain",
                                                "roofline", "kernel_ms_per_step", "host_breakdown_s", "config")}
+    if not args.no_autotune:
+        # launch-shape autotuner on one plain FP64 apply (reported beside the
+        # headline, which keeps the default shape); defaults restored after
+        t = mr.autotune_band(lambda: plain_one(), repeats=3)
+        default = [c for c in t["candidates"] if c["layout"] == "rows" and c["unit_groups"] == 32 and c["workgroups_per_cu"] == 0]
+        res["autotuned"] = {"what": "band launch shape timed on one plain FP64 apply (median of 3 per candidate)",
+                            "best": t["best"], "default_seconds": default[0]["seconds"] if default else None,
+                            "candidates": t["candidates"]}
+        mr.set_band_layout("auto")
+        mr.set_band_launch(32, 0)
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]

@@ -130,6 +130,14 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
  * counterpart (the reference computes in FP64 only). */
 int mrcpp_set_sweep_precision(int precision);
 int mrcpp_get_sweep_precision(void);
+
+/* Autotune the band sum's launch shape on the caller's workload: run(ctx) is
+ * timed under each candidate (layout x minimum unit groups x workgroups per
+ * CU; see mwg_set_band_layout, mwg_set_band_launch) and the fastest left set.
+ * best (4 doubles): layout, unit groups, workgroups per CU, seconds per run;
+ * table (table_cap rows of 4, may be NULL): every candidate.  Returns the
+ * number of candidates timed, or -1.  No reference counterpart. */
+int mrcpp_autotune_band(void (*run)(void *), void *ctx, int repeats, double *best, double *table, int table_cap);
 /* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
  * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
  * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,

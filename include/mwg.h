@@ -302,6 +302,16 @@ enum { MWG_BAND_AUTO = 0, MWG_BAND_TILES = 1, MWG_BAND_ROWS = 2 };
 mwg_status mwg_set_band_layout(int layout);
 int mwg_band_layout(void);
 
+/* Launch parameters of the band sum, process-wide: each work unit holds at
+ * least unit_groups groups of an output node (default 32; 8..1024), and the
+ * persistent band kernels run at most workgroups_per_cu workgroups per
+ * compute unit (0, the default: as many as fit).  Results depend on
+ * unit_groups only through the order in which a node's partial sums are
+ * added (last bits); the defaults are what the parity tests hold.  No
+ * reference counterpart (the launch shape of calcNodeVector's work). */
+mwg_status mwg_set_band_launch(int unit_groups, int workgroups_per_cu);
+void mwg_band_launch(int *unit_groups, int *workgroups_per_cu);
+
 /* Add each parent's reconstructed s/d into the scaling block of its 8 children
  * (MWTree::mwTransform(TopDown, false) -> tree_utils::mw_transform,
  * src/trees/MWTree.cpp:194-216, src/utils/tree_utils.cpp:113-216).  Children

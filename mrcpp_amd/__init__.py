@@ -101,6 +101,9 @@ SIGNATURES = {
     "mwg_profile_kinds": (ctypes.c_int, [ctypes.c_uint]),
     "mwg_set_band_layout": (ctypes.c_int, [ctypes.c_int]),
     "mwg_band_layout": (ctypes.c_int, []),
+    "mwg_set_band_launch": (ctypes.c_int, [ctypes.c_int, ctypes.c_int]),
+    "mwg_band_launch": (None, [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+    "mrcpp_autotune_band": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _dp, _dp, ctypes.c_int]),
     "mwg_profile_read": (ctypes.c_int, [_dp, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "mwg_release_pool": (ctypes.c_int, []),
     "mwg_init": (ctypes.c_int, [ctypes.c_int]),
@@ -516,6 +519,58 @@ def set_band_layout(name):
 
 def band_layout():
     return {v: k for k, v in BAND_LAYOUTS.items()}[lib().mwg_band_layout()]
+
+
+def set_band_launch(unit_groups=32, workgroups_per_cu=0):
+    """Minimum groups per work unit of the band sum (8..1024, default 32) and
+    workgroups per CU of the persistent band kernels (0: as many as fit).
+    Process-wide; results change only in the last bits (the order of a
+    node's partial sums)."""
+    if lib().mwg_set_band_launch(int(unit_groups), int(workgroups_per_cu)) != 0:
+        raise MRCPPError(lib().mwg_last_error().decode())
+
+
+def band_launch():
+    ug, wg = ctypes.c_int(), ctypes.c_int()
+    lib().mwg_band_launch(ctypes.byref(ug), ctypes.byref(wg))
+    return ug.value, wg.value
+
+
+_RUN_CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+
+
+def autotune_band(run, repeats=3):
+    """Time run() (the caller's workload, e.g. one apply) under each launch
+    shape of the band sum -- layout x minimum unit groups (24, 32, 48) x
+    workgroups per CU (as many as fit, or 3) -- and leave the fastest set.
+    Returns {"best": {...}, "candidates": [...]} with seconds per run."""
+    errors = []
+
+    def call(_ctx):
+        if errors:
+            return
+        try:
+            run()
+        except BaseException as e:  # re-raised after the tuner returns
+            errors.append(e)
+
+    cb = _RUN_CB(call)
+    best = np.zeros(4)
+    table = np.zeros(4 * 64)
+    before = (band_layout(), band_launch())
+    n = lib().mrcpp_autotune_band(ctypes.cast(cb, ctypes.c_void_p), None, int(repeats), _p(best), _p(table), 64)
+    if errors:  # the workload failed: no timing is valid, the settings go back
+        set_band_layout(before[0])
+        set_band_launch(*before[1])
+        raise errors[0]
+    if n < 0:
+        raise MRCPPError(lib().mrcpp_last_error().decode())
+    names = {v: k for k, v in BAND_LAYOUTS.items()}
+
+    def row(v):
+        return {"layout": names[int(v[0])], "unit_groups": int(v[1]), "workgroups_per_cu": int(v[2]), "seconds": float(v[3])}
+
+    return {"best": row(best), "candidates": [row(table[4 * i:4 * i + 4]) for i in range(min(n, 64))]}
 
 
 _fp = ctypes.POINTER(ctypes.c_float)

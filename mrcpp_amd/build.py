@@ -18,7 +18,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 ARCH = os.environ.get("MRCPP_AMD_ARCH", "gfx950")
 
-HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "tree_io.cpp", "arith.cpp", "comp_function.cpp", "apply.cpp", "predictor.cpp", "capi.cpp"]
+HOST_SOURCES = ["core.cpp", "operators.cpp", "device.cpp", "function_tree.cpp", "tree_io.cpp", "arith.cpp", "comp_function.cpp", "apply.cpp", "predictor.cpp", "tuning.cpp", "capi.cpp"]
 HIP_SOURCES = ["runtime.hip", "scan.hip", "level.hip", "screen.hip", "band.hip", "transform.hip", "project.hip", "opbuild.hip", "arith.hip", "mlp.hip"]
 
 # IEEE evaluation order on the host (no FMA contraction) keeps host results

@@ -30,7 +30,9 @@ namespace mwgdev {
 
 namespace {
 
-constexpr int kGroupMinPerUnit = 32;
+constexpr int kGroupMinPerUnit = 32; // default minimum groups per work unit
+int g_unit_groups = kGroupMinPerUnit;  // mwg_set_band_launch
+int g_band_wg_per_cu = 0;              // workgroups per CU of the persistent band kernels, 0: as many as fit
 constexpr int kPrecValuBand = 3; // FP64 sweeps on the vector ALUs (comparison path)
 constexpr int kMaxUnitsPerNode = 512;
 constexpr double kPartialBudget = 1024.0 * 1024.0 * 1024.0; // bytes of partial slots
@@ -239,16 +241,16 @@ __global__ __launch_bounds__(64 * kScreenWaves) void k_group_screen(const mwg_se
 // 2. work units
 // ---------------------------------------------------------------------------
 // Groups per unit of an output node with T groups, total groups in the level:
-// at least kGroupMinPerUnit, few enough partial slots for the budget, at most
+// at least gmin, few enough partial slots for the budget, at most
 // kMaxUnitsPerNode units per node (bounds the node's reduction).
-__device__ inline int units_group_size(int total, int slots, int T) {
-    const int G = max(kGroupMinPerUnit, (int)(((long long)total + slots - 1) / slots));
+__device__ inline int units_group_size(int gmin, int total, int slots, int T) {
+    const int G = max(gmin, (int)(((long long)total + slots - 1) / slots));
     return max(G, (T + kMaxUnitsPerNode - 1) / kMaxUnitsPerNode);
 }
 
 // Units per output node, packed with a nonzero flag so one scan yields both
 // the first unit and the first partial slot of every node.
-__global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npairs, const int *gstart, int slots, long long *packed, int *counter,
+__global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npairs, const int *gstart, int gmin, int slots, long long *packed, int *counter,
                                                     const mwg_segment *segs, const int *node_seg, const int *papp, const double *pmg, double *g_margin) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g == 0) *counter = 0;
@@ -268,7 +270,7 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
         for (int j = 0; j < MWG_MARGINS; j++) g_margin[MWG_MARGINS * (size_t)g + j] = m[j];
     }
     const int T = gstart[gf + np] - gstart[gf];
-    const int G = units_group_size(gstart[npairs], slots, T);
+    const int G = units_group_size(gmin, gstart[npairs], slots, T);
     const int nu = (T + G - 1) / G;
     packed[g] = (long long)nu + ((long long)(nu > 0) << 32);
 }
@@ -280,7 +282,7 @@ __device__ inline int nonzero_before(long long v) { return (int)(v >> 32); }
 // partial slot or -1} of the units that start inside the pair and, when grec
 // is given, the pair's group codes in level order (gstart numbering) with the
 // pair index.
-__global__ __launch_bounds__(64) void k_unit_write(int npairs, mwg_pairs pr, const int *gstart, const long long *uscan, int slots, int4 *units,
+__global__ __launch_bounds__(64) void k_unit_write(int npairs, mwg_pairs pr, const int *gstart, const long long *uscan, int gmin, int slots, int4 *units,
                                                    const unsigned *codes, int stride, int2 *grec) {
     const int p = blockIdx.x;
     const int base = gstart[p], cnt = gstart[p + 1] - base;
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(64) void k_unit_write(int npairs, mwg_pairs pr, con
     const int gf = pr.g_first[g], np = pr.g_npairs[g];
     const int B = gstart[gf];
     const int T = gstart[gf + np] - B;
-    const int G = units_group_size(gstart[npairs], slots, T);
+    const int G = units_group_size(gmin, gstart[npairs], slots, T);
     const int a = base - B, e = a + cnt;
     const long long v = uscan[g];
     const int Ug = unit_first(v);
@@ -1257,17 +1259,22 @@ template <typename K> int resident_grid(K kernel, int threads) {
     }
     return per_cu * device_cus();
 }
+// the resident grid, or g_band_wg_per_cu workgroups per CU when that is smaller
+int capped_grid(int resident) {
+    if (g_band_wg_per_cu <= 0) return resident;
+    return std::min(resident, g_band_wg_per_cu * device_cus());
+}
 template <int N1> int unit_grid() {
     static const int grid = resident_grid(k_band_units<N1>, UnitShape<N1>::NT);
-    return grid;
+    return capped_grid(grid);
 }
 template <int N1, int PREC> int mfma_grid() {
     static const int grid = resident_grid(k_band_mfma<N1, PREC>, 256);
-    return grid;
+    return capped_grid(grid);
 }
 template <int PREC> int rows_grid() {
     static const int grid = resident_grid(k_band_rows<PREC>, 256);
-    return grid;
+    return capped_grid(grid);
 }
 int g_band_layout = MWG_BAND_AUTO;
 template <int N1, int PREC>
@@ -1331,12 +1338,12 @@ template <int N1> struct BandL {
             MWG_LAUNCHED();
             mwg_status s = scan_exclusive(gstart, gstart, npairs);
             if (s != MWG_OK) return s;
-            hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, slots, uscan, w.counter.as<int>(),
+            hipLaunchKernelGGL(k_unit_count, dim3((L + 255) / 256), dim3(256), 0, 0, L, *pr, npairs, gstart, g_unit_groups, slots, uscan, w.counter.as<int>(),
                                dev.segs, dev.node_seg, w.papp.as<int>(), mg.pair, lvl->g_margin);
             MWG_LAUNCHED();
             s = scan_exclusive(uscan, uscan, L);
             if (s != MWG_OK) return s;
-            hipLaunchKernelGGL(k_unit_write, dim3(npairs), dim3(64), 0, 0, npairs, *pr, gstart, uscan, slots, w.units.as<int4>(), w.codes.as<unsigned>(),
+            hipLaunchKernelGGL(k_unit_write, dim3(npairs), dim3(64), 0, 0, npairs, *pr, gstart, uscan, g_unit_groups, slots, w.units.as<int4>(), w.codes.as<unsigned>(),
                                stride, mfma ? w.grec.as<int2>() : nullptr);
             MWG_LAUNCHED();
         }
@@ -1432,6 +1439,19 @@ extern "C" mwg_status mwg_set_band_layout(int layout) {
 }
 
 extern "C" int mwg_band_layout(void) { return g_band_layout; }
+
+extern "C" mwg_status mwg_set_band_launch(int unit_groups, int workgroups_per_cu) {
+    if (unit_groups < 8 || unit_groups > 1024) return fail(MWG_ERR_ARG, "unit groups must be in 8..1024");
+    if (workgroups_per_cu < 0 || workgroups_per_cu > 64) return fail(MWG_ERR_ARG, "workgroups per CU must be in 0..64");
+    g_unit_groups = unit_groups;
+    g_band_wg_per_cu = workgroups_per_cu;
+    return MWG_OK;
+}
+
+extern "C" void mwg_band_launch(int *unit_groups, int *workgroups_per_cu) {
+    if (unit_groups) *unit_groups = g_unit_groups;
+    if (workgroups_per_cu) *workgroups_per_cu = g_band_wg_per_cu;
+}
 
 extern "C" mwg_status mwg_band_apply(const mwg_level *lvl, const mwg_pairs *pairs, int npairs, int precision) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;

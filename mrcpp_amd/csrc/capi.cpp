@@ -293,6 +293,29 @@ int mrcpp_set_sweep_precision(int precision) {
 
 int mrcpp_get_sweep_precision(void) { return (int)sweep_precision(); }
 
+int mrcpp_autotune_band(void (*run)(void *), void *ctx, int repeats, double *best, double *table, int table_cap) {
+    int n = 0;
+    const int rc = guarded([&] {
+        if (!run || !best) throw Error("autotune: null argument");
+        std::vector<BandTuning> all;
+        const BandTuning b = autotune_band([&] { run(ctx); }, repeats, &all);
+        best[0] = b.layout;
+        best[1] = b.unitGroups;
+        best[2] = b.wgPerCu;
+        best[3] = b.seconds;
+        for (const BandTuning &t : all) {
+            if (table && n < table_cap) {
+                table[4 * n] = t.layout;
+                table[4 * n + 1] = t.unitGroups;
+                table[4 * n + 2] = t.wgPerCu;
+                table[4 * n + 3] = t.seconds;
+            }
+            n++;
+        }
+    });
+    return rc < 0 ? -1 : n;
+}
+
 namespace {
 void pack_stats(const ApplyStats &s, double *out) {
     double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,

@@ -182,6 +182,19 @@ enum class SweepPrecision { FP64 = 0, FP32 = 1, BF16x3 = 2, FP64Vector = 3 };
 void set_sweep_precision(SweepPrecision p);
 SweepPrecision sweep_precision();
 
+// Launch shape of the band sum chosen by timing run() (the caller's own
+// workload, e.g. one apply) under each candidate: layout (MWG_BAND_ROWS or
+// MWG_BAND_TILES) x minimum groups per work unit (24, 32, 48) x workgroups
+// per CU of the persistent kernels (as many as fit, or 3); run() is called
+// once untimed and `repeats` times timed per candidate, the median kept.
+// The fastest is left set (mwg_set_band_layout, mwg_set_band_launch) and
+// returned; timings, if given, receives every candidate.  Process-wide.
+struct BandTuning {
+    int layout, unitGroups, wgPerCu;
+    double seconds;
+};
+BandTuning autotune_band(const std::function<void()> &run, int repeats = 3, std::vector<BandTuning> *timings = nullptr);
+
 // apply(prec, out, oper, inp, maxIter, absPrec) (src/treebuilders/apply.cpp:68-93),
 // the band sum running on the MI355X.
 void apply(double prec, FunctionTree<3> &out, ConvolutionOperator<3> &oper, FunctionTree<3> &inp, int maxIter = -1,

@@ -40,3 +40,14 @@ def test_device_calls_fail_cleanly_without_init(mr):
 
 def test_oracle_library_loads(orc):
     assert orc.Oracle.max_threads() >= 1
+
+
+def test_band_launch_settings_are_checked(mr):
+    """Launch-shape setters need no device: they check their arguments."""
+    assert mr.band_launch() == (32, 0)
+    for bad in ((4, 0), (2048, 0), (32, -1), (32, 65)):
+        with pytest.raises(mr.MRCPPError):
+            mr.set_band_launch(*bad)
+    assert mr.band_launch() == (32, 0)
+    with pytest.raises(ValueError):
+        mr.set_band_layout("diagonal")

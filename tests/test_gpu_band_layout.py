@@ -90,3 +90,67 @@ def test_rows_equal_tiles_batch(mr, layouts):
         assert np.array_equal(ti, ri), k
         assert np.array_equal(tc, rc), (k, float(np.abs(tc - rc).max()))
         assert np.array_equal(tn, rn), k
+
+
+def test_unit_groups_change_only_the_last_bits(mr, layouts):
+    """The minimum groups per work unit reorders only the additions of a
+    node's partial sums: same node set and applied count, coefficients equal
+    to 1e-14."""
+    src = W.WORKLOADS["gauss16_k7"]["sources"]()
+    try:
+        (ri, rc, _), rst = _apply(mr, src, 7, 32, 1e-6, None, "auto", "fp64")
+        for ug, wg in ((8, 0), (64, 2), (1024, 0)):
+            mr.set_band_launch(ug, wg)
+            (ti, tc, _), tst = _apply(mr, src, 7, 32, 1e-6, None, "auto", "fp64")
+            rel = np.linalg.norm(tc - rc) / np.linalg.norm(rc)
+            _say("unit groups", ug, "wg/cu", wg, "rel", rel)
+            assert np.array_equal(ti, ri) and tst["n_applied"] == rst["n_applied"]
+            assert rel <= 1e-14
+    finally:
+        mr.set_band_launch(32, 0)
+
+
+def test_autotune_band(mr, layouts):
+    src = W.WORKLOADS["gauss16_k7"]["sources"]()
+    mra = mr.MultiResolutionAnalysis((-32, 32), 7)
+    f = mr.FunctionTree(mra)
+    gx = mr.GaussExp.from_sources(src)
+    mr.build_grid(f, gx)
+    mr.project(1e-6, f, gx)
+    P = mr.PoissonOperator(mra, 1e-6)
+    calls = []
+
+    def run():
+        calls.append(1)
+        mr.apply(1e-6, mr.FunctionTree(mra), P, f)
+
+    try:
+        res = mr.autotune_band(run, repeats=2)
+        cands = res["candidates"]
+        _say("autotune best", res["best"], "of", len(cands))
+        assert len(cands) == 12 and len(calls) == 12 * 3
+        assert res["best"]["seconds"] == min(c["seconds"] for c in cands)
+        assert mr.band_layout() == res["best"]["layout"]
+        assert mr.band_launch() == (res["best"]["unit_groups"], res["best"]["workgroups_per_cu"])
+        # the tuned shape computes the same tree (to the last bits)
+        g = mr.FunctionTree(mra)
+        mr.set_sweep_precision("fp64")
+        mr.apply(1e-6, g, P, f)
+        mr.set_band_launch(32, 0)
+        mr.set_band_layout("auto")
+        h = mr.FunctionTree(mra)
+        mr.apply(1e-6, h, P, f)
+        (gi, gc, _), (hi, hc, _) = g.export(), h.export()
+        assert np.array_equal(gi, hi)
+        assert np.linalg.norm(gc - hc) / np.linalg.norm(hc) <= 1e-14
+        # an exception in the workload propagates and the settings are restored
+        mr.set_band_launch(48, 3)
+
+        def bad():
+            raise RuntimeError("workload failed")
+
+        with pytest.raises(RuntimeError):
+            mr.autotune_band(bad, repeats=1)
+        assert mr.band_launch() == (48, 3) and mr.band_layout() == "auto"
+    finally:
+        mr.set_band_launch(32, 0)
