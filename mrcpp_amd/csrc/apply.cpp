@@ -847,6 +847,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
         job.fm = &job.task.inp->inputMirror();
         job.fstore = &job.task.inp->topology();
         job.g = &job.task.out->nodes();
+        job.g->reserve(std::max(job.g->size(), 2 * job.fm->nReal)); // node arrays and index without regrowth
         job.filters = device::filters(job.mra->getOrder());
         // output pool sized for an output tree as large as the input (grown on demand)
         job.task.out->deviceCoefs().reserve((size_t)std::max(job.g->size(), job.fm->nReal) * sh.ncoef);

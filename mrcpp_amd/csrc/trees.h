@@ -207,6 +207,28 @@ public:
     int kp1_d() const { return kp1_d_; }
     int ncoef() const { return ncoef_; }
     int size() const { return (int)n_.size(); }
+    // Room for n nodes without reallocation: the per-node arrays and the
+    // index (kept at most a quarter full up to n nodes).
+    void reserve(int n) {
+        if (n <= 0) return;
+        n_.reserve(n);
+        l_.reserve(n);
+        parent_.reserve(n);
+        child0_.reserve(n);
+        gen_.reserve(n);
+        has_coefs_.reserve(n);
+        norm_.reserve((size_t)n * TD);
+        sq_.reserve(n);
+        size_t want = 1024;
+        while (want < 4 * (size_t)n) want *= 2;
+        if (want > slots_.size()) {
+            std::vector<int> old(want, -1);
+            old.swap(slots_);
+            indexed_ = 0;
+            for (int v : old)
+                if (v >= 0) place(v);
+        }
+    }
     const std::vector<int> &roots() const { return roots_; }
     int rootScale() const { return mra_->getRootScale(); }
 
@@ -330,7 +352,7 @@ public:
     }
 
 private:
-    static constexpr int kChunkBits = 8;
+    static constexpr int kChunkBits = 11; // 2048 nodes per host coefficient chunk (virtual until touched)
     static constexpr int kChunkMask = (1 << kChunkBits) - 1;
 
     int add_node(int n, const Index &l, int parent, bool gen) {
