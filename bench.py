@@ -140,16 +140,21 @@ def time_operator_build(mr, w, mra):
             "what": "operator construction (kernel expansion, 1D kernel trees, 2D operator trees, transforms, caches)"}
 
 
-def time_tree_io(g):
+def time_tree_io(g, box):
     """Writing and reading the apply result (its coefficients in HBM) as a
-    binary tree file and as MADNESS-convention text."""
+    binary tree file and as MADNESS-convention text (the text format needs a
+    world cube [-2^m, 2^m], as MRCPP's reader does)."""
     import shutil
     import tempfile
+    text_ok = box > 0 and (box & (box - 1)) == 0
     d = tempfile.mkdtemp()
     try:
         base, txt = os.path.join(d, "g"), os.path.join(d, "g.txt")
         out = {}
         for name, arg, path in (("binary", base, base + ".tree"), ("text", txt, txt)):
+            if name == "text" and not text_ok:
+                out[name] = "not applicable: the world is not a cube [-2^m, 2^m]"
+                continue
             t0 = time.perf_counter()
             (g.saveTree if name == "binary" else g.saveTreeTXT)(arg)
             t1 = time.perf_counter()
@@ -603,7 +608,7 @@ def main():
             "kernel_ms_per_step": {k: round(v, 4) for k, v in variant["breakdown"].items()}}
     res["projection"] = time_projection(mr, w, src, mra)
     res["arithmetic"] = time_arithmetic(mr, w, f, P)
-    res["tree_io"] = time_tree_io(plain["out"])
+    res["tree_io"] = time_tree_io(plain["out"], w["box"])
     res["operator_build"] = time_operator_build(mr, w, mra)
     # the north star's comparison on the 64-tree batch: one apply_batch of all
     # trees in the chosen form (with and without the refinement predictor)
