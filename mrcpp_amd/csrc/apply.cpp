@@ -6,7 +6,11 @@
 // apply_batch() runs many applications together: each breadth-first step
 // gathers the work nodes of every unfinished tree into one level of
 // (tree, scale) segments, so one screen and one band launch serve all trees.
-// A tree's result does not depend on the other trees of the batch.
+// The other trees of a batch change a tree's coefficients only in the last
+// bits: a level batch of more groups than the minimum unit size times the
+// partial slots (32 x 32768 at order 7) uses larger work units,
+// which adds the band sum's partial results in another order (the screening,
+// and so the applied tuples, is per tree).
 #include <algorithm>
 #include <climits>
 #include <chrono>
