@@ -130,6 +130,15 @@ int mrcpp_kernel_terms(int kind, double mu, double eps, double r_min, double r_m
  * counterpart (the reference computes in FP64 only). */
 int mrcpp_set_sweep_precision(int precision);
 int mrcpp_get_sweep_precision(void);
+/* Whether later mrcpp_apply calls on this thread form each breadth-first
+ * step's next level and its pair screen on the device before the host's
+ * split (single plain trees, steps of at most 2048 nodes; the host checks the
+ * device's split against its own and takes it only when they agree bit for
+ * bit).  1 or 0 (default); the trees are the same either way.  No reference
+ * counterpart (TreeBuilder::build, src/treebuilders/TreeBuilder.cpp:48-80,
+ * splits on the host). */
+int mrcpp_set_level_ahead(int on);
+int mrcpp_level_ahead(void);
 
 /* Autotune the band sum's launch shape on the caller's workload: run(ctx) is
  * timed under each candidate (layout x minimum unit groups x workgroups per
@@ -141,7 +150,8 @@ int mrcpp_autotune_band(void (*run)(void *), void *ctx, int repeats, double *bes
 /* apply(prec, out, oper, inp, max_iter, abs_prec) (src/treebuilders/apply.h:41).
  * stats (may be NULL, 16 doubles): applied tuples, candidate pairs, generated
  * nodes, output nodes, seconds, levels, t_screen, t_generate, t_band,
- * t_setup, t_transform, t_host, t_cleanup */
+ * t_setup, t_transform, t_host, t_cleanup, steps formed ahead on the device
+ * (their nodes and pair screen before the host's split) */
 int mrcpp_apply(double prec, void *out, void *oper, void *inp, int max_iter, int abs_prec, double *stats);
 /* apply(prec, out, oper, inp, precTrees, max_iter, abs_prec) (src/treebuilders/apply.h:43,
  * src/treebuilders/apply.cpp:214-251): prec_trees holds n_prec tree handles

@@ -54,6 +54,8 @@ mwg_status mwg_host_alloc(void **ptr, size_t bytes);
 mwg_status mwg_host_free(void *ptr);
 mwg_status mwg_upload(void *dst, const void *src, size_t bytes);
 mwg_status mwg_download(void *dst, const void *src, size_t bytes);
+/* n downloads (dst[k] <- src[k], bytes[k]) with one wait for the stream. */
+mwg_status mwg_download_many(int n, void *const *dst, const void *const *src, const size_t *bytes);
 /* Upload through a pinned staging ring and return without waiting: src may be
  * reused on return; the copy is ordered before later work of the library's
  * stream.  One host thread at a time. */
@@ -176,7 +178,7 @@ typedef struct {
     mwg_ftree f;         /* the tree's input */
     mwg_band_tables op;  /* its operator's tables */
     double *g_coef;      /* the tree's output coefficient pool */
-    unsigned long long *counters; /* device [2] the segment adds to: applied (f, ft, gt, term) tuples, candidate pairs */
+    unsigned long long *counters; /* device [2] mwg_band_apply adds to: applied (f, ft, gt, term) tuples, candidate pairs */
 } mwg_segment;
 
 /* Screening-margin bins of a level (mwg_level::g_margin). */
@@ -206,6 +208,10 @@ typedef struct {
      * threshold up to margin_ratio[j] times higher. */
     double *g_margin;        /* [count][MWG_MARGINS] */
     double margin_ratio[MWG_MARGINS];
+    /* Optional (pair screen only): the level's node count in device memory,
+     * when the host knows only the upper bound `count` (a level formed on
+     * the device by mwg_next_level); nodes at or above it are skipped. */
+    const int *count_dev;
 } mwg_level;
 
 /* Candidate (output node, band node) pairs of a level.  pair_f >= 0 is the
@@ -243,7 +249,9 @@ typedef struct {
  * components; neither can reject a tuple the exact test would apply.  The
  * pairs of one output node are contiguous and in band order (x fastest);
  * *count receives the number of pairs (pairs beyond capacity are counted but
- * not written: grow the arrays and screen again). */
+ * not written: grow the arrays and screen again).  With lvl->count_dev the
+ * level's count is an upper bound and only nodes below *count_dev are
+ * screened (their g_first / g_npairs written). */
 mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
 
 /* One upload for a level: `bytes` of the caller's own level data (returned at
@@ -255,6 +263,43 @@ mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out);
  * Queued on the null stream; *data_dev stays valid until the next call.  No
  * reference counterpart. */
 mwg_status mwg_level_upload(const mwg_level *lvl, const void *data, size_t bytes, void **data_dev);
+
+/* The split decisions of a finished breadth-first step of one tree at one
+ * scale and the next step's nodes, formed on the device (TreeBuilder::build's
+ * norm bookkeeping and WaveletAdaptor::splitNode,
+ * src/treebuilders/TreeBuilder.cpp:48-80, src/treebuilders/WaveletAdaptor.h:
+ * 45-56, src/utils/tree_utils.cpp:47-65), for a level whose nodes have no
+ * children yet.  Every quantity is formed with the host's IEEE operations in
+ * the host's order (no fused multiply-add; the step's wavelet norms summed in
+ * node order), so the decisions equal the host's bit for bit:
+ *   w_i = sum_{t=1..7} n_t^2 over the node's component norms (-1 if one is negative);
+ *   w' = w_norm + sum_i w_i; tsn = s_norm < 0 || w' < 0 ? -1 : s_norm + w';
+ *   thr = max(thr_floor, prec * (tsn > 0 && !abs_prec ? sqrt(tsn) : 1) * scale_fac);
+ *   node i splits iff may_split && prec > 0 && sqrt(w_i) > thr;
+ *   screening threshold of the next step = tsn > 0 ? prec * 1.0 * sqrt(tsn / n_terms) : tsn.
+ * The children of the k-th splitting node (node order) get ids base_id + 8 k + c,
+ * c the child index, translation 2 l + bit d of c.  One workgroup; at most
+ * 4096 nodes (the sum over the level is sequential).
+ * No reference counterpart as an entry point. */
+typedef struct {
+    int count;                 /* nodes of the finished level */
+    int may_split;             /* scale + 2 <= the MRA's max scale, and not the last step (maxIter) */
+    int abs_prec;
+    double prec, s_norm, w_norm;
+    double scale_fac;          /* 2^-(0.5 (scale + 1)), as the host forms it */
+    double n_terms;            /* separation rank of the operator */
+    double thr_floor;          /* 2 MachinePrec */
+    long long base_id, ncoef;
+    const double *g_norms;     /* [count][9] the finished level's component norms and square norm */
+    const int *l;              /* [count][3] its translations */
+    double *head;              /* out [4]: tsn, next screening threshold, w', number of children */
+    int *split;                /* out [count] 0 / 1 */
+    long long *next_off;       /* out [8 count]: child id * ncoef */
+    int *next_l;               /* out [8 count][3] */
+    double *next_thr;          /* out [8 count]: the next screening threshold */
+    int *next_count;           /* out: number of children */
+} mwg_next_level_args;
+mwg_status mwg_next_level(const mwg_next_level_args *a);
 
 /* For each of the count (= count[3] of the screen) pairs whose band node was
  * missing: pair_f = node_of_record[its record] (band nodes generated on the

@@ -92,6 +92,8 @@ SIGNATURES = {
     "mrcpp_samples_fetch": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float), ctypes.c_int]),
     "mrcpp_set_sweep_precision": (ctypes.c_int, [ctypes.c_int]),
     "mrcpp_get_sweep_precision": (ctypes.c_int, []),
+    "mrcpp_set_level_ahead": (ctypes.c_int, [ctypes.c_int]),
+    "mrcpp_level_ahead": (ctypes.c_int, []),
     "mrcpp_last_apply_levels": (ctypes.c_int, [_ip, ctypes.c_int]),
     # device layer (include/mwg.h) used by the harness
     "mwg_last_error": (ctypes.c_char_p, []),
@@ -454,7 +456,7 @@ def dot(bra, ket):
 
 
 STAT_KEYS = ("n_applied", "n_pairs", "n_generated", "n_output_nodes", "seconds", "n_levels", "t_screen", "t_generate",
-             "t_band", "t_setup", "t_transform", "t_host", "t_cleanup")
+             "t_band", "t_setup", "t_transform", "t_host", "t_cleanup", "n_ahead")
 
 
 def device_synchronize():
@@ -501,6 +503,18 @@ def set_sweep_precision(name):
 def sweep_precision():
     code = lib().mrcpp_get_sweep_precision()
     return {v: k for k, v in SWEEP_PRECISIONS.items()}[code]
+
+
+def set_level_ahead(on):
+    """Whether later apply() calls on this thread form each breadth-first
+    step's next level and its pair screen on the GPU before the host's split
+    (single plain trees, steps of at most 2048 nodes; taken only where the
+    host's split agrees bit for bit).  Default False; the same trees either way."""
+    _check(lib().mrcpp_set_level_ahead(1 if on else 0))
+
+
+def level_ahead():
+    return bool(lib().mrcpp_level_ahead())
 
 
 BAND_LAYOUTS = {"auto": 0, "tiles": 1, "rows": 2}
@@ -657,7 +671,7 @@ def apply_predicted(tasks, predictor=None, centers=None, collect=False):
                                              _p(cflat), _p(ncen, _ip), int(bool(collect)), _p(st)))
     stats = []
     for i in range(n):
-        d = {k: float(v) for k, v in zip(STAT_KEYS, st[16 * i:16 * i + 16])}
+        d = {k: float(v) for k, v in zip(STAT_KEYS[:13], st[16 * i:16 * i + 13])}
         d.update(n_predicted=st[16 * i + 13], n_cropped=st[16 * i + 14], n_fallback=st[16 * i + 15])
         stats.append(d)
     if not collect:

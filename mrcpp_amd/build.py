@@ -32,7 +32,9 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Xclang", "
 HIP_FILE_FLAGS = {"band.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"],
                   # operator nodes are formed bit-identically to the host construction
                   "opbuild.hip": ["-ffp-contract=off"],
-                  "arith.hip": ["-ffp-contract=off"]}
+                  "arith.hip": ["-ffp-contract=off"],
+                  # split decisions formed with the host's rounding
+                  "level.hip": ["-ffp-contract=off"]}
 
 
 def _run(cmd):

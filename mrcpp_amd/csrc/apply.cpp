@@ -32,6 +32,7 @@ namespace {
 thread_local ApplyStats g_stats;
 thread_local std::vector<ApplyStats> g_batch_stats;
 thread_local SweepPrecision g_sweep = SweepPrecision::FP64;
+thread_local bool g_ahead = false;
 
 using clk = std::chrono::steady_clock;
 double since(clk::time_point t) { return std::chrono::duration<double>(clk::now() - t).count(); }
@@ -39,6 +40,13 @@ double since(clk::time_point t) { return std::chrono::duration<double>(clk::now(
 // Device scratch reused across levels and applies.
 struct Scratch {
     device::DeviceArray<int> pairG, pairF, pairL, gFirst, gCount, missing, missingPairs, dedupRec;
+    // levels formed on the device (mwg_next_level): two sets, the level being
+    // computed and the next one
+    device::DeviceArray<long long> nextOff[2];
+    device::DeviceArray<int> nextL[2];
+    device::DeviceArray<double> nextThr[2];
+    device::DeviceArray<int> split, nextCount;
+    device::DeviceArray<double> nextHead;
     device::DeviceArray<unsigned long long> dedupKeys;
     bool dedupClean = false; // every slot of dedupKeys is zero
     device::DeviceArray<long long> gen; // one upload per level: generated nodes and pair patches
@@ -201,6 +209,37 @@ struct SegmentSpec {
     std::vector<int> nodes;
 };
 
+// The next level of one plain tree formed and screened on the device before
+// the host has split the current one.  Right after a level's band sums,
+// mwg_next_level forms its split decisions and the next level's node data,
+// and the pair screen of the next level runs over an upper bound of its node
+// count (eight per node) with the device's count, so that one download brings
+// back the level's norms, the device's decisions and the next level's screen
+// counters.  The host then splits with its own arithmetic and takes the
+// device's level only when both agree bit for bit; otherwise that level is
+// formed and screened again from the host's split.
+constexpr int kAheadMaxNodes = 2048; // the device sums a level's norms sequentially
+struct Ahead {
+    bool want = false;  // in: form and screen the next level on the device
+    bool ready = false; // in: this level was formed and screened ahead (device set `cur`)
+    int cur = 0;
+    bool formed = false; // out: the next level was formed (device set `set`)
+    int set = 0;
+    long long base = 0;      // first child id the device used
+    std::vector<int> nodes;  // the level's nodes, in the order of the device's decisions
+    std::vector<int> split;  // the device's split decisions
+    double k1[4] = {};       // tree square norm, next screening threshold, wavelet norm sum, children
+    std::vector<int> head;   // the next level's screen counters and first missing-node records
+    mwg_pairs pairs{};       // the arrays that screen wrote
+};
+
+// Whether two bindings name the same pair arrays and sizes.
+bool same_arrays(const mwg_pairs &a, const mwg_pairs &b) {
+    return a.pair_g == b.pair_g && a.pair_f == b.pair_f && a.pair_l == b.pair_l && a.g_first == b.g_first && a.g_npairs == b.g_npairs &&
+           a.missing == b.missing && a.missing_pairs == b.missing_pairs && a.dedup_keys == b.dedup_keys && a.dedup_rec == b.dedup_rec &&
+           a.count == b.count && a.capacity == b.capacity && a.missing_cap == b.missing_cap && a.dedup_cap == b.dedup_cap;
+}
+
 mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned long long *counters) {
     mwg_segment s{};
     s.first = first;
@@ -218,10 +257,89 @@ mwg_segment make_segment(Job &job, int scale, int first, int count, unsigned lon
     return s;
 }
 
-// Screen, generate, band-sum and reduce one level batch of segments; writes
-// the computed nodes' norms back into their trees.
-void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<SegmentSpec> &specs, unsigned long long *counters) {
+// S.missing: the screen's counters (4 ints) then the missing-node records
+// (6 ints each), so one download returns the counts and the first records.
+constexpr size_t kHead = 4, kEagerRecords = 512;
+
+// After the band sums of a level of one plain tree (its norms in S.gNorms):
+// form the next level on the device, screen it over an upper bound of its
+// node count, and bring back the level's norms (into nrm), the device's split
+// decisions and the next level's screen head with one wait.
+void form_ahead(Job &job, const Shape &sh, const SegmentSpec &sp, const mwg_level &lvl, mwg_pairs &pr, unsigned long long *counters, Ahead &ah,
+                std::vector<double> &nrm) {
     Scratch &S = scratch();
+    const int L = lvl.count, Lmax = 8 * L;
+    const int nxt = ah.ready ? 1 - ah.cur : 0;
+    S.nextOff[nxt].reserve(Lmax);
+    S.nextL[nxt].reserve(3 * (size_t)Lmax);
+    S.nextThr[nxt].reserve(Lmax);
+    S.split.reserve(L);
+    S.nextCount.reserve(1);
+    S.nextHead.reserve(4);
+    mwg_next_level_args a{};
+    a.count = L;
+    a.may_split = 1;
+    a.abs_prec = job.task.absPrec ? 1 : 0;
+    a.prec = job.task.prec;
+    a.s_norm = job.sNorm;
+    a.w_norm = job.wNorm;
+    a.scale_fac = detail::wavelet_scale_factor(sp.scale);
+    a.n_terms = (double)job.task.oper->size();
+    a.thr_floor = 2.0 * MachinePrec;
+    a.base_id = job.g->size();
+    a.ncoef = sh.ncoef;
+    a.g_norms = lvl.g_norms;
+    a.l = lvl.l;
+    a.head = S.nextHead.get();
+    a.split = S.split.get();
+    a.next_off = S.nextOff[nxt].get();
+    a.next_l = S.nextL[nxt].get();
+    a.next_thr = S.nextThr[nxt].get();
+    a.next_count = S.nextCount.get();
+    device::check(mwg_next_level(&a), "next level");
+    // the next level: one segment of at most Lmax nodes, per-node thresholds
+    std::vector<mwg_segment> seg = {make_segment(job, sp.scale + 1, 0, Lmax, counters)};
+    mwg_level nl{};
+    nl.kp1 = sh.kp1;
+    nl.count = Lmax;
+    nl.nseg = 1;
+    nl.segs = seg.data();
+    void *unused = nullptr;
+    device::check(mwg_level_upload(&nl, nullptr, 0, &unused), "level upload");
+    nl.g_off = a.next_off;
+    nl.l = a.next_l;
+    nl.node_thr = a.next_thr;
+    nl.g_norms = lvl.g_norms;
+    nl.count_dev = a.next_count;
+    // per-node pair lists for the bound (the pair arrays keep their size;
+    // the work still queued reads the old blocks)
+    S.gFirst.reserve(Lmax);
+    S.gCount.reserve(Lmax);
+    pr.g_first = S.gFirst.get();
+    pr.g_npairs = S.gCount.get();
+    S.dedupClean = false;
+    device::check(mwg_pair_screen(&nl, &pr), "pair screen");
+    ah.split.resize(L);
+    ah.head.resize(kHead + 6 * std::min<size_t>(kEagerRecords, (size_t)pr.missing_cap));
+    void *dst[4] = {nrm.data(), ah.k1, ah.split.data(), ah.head.data()};
+    const void *src[4] = {lvl.g_norms, a.head, a.split, S.missing.get()};
+    const size_t bytes[4] = {sizeof(double) * nrm.size(), sizeof(ah.k1), sizeof(int) * ah.split.size(), sizeof(int) * ah.head.size()};
+    device::check(mwg_download_many(4, dst, src, bytes), "download");
+    ah.formed = true;
+    ah.pairs = pr;
+    ah.set = nxt;
+    ah.base = a.base_id;
+    ah.nodes = sp.nodes;
+}
+
+// Screen, generate, band-sum and reduce one level batch of segments; writes
+// the computed nodes' norms back into their trees.  With `ahead` (one plain
+// tree, one segment) the level may come formed and screened from the device,
+// and the next level is formed and screened before the norms come back.
+void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<SegmentSpec> &specs, unsigned long long *counters,
+                     Ahead *ahead = nullptr) {
+    Scratch &S = scratch();
+    const bool ready = ahead && ahead->ready;
     int L = 0;
     for (const auto &sp : specs) L += (int)sp.nodes.size();
     std::vector<int> lv(3 * (size_t)L), nodeJob(L);
@@ -263,13 +381,21 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     lvl.count = L;
     lvl.nseg = (int)segs.size();
     lvl.segs = segs.data();
-    // the level data goes with the segment tables in one upload
     void *lvlDev = nullptr;
-    device::check(mwg_level_upload(&lvl, packed.data(), sizeof(long long) * packed.size(), &lvlDev), "level upload");
-    const long long *lvlData = static_cast<const long long *>(lvlDev);
-    lvl.g_off = lvlData;
-    lvl.node_thr = perNode ? reinterpret_cast<const double *>(lvlData + L) : nullptr;
-    lvl.l = reinterpret_cast<const int *>(lvlData + L + nThr);
+    if (ready) {
+        // offsets, translations and screening thresholds formed on the device
+        device::check(mwg_level_upload(&lvl, nullptr, 0, &lvlDev), "level upload");
+        lvl.g_off = S.nextOff[ahead->cur].get();
+        lvl.node_thr = S.nextThr[ahead->cur].get();
+        lvl.l = S.nextL[ahead->cur].get();
+    } else {
+        // the level data goes with the segment tables in one upload
+        device::check(mwg_level_upload(&lvl, packed.data(), sizeof(long long) * packed.size(), &lvlDev), "level upload");
+        const long long *lvlData = static_cast<const long long *>(lvlDev);
+        lvl.g_off = lvlData;
+        lvl.node_thr = perNode ? reinterpret_cast<const double *>(lvlData + L) : nullptr;
+        lvl.l = reinterpret_cast<const int *>(lvlData + L + nThr);
+    }
     lvl.g_norms = S.gNorms.get();
     if (margins) {
         S.gMargin.reserve(MWG_MARGINS * (size_t)L);
@@ -279,12 +405,10 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
 
     mwg_pairs pr{};
     int npairs = 0, nmissing = 0, nMissingPairs = 0;
-    size_t cap = std::max<size_t>(S.pairF.capacity(), 4096);
-    // S.missing: the screen's counters (4 ints) then the missing-node records
-    // (6 ints each), so one download returns the counts and the first records
-    constexpr size_t kHead = 4, kEagerRecords = 512;
+    // the pair capacity the arrays hold now (binding it grows nothing)
+    size_t cap = std::max<size_t>(std::min(S.pairF.capacity(), S.pairL.capacity() / 3), 4096);
     size_t mcap = std::max<size_t>(S.missing.capacity() > kHead ? (S.missing.capacity() - kHead) / 6 : 0, 1024);
-    auto bind_pairs = [&]() {
+    auto bind_pairs = [&](bool keepTable) {
         S.pairG.reserve(cap);
         S.pairF.reserve(cap);
         S.pairL.reserve(3 * cap);
@@ -312,19 +436,33 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         pr.dedup_keys = S.dedupKeys.get();
         pr.dedup_rec = S.dedupRec.get();
         pr.dedup_cap = (int)dcap;
+        if (keepTable) return; // the slots claimed by the level's own screen
         if (!S.dedupClean) S.dedupKeys.zero(S.dedupKeys.capacity());
         S.dedupClean = true;
     };
-    bind_pairs();
+    // a level screened ahead left its pairs, records and table slots in
+    // these same arrays (none grows here; if one had moved, the level is
+    // screened again on a cleared table)
+    bind_pairs(ready);
+    bool screened = ready;
+    if (ready && !same_arrays(pr, ahead->pairs)) {
+        screened = false;
+        bind_pairs(false);
+    }
     auto t0 = clk::now();
     std::vector<int> head;
     for (;;) {
-        // a screen claims table slots for missing band nodes; the patch of
-        // their pairs returns them to zero
-        S.dedupClean = false;
-        device::check(mwg_pair_screen(&lvl, &pr), "pair screen");
-        head.resize(kHead + 6 * std::min<size_t>(kEagerRecords, (size_t)pr.missing_cap));
-        S.missing.download(head.data(), head.size());
+        if (screened) {
+            head = ahead->head; // came back with the previous level's norms
+            screened = false;
+        } else {
+            // a screen claims table slots for missing band nodes; the patch of
+            // their pairs returns them to zero
+            S.dedupClean = false;
+            device::check(mwg_pair_screen(&lvl, &pr), "pair screen");
+            head.resize(kHead + 6 * std::min<size_t>(kEagerRecords, (size_t)pr.missing_cap));
+            S.missing.download(head.data(), head.size());
+        }
         if (head[2]) throw Error("pair screen: band width does not match the operator tables");
         npairs = head[0];
         nmissing = head[1];
@@ -335,7 +473,7 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         }
         cap = std::max(cap, (size_t)npairs + npairs / 4);
         mcap = std::max(mcap, (size_t)nmissing + nmissing / 4);
-        bind_pairs();
+        bind_pairs(false);
     }
     const double tScreen = since(t0);
     double tGenerate = 0.0;
@@ -407,8 +545,12 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
     // band sums, output coefficients and norms (zero when no pair survives)
     device::check(mwg_band_apply(&lvl, &pr, npairs, (int)g_sweep), "band apply");
     std::vector<double> nrm(9 * (size_t)L), mgn(margins ? MWG_MARGINS * (size_t)L : 0);
-    S.gNorms.download(nrm.data(), nrm.size());
-    if (margins) S.gMargin.download(mgn.data(), mgn.size());
+    if (ahead && ahead->want) {
+        form_ahead(jobs[specs[0].job], sh, specs[0], lvl, pr, counters + 2 * specs[0].job, *ahead, nrm);
+    } else {
+        S.gNorms.download(nrm.data(), nrm.size());
+        if (margins) S.gMargin.download(mgn.data(), mgn.size());
+    }
     const double tBand = since(t0);
     std::set<int> touched;
     for (const auto &sp : specs) touched.insert(sp.job);
@@ -446,6 +588,43 @@ double node_budget(const Job &job, int scale) {
     const int W = (o_depth >= 0 && o_depth < ht.nDepth) ? std::max(ht.bandMax[o_depth], 0) : 0;
     const double box = std::pow(2.0 * W + 1.0, 3.0);
     return 8.0 * (box + 256.0) + 128.0 * (96.0 * ht.M + 48.0); // screen candidates; per pair: group codes, level-order records, pair record
+}
+
+// Screening threshold of a step from the approximate tree norm
+// (ConvolutionCalculator::calcNode, src/treebuilders/ConvolutionCalculator.cpp:241-246).
+double screening_threshold(const Job &job) {
+    double t = job.g->treeSquareNorm;
+    if (t > 0.0) t = job.task.prec * 1.0 * std::sqrt(t / (double)job.task.oper->size());
+    return t;
+}
+
+// Whether the next level of this step's one segment can be formed and
+// screened on the device: a plain tree past its first step (the scaling norm
+// is known), a step that may split, at most kAheadMaxNodes nodes, none with
+// children yet.
+bool ahead_eligible(const Job &job, const SegmentSpec &sp) {
+    const detail::NodeStore<3> &g = *job.g;
+    if (job.mode != Mode::Plain || job.precFunc || job.task.predictor || job.task.samples || job.marginPass) return false;
+    if (job.iter < 1 || sp.nodes.empty() || (int)sp.nodes.size() > kAheadMaxNodes) return false;
+    if (job.iter >= job.task.maxIter && job.task.maxIter >= 0) return false;
+    if (sp.scale + 2 > job.mra->getMaxScale() || sp.scale + 1 - job.task.oper->getOperatorRoot() > mwgkey::kMaxDepth) return false;
+    for (int u : sp.nodes)
+        if (g.hasChildren(u) || g.depth(u) < 0) return false;
+    return true;
+}
+
+bool same_bits(double a, double b) { return std::memcmp(&a, &b, sizeof a) == 0; }
+
+// After the host's split of a level formed ahead: whether the device formed
+// the same next level (children, ids, approximate norm and threshold).
+bool ahead_agrees(const Job &job, const Ahead &ah) {
+    const detail::NodeStore<3> &g = *job.g;
+    if ((double)job.work.size() != ah.k1[3] || ah.split.size() != ah.nodes.size()) return false;
+    for (size_t i = 0; i < job.work.size(); i++)
+        if (job.work[i] != ah.base + (long long)i) return false;
+    for (size_t i = 0; i < ah.nodes.size(); i++)
+        if ((g.hasChildren(ah.nodes[i]) ? 1 : 0) != ah.split[i]) return false;
+    return same_bits(g.treeSquareNorm, ah.k1[0]) && same_bits(screening_threshold(job), ah.k1[1]) && same_bits(job.wNorm, ah.k1[2]);
 }
 
 // The final transforms of every tree of a batch together: TopDown
@@ -814,6 +993,8 @@ const std::vector<ApplyStats> &last_batch_stats() { return g_batch_stats; }
 
 void set_sweep_precision(SweepPrecision p) { g_sweep = p; }
 SweepPrecision sweep_precision() { return g_sweep; }
+void set_level_ahead(bool on) { g_ahead = on; }
+bool level_ahead() { return g_ahead; }
 
 void apply_batch(const std::vector<ApplyTask> &tasks) {
     std::lock_guard<std::recursive_mutex> lock(device::api_mutex());
@@ -863,6 +1044,7 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
     S.counters.zero(2 * jobs.size());
     const double tSetup = since(t_start);
 
+    Ahead ahead; // one plain tree: levels formed and screened on the device
     for (;;) {
         std::vector<int> active;
         for (size_t i = 0; i < jobs.size(); i++)
@@ -874,13 +1056,23 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
             Job &job = jobs[j];
             job.stats.levelNodes.push_back((int)job.work.size());
             eval_prec_factors(job);
-            double gthr = job.g->treeSquareNorm;
-            if (gthr > 0.0) gthr = job.task.prec * 1.0 * std::sqrt(gthr / (double)job.task.oper->size());
-            job.gthr = gthr;
+            job.gthr = screening_threshold(job);
             job.task.out->deviceCoefs().reserve((size_t)job.g->size() * sh.ncoef, (size_t)job.g->size() * sh.ncoef);
             std::map<int, std::vector<int>> byScale;
             for (int u : job.work) byScale[job.g->scale(u)].push_back(u);
             for (auto &kv : byScale) specs.push_back({j, kv.first, std::move(kv.second)});
+        }
+        // one tree whose step is one segment within the batch budget: the
+        // step runs whole, with its next level formed ahead where it can be
+        const bool whole =
+                jobs.size() == 1 && specs.size() == 1 && node_budget(jobs[0], specs[0].scale) * (double)specs[0].nodes.size() <= kBatchBudget;
+        ahead.want = whole && g_ahead && ahead_eligible(jobs[0], specs[0]);
+        if (whole) {
+            run_level_batch(jobs, sh, specs, S.counters.get(), &ahead);
+            if (ahead.ready) jobs[0].stats.nAhead++;
+            specs.clear();
+        } else {
+            ahead.ready = false;
         }
         std::vector<SegmentSpec> batch;
         double used = 0.0;
@@ -917,6 +1109,19 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
                 job.task.out->deviceCoefs().reserve(n, std::min(n, job.task.out->deviceCoefs().capacity()));
             }
             job.stats.tHost += since(th);
+        }
+        if (ahead.formed) {
+            // the device's next level stands when the host formed the same
+            // one (and it runs as one level batch); else its screen's table
+            // slots are cleared before the next screen
+            Job &job = jobs[0];
+            ahead.ready = ahead_agrees(job, ahead) &&
+                          (job.work.empty() || node_budget(job, job.g->scale(job.work[0])) * (double)job.work.size() <= kBatchBudget);
+            ahead.cur = ahead.set;
+            ahead.formed = false;
+            if (!ahead.ready) S.dedupClean = false;
+        } else {
+            ahead.ready = false;
         }
     }
     std::vector<unsigned long long> na(2 * jobs.size());

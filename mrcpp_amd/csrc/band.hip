@@ -259,6 +259,7 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
     unsigned long long app = 0; // applied tuples of the node's pairs
     for (int q = gf; q < gf + np; q++) app += (unsigned)papp[q];
     if (app) atomicAdd(segs[node_seg[g]].counters, app);
+    if (np) atomicAdd(segs[node_seg[g]].counters + 1, (unsigned long long)np); // candidate pairs
     if (g_margin) { // the node's screening margins, its pairs in order
         double m[MWG_MARGINS];
 #pragma unroll

@@ -293,6 +293,12 @@ int mrcpp_set_sweep_precision(int precision) {
 
 int mrcpp_get_sweep_precision(void) { return (int)sweep_precision(); }
 
+int mrcpp_set_level_ahead(int on) {
+    return guarded([&] { set_level_ahead(on != 0); });
+}
+
+int mrcpp_level_ahead(void) { return level_ahead() ? 1 : 0; }
+
 int mrcpp_autotune_band(void (*run)(void *), void *ctx, int repeats, double *best, double *table, int table_cap) {
     int n = 0;
     const int rc = guarded([&] {
@@ -319,7 +325,7 @@ int mrcpp_autotune_band(void (*run)(void *), void *ctx, int repeats, double *bes
 namespace {
 void pack_stats(const ApplyStats &s, double *out) {
     double v[16] = {(double)s.nApplied, (double)s.nPairs, (double)s.nGenerated, (double)s.nOutputNodes, s.seconds,
-                    (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tSetup, s.tTransform, s.tHost, s.tCleanup, 0, 0, 0};
+                    (double)s.levelNodes.size(), s.tScreen, s.tGenerate, s.tBand, s.tSetup, s.tTransform, s.tHost, s.tCleanup, (double)s.nAhead, 0, 0};
     std::memcpy(out, v, sizeof(v));
 }
 } // namespace

@@ -65,7 +65,9 @@ const int *staged_seg_blk(const mwg_level *lvl, const std::vector<int> &seg_blk)
 
 // Exclusive prefix sum of in[0..n) into out[0..n], out[n] = total; in and
 // out may alias.  Enqueued on the null stream.
-mwg_status scan_exclusive(const int *in, int *out, int n);
+// With n_dev, only the first min(n, *n_dev * mult) items (a length known on
+// the device) are scanned, their total written after them.
+mwg_status scan_exclusive(const int *in, int *out, int n, const int *n_dev = nullptr, int mult = 0);
 mwg_status scan_exclusive(const long long *in, long long *out, int n);
 
 #define MWG_HIP(call)                                                                                                   \

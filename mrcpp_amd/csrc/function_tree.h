@@ -166,6 +166,7 @@ struct ApplyStats {
     long long nPredicted = 0;  // output nodes created by the refinement predictor
     long long nCropped = 0;    // predicted nodes the wavelet test rejected (removed)
     long long nFallback = 0;   // nodes the predictor missed, computed in extra passes
+    long long nAhead = 0;      // steps whose nodes and pair screen were formed on the device ahead of the host's split
     double seconds = 0.0;      // host wall time of apply()
     double tScreen = 0.0, tGenerate = 0.0, tBand = 0.0, tReduce = 0.0, tTransform = 0.0, tHost = 0.0;
     double tSetup = 0.0;   // tables, mirrors and output pool before the first step
@@ -181,6 +182,13 @@ enum class SweepPrecision { FP64 = 0, FP32 = 1, BF16x3 = 2, FP64Vector = 3 };
 // Sweep precision used by apply() on this thread.
 void set_sweep_precision(SweepPrecision p);
 SweepPrecision sweep_precision();
+// Whether apply() on this thread forms the next breadth-first level and its
+// pair screen on the device before the host's split of a single plain tree
+// (the same tree either way, one host round trip fewer per step; off by
+// default: at the step sizes measured the extra launches cost more than the
+// round trip saves).
+void set_level_ahead(bool on);
+bool level_ahead();
 
 // Launch shape of the band sum chosen by timing run() (the caller's own
 // workload, e.g. one apply) under each candidate: layout (MWG_BAND_ROWS or

@@ -1,7 +1,11 @@
-// Device copy of a level's segments and the segment of every output node.
+// Device copy of a level's segments and the segment of every output node;
+// the next breadth-first level formed on the device.
 #include "device_common.h"
 
 #include <cstring>
+
+// split decisions equal to the host's: every product and sum rounded on its own
+#pragma clang fp contract(off)
 
 namespace mwgdev {
 
@@ -17,6 +21,91 @@ __global__ __launch_bounds__(256) void k_node_seg(const mwg_segment *segs, int n
         else hi = mid - 1;
     }
     node_seg[g] = lo;
+}
+
+constexpr int kNextLevelMax = 4096; // nodes of a finished level (their wavelet norms held in LDS)
+
+// Wavelet square norm of a node from its component norms
+// (NodeStore::waveletNorm: sequential, -1 once a component norm is unset).
+__device__ inline double node_wavelet_sq(const double *n) {
+    double w = 0.0;
+    for (int t = 1; t < 8; t++) {
+        const double v = n[t];
+        if (v >= 0.0) w += v * v;
+        else w = -1.0;
+    }
+    return w;
+}
+
+// The next breadth-first level from a finished one (mwg_next_level).  One
+// workgroup: the nodes' wavelet norms are formed in parallel into LDS and
+// summed by one thread in node order; then the split flags, their ranks
+// (wave ballots) and the children, a chunk of kNextThreads nodes at a time.
+constexpr int kNextThreads = 256;
+__global__ __launch_bounds__(kNextThreads) void k_next_level(mwg_next_level_args a) {
+    __shared__ double s_w[kNextLevelMax];
+    __shared__ double s_thr[2];
+    __shared__ int s_wave[kNextThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < a.count; i += kNextThreads) s_w[i] = node_wavelet_sq(a.g_norms + 9 * (size_t)i);
+    __syncthreads();
+    if (tid == 0) {
+        double step = 0.0; // in node order; eight LDS reads issued ahead of their additions
+        int k = 0;
+        for (; k + 8 <= a.count; k += 8) {
+            double v[8];
+#pragma unroll
+            for (int j = 0; j < 8; j++) v[j] = s_w[k + j];
+#pragma unroll
+            for (int j = 0; j < 8; j++) step += v[j];
+        }
+        for (; k < a.count; k++) step += s_w[k];
+        const double wn = a.w_norm + step;
+        const double tsn = (a.s_norm < 0.0 || wn < 0.0) ? -1.0 : a.s_norm + wn;
+        const double tn = (tsn > 0.0 && !a.abs_prec) ? sqrt(tsn) : 1.0;
+        const double x = a.prec * tn * a.scale_fac;
+        const double gthr = tsn > 0.0 ? a.prec * 1.0 * sqrt(tsn / a.n_terms) : tsn;
+        s_thr[0] = a.thr_floor < x ? x : a.thr_floor; // std::max(floor, x)
+        s_thr[1] = gthr;
+        a.head[0] = tsn;
+        a.head[1] = gthr;
+        a.head[2] = wn;
+    }
+    __syncthreads();
+    const double thr = s_thr[0], gthr = s_thr[1];
+    const bool may = a.may_split && a.prec > 0.0;
+    int before = 0;
+    for (int c0 = 0; c0 < a.count; c0 += kNextThreads) {
+        const int i = c0 + tid;
+        bool sp = false;
+        if (i < a.count) {
+            sp = may && sqrt(s_w[i]) > thr;
+            a.split[i] = sp ? 1 : 0;
+        }
+        const unsigned long long m = __ballot(sp);
+        if (lane == 0) s_wave[wave] = __popcll(m);
+        __syncthreads();
+        int off = before, total = 0;
+        for (int w = 0; w < kNextThreads / 64; w++) {
+            if (w < wave) off += s_wave[w];
+            total += s_wave[w];
+        }
+        if (sp) {
+            const int k = off + __popcll(m & ((1ull << lane) - 1ull));
+            for (int c = 0; c < 8; c++) {
+                const int j = 8 * k + c;
+                a.next_off[j] = (a.base_id + j) * a.ncoef;
+                for (int d = 0; d < 3; d++) a.next_l[3 * (size_t)j + d] = 2 * a.l[3 * (size_t)i + d] + ((c >> d) & 1);
+                a.next_thr[j] = gthr;
+            }
+        }
+        before += total;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        *a.next_count = 8 * before;
+        a.head[3] = (double)(8 * before);
+    }
 }
 
 Buffer g_segs, g_node_seg, g_arena;
@@ -92,6 +181,19 @@ const int *staged_seg_blk(const mwg_level *lvl, const std::vector<int> &seg_blk)
 } // namespace mwgdev
 
 using namespace mwgdev;
+
+extern "C" mwg_status mwg_next_level(const mwg_next_level_args *a) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (!a || a->count < 0) return fail(MWG_ERR_ARG, "bad next-level arguments");
+    if (a->count == 0) return MWG_OK;
+    if (!a->g_norms || !a->l || !a->head || !a->split || !a->next_off || !a->next_l || !a->next_thr || !a->next_count)
+        return fail(MWG_ERR_ARG, "null next-level array");
+    if (!(a->n_terms > 0.0) || a->ncoef <= 0 || a->base_id < 0) return fail(MWG_ERR_ARG, "bad next-level sizes");
+    if (a->count > kNextLevelMax) return fail(MWG_ERR_UNSUPPORTED, "more than 4096 nodes for the next-level kernel");
+    hipLaunchKernelGGL(k_next_level, dim3(1), dim3(kNextThreads), 0, 0, *a);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
 
 extern "C" mwg_status mwg_level_upload(const mwg_level *lvl, const void *data, size_t bytes, void **data_dev) {
     if (need_init() != MWG_OK) return MWG_ERR_ARG;

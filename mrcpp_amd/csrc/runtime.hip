@@ -87,6 +87,18 @@ __global__ __launch_bounds__(256) void k_write_back(unsigned *dst, const unsigne
     for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < words; i += (size_t)gridDim.x * 256) dst[i] = src[i];
 }
 
+// Up to kGatherMax ranges in one launch (blockIdx.y: the range).
+constexpr int kGatherMax = 4;
+struct GatherRanges {
+    unsigned *dst[kGatherMax];
+    const unsigned *src[kGatherMax];
+    size_t words[kGatherMax];
+};
+__global__ __launch_bounds__(256) void k_write_back_many(GatherRanges r) {
+    const int k = blockIdx.y;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < r.words[k]; i += (size_t)gridDim.x * 256) r.dst[k][i] = r.src[k][i];
+}
+
 mwg_status fail(mwg_status s, const std::string &msg) {
     g_err = msg;
     return s;
@@ -306,16 +318,22 @@ mwg_status mwg_upload_async(void *dst, const void *src, size_t bytes) {
     return MWG_OK;
 }
 
+// The pinned landing buffer of downloads and its device address (when the
+// runtime maps it).
+static mwg_status ensure_landing() {
+    if (g_landing) return MWG_OK;
+    MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_landing), kStageMaxCopy, hipHostMallocMapped | hipHostMallocCoherent));
+    void *dev = nullptr;
+    if (hipHostGetDevicePointer(&dev, g_landing, 0) == hipSuccess) g_landing_dev = static_cast<unsigned char *>(dev);
+    else (void)hipGetLastError();
+    return MWG_OK;
+}
+
 mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
     if (!bytes) return MWG_OK;
     if (!dst || !src) return fail(MWG_ERR_ARG, "null pointer");
     if (bytes <= kStageMaxCopy) {
-        if (!g_landing) {
-            MWG_HIP(hipHostMalloc(reinterpret_cast<void **>(&g_landing), kStageMaxCopy, hipHostMallocMapped | hipHostMallocCoherent));
-            void *dev = nullptr;
-            if (hipHostGetDevicePointer(&dev, g_landing, 0) == hipSuccess) g_landing_dev = static_cast<unsigned char *>(dev);
-            else (void)hipGetLastError();
-        }
+        if (mwg_status s = ensure_landing(); s != MWG_OK) return s;
         const bool words = bytes % 4 == 0 && reinterpret_cast<uintptr_t>(src) % 4 == 0;
         if (g_landing_dev && words && bytes <= kWriteBackMax) {
             const size_t n = bytes / 4;
@@ -330,6 +348,45 @@ mwg_status mwg_download(void *dst, const void *src, size_t bytes) {
         std::memcpy(dst, g_landing, bytes);
     } else {
         MWG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    }
+    g_stage_used = 0; // the null stream has drained
+    return MWG_OK;
+}
+
+mwg_status mwg_download_many(int n, void *const *dst, const void *const *src, const size_t *bytes) {
+    if (n < 0 || (n > 0 && (!dst || !src || !bytes))) return fail(MWG_ERR_ARG, "null argument");
+    size_t total = 0;
+    bool words = n <= kGatherMax;
+    for (int k = 0; k < n; k++) {
+        if (bytes[k] && (!dst[k] || !src[k])) return fail(MWG_ERR_ARG, "null pointer");
+        total += (bytes[k] + 255) & ~size_t(255);
+        words = words && bytes[k] % 4 == 0 && reinterpret_cast<uintptr_t>(src[k]) % 4 == 0;
+    }
+    if (mwg_status s = ensure_landing(); s != MWG_OK) return s;
+    if (n == 1 || total > kWriteBackMax || !words || !g_landing_dev) { // one download each
+        for (int k = 0; k < n; k++)
+            if (mwg_status s = mwg_download(dst[k], src[k], bytes[k]); s != MWG_OK) return s;
+        return MWG_OK;
+    }
+    GatherRanges r{};
+    size_t at = 0, most = 0;
+    for (int k = 0; k < n; k++) {
+        r.dst[k] = reinterpret_cast<unsigned *>(g_landing_dev + at);
+        r.src[k] = static_cast<const unsigned *>(src[k]);
+        r.words[k] = bytes[k] / 4;
+        most = std::max(most, r.words[k]);
+        at += (bytes[k] + 255) & ~size_t(255);
+    }
+    if (most) {
+        const unsigned blocks = (unsigned)std::min<size_t>((most + 255) / 256, 1024);
+        hipLaunchKernelGGL(k_write_back_many, dim3(blocks, (unsigned)n), dim3(256), 0, 0, r);
+        MWG_LAUNCHED();
+    }
+    MWG_HIP(hipStreamSynchronize(0));
+    at = 0;
+    for (int k = 0; k < n; k++) {
+        if (bytes[k]) std::memcpy(dst[k], g_landing + at, bytes[k]);
+        at += (bytes[k] + 255) & ~size_t(255);
     }
     g_stage_used = 0; // the null stream has drained
     return MWG_OK;

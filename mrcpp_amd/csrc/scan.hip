@@ -10,9 +10,13 @@ namespace {
 constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4 * kScanThreads;
 
+// Items scanned: n, or with a device count n_dev the first min(n, *n_dev * mult).
+__device__ inline int scan_length(int n, const int *n_dev, int mult) { return n_dev ? min(n, *n_dev * mult) : n; }
+
 // out[i] = exclusive scan of in within the block's 1024 items; bsum[block] = block total.
-template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_block(const T *in, T *out, T *bsum, int n) {
+template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_block(const T *in, T *out, T *bsum, int n, const int *n_dev, int mult) {
     __shared__ T tmp[kScanThreads / 64];
+    n = scan_length(n, n_dev, mult);
     const int base = blockIdx.x * kScanItems + 4 * threadIdx.x;
     T v[4], s = 0;
 #pragma unroll
@@ -31,7 +35,8 @@ template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_blo
 }
 
 // out[i] += boff[block]; out[n] = boff[nblocks].
-template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_add(T *out, const T *boff, int n, int nblocks) {
+template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_add(T *out, const T *boff, int n, int nblocks, const int *n_dev, int mult) {
+    n = scan_length(n, n_dev, mult);
     const T add = boff[blockIdx.x];
     const int base = blockIdx.x * kScanItems;
     for (int i = threadIdx.x; i < kScanItems; i += kScanThreads)
@@ -44,8 +49,9 @@ template <typename T> __global__ __launch_bounds__(kScanThreads) void k_scan_add
 // write), so in and out may alias.
 constexpr int kSingleThreads = 1024;
 constexpr int kSingleMax = 64 * kSingleThreads;
-template <typename T> __global__ __launch_bounds__(kSingleThreads) void k_scan_single(const T *in, T *out, int n) {
+template <typename T> __global__ __launch_bounds__(kSingleThreads) void k_scan_single(const T *in, T *out, int n, const int *n_dev, int mult) {
     __shared__ T tmp[kSingleThreads / 64];
+    n = scan_length(n, n_dev, mult);
     const int c = (n + kSingleThreads - 1) / kSingleThreads;
     const int a = min(n, (int)threadIdx.x * c), b = min(n, a + c);
     T s = 0;
@@ -65,15 +71,20 @@ template <typename T> Buffer &scan_temp(int level) {
     return b[level];
 }
 
-template <typename T> mwg_status scan_rec(const T *in, T *out, int n, int level) {
+// The total goes to out[n], or out[min(n, *n_dev * mult)] with a device count.
+template <typename T> mwg_status scan_rec(const T *in, T *out, int n, int level, const int *n_dev, int mult) {
     const int nb = (n + kScanItems - 1) / kScanItems;
     if (nb > 1 && n <= kSingleMax) {
-        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kSingleThreads), 0, 0, in, out, n);
+        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kSingleThreads), 0, 0, in, out, n, n_dev, mult);
         MWG_LAUNCHED();
         return MWG_OK;
     }
     if (nb <= 1) {
-        hipLaunchKernelGGL(k_scan_block<T>, dim3(1), dim3(kScanThreads), 0, 0, in, out, out + n, n);
+        if (n_dev) { // the total's place is known on the device only
+            hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(kSingleThreads), 0, 0, in, out, n, n_dev, mult);
+        } else {
+            hipLaunchKernelGGL(k_scan_block<T>, dim3(1), dim3(kScanThreads), 0, 0, in, out, out + n, n, nullptr, 0);
+        }
         MWG_LAUNCHED();
         return MWG_OK;
     }
@@ -81,25 +92,26 @@ template <typename T> mwg_status scan_rec(const T *in, T *out, int n, int level)
     Buffer &tb = scan_temp<T>(level);
     if (tb.ensure(sizeof(T) * ((size_t)nb + 1)) != MWG_OK) return MWG_ERR_NOMEM;
     T *bs = tb.as<T>();
-    hipLaunchKernelGGL(k_scan_block<T>, dim3(nb), dim3(kScanThreads), 0, 0, in, out, bs, n);
+    // blocks past the device count have zero totals
+    hipLaunchKernelGGL(k_scan_block<T>, dim3(nb), dim3(kScanThreads), 0, 0, in, out, bs, n, n_dev, mult);
     MWG_LAUNCHED();
-    const mwg_status s = scan_rec<T>(bs, bs, nb, level + 1);
+    const mwg_status s = scan_rec<T>(bs, bs, nb, level + 1, nullptr, 0);
     if (s != MWG_OK) return s;
-    hipLaunchKernelGGL(k_scan_add<T>, dim3(nb), dim3(kScanThreads), 0, 0, out, bs, n, nb);
+    hipLaunchKernelGGL(k_scan_add<T>, dim3(nb), dim3(kScanThreads), 0, 0, out, bs, n, nb, n_dev, mult);
     MWG_LAUNCHED();
     return MWG_OK;
 }
 
 } // namespace
 
-mwg_status scan_exclusive(const int *in, int *out, int n) {
-    if (n < 0) return fail(MWG_ERR_ARG, "negative scan length");
-    return scan_rec<int>(in, out, n, 0);
+mwg_status scan_exclusive(const int *in, int *out, int n, const int *n_dev, int mult) {
+    if (n < 0 || (n_dev && mult <= 0)) return fail(MWG_ERR_ARG, "bad scan length");
+    return scan_rec<int>(in, out, n, 0, n_dev, mult);
 }
 
 mwg_status scan_exclusive(const long long *in, long long *out, int n) {
     if (n < 0) return fail(MWG_ERR_ARG, "negative scan length");
-    return scan_rec<long long>(in, out, n, 0);
+    return scan_rec<long long>(in, out, n, 0, nullptr, 0);
 }
 
 } // namespace mwgdev

@@ -740,10 +740,12 @@ template <int D> std::vector<int> build_tree(NodeStore<D> &s, const BuildHooks<D
 
 // tree_utils::split_check (src/utils/tree_utils.cpp:47-65), split factor 1
 // Wavelet-norm threshold of the split check at a scale.
+inline double wavelet_scale_factor(int scale) { return std::pow(2.0, -(0.5 * 1.0 * (scale + 1))); }
+
 inline double wavelet_threshold(double prec, double tree_sq_norm, bool abs_prec, int scale) {
     double t_norm = 1.0;
     if (tree_sq_norm > 0.0 && !abs_prec) t_norm = std::sqrt(tree_sq_norm);
-    const double scale_fac = std::pow(2.0, -(0.5 * 1.0 * (scale + 1)));
+    const double scale_fac = wavelet_scale_factor(scale);
     return std::max(2.0 * MachinePrec, prec * t_norm * scale_fac);
 }
 
