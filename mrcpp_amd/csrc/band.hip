@@ -254,12 +254,29 @@ __global__ __launch_bounds__(256) void k_unit_count(int L, mwg_pairs pr, int npa
                                                     const mwg_segment *segs, const int *node_seg, const int *papp, const double *pmg, double *g_margin) {
     const int g = blockIdx.x * 256 + threadIdx.x;
     if (g == 0) *counter = 0;
-    if (g >= L) return;
-    const int gf = pr.g_first[g], np = pr.g_npairs[g];
-    unsigned long long app = 0; // applied tuples of the node's pairs
+    // applied tuples and candidate pairs of the node's pairs, added to its
+    // segment's counters: one atomic per wave when the wave's nodes share a
+    // segment (lane 0 holds the wave's first node)
+    const bool live = g < L;
+    const int gf = live ? pr.g_first[g] : 0, np = live ? pr.g_npairs[g] : 0;
+    const int s = live ? node_seg[g] : -1;
+    unsigned long long app = 0, npl = (unsigned long long)np;
     for (int q = gf; q < gf + np; q++) app += (unsigned)papp[q];
-    if (app) atomicAdd(segs[node_seg[g]].counters, app);
-    if (np) atomicAdd(segs[node_seg[g]].counters + 1, (unsigned long long)np); // candidate pairs
+    const int s0 = __shfl(s, 0);
+    if (__all(!live || s == s0)) {
+        for (int off = 32; off > 0; off >>= 1) {
+            app += __shfl_xor(app, off);
+            npl += __shfl_xor(npl, off);
+        }
+        if ((threadIdx.x & 63) == 0 && s0 >= 0) {
+            if (app) atomicAdd(segs[s0].counters, app);
+            if (npl) atomicAdd(segs[s0].counters + 1, npl);
+        }
+    } else if (live) {
+        if (app) atomicAdd(segs[s].counters, app);
+        if (np) atomicAdd(segs[s].counters + 1, npl);
+    }
+    if (!live) return;
     if (g_margin) { // the node's screening margins, its pairs in order
         double m[MWG_MARGINS];
 #pragma unroll

@@ -174,9 +174,36 @@ __device__ inline int live_blocks(int nblk, const int *count_dev, const_as4<mwg_
     return count_dev ? min(nblk, *count_dev * screen_blocks(segs[0].band_w)) : nblk;
 }
 
-// One thread per band position; block-local compaction in position order.
-// Workgroups stride over the screen blocks: all nblk, or with a device node
-// count (count_dev, one segment) the blocks of the nodes below it.
+// One thread per band position of screen block blk; block-local compaction
+// in position order.
+__device__ __forceinline__ void screen_block(const_as4<mwg_segment> *segs, int nseg, const int *seg_blk, int blk, const int *l, const double *node_thr,
+                                             int *cand_pos, int *cand_code, int *blk_count, int *tmp) {
+    const ScreenBlock sb = locate(seg_blk, nseg, segs, blk);
+    const_as4<mwg_segment> &S = segs[sb.s];
+    const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
+    const Box bx = segment_box(S, gl);
+    const int P = bx.n[0] * bx.n[1] * bx.n[2];
+    const int p = sb.b * kScreenThreads + threadIdx.x;
+    int code = 0;
+    bool ok = false;
+    if (p < P) {
+        const int x = bx.s[0] + p % bx.n[0];
+        const int y = bx.s[1] + (p / bx.n[0]) % bx.n[1];
+        const int z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
+        ok = screen_position(S, node_thr ? node_thr[sb.g] : S.g_thr, gl, x, y, z, &code);
+    }
+    int excl;
+    const int total = block_exclusive_scan<int, kScreenThreads>(ok ? 1 : 0, tmp, excl);
+    if (ok) {
+        cand_pos[(size_t)blk * kScreenThreads + excl] = p;
+        cand_code[(size_t)blk * kScreenThreads + excl] = code;
+    }
+    if (threadIdx.x == 0) blk_count[blk] = total;
+}
+
+// One workgroup per screen block; with a device node count (DEV, one
+// segment) the workgroups stride over the blocks of the nodes below it.
+template <bool DEV>
 __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_segment *segs_, int nseg, const int *seg_blk, int nblk, const int *count_dev,
                                                                     const int *l, const double *node_thr, int *cand_pos, int *cand_code, int *blk_count,
                                                                     int *count) {
@@ -187,29 +214,11 @@ __global__ __launch_bounds__(kScreenThreads) void k_screen_positions(const mwg_s
         count[2] = 0;
         count[3] = 0;
     }
-    const int live = live_blocks(nblk, count_dev, segs);
-    for (int blk = blockIdx.x; blk < live; blk += gridDim.x) {
-        const ScreenBlock sb = locate(seg_blk, nseg, segs, blk);
-        const_as4<mwg_segment> &S = segs[sb.s];
-        const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
-        const Box bx = segment_box(S, gl);
-        const int P = bx.n[0] * bx.n[1] * bx.n[2];
-        const int p = sb.b * kScreenThreads + threadIdx.x;
-        int code = 0;
-        bool ok = false;
-        if (p < P) {
-            const int x = bx.s[0] + p % bx.n[0];
-            const int y = bx.s[1] + (p / bx.n[0]) % bx.n[1];
-            const int z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
-            ok = screen_position(S, node_thr ? node_thr[sb.g] : S.g_thr, gl, x, y, z, &code);
-        }
-        int excl;
-        const int total = block_exclusive_scan<int, kScreenThreads>(ok ? 1 : 0, tmp, excl);
-        if (ok) {
-            cand_pos[(size_t)blk * kScreenThreads + excl] = p;
-            cand_code[(size_t)blk * kScreenThreads + excl] = code;
-        }
-        if (threadIdx.x == 0) blk_count[blk] = total;
+    if constexpr (!DEV) {
+        screen_block(segs, nseg, seg_blk, blockIdx.x, l, node_thr, cand_pos, cand_code, blk_count, tmp);
+    } else {
+        const int live = live_blocks(nblk, count_dev, segs);
+        for (int blk = blockIdx.x; blk < live; blk += gridDim.x) screen_block(segs, nseg, seg_blk, blk, l, node_thr, cand_pos, cand_code, blk_count, tmp);
     }
 }
 
@@ -234,87 +243,95 @@ __device__ inline unsigned dedup_slot(unsigned long long k) {
     return (unsigned)k;
 }
 
-// Place the survivors of every block at their scanned offset (workgroups
-// stride over the blocks as in k_screen_positions).
+// Place the survivors of screen block blk at their scanned offset.
+__device__ __forceinline__ void scatter_block(const_as4<mwg_segment> *segs, int nseg, const int *seg_blk, int blk, const int *l, const int *cand_pos,
+                                              const int *cand_code, const int *blk_off, const mwg_pairs &out) {
+    const ScreenBlock sb = locate(seg_blk, nseg, segs, blk);
+    const int base = blk_off[blk];
+    const int cnt = blk_off[blk + 1] - base;
+    if (threadIdx.x < cnt) {
+        const_as4<mwg_segment> &S = segs[sb.s];
+        const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
+        const Box bx = segment_box(S, gl);
+        const int p = cand_pos[(size_t)blk * kScreenThreads + threadIdx.x];
+        const int q = base + threadIdx.x;
+        const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
+        const int x = bx.s[0] + p % bx.n[0], y = bx.s[1] + (p / bx.n[0]) % bx.n[1], z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
+        if (code < 0) {
+            // one record per distinct missing band node of the segment: the
+            // first pair to claim its key in the table writes it; every
+            // missing pair lists its slot (or, without a table entry, its
+            // own record as -(record + 1)).  A probe visits each slot at
+            // most once: a table filled by a level with more pairs than
+            // the capacity (screened again with larger arrays) leaves the
+            // remaining pairs without an entry.
+            const int k = atomicAdd(out.count + 3, 1);
+            int e = 0;
+            bool own = true, slot = false;
+            const unsigned long long key = dedup_key(S, sb.s, x, y, z);
+            if (key && out.dedup_cap > 0) {
+                const unsigned mask = (unsigned)out.dedup_cap - 1;
+                unsigned h = dedup_slot(key) & mask;
+                for (int probe = 0; probe < out.dedup_cap; probe++, h = (h + 1) & mask) {
+                    const unsigned long long prev = atomicCAS(out.dedup_keys + h, 0ull, key);
+                    if (prev == 0ull || prev == key) {
+                        own = prev == 0ull;
+                        e = (int)h;
+                        slot = true;
+                        break;
+                    }
+                }
+            }
+            if (own) {
+                const int m = atomicAdd(out.count + 1, 1);
+                if (m < out.missing_cap) {
+                    int *rec = out.missing + 6 * (size_t)m;
+                    rec[0] = q;
+                    rec[1] = code;
+                    rec[2] = sb.g;
+                    rec[3] = x;
+                    rec[4] = y;
+                    rec[5] = z;
+                }
+                if (slot) out.dedup_rec[e] = m;
+                else e = -(m + 1);
+            }
+            if (k < out.capacity) {
+                out.missing_pairs[2 * (size_t)k] = q;
+                out.missing_pairs[2 * (size_t)k + 1] = e;
+            }
+        }
+        if (q < out.capacity) {
+            out.pair_g[q] = sb.g;
+            out.pair_f[q] = code;
+            out.pair_l[3 * q + 0] = x;
+            out.pair_l[3 * q + 1] = y;
+            out.pair_l[3 * q + 2] = z;
+        }
+    }
+    if (sb.b == 0 && threadIdx.x == 0) {
+        const_as4<mwg_segment> &S = segs[sb.s];
+        const int nb = screen_blocks(S.band_w);
+        const int np = blk_off[blk + nb] - base;
+        out.g_first[sb.g] = base;
+        out.g_npairs[sb.g] = np;
+        // the block count covers the box only if band_w is the table's band width
+        if (sb.g == S.first && S.band_w >= 0 && S.op.band_max[S.o_depth] != S.band_w) out.count[2] = 1;
+    }
+
+}
+
+template <bool DEV>
 __global__ __launch_bounds__(kScreenThreads) void k_screen_scatter(const mwg_segment *segs_, int nseg, const int *seg_blk, int nblk, const int *count_dev,
                                                                   const int *l, const int *cand_pos, const int *cand_code, const int *blk_off, mwg_pairs out) {
     const auto segs = ro(segs_);
-    const int live = live_blocks(nblk, count_dev, segs);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *out.count = blk_off[live];
-    for (int blk = blockIdx.x; blk < live; blk += gridDim.x) {
-        const ScreenBlock sb = locate(seg_blk, nseg, segs, blk);
-        const int base = blk_off[blk];
-        const int cnt = blk_off[blk + 1] - base;
-        if (threadIdx.x < cnt) {
-            const_as4<mwg_segment> &S = segs[sb.s];
-            const int gl[3] = {l[3 * sb.g], l[3 * sb.g + 1], l[3 * sb.g + 2]};
-            const Box bx = segment_box(S, gl);
-            const int p = cand_pos[(size_t)blk * kScreenThreads + threadIdx.x];
-            const int q = base + threadIdx.x;
-            const int code = cand_code[(size_t)blk * kScreenThreads + threadIdx.x];
-            const int x = bx.s[0] + p % bx.n[0], y = bx.s[1] + (p / bx.n[0]) % bx.n[1], z = bx.s[2] + p / (bx.n[0] * bx.n[1]);
-            if (code < 0) {
-                // one record per distinct missing band node of the segment: the
-                // first pair to claim its key in the table writes it; every
-                // missing pair lists its slot (or, without a table entry, its
-                // own record as -(record + 1)).  A probe visits each slot at
-                // most once: a table filled by a level with more pairs than
-                // the capacity (screened again with larger arrays) leaves the
-                // remaining pairs without an entry.
-                const int k = atomicAdd(out.count + 3, 1);
-                int e = 0;
-                bool own = true, slot = false;
-                const unsigned long long key = dedup_key(S, sb.s, x, y, z);
-                if (key && out.dedup_cap > 0) {
-                    const unsigned mask = (unsigned)out.dedup_cap - 1;
-                    unsigned h = dedup_slot(key) & mask;
-                    for (int probe = 0; probe < out.dedup_cap; probe++, h = (h + 1) & mask) {
-                        const unsigned long long prev = atomicCAS(out.dedup_keys + h, 0ull, key);
-                        if (prev == 0ull || prev == key) {
-                            own = prev == 0ull;
-                            e = (int)h;
-                            slot = true;
-                            break;
-                        }
-                    }
-                }
-                if (own) {
-                    const int m = atomicAdd(out.count + 1, 1);
-                    if (m < out.missing_cap) {
-                        int *rec = out.missing + 6 * (size_t)m;
-                        rec[0] = q;
-                        rec[1] = code;
-                        rec[2] = sb.g;
-                        rec[3] = x;
-                        rec[4] = y;
-                        rec[5] = z;
-                    }
-                    if (slot) out.dedup_rec[e] = m;
-                    else e = -(m + 1);
-                }
-                if (k < out.capacity) {
-                    out.missing_pairs[2 * (size_t)k] = q;
-                    out.missing_pairs[2 * (size_t)k + 1] = e;
-                }
-            }
-            if (q < out.capacity) {
-                out.pair_g[q] = sb.g;
-                out.pair_f[q] = code;
-                out.pair_l[3 * q + 0] = x;
-                out.pair_l[3 * q + 1] = y;
-                out.pair_l[3 * q + 2] = z;
-            }
-        }
-        if (sb.b == 0 && threadIdx.x == 0) {
-            const_as4<mwg_segment> &S = segs[sb.s];
-            const int nb = screen_blocks(S.band_w);
-            const int np = blk_off[blk + nb] - base;
-            out.g_first[sb.g] = base;
-            out.g_npairs[sb.g] = np;
-            // the block count covers the box only if band_w is the table's band width
-            if (sb.g == S.first && S.band_w >= 0 && S.op.band_max[S.o_depth] != S.band_w) out.count[2] = 1;
-        }
-
+    if constexpr (!DEV) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) *out.count = blk_off[nblk];
+        scatter_block(segs, nseg, seg_blk, blockIdx.x, l, cand_pos, cand_code, blk_off, out);
+    } else {
+        const int live = live_blocks(nblk, count_dev, segs);
+        if (blockIdx.x == 0 && threadIdx.x == 0) *out.count = blk_off[live];
+        for (int blk = blockIdx.x; blk < live; blk += gridDim.x) scatter_block(segs, nseg, seg_blk, blk, l, cand_pos, cand_code, blk_off, out);
     }
 }
 
@@ -429,12 +446,20 @@ extern "C" mwg_status mwg_pair_screen(const mwg_level *lvl, mwg_pairs *out) {
     // bound on the host) a grid of at most 8 workgroups per compute unit
     // strides over the blocks
     const unsigned grid = lvl->count_dev ? (unsigned)std::min<long long>(nblk, 8ll * device_cus()) : (unsigned)nblk;
-    hipLaunchKernelGGL(k_screen_positions, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, lvl->count_dev, lvl->l,
-                       lvl->node_thr, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
+    if (lvl->count_dev)
+        hipLaunchKernelGGL(k_screen_positions<true>, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, lvl->count_dev,
+                           lvl->l, lvl->node_thr, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
+    else
+        hipLaunchKernelGGL(k_screen_positions<false>, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, nullptr, lvl->l,
+                           lvl->node_thr, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), out->count);
     MWG_LAUNCHED();
     if ((st = scan_exclusive(g_blk.as<int>(), g_blk.as<int>(), (int)nblk, lvl->count_dev, screen_blocks(lvl->segs[0].band_w))) != MWG_OK) return st;
-    hipLaunchKernelGGL(k_screen_scatter, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, lvl->count_dev, lvl->l,
-                       g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), *out);
+    if (lvl->count_dev)
+        hipLaunchKernelGGL(k_screen_scatter<true>, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, lvl->count_dev,
+                           lvl->l, g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), *out);
+    else
+        hipLaunchKernelGGL(k_screen_scatter<false>, dim3(grid), dim3(kScreenThreads), 0, 0, dev.segs, lvl->nseg, seg_blk_dev, (int)nblk, nullptr, lvl->l,
+                           g_cand_pos.as<int>(), g_cand_code.as<int>(), g_blk.as<int>(), *out);
     MWG_LAUNCHED();
     return MWG_OK;
 }
