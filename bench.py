@@ -510,6 +510,7 @@ def main():
     ap.add_argument("--no-predictor", action="store_true", help="do not time the refinement-predictor variant beside the chosen form")
     ap.add_argument("--no-batch", action="store_true", help="leave out the 64-tree batch comparison (profiling runs of the single tree)")
     ap.add_argument("--no-autotune", action="store_true", help="leave out the launch-shape autotuner run")
+    ap.add_argument("--no-ahead", action="store_true", help="leave out the plain FP64 apply with levels formed ahead on the GPU")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
@@ -628,6 +629,21 @@ def main():
                             "candidates": t["candidates"]}
         mr.set_band_layout("auto")
         mr.set_band_launch(32, 0)
+    if not args.no_ahead:
+        # the plain FP64 apply with each next level formed and screened on the
+        # GPU before the host's split (opt-in; the same tree bit for bit)
+        mr.set_level_ahead(True)
+        try:
+            ah = time_form(mr, plain_one, args.steps, args.warmup)
+        finally:
+            mr.set_level_ahead(False)
+        ia, ca, _ = ah["out"].export()
+        ast = ah["stats"][-1]
+        res["level_ahead"] = {"form": "FP64 sweeps, next level formed and screened on the GPU (set_level_ahead)",
+                              "ms_per_step": round(ah["s"] * 1e3, 4), "speedup_vs_plain": round(plain["s"] / ah["s"], 3),
+                              "steps_formed_ahead": int(ast["n_ahead"]), "passes": int(ast["n_levels"]),
+                              "identical_to_plain": bool(np.array_equal(ia, i0) and np.array_equal(ca, c0)),
+                              "kernel_ms_per_step": {k: round(v, 4) for k, v in ah["breakdown"].items()}}
     if not args.no_cpu_baseline:
         res["cpu_baseline"] = cpu_baseline(args.workload, w, src)
         res["gpu_vs_cpu"] = res["cpu_baseline"]["s_per_tree"] / plain["s"]
