@@ -1,7 +1,9 @@
 // apply(): MRCPP's breadth-first operator application with every level's band
 // sum, generated band nodes and final multiwavelet transforms on the MI355X.
 // The host keeps the tree topology and the split decisions
-// (src/treebuilders/apply.cpp:68-93, src/treebuilders/TreeBuilder.cpp:38-87).
+// (src/treebuilders/apply.cpp:68-93, src/treebuilders/TreeBuilder.cpp:38-87);
+// optionally (set_level_ahead) the device forms each next level and its pair
+// screen first, and the host keeps it where its own split agrees bit for bit.
 //
 // apply_batch() runs many applications together: each breadth-first step
 // gathers the work nodes of every unfinished tree into one level of
