@@ -18,6 +18,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <mutex>
 #include <map>
 #include <memory>
 #include <set>
@@ -160,11 +162,12 @@ struct Shape {
     int kp1, n3, ncoef;
 };
 
-// Plan the generated nodes some pairs of a tree need into its input mirror
-// (queued in `batch`) and set those pairs' band nodes (pf[i] < 0: ancestor
-// code; band translation pl, target scale sc[i]).
-void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const std::vector<int> &pl, const std::vector<int> &sc,
-                         device::MirrorBatch &batch) {
+// Plan the generated nodes some pairs of a tree need into its input mirror's
+// host table and set those pairs' band nodes (pf[i] < 0: ancestor code; band
+// translation pl, target scale sc[i]); returns the nodes to generate, for
+// MirrorBatch::add.  Touches only the tree's input mirror: trees with
+// different inputs may be planned concurrently.
+std::vector<device::GenNode> plan_band_nodes(Job &job, std::vector<int> &pf, const std::vector<int> &pl, const std::vector<int> &sc) {
     device::TreeMirror &fm = *job.fm;
     struct NewNode {
         int id, parent, child, scale;
@@ -197,12 +200,27 @@ void generate_band_nodes(Job &job, const Shape &sh, std::vector<int> &pf, const 
         }
         pf[p] = prev;
     }
-    if (made.empty()) return;
     std::vector<device::GenNode> plan;
     plan.reserve(made.size());
     for (const auto &m : made) plan.push_back({m.id, m.parent, m.child, m.scale});
-    batch.add(fm, *job.mra, job.filters, plan);
-    job.stats.nGenerated += (long long)made.size();
+    return plan;
+}
+
+// Runs body(i) for i in [0, n) on the host threads when n >= 4; the first
+// exception is rethrown once all have finished.
+template <typename F> void host_parallel(int n, F &&body) {
+    std::exception_ptr err;
+    std::mutex m;
+#pragma omp parallel for schedule(dynamic, 1) if (n >= 4)
+    for (int i = 0; i < n; i++) {
+        try {
+            body(i);
+        } catch (...) {
+            std::lock_guard<std::mutex> lk(m);
+            if (!err) err = std::current_exception();
+        }
+    }
+    if (err) std::rethrow_exception(err);
 }
 
 // A run of one level batch: nodes of one tree at one scale.
@@ -496,43 +514,71 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         struct Miss {
             int job, anc, scale, l[3], rec;
         };
+        // records grouped by tree (in record order), then each tree's sorted
+        const int J = (int)jobs.size();
+        std::vector<size_t> jb(J + 1, 0);
+        for (int i = 0; i < nmissing; i++) jb[nodeJob[rec[6 * (size_t)i + 2]] + 1]++;
+        for (int j = 0; j < J; j++) jb[j + 1] += jb[j];
         std::vector<Miss> ms(nmissing);
-        for (int i = 0; i < nmissing; i++) {
-            const int *r = rec.data() + 6 * (size_t)i;
-            ms[i] = Miss{nodeJob[r[2]], r[1], segs[segOf[r[2]]].scale, {r[3], r[4], r[5]}, i};
+        {
+            std::vector<size_t> put(jb.begin(), jb.end() - 1);
+            for (int i = 0; i < nmissing; i++) {
+                const int *r = rec.data() + 6 * (size_t)i;
+                const int j = nodeJob[r[2]];
+                ms[put[j]++] = Miss{j, r[1], segs[segOf[r[2]]].scale, {r[3], r[4], r[5]}, i};
+            }
         }
         auto same = [](const Miss &a, const Miss &b) {
             return a.job == b.job && a.anc == b.anc && a.scale == b.scale && a.l[0] == b.l[0] && a.l[1] == b.l[1] && a.l[2] == b.l[2];
         };
-        std::sort(ms.begin(), ms.end(), [](const Miss &a, const Miss &b) {
-            if (a.job != b.job) return a.job < b.job;
-            if (a.scale != b.scale) return a.scale < b.scale;
-            if (a.anc != b.anc) return a.anc < b.anc;
-            for (int d = 0; d < 3; d++)
-                if (a.l[d] != b.l[d]) return a.l[d] < b.l[d];
-            return a.rec < b.rec;
-        });
-        std::vector<int> nodeOfRecord(nmissing);
-        device::MirrorBatch batch;
-        for (size_t a = 0; a < ms.size();) {
-            // the distinct band nodes of one tree
-            const int job = ms[a].job;
-            std::vector<int> f, l3, sc;
-            std::vector<size_t> first;
-            size_t b = a;
-            for (; b < ms.size() && ms[b].job == job; b++)
-                if (b == a || !same(ms[b], ms[b - 1])) {
-                    first.push_back(b);
-                    f.push_back(ms[b].anc);
-                    l3.insert(l3.end(), ms[b].l, ms[b].l + 3);
-                    sc.push_back(ms[b].scale);
-                }
-            generate_band_nodes(jobs[job], sh, f, l3, sc, batch);
-            for (size_t k = 0; k < first.size(); k++) {
-                const size_t e = k + 1 < first.size() ? first[k + 1] : b;
-                for (size_t i = first[k]; i < e; i++) nodeOfRecord[ms[i].rec] = f[k];
+        // trees sharing an input mirror are planned in tree order by one thread
+        std::vector<int> withRecords;
+        for (int j = 0; j < J; j++)
+            if (jb[j + 1] > jb[j]) withRecords.push_back(j);
+        std::vector<std::vector<int>> byMirror;
+        {
+            std::map<const device::TreeMirror *, size_t> slot;
+            for (int j : withRecords) {
+                auto it = slot.emplace(jobs[j].fm, byMirror.size()).first;
+                if (it->second == byMirror.size()) byMirror.emplace_back();
+                byMirror[it->second].push_back(j);
             }
-            a = b;
+        }
+        std::vector<int> nodeOfRecord(nmissing);
+        std::vector<std::vector<device::GenNode>> made(J);
+        host_parallel((int)byMirror.size(), [&](int m) {
+            for (int job : byMirror[m]) {
+                const auto lo = ms.begin() + jb[job], hi = ms.begin() + jb[job + 1];
+                std::sort(lo, hi, [](const Miss &a, const Miss &b) {
+                    if (a.scale != b.scale) return a.scale < b.scale;
+                    if (a.anc != b.anc) return a.anc < b.anc;
+                    for (int d = 0; d < 3; d++)
+                        if (a.l[d] != b.l[d]) return a.l[d] < b.l[d];
+                    return a.rec < b.rec;
+                });
+                // the distinct band nodes of the tree
+                std::vector<int> f, l3, sc;
+                std::vector<size_t> first;
+                const size_t a = jb[job], b = jb[job + 1];
+                for (size_t i = a; i < b; i++)
+                    if (i == a || !same(ms[i], ms[i - 1])) {
+                        first.push_back(i);
+                        f.push_back(ms[i].anc);
+                        l3.insert(l3.end(), ms[i].l, ms[i].l + 3);
+                        sc.push_back(ms[i].scale);
+                    }
+                made[job] = plan_band_nodes(jobs[job], f, l3, sc);
+                for (size_t k = 0; k < first.size(); k++) {
+                    const size_t e = k + 1 < first.size() ? first[k + 1] : b;
+                    for (size_t i = first[k]; i < e; i++) nodeOfRecord[ms[i].rec] = f[k];
+                }
+            }
+        });
+        device::MirrorBatch batch;
+        for (int j : withRecords) {
+            if (made[j].empty()) continue;
+            jobs[j].stats.nGenerated += (long long)made[j].size();
+            batch.add(*jobs[j].fm, *jobs[j].mra, jobs[j].filters, made[j]);
         }
         // one upload for every tree's new nodes and the node of every record
         const size_t at = batch.addInts(nodeOfRecord.data(), nodeOfRecord.size());
@@ -1100,17 +1146,26 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
             }
         }
         flush();
+        // the trees' split steps are independent: on the host threads when
+        // every tree is a plain one (a predictor's steps call the device)
+        bool plainAll = true;
+        for (int j : active) plainAll = plainAll && jobs[j].mode == Mode::Plain && !jobs[j].task.predictor && !jobs[j].task.samples;
+        auto split_step = [&](int i) {
+            const auto th = clk::now();
+            advance(jobs[active[i]]);
+            jobs[active[i]].stats.tHost += since(th);
+        };
+        if (plainAll) host_parallel((int)active.size(), split_step);
+        else
+            for (int i = 0; i < (int)active.size(); i++) split_step(i);
         for (int j : active) {
-            auto th = clk::now();
             Job &job = jobs[j];
-            advance(job);
             if (!job.work.empty()) {
                 // grow the device pool, keeping every slot (recomputed nodes
                 // need not be the newest)
                 const size_t n = (size_t)job.g->size() * sh.ncoef;
                 job.task.out->deviceCoefs().reserve(n, std::min(n, job.task.out->deviceCoefs().capacity()));
             }
-            job.stats.tHost += since(th);
         }
         if (ahead.formed) {
             // the device's next level stands when the host formed the same
