@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstring>
 #include <exception>
+#include <omp.h>
 #include <mutex>
 #include <map>
 #include <memory>
@@ -206,12 +207,14 @@ std::vector<device::GenNode> plan_band_nodes(Job &job, std::vector<int> &pf, con
     return plan;
 }
 
-// Runs body(i) for i in [0, n) on the host threads when n >= 4; the first
-// exception is rethrown once all have finished.
+// Runs body(i) for i in [0, n) on the host threads when n >= 4 (two cores
+// left to the runtime's own threads, which feed the GPU); the first exception
+// is rethrown once all have finished.
 template <typename F> void host_parallel(int n, F &&body) {
     std::exception_ptr err;
     std::mutex m;
-#pragma omp parallel for schedule(dynamic, 1) if (n >= 4)
+    const int threads = std::max(1, std::min(n, omp_get_max_threads() - 2));
+#pragma omp parallel for schedule(dynamic, 1) if (n >= 4) num_threads(threads)
     for (int i = 0; i < n; i++) {
         try {
             body(i);
