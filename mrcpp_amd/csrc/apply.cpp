@@ -211,10 +211,14 @@ std::vector<device::GenNode> plan_band_nodes(Job &job, std::vector<int> &pf, con
 // left to the runtime's own threads, which feed the GPU); the first exception
 // is rethrown once all have finished.
 template <typename F> void host_parallel(int n, F &&body) {
+    if (n < 4) { // no thread team for a few items
+        for (int i = 0; i < n; i++) body(i);
+        return;
+    }
     std::exception_ptr err;
     std::mutex m;
     const int threads = std::max(1, std::min(n, omp_get_max_threads() - 2));
-#pragma omp parallel for schedule(dynamic, 1) if (n >= 4) num_threads(threads)
+#pragma omp parallel for schedule(dynamic, 1) num_threads(threads)
     for (int i = 0; i < n; i++) {
         try {
             body(i);
@@ -610,26 +614,36 @@ void run_level_batch(std::vector<Job> &jobs, const Shape &sh, const std::vector<
         jobs[j].stats.tGenerate += tGenerate;
         jobs[j].stats.tBand += tBand;
     }
+    // the norms back into the trees, segments on the host threads (each
+    // writes its own nodes; per-tree arrays sized first)
+    std::vector<int> segAt(specs.size());
     at = 0;
-    for (const auto &sp : specs) {
-        Job &job = jobs[sp.job];
-        detail::NodeStore<3> &g = *job.g;
+    for (size_t s = 0; s < specs.size(); s++) {
+        segAt[s] = at;
+        at += (int)specs[s].nodes.size();
+        Job &job = jobs[specs[s].job];
         if (job.marginPass) {
-            job.margin.resize(g.size());
-            job.thrUsed.resize(g.size(), -1.0);
-        }
-        for (int u : sp.nodes) {
-            double *cn = g.norms(u);
-            for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)at + t];
-            g.setSquareNorm(u, nrm[9 * (size_t)at + 8]);
-            g.setHasCoefs(u, true);
-            if (job.marginPass) {
-                for (int j = 0; j < MWG_MARGINS; j++) job.margin[u][j] = mgn[MWG_MARGINS * (size_t)at + j];
-                job.thrUsed[u] = job.precFunc ? node_threshold(job, u) : job.gthr;
-            }
-            at++;
+            job.margin.resize(job.g->size());
+            job.thrUsed.resize(job.g->size(), -1.0);
         }
     }
+    host_parallel((int)specs.size(), [&](int s) {
+        const SegmentSpec &sp = specs[s];
+        Job &job = jobs[sp.job];
+        detail::NodeStore<3> &g = *job.g;
+        int k = segAt[s];
+        for (int u : sp.nodes) {
+            double *cn = g.norms(u);
+            for (int t = 0; t < 8; t++) cn[t] = nrm[9 * (size_t)k + t];
+            g.setSquareNorm(u, nrm[9 * (size_t)k + 8]);
+            g.setHasCoefs(u, true);
+            if (job.marginPass) {
+                for (int j = 0; j < MWG_MARGINS; j++) job.margin[u][j] = mgn[MWG_MARGINS * (size_t)k + j];
+                job.thrUsed[u] = job.precFunc ? node_threshold(job, u) : job.gthr;
+            }
+            k++;
+        }
+    });
 }
 
 // Device bytes of one output node in a level batch (screen candidates and group lists).
@@ -695,15 +709,19 @@ void transform_and_norms(std::vector<Job> &jobs, const Shape &sh) {
     std::vector<std::vector<std::vector<int>>> br(jobs.size()); // [job][depth] parents with children
     std::vector<long long> poolOff(jobs.size());
     size_t depths = 0;
-    for (size_t j = 0; j < jobs.size(); j++) {
-        detail::NodeStore<3> &g = *jobs[j].g;
-        poolOff[j] = (long long)(((uintptr_t)jobs[j].task.out->deviceCoefs().get() - (uintptr_t)base) / sizeof(double));
+    std::vector<std::vector<int>> all(jobs.size()); // every real node of each tree
+    host_parallel((int)jobs.size(), [&](int j) {
+        const detail::NodeStore<3> &g = *jobs[j].g;
         const auto tab = detail::real_nodes_by_depth(g);
         br[j].resize(tab.size());
         for (size_t d = 0; d < tab.size(); d++)
             for (int u : tab[d])
                 if (g.hasRealChildren(u)) br[j][d].push_back(u);
-        depths = std::max(depths, tab.size());
+        all[j] = detail::real_nodes(g);
+    });
+    for (size_t j = 0; j < jobs.size(); j++) {
+        poolOff[j] = (long long)(((uintptr_t)jobs[j].task.out->deviceCoefs().get() - (uintptr_t)base) / sizeof(double));
+        depths = std::max(depths, br[j].size());
     }
     struct Pass {
         size_t at, n;
@@ -726,12 +744,9 @@ void transform_and_norms(std::vector<Job> &jobs, const Shape &sh) {
     for (size_t d = 0; d < depths; d++) add(d, false);
     for (int d = (int)depths - 2; d >= 0; d--) add((size_t)d, true);
     // then the component norms of every node: offsets appended to po
-    std::vector<std::vector<int>> all(jobs.size());
     const size_t normAt = po.size();
-    for (size_t j = 0; j < jobs.size(); j++) {
-        all[j] = detail::real_nodes(*jobs[j].g);
+    for (size_t j = 0; j < jobs.size(); j++)
         for (int u : all[j]) po.push_back(poolOff[j] + (long long)u * sh.ncoef);
-    }
     const size_t nAll = po.size() - normAt;
     S.pOff.reserve(po.size());
     S.cOff.reserve(std::max<size_t>(co.size(), 1));
@@ -1191,18 +1206,30 @@ void apply_batch(const std::vector<ApplyTask> &tasks) {
     auto tt = clk::now();
     transform_and_norms(jobs, sh);
     const double tTransform = since(tt);
+    std::vector<long long> nOut(jobs.size());
+    host_parallel((int)jobs.size(), [&](int i) {
+        detail::calc_square_norm(*jobs[i].g);
+        nOut[i] = (long long)detail::real_nodes(*jobs[i].g).size();
+    });
     for (size_t i = 0; i < jobs.size(); i++) {
         Job &job = jobs[i];
-        detail::calc_square_norm(*job.g);
         job.task.out->markDeviceResident();
         job.stats.nApplied = (long long)na[2 * i];
         job.stats.nPairs = (long long)na[2 * i + 1];
         job.stats.tTransform = tTransform;
-        job.stats.nOutputNodes = (long long)detail::real_nodes(*job.g).size();
+        job.stats.nOutputNodes = nOut[i];
     }
     const auto tc = clk::now();
-    for (auto &job : jobs)
-        if (mirrors.insert(job.fm).second) device::drop_generated(*job.fm, *job.mra, *job.fstore);
+    std::vector<int> owner; // first job of every distinct input mirror
+    for (size_t j = 0; j < jobs.size(); j++)
+        if (mirrors.insert(jobs[j].fm).second) owner.push_back((int)j);
+    std::vector<char> dropped(owner.size(), 0);
+    host_parallel((int)owner.size(), [&](int i) {
+        const Job &job = jobs[owner[i]];
+        dropped[i] = device::drop_generated_host(*job.fm, *job.mra, *job.fstore) ? 1 : 0;
+    });
+    for (size_t i = 0; i < owner.size(); i++)
+        if (dropped[i]) device::drop_generated_device(*jobs[owner[i]].fm);
     const double tCleanup = since(tc);
     const double seconds = since(t_start);
     for (auto &job : jobs) {

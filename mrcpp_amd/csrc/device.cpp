@@ -356,8 +356,8 @@ void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra
     b.run(staging);
 }
 
-void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f) {
-    if (fm.nGen == 0) return;
+bool drop_generated_host(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f) {
+    if (fm.nGen == 0) return false;
     // the generated nodes' slots emptied (only those go to the device), or
     // after a rehash the real nodes' table rebuilt
     if (!fm.table.rollback()) {
@@ -370,7 +370,13 @@ void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const
     fm.genScale.clear();
     fm.genL.clear();
     fm.hostOff.resize(fm.nReal);
-    fm.uploadTable();
+    return true;
+}
+
+void drop_generated_device(TreeMirror &fm) { fm.uploadTable(); }
+
+void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f) {
+    if (drop_generated_host(fm, mra, f)) drop_generated_device(fm);
 }
 
 } // namespace device

@@ -249,6 +249,11 @@ void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra
 
 // Forget a mirror's generated nodes (FunctionTree::deleteGenerated).
 void drop_generated(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f);
+// The same in two parts: the host table and lists (no device call; mirrors
+// may be dropped concurrently), then the device table (returns at once when
+// nothing was dropped).
+bool drop_generated_host(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const detail::NodeStore<3> &f);
+void drop_generated_device(TreeMirror &fm);
 
 } // namespace device
 } // namespace mrcpp
