@@ -322,6 +322,28 @@ mwg_status mwg_generate_nodes(const mwg_ftree *f, const double *filters, const i
 mwg_status mwg_generate_nodes_recorded(const mwg_ftree *f, const double *filters, const int *ids, const int *parent, const int *child_idx,
                                        const long long *offs, long long *coef_off_w, unsigned char *is_gen_w, int count);
 
+/* The band nodes of several trees in one launch: node i of the call is
+ * generated and recorded as by mwg_generate_nodes_recorded in tree
+ * trees[tree_of[i]] (trees: a device array of ntrees entries, all of order
+ * kp1, coef_off_w / is_gen_w the tree's own f.coef_off / f.is_gen).  Nodes of
+ * one call must not be each other's parents (one call per scale, coarse to
+ * fine).  No reference counterpart (one launch for a level batch of trees). */
+typedef struct {
+    mwg_ftree f;
+    long long *coef_off_w;
+    unsigned char *is_gen_w;
+} mwg_gen_tree;
+mwg_status mwg_generate_nodes_multi(int kp1, const mwg_gen_tree *trees, int ntrees, const double *filters, const int *tree_of, const int *ids,
+                                    const int *parent, const int *child_idx, const long long *offs, int count);
+/* Key-table slots of several trees in one launch: upd is a device array of n
+ * records (table, slot, key hi, key lo, value), each written into
+ * tables[table] (a device array of ntables).  No reference counterpart. */
+typedef struct {
+    unsigned long long *keys; /* [cap][2] */
+    int *vals;                /* [cap] */
+} mwg_hash_table;
+mwg_status mwg_hash_update_multi(const mwg_hash_table *tables, int ntables, const long long *upd, int n);
+
 /* Exact screening and the 3-sweep band sum of every candidate pair, then the
  * output coefficients of the level and their component norms
  * (ConvolutionCalculator::calcNode, applyOperComp, applyOperator and

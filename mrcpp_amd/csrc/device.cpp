@@ -312,6 +312,7 @@ void MirrorBatch::add(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, con
         size_t j = i;
         while (j < n && made[j].scale == made[i].scale) j++;
         e.runs.push_back({i, j - i});
+        e.runScale.push_back(made[i].scale);
         i = j;
     }
     entries_.push_back(std::move(e));
@@ -324,6 +325,11 @@ size_t MirrorBatch::addInts(const int *p, size_t n) {
 }
 
 const int *MirrorBatch::run(DeviceArray<long long> &staging) {
+    // trees of one order and filter bank: one key-table launch for all of
+    // them and one generation launch per scale (coarse to fine)
+    bool one = entries_.size() > 1;
+    for (const Entry &e : entries_) one = one && e.filters == entries_[0].filters && e.fm->kp1 == entries_[0].fm->kp1;
+    if (one) return runTogether(staging);
     const size_t nw = words_.size(), total = nw + (ints_.size() + 1) / 2;
     if (total == 0) return nullptr;
     std::vector<long long> buf(total, 0);
@@ -347,6 +353,71 @@ const int *MirrorBatch::run(DeviceArray<long long> &staging) {
     words_.clear();
     ints_.clear();
     return I;
+}
+
+const int *MirrorBatch::runTogether(DeviceArray<long long> &staging) {
+    const size_t nEnt = entries_.size();
+    std::vector<mwg_gen_tree> trees(nEnt);
+    std::vector<mwg_hash_table> tables(nEnt);
+    std::vector<long long> upd; // (tree, slot, key hi, key lo, value)
+    struct G {
+        int scale, tree;
+        size_t k; // node k of the tree's entry
+    };
+    std::vector<G> gen;
+    for (size_t t = 0; t < nEnt; t++) {
+        const Entry &e = entries_[t];
+        TreeMirror &fm = *e.fm;
+        trees[t] = mwg_gen_tree{fm.view(*e.mra), fm.coefOff.get(), fm.isGen.get()}; // after every reserve of the batch
+        tables[t] = mwg_hash_table{fm.keys.get(), fm.vals.get()};
+        const long long *u = words_.data() + e.upd;
+        for (size_t k = 0; k < e.nupd; k++)
+            upd.insert(upd.end(), {(long long)t, u[k], u[e.nupd + 2 * k], u[e.nupd + 2 * k + 1], u[3 * e.nupd + k]});
+        for (size_t r = 0; r < e.runs.size(); r++)
+            for (size_t k = e.runs[r].first; k < e.runs[r].first + e.runs[r].second; k++) gen.push_back({e.runScale[r], (int)t, k});
+    }
+    std::stable_sort(gen.begin(), gen.end(), [](const G &a, const G &b) { return a.scale < b.scale; });
+    // one upload: [updates | node offsets | tree views | tables | integers of addInts | tree, id, parent, child of every node]
+    static_assert(sizeof(mwg_gen_tree) % 8 == 0 && sizeof(mwg_hash_table) % 8 == 0, "upload layout");
+    const size_t ng = gen.size(), wTree = sizeof(mwg_gen_tree) / 8, wTab = sizeof(mwg_hash_table) / 8;
+    const size_t oOffs = upd.size(), oTrees = oOffs + ng, oTabs = oTrees + wTree * nEnt, oInts = oTabs + wTab * nEnt;
+    const size_t nInts = ints_.size() + 4 * ng;
+    std::vector<long long> buf(oInts + (nInts + 1) / 2, 0);
+    std::copy(upd.begin(), upd.end(), buf.begin());
+    std::memcpy(buf.data() + oTrees, trees.data(), sizeof(mwg_gen_tree) * nEnt);
+    std::memcpy(buf.data() + oTabs, tables.data(), sizeof(mwg_hash_table) * nEnt);
+    int *I = reinterpret_cast<int *>(buf.data() + oInts);
+    if (!ints_.empty()) std::memcpy(I, ints_.data(), sizeof(int) * ints_.size());
+    int *gt = I + ints_.size(), *gid = gt + ng, *gpar = gid + ng, *gch = gpar + ng;
+    for (size_t i = 0; i < ng; i++) {
+        const Entry &e = entries_[gen[i].tree];
+        const size_t k = gen[i].k;
+        buf[oOffs + i] = words_[e.offs + k];
+        gt[i] = gen[i].tree;
+        gid[i] = ints_[e.ids + k];
+        gpar[i] = ints_[e.ids + e.n + k];
+        gch[i] = ints_[e.ids + 2 * e.n + k];
+    }
+    staging.reserve(buf.size());
+    staging.upload(buf.data(), buf.size());
+    const long long *W = staging.get();
+    const int *ID = reinterpret_cast<const int *>(W + oInts);
+    const auto *treesDev = reinterpret_cast<const mwg_gen_tree *>(W + oTrees);
+    if (!upd.empty())
+        check(mwg_hash_update_multi(reinterpret_cast<const mwg_hash_table *>(W + oTabs), (int)nEnt, W, (int)(upd.size() / 5)), "key table update");
+    const int *dt = ID + ints_.size(), *did = dt + ng, *dpar = did + ng, *dch = dpar + ng;
+    for (size_t a = 0; a < ng;) {
+        size_t b = a;
+        while (b < ng && gen[b].scale == gen[a].scale) b++;
+        check(mwg_generate_nodes_multi(entries_[0].fm->kp1, treesDev, (int)nEnt, entries_[0].filters, dt + a, did + a, dpar + a, dch + a,
+                                       W + oOffs + a, (int)(b - a)),
+              "generate nodes");
+        a = b;
+    }
+    entries_.clear();
+    words_.clear();
+    ints_.clear();
+    return ID;
 }
 
 void generate_mirror_nodes(TreeMirror &fm, const MultiResolutionAnalysis<3> &mra, const double *filters, std::vector<GenNode> &made,

@@ -230,6 +230,7 @@ public:
     bool empty() const { return entries_.empty() && ints_.empty(); }
 
 private:
+    const int *runTogether(DeviceArray<long long> &staging);
     struct Entry {
         TreeMirror *fm;
         const MultiResolutionAnalysis<3> *mra;
@@ -237,6 +238,7 @@ private:
         size_t upd, nupd;              // table updates: words_[upd ..), 4 nupd words
         size_t offs, ids, n;           // offsets words_[offs ..), ids / parents / children ints_[ids ..)
         std::vector<std::pair<size_t, size_t>> runs; // (first, count) of each scale
+        std::vector<int> runScale;                    // the scale of each run
     };
     std::vector<Entry> entries_;
     std::vector<long long> words_;

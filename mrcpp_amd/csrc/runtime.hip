@@ -152,6 +152,17 @@ __global__ __launch_bounds__(256) void k_hash_update(unsigned long long *keys, i
     vals[slot] = (int)upd[3 * (size_t)n + i];
 }
 
+__global__ __launch_bounds__(256) void k_hash_update_multi(const mwg_hash_table *tables, const long long *upd, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const long long *r = upd + 5 * (size_t)i;
+    const mwg_hash_table t = tables[r[0]];
+    const long long slot = r[1];
+    t.keys[2 * slot] = (unsigned long long)r[2];
+    t.keys[2 * slot + 1] = (unsigned long long)r[3];
+    t.vals[slot] = (int)r[4];
+}
+
 } // namespace mwgdev
 
 using namespace mwgdev;
@@ -164,6 +175,16 @@ mwg_status mwg_hash_update(unsigned long long *keys, int *vals, const long long 
     if (n == 0) return MWG_OK;
     if (!keys || !vals || !upd) return fail(MWG_ERR_ARG, "null argument");
     hipLaunchKernelGGL(k_hash_update, dim3((n + 255) / 256), dim3(256), 0, 0, keys, vals, upd, n);
+    MWG_LAUNCHED();
+    return MWG_OK;
+}
+
+mwg_status mwg_hash_update_multi(const mwg_hash_table *tables, int ntables, const long long *upd, int n) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (n < 0 || ntables < 0) return fail(MWG_ERR_ARG, "negative count");
+    if (n == 0) return MWG_OK;
+    if (!tables || !upd || ntables == 0) return fail(MWG_ERR_ARG, "null argument");
+    hipLaunchKernelGGL(k_hash_update_multi, dim3((n + 255) / 256), dim3(256), 0, 0, tables, upd, n);
     MWG_LAUNCHED();
     return MWG_OK;
 }

@@ -7,20 +7,13 @@ namespace mwgdev {
 
 namespace {
 
-// Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
-// + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
-// With offs given, the node is also recorded in the mirror: coef_off_w[id] =
-// offs[k] (where its block goes) and is_gen_w[id] = 1.
+// The scaling block of child c of node pid (staged filters sfl; a, b, red:
+// LDS of the workgroup), written at offs[k] (recorded in the tree) or at the
+// node's own offset, and its norms.
 template <int N1>
-__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child,
-                                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w) {
+__device__ __forceinline__ void generate_node(const mwg_ftree &f, const double *sfl, int k, int id, int pid, int c, const long long *offs,
+                                              long long *coef_off_w, unsigned char *is_gen_w, double *a, double *b, double *red) {
     constexpr int N3 = N1 * N1 * N1;
-    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
-    __shared__ double red[8 * (kThreads / 64)];
-    __shared__ double sfl[8 * N1 * N1];
-    stage_filters<N1>(sfl, filt, 8);
-    const int k = blockIdx.x;
-    const int pid = parent[k], id = ids[k], c = child[k];
     const double *src = f.coef + f.coef_off[pid];
     const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
     for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[pad<N1>(o)] = (o < nblk * N3) ? src[o] : 0.0;
@@ -54,6 +47,34 @@ __global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double
         for (int t = 1; t < 8; t++) nr[t] = 0.0;
         nr[8] = n0 * n0;
     }
+}
+
+// Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
+// + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
+// With offs given, the node is also recorded in the mirror: coef_off_w[id] =
+// offs[k] (where its block goes) and is_gen_w[id] = 1.
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child,
+                                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w) {
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double sfl[8 * N1 * N1];
+    stage_filters<N1>(sfl, filt, 8);
+    const int k = blockIdx.x;
+    generate_node<N1>(f, sfl, k, ids[k], parent[k], child[k], offs, coef_off_w, is_gen_w, a, b, red);
+}
+
+// The same for nodes of several trees: node k in trees[tree_of[k]].
+template <int N1>
+__global__ __launch_bounds__(kThreads) void k_generate_multi(const mwg_gen_tree *trees, const double *filt, const int *tree_of, const int *ids,
+                                                             const int *parent, const int *child, const long long *offs) {
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double sfl[8 * N1 * N1];
+    stage_filters<N1>(sfl, filt, 8);
+    const int k = blockIdx.x;
+    const mwg_gen_tree &t = trees[tree_of[k]];
+    generate_node<N1>(t.f, sfl, k, ids[k], parent[k], child[k], offs, t.coef_off_w, t.is_gen_w, a, b, red);
 }
 
 template <int N1>
@@ -110,6 +131,15 @@ template <int N1> struct GenerateL {
         return MWG_OK;
     }
 };
+template <int N1> struct GenerateMultiL {
+    static mwg_status run(const mwg_gen_tree *trees, const double *filt, const int *tree_of, const int *ids, const int *par, const int *ch,
+                          const long long *offs, int count) {
+        ProfileScope prof_(MWG_KERNEL_GENERATE);
+        hipLaunchKernelGGL(k_generate_multi<N1>, dim3(count), dim3(kThreads), 0, 0, trees, filt, tree_of, ids, par, ch, offs);
+        MWG_LAUNCHED();
+        return MWG_OK;
+    }
+};
 template <int N1> struct ReconL {
     static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
         ProfileScope prof_(MWG_KERNEL_RECONSTRUCT);
@@ -158,6 +188,14 @@ mwg_status mwg_generate_nodes_recorded(const mwg_ftree *f, const double *filters
     if (!f || !filters || !ids || !parent || !child_idx || !offs || !coef_off_w || !is_gen_w) return fail(MWG_ERR_ARG, "null argument");
     if (coef_off_w != f->coef_off || is_gen_w != f->is_gen) return fail(MWG_ERR_ARG, "recorded offsets and flags must be the tree's own arrays");
     return dispatch<GenerateL>(f->kp1, f, filters, ids, parent, child_idx, count, offs, coef_off_w, is_gen_w);
+}
+
+mwg_status mwg_generate_nodes_multi(int kp1, const mwg_gen_tree *trees, int ntrees, const double *filters, const int *tree_of, const int *ids,
+                                    const int *parent, const int *child_idx, const long long *offs, int count) {
+    if (need_init() != MWG_OK) return MWG_ERR_ARG;
+    if (count <= 0) return MWG_OK;
+    if (!trees || ntrees <= 0 || !filters || !tree_of || !ids || !parent || !child_idx || !offs) return fail(MWG_ERR_ARG, "null argument");
+    return dispatch<GenerateMultiL>(kp1, trees, filters, tree_of, ids, parent, child_idx, offs, count);
 }
 
 mwg_status mwg_reconstruct_add(int kp1, const double *filters, double *coef, const long long *parent_off, const long long *child_off, int count,
