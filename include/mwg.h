@@ -164,6 +164,9 @@ typedef struct {
     const long long *coef_off;           /* [nodes] offset into coef (doubles) */
     double *norms;                       /* [nodes][9] */
     const unsigned char *is_gen;         /* [nodes] */
+    double norm_max;                     /* at least every node's component norms and sqrt(square norm),
+                                            generated nodes included (the screen skips a band position
+                                            whose operator bound times it fails the threshold); +inf: no bound */
 } mwg_ftree;
 
 /* One run of output nodes of a level that share a tree, a scale and an

@@ -230,6 +230,7 @@ mwg_ftree TreeMirror::view(const MultiResolutionAnalysis<3> &mra) const {
     f.coef_off = coefOff.get();
     f.norms = norms.get();
     f.is_gen = isGen.get();
+    f.norm_max = normMax;
     return f;
 }
 

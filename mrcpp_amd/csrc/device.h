@@ -3,6 +3,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -191,6 +192,10 @@ struct TreeMirror {
     std::vector<int> genParent;        // host bookkeeping for generated nodes
     std::vector<int> genScale;
     std::vector<std::array<int, 3>> genL;
+    // at least every node's norms (mwg_ftree::norm_max): the real nodes'
+    // largest, with room for the rounding of generated nodes, which the
+    // orthogonal two-scale transform bounds by their real ancestors
+    double normMax = HUGE_VAL;
     bool valid = false;
     mwg_ftree view(const MultiResolutionAnalysis<3> &mra) const;
     void uploadTable();

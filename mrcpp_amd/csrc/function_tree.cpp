@@ -120,11 +120,17 @@ device::TreeMirror &FunctionTree<3>::inputMirror() {
     const size_t nc = m->ncoef;
     std::vector<double> nrm((size_t)N * 9);
     m->hostOff.resize(N);
+    double normMax = 0.0;
     for (int u = 0; u < N; u++) {
-        for (int t = 0; t < 8; t++) nrm[(size_t)u * 9 + t] = s.compNorm(u, t);
+        for (int t = 0; t < 8; t++) {
+            nrm[(size_t)u * 9 + t] = s.compNorm(u, t);
+            normMax = std::max(normMax, nrm[(size_t)u * 9 + t]);
+        }
         nrm[(size_t)u * 9 + 8] = s.squareNorm(u);
+        normMax = std::max(normMax, std::sqrt(std::max(nrm[(size_t)u * 9 + 8], 0.0)));
         m->hostOff[u] = (long long)u * nc;
     }
+    m->normMax = std::isfinite(normMax) ? normMax * (1.0 + 1e-10) : HUGE_VAL;
     m->coef.reserve((size_t)N * nc + 1024 * (size_t)s.kp1_d());
     if (onDevice) {
         device::check(mwg_copy(m->coef.get(), dev_coef_.get(), sizeof(double) * (size_t)N * nc), "mirror copy");

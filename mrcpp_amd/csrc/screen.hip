@@ -51,6 +51,10 @@ __device__ inline bool screen_position(const_as4<mwg_segment> &S, double g_thr, 
     // a zero bound cannot pass a non-negative threshold: skip the lookup
     if (a == 0.0 && g_thr >= 0.0) return false;
     const mwg_ftree f = ld(S.f);
+    // nor can a bound that fails with the largest norm of any node of the
+    // tree (the test below with maxf <= norm_max; products of non-negative
+    // numbers round monotonically)
+    if (g_thr >= 0.0 && !(a * f.norm_max * kBoundSlack > g_thr)) return false;
     const int id = f_lookup(f, d, x, y, z);
     double maxf;
     double fnv[8]; // per-component norms of the band node (or their bound)
