@@ -77,43 +77,128 @@ __global__ __launch_bounds__(kThreads) void k_generate_multi(const mwg_gen_tree 
     generate_node<N1>(t.f, sfl, k, ids[k], parent[k], child[k], offs, t.coef_off_w, t.is_gen_w, a, b, red);
 }
 
+// The top-down and bottom-up passes are latency-bound for the few nodes of a
+// depth: their workgroups have kPassThreads threads, each forming a 2 x 2
+// register tile of a pass (the same per-output sum, over i in increasing
+// order, as the 4 x 4 tiles of mw_pass, so the same results).
+constexpr int kPassThreads = 1024;
+template <int N1> struct SmallTile {
+    static constexpr int N2 = N1 * N1, J = 2, R = 2, RB = N2 / R, JG = N1 / J, TILES = 8 * JG * RB;
+    static_assert(N1 % J == 0 && N2 % R == 0, "small pass tile shape");
+};
 template <int N1>
-__global__ __launch_bounds__(kThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
-    constexpr int N3 = N1 * N1 * N1;
-    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
-    __shared__ double sfl[8 * N1 * N1];
-    stage_filters<N1>(sfl, filt, 8);
-    const int k = blockIdx.x;
-    const double *src = coef + poff[k];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[pad<N1>(o)] = src[o];
-    __syncthreads();
-    mw_pass<N1>(b, a, sfl, 4, 0);
-    __syncthreads();
-    mw_pass<N1>(a, b, sfl, 4, 1);
-    __syncthreads();
-    mw_pass_add_children<N1>(coef, coff[k], stride, a, sfl);
+__device__ __forceinline__ void small_tile(const double *in, const double *filt, int dir_base, int dim, int t, int &gt, int &j0, int &rb, double (&s0)[2][2],
+                                           double (&s1)[2][2]) {
+    using P = SmallTile<N1>;
+    constexpr int N2 = P::N2, N3 = N2 * N1, J = P::J, R = P::R, RB = P::RB;
+    gt = t / (P::JG * RB);
+    j0 = ((t / RB) % P::JG) * J;
+    rb = t % RB;
+    const int bit = 1 << dim;
+    const int ft0 = gt & ~bit, ft1 = gt | bit;
+    const double *F0 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 0) * N2 + N1 * j0;
+    const double *F1 = filt + (size_t)(dir_base + 2 * ((gt >> dim) & 1) + 1) * N2 + N1 * j0;
+#pragma unroll
+    for (int p = 0; p < J; p++)
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            s0[p][q] = 0.0;
+            s1[p][q] = 0.0;
+        }
+#pragma unroll 2
+    for (int i = 0; i < N1; i++) {
+        double x0[R], x1[R], f0[J], f1[J];
+#pragma unroll
+        for (int q = 0; q < R; q++) {
+            x0[q] = in[pad<N1>(ft0 * N3 + N1 * (rb + q * RB) + i)];
+            x1[q] = in[pad<N1>(ft1 * N3 + N1 * (rb + q * RB) + i)];
+        }
+#pragma unroll
+        for (int p = 0; p < J; p++) {
+            f0[p] = F0[N1 * p + i];
+            f1[p] = F1[N1 * p + i];
+        }
+#pragma unroll
+        for (int p = 0; p < J; p++)
+#pragma unroll
+            for (int q = 0; q < R; q++) {
+                s0[p][q] += x0[q] * f0[p];
+                s1[p][q] += x1[q] * f1[p];
+            }
+    }
+}
+template <int N1> __device__ __forceinline__ void small_pass(double *out, const double *in, const double *filt, int dir_base, int dim) {
+    using P = SmallTile<N1>;
+    constexpr int N2 = P::N2, N3 = N2 * N1;
+    for (int t = threadIdx.x; t < P::TILES; t += kPassThreads) {
+        double s0[2][2], s1[2][2];
+        int gt, j0, rb;
+        small_tile<N1>(in, filt, dir_base, dim, t, gt, j0, rb, s0, s1);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) out[pad<N1>(gt * N3 + (j0 + p) * N2 + rb + q * P::RB)] = s0[p][q] + s1[p][q];
+    }
+}
+template <int N1>
+__device__ __forceinline__ void small_pass_add_children(double *coef, long long child_off, long long stride, const double *in, const double *filt) {
+    using P = SmallTile<N1>;
+    constexpr int N2 = P::N2;
+    for (int t = threadIdx.x; t < P::TILES; t += kPassThreads) {
+        double s0[2][2], s1[2][2];
+        int gt, j0, rb;
+        small_tile<N1>(in, filt, 4, 2, t, gt, j0, rb, s0, s1);
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                double *dst = coef + child_off + (long long)gt * stride + (j0 + p) * N2 + rb + q * P::RB;
+                double v = *dst;
+                v += s0[p][q];
+                v += s1[p][q];
+                *dst = v;
+            }
+    }
 }
 
 template <int N1>
-__global__ __launch_bounds__(kThreads) void k_compress(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
+__global__ __launch_bounds__(kPassThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff,
+                                                                  long long stride) {
+    constexpr int N3 = N1 * N1 * N1;
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double sfl[8 * N1 * N1];
+    for (int i = threadIdx.x; i < 8 * N1 * N1; i += kPassThreads) sfl[i] = filt[i];
+    const int k = blockIdx.x;
+    const double *src = coef + poff[k];
+    for (int o = threadIdx.x; o < 8 * N3; o += kPassThreads) a[pad<N1>(o)] = src[o];
+    __syncthreads();
+    small_pass<N1>(b, a, sfl, 4, 0);
+    __syncthreads();
+    small_pass<N1>(a, b, sfl, 4, 1);
+    __syncthreads();
+    small_pass_add_children<N1>(coef, coff[k], stride, a, sfl);
+}
+
+template <int N1>
+__global__ __launch_bounds__(kPassThreads) void k_compress(const double *filt, double *coef, const long long *poff, const long long *coff, long long stride) {
     constexpr int N3 = N1 * N1 * N1;
     __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
     __shared__ double sfl[4 * N1 * N1];
-    stage_filters<N1>(sfl, filt, 4);
+    for (int i = threadIdx.x; i < 4 * N1 * N1; i += kPassThreads) sfl[i] = filt[i];
     const int k = blockIdx.x;
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) {
+    for (int o = threadIdx.x; o < 8 * N3; o += kPassThreads) {
         const int t = o / N3, e = o % N3;
         a[pad<N1>(o)] = coef[coff[k] + (long long)t * stride + e];
     }
     __syncthreads();
-    mw_pass<N1>(b, a, sfl, 0, 0);
+    small_pass<N1>(b, a, sfl, 0, 0);
     __syncthreads();
-    mw_pass<N1>(a, b, sfl, 0, 1);
+    small_pass<N1>(a, b, sfl, 0, 1);
     __syncthreads();
-    mw_pass<N1>(b, a, sfl, 0, 2);
+    small_pass<N1>(b, a, sfl, 0, 2);
     __syncthreads();
     double *dst = coef + poff[k];
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) dst[o] = b[pad<N1>(o)];
+    for (int o = threadIdx.x; o < 8 * N3; o += kPassThreads) dst[o] = b[pad<N1>(o)];
 }
 
 template <int N1> __global__ __launch_bounds__(kThreads) void k_node_norms(const double *coef, const long long *off, double *norms) {
@@ -143,7 +228,7 @@ template <int N1> struct GenerateMultiL {
 template <int N1> struct ReconL {
     static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
         ProfileScope prof_(MWG_KERNEL_RECONSTRUCT);
-        hipLaunchKernelGGL(k_reconstruct_add<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        hipLaunchKernelGGL(k_reconstruct_add<N1>, dim3(count), dim3(kPassThreads), 0, 0, filt, coef, po, co, stride);
         MWG_LAUNCHED();
         return MWG_OK;
     }
@@ -151,7 +236,7 @@ template <int N1> struct ReconL {
 template <int N1> struct CompressL {
     static mwg_status run(const double *filt, double *coef, const long long *po, const long long *co, int count, long long stride) {
         ProfileScope prof_(MWG_KERNEL_COMPRESS);
-        hipLaunchKernelGGL(k_compress<N1>, dim3(count), dim3(kThreads), 0, 0, filt, coef, po, co, stride);
+        hipLaunchKernelGGL(k_compress<N1>, dim3(count), dim3(kPassThreads), 0, 0, filt, coef, po, co, stride);
         MWG_LAUNCHED();
         return MWG_OK;
     }
