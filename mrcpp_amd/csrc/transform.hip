@@ -7,78 +7,8 @@ namespace mwgdev {
 
 namespace {
 
-// The scaling block of child c of node pid (staged filters sfl; a, b, red:
-// LDS of the workgroup), written at offs[k] (recorded in the tree) or at the
-// node's own offset, and its norms.
-template <int N1>
-__device__ __forceinline__ void generate_node(const mwg_ftree &f, const double *sfl, int k, int id, int pid, int c, const long long *offs,
-                                              long long *coef_off_w, unsigned char *is_gen_w, double *a, double *b, double *red) {
-    constexpr int N3 = N1 * N1 * N1;
-    const double *src = f.coef + f.coef_off[pid];
-    const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
-    for (int o = threadIdx.x; o < 8 * N3; o += kThreads) a[pad<N1>(o)] = (o < nblk * N3) ? src[o] : 0.0;
-    __syncthreads();
-    mw_pass<N1>(b, a, sfl, 4, 0);
-    __syncthreads();
-    mw_pass<N1>(a, b, sfl, 4, 1);
-    __syncthreads();
-    mw_pass<N1>(b, a, sfl, 4, 2);
-    __syncthreads();
-    double *dst = f.coef + (offs ? offs[k] : f.coef_off[id]);
-    if (offs && threadIdx.x == 0) {
-        coef_off_w[id] = offs[k];
-        is_gen_w[id] = 1;
-    }
-    double ss = 0.0;
-    for (int e = threadIdx.x; e < N3; e += kThreads) {
-        const double v = b[pad<N1>(c * N3 + e)];
-        dst[e] = v;
-        ss += v * v;
-    }
-    for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double v = 0.0;
-        for (int w = 0; w < kThreads / 64; w++) v += red[w];
-        const double n0 = sqrt(v);
-        double *nr = f.norms + (size_t)id * 9;
-        nr[0] = n0;
-        for (int t = 1; t < 8; t++) nr[t] = 0.0;
-        nr[8] = n0 * n0;
-    }
-}
-
-// Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
-// + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
-// With offs given, the node is also recorded in the mirror: coef_off_w[id] =
-// offs[k] (where its block goes) and is_gen_w[id] = 1.
-template <int N1>
-__global__ __launch_bounds__(kThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child,
-                                                       const long long *offs, long long *coef_off_w, unsigned char *is_gen_w) {
-    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
-    __shared__ double red[8 * (kThreads / 64)];
-    __shared__ double sfl[8 * N1 * N1];
-    stage_filters<N1>(sfl, filt, 8);
-    const int k = blockIdx.x;
-    generate_node<N1>(f, sfl, k, ids[k], parent[k], child[k], offs, coef_off_w, is_gen_w, a, b, red);
-}
-
-// The same for nodes of several trees: node k in trees[tree_of[k]].
-template <int N1>
-__global__ __launch_bounds__(kThreads) void k_generate_multi(const mwg_gen_tree *trees, const double *filt, const int *tree_of, const int *ids,
-                                                             const int *parent, const int *child, const long long *offs) {
-    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
-    __shared__ double red[8 * (kThreads / 64)];
-    __shared__ double sfl[8 * N1 * N1];
-    stage_filters<N1>(sfl, filt, 8);
-    const int k = blockIdx.x;
-    const mwg_gen_tree &t = trees[tree_of[k]];
-    generate_node<N1>(t.f, sfl, k, ids[k], parent[k], child[k], offs, t.coef_off_w, t.is_gen_w, a, b, red);
-}
-
-// The top-down and bottom-up passes are latency-bound for the few nodes of a
-// depth: their workgroups have kPassThreads threads, each forming a 2 x 2
+// The generation, top-down and bottom-up passes are latency-bound for the
+// few nodes of a launch: their workgroups have kPassThreads threads, each forming a 2 x 2
 // register tile of a pass (the same per-output sum, over i in increasing
 // order, as the 4 x 4 tiles of mw_pass, so the same results).
 constexpr int kPassThreads = 1024;
@@ -161,6 +91,80 @@ __device__ __forceinline__ void small_pass_add_children(double *coef, long long 
     }
 }
 
+// The scaling block of child c of node pid (staged filters sfl; a, b, red:
+// LDS of the workgroup), written at offs[k] (recorded in the tree) or at the
+// node's own offset, and its norms.
+template <int N1>
+__device__ __forceinline__ void generate_node(const mwg_ftree &f, const double *sfl, int k, int id, int pid, int c, const long long *offs,
+                                              long long *coef_off_w, unsigned char *is_gen_w, double *a, double *b, double *red) {
+    constexpr int N3 = N1 * N1 * N1;
+    const double *src = f.coef + f.coef_off[pid];
+    const int nblk = f.is_gen[pid] ? 1 : 8; // generated parents carry the scaling block only
+    for (int o = threadIdx.x; o < 8 * N3; o += kPassThreads) a[pad<N1>(o)] = (o < nblk * N3) ? src[o] : 0.0;
+    __syncthreads();
+    small_pass<N1>(b, a, sfl, 4, 0);
+    __syncthreads();
+    small_pass<N1>(a, b, sfl, 4, 1);
+    __syncthreads();
+    small_pass<N1>(b, a, sfl, 4, 2);
+    __syncthreads();
+    double *dst = f.coef + (offs ? offs[k] : f.coef_off[id]);
+    if (offs && threadIdx.x == 0) {
+        coef_off_w[id] = offs[k];
+        is_gen_w[id] = 1;
+    }
+    for (int e = threadIdx.x; e < N3; e += kPassThreads) dst[e] = b[pad<N1>(c * N3 + e)];
+    // the norm summed by the first kThreads threads in their order (the
+    // order of the band screen's other norms)
+    if (threadIdx.x < kThreads) {
+        double ss = 0.0;
+        for (int e = threadIdx.x; e < N3; e += kThreads) {
+            const double v = b[pad<N1>(c * N3 + e)];
+            ss += v * v;
+        }
+        for (int off = 32; off > 0; off >>= 1) ss += __shfl_xor(ss, off);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = ss;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = 0.0;
+        for (int w = 0; w < kThreads / 64; w++) v += red[w];
+        const double n0 = sqrt(v);
+        double *nr = f.norms + (size_t)id * 9;
+        nr[0] = n0;
+        for (int t = 1; t < 8; t++) nr[t] = 0.0;
+        nr[8] = n0 * n0;
+    }
+}
+
+// Scaling block of one child of an existing node (MWNode::threadSafeGenChildren
+// + giveChildrenCoefs, src/trees/MWNode.cpp:312-335,410-418).
+// With offs given, the node is also recorded in the mirror: coef_off_w[id] =
+// offs[k] (where its block goes) and is_gen_w[id] = 1.
+template <int N1>
+__global__ __launch_bounds__(kPassThreads) void k_generate(mwg_ftree f, const double *filt, const int *ids, const int *parent, const int *child,
+                                                           const long long *offs, long long *coef_off_w, unsigned char *is_gen_w) {
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double sfl[8 * N1 * N1];
+    for (int i = threadIdx.x; i < 8 * N1 * N1; i += kPassThreads) sfl[i] = filt[i];
+    const int k = blockIdx.x;
+    generate_node<N1>(f, sfl, k, ids[k], parent[k], child[k], offs, coef_off_w, is_gen_w, a, b, red);
+}
+
+// The same for nodes of several trees: node k in trees[tree_of[k]].
+template <int N1>
+__global__ __launch_bounds__(kPassThreads) void k_generate_multi(const mwg_gen_tree *trees, const double *filt, const int *tree_of, const int *ids,
+                                                                 const int *parent, const int *child, const long long *offs) {
+    __shared__ double a[Lds<N1>::size], b[Lds<N1>::size];
+    __shared__ double red[8 * (kThreads / 64)];
+    __shared__ double sfl[8 * N1 * N1];
+    for (int i = threadIdx.x; i < 8 * N1 * N1; i += kPassThreads) sfl[i] = filt[i];
+    const int k = blockIdx.x;
+    const mwg_gen_tree &t = trees[tree_of[k]];
+    generate_node<N1>(t.f, sfl, k, ids[k], parent[k], child[k], offs, t.coef_off_w, t.is_gen_w, a, b, red);
+}
+
 template <int N1>
 __global__ __launch_bounds__(kPassThreads) void k_reconstruct_add(const double *filt, double *coef, const long long *poff, const long long *coff,
                                                                   long long stride) {
@@ -211,7 +215,7 @@ template <int N1> struct GenerateL {
     static mwg_status run(const mwg_ftree *f, const double *filt, const int *ids, const int *par, const int *ch, int count, const long long *offs,
                           long long *coef_off_w, unsigned char *is_gen_w) {
         ProfileScope prof_(MWG_KERNEL_GENERATE);
-        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kThreads), 0, 0, *f, filt, ids, par, ch, offs, coef_off_w, is_gen_w);
+        hipLaunchKernelGGL(k_generate<N1>, dim3(count), dim3(kPassThreads), 0, 0, *f, filt, ids, par, ch, offs, coef_off_w, is_gen_w);
         MWG_LAUNCHED();
         return MWG_OK;
     }
@@ -220,7 +224,7 @@ template <int N1> struct GenerateMultiL {
     static mwg_status run(const mwg_gen_tree *trees, const double *filt, const int *tree_of, const int *ids, const int *par, const int *ch,
                           const long long *offs, int count) {
         ProfileScope prof_(MWG_KERNEL_GENERATE);
-        hipLaunchKernelGGL(k_generate_multi<N1>, dim3(count), dim3(kThreads), 0, 0, trees, filt, tree_of, ids, par, ch, offs);
+        hipLaunchKernelGGL(k_generate_multi<N1>, dim3(count), dim3(kPassThreads), 0, 0, trees, filt, tree_of, ids, par, ch, offs);
         MWG_LAUNCHED();
         return MWG_OK;
     }
