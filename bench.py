@@ -514,7 +514,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one MI355X: the apply path runs on a single GPU "
-              "(multi-GPU sharding is described in DESIGN.md, not launched); use --gpus 1", file=sys.stderr)
+              "(sharding across GPUs is not launched); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
@@ -611,7 +611,7 @@ def main():
     res["arithmetic"] = time_arithmetic(mr, w, f, P)
     res["tree_io"] = time_tree_io(plain["out"], w["box"])
     res["operator_build"] = time_operator_build(mr, w, mra)
-    # the north star's comparison on the 64-tree batch: one apply_batch of all
+    # the comparison on the 64-tree batch: one apply_batch of all
     # trees in the chosen form (with and without the refinement predictor)
     # against the plain FP64 apply of each tree in turn
     if not args.no_batch:
