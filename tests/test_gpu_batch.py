@@ -4,7 +4,8 @@ precision in one run of shared launches.
 Bar: every tree of the batch has the node set and applied-tuple count of its
 own apply() and of the CPU oracle, and coefficients equal to its own apply()
 up to rounding (relative L2 <= 1e-13; unit boundaries may differ when a level
-batch holds very many groups, which reorders the band sum).
+batch holds very many groups, which reorders the band sum), bit for bit for a
+small batch.
 """
 import sys
 
@@ -55,6 +56,11 @@ def test_batch_matches_single_applies(mr):
         assert st[i]["n_output_nodes"] == solo_st[i]["n_output_nodes"]
         rel = np.linalg.norm(bc - sc) / np.linalg.norm(sc)
         assert rel <= 1e-13, rel
+        # a small batch uses the single applies' work units, and the band
+        # nodes of all its trees generated together are the trees' own: the
+        # same coefficients bit for bit
+        assert np.array_equal(bc, sc), float(np.abs(bc - sc).max())
+        assert st[i]["n_generated"] == solo_st[i]["n_generated"]
 
 
 def test_batch_matches_oracle(mr, orc):
@@ -84,6 +90,11 @@ def test_batch_shared_input_and_errors(mr):
     bi, bc, _ = b.export()
     assert np.array_equal(ai, bi) and np.array_equal(ac, bc)
     assert st[0]["n_applied"] == st[1]["n_applied"]
+    # and both equal the input's single apply (planned in task order on one mirror)
+    s = mr.FunctionTree(mra)
+    mr.apply(t["prec"], s, t["P"], t["f"])
+    si, sc, _ = s.export()
+    assert np.array_equal(ai, si) and np.array_equal(ac, sc)
     # an output tree twice is rejected
     c = mr.FunctionTree(mra)
     with pytest.raises(mr.MRCPPError):
