@@ -958,8 +958,10 @@ template <> struct RowGeom<MWG_PREC_FP32> {
     __device__ static inline int atT2(int b, int kk, int r) { return b * 144 + kk * 16 + (r ^ (8 * ((kk >> 1) & 1))); }
 };
 
+// FP32: at least 5 waves per SIMD (the registers allow 4 otherwise; a few
+// spilled bytes cost less than the latency the fifth wave hides).
 template <int PREC>
-__global__ __launch_bounds__(256) void k_band_rows(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
+__global__ __launch_bounds__(256, PREC == MWG_PREC_FP32 ? 5 : 1) void k_band_rows(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
                                                    const int4 *pdl, const long long *pfo, const int *gstart, const int4 *units,
                                                    const long long *uscan, int L, double *partial, int *counter) {
     constexpr int N1 = 8, N3 = 512, KSTEPS = 2;
