@@ -692,8 +692,10 @@ __device__ inline void settled(const uint4 &x) {
     asm volatile("" ::"v"(a), "v"(b), "v"(c), "v"(d));
 }
 
+// Order 9 (N1 = 10): at least 2 waves per SIMD (the registers allowed one;
+// 3 spills too much).
 template <int N1, int PREC>
-__global__ __launch_bounds__(256) void k_band_mfma(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
+__global__ __launch_bounds__(256, N1 == 10 ? 2 : 1) void k_band_mfma(const mwg_segment *segs, const int *node_seg, const long long *g_off, mwg_pairs pr, const int2 *grec,
                                                    const int4 *pdl, const long long *pfo, const int *gstart, const int4 *units,
                                                    const long long *uscan, int L, double *partial, int *counter) {
     using S = SweepShape<N1>;
